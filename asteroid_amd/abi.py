@@ -1,0 +1,135 @@
+"""ctypes mirror of include/asteroid.h and the loader of libasteroid_hip.so.
+
+The structures here are byte-for-byte the C ABI's; the loader fails loudly
+when the HIP library has not been built (there is no CPU fallback).
+"""
+import ctypes as C
+import os
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libasteroid_hip.so")
+
+# Property columns and periodic codes (asteroid.h:115-131).
+P_TYPE, P_INDEPENDENT, P_FROZEN, P_ISEG, P_MULTI_SEGMENT, P_PERIODIC = range(6)
+N_COLUMNS = 6
+PERIODIC, PERIODIC_LAM, SOFT_BOTH, HARD_LEFT, HARD_RIGHT, HARD_BOTH = 1, 0, -1, -2, -3, -4
+MAX_PARAMS, MAX_TYPES, MAX_FILTERS = 16, 16, 8
+
+PREC_FP64, PREC_F32SIMPLEX, PREC_MIXED = 0, 1, 2
+
+ERRORS = {
+    0: "AST_OK", -1: "AST_E_ARG", -2: "AST_E_STATE", -3: "AST_E_HIP", -4: "AST_E_NOMEM",
+    -5: "AST_E_DATA", -6: "AST_E_UNSUPPORTED", -7: "AST_E_NODEVICE",
+}
+
+
+class Variant(C.Structure):
+    _fields_ = [("torque", C.c_int), ("bc", C.c_int), ("bw_ball", C.c_int), ("p_psi", C.c_int),
+                ("interp", C.c_int), ("precision", C.c_int), ("time_step", C.c_double)]
+
+
+class TypeIds(C.Structure):
+    _fields_ = [(n, C.c_int) for n in (
+        "theta_M", "phi_M", "phi_0", "L", "Ti", "Ts", "Tl", "c_tumb", "b_tumb", "Es", "psi_0",
+        "c", "b", "theta_R", "phi_R", "kappa", "n_types", "n_params")]
+
+
+class Obs(C.Structure):
+    _fields_ = [("mjd", C.c_double), ("V", C.c_float), ("w", C.c_float), ("filter", C.c_int),
+                ("pad_", C.c_int), ("E", C.c_double * 3), ("S", C.c_double * 3)]
+
+
+class Ephem3(C.Structure):
+    _fields_ = [("E", (C.c_double * 3) * 3), ("S", (C.c_double * 3) * 3), ("mjd", C.c_double * 3)]
+
+
+class X2(C.Structure):
+    _fields_ = [("reopt", C.c_int), ("ppsi1", C.c_float), ("ppsi2", C.c_float)]
+
+
+class RunCfg(C.Structure):
+    _fields_ = [("n_groups", C.c_int), ("group_size", C.c_int), ("nstages", C.c_int),
+                ("n_steps", C.c_int), ("eval_budget", C.c_int64), ("size_min", C.c_double),
+                ("ticks_per_launch", C.c_int)]
+
+
+assert C.sizeof(Obs) == 72
+
+
+def type_ids(variant):
+    """ast_variant_type_ids of include/asteroid.h (asteroid.h:137-178 order)."""
+    ids = {}
+    k = 0
+    for name in ("theta_M", "phi_M", "phi_0", "L"):
+        ids[name] = k
+        k += 1
+    for name in ("Ti", "Ts", "Tl"):
+        ids[name] = -1
+    if variant.torque:
+        for name in ("Ti", "Ts", "Tl"):
+            ids[name] = k
+            k += 1
+    for name in ("c_tumb", "b_tumb", "Es", "psi_0"):
+        ids[name] = k
+        k += 1
+    ids["c"] = ids["b"] = -1
+    if variant.bc:
+        ids["c"], ids["b"] = k, k + 1
+        k += 2
+    ids["theta_R"] = ids["phi_R"] = ids["kappa"] = -1
+    if variant.bw_ball:
+        ids["theta_R"], ids["phi_R"], ids["kappa"] = k, k + 1, k + 2
+        k += 3
+    ids["n_types"] = k
+    ids["n_params"] = 8 + 2 * bool(variant.bc) + 3 * bool(variant.torque) + 3 * bool(variant.bw_ball)
+    return ids
+
+
+_lib = None
+
+
+def load():
+    """Load libasteroid_hip.so; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(there is no CPU fallback for the fitting kernels)")
+    lib = C.CDLL(LIB_PATH)
+    P, I, D = C.c_void_p, C.c_int, C.c_double
+    pI, pD, pI64 = C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int64)
+    sig = {
+        "ast_ctx_create": (I, [C.POINTER(C.c_void_p), C.POINTER(Variant), I]),
+        "ast_ctx_destroy": (I, [P]),
+        "ast_set_model_space": (I, [P, I, pI, I, pI, pD, C.POINTER(X2)]),
+        "ast_upload_curve": (I, [P, C.POINTER(Obs), I, I, C.POINTER(Ephem3)]),
+        "ast_rng_seed": (I, [P, C.c_uint64]),
+        "ast_reset_best": (I, [P]),
+        "ast_set_start": (I, [P, pD]),
+        "ast_run": (I, [P, C.POINTER(RunCfg)]),
+        "ast_fetch": (I, [P, pD, pD, pD]),
+        "ast_fetch_lanes": (I, [P, pD, pI64, pI64]),
+        "ast_chi2_batch": (I, [P, pD, I, pD, pD]),
+        "ast_last_timing": (I, [P, pD, pI]),
+        "ast_last_counts": (I, [P, pI64, pI64]),
+        "ast_default_model_space": (I, [C.POINTER(Variant), pD, pI, pI, pI, pI, pD, C.POINTER(X2)]),
+        "ast_strerror": (C.c_char_p, [I]),
+        "ast_last_hip_error": (C.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+EXPORTED = (
+    "ast_ctx_create", "ast_ctx_destroy", "ast_set_model_space", "ast_upload_curve", "ast_rng_seed",
+    "ast_reset_best", "ast_set_start", "ast_run", "ast_fetch", "ast_fetch_lanes", "ast_chi2_batch",
+    "ast_last_timing", "ast_last_counts", "ast_default_model_space", "ast_strerror",
+    "ast_last_hip_error",
+)

@@ -1,0 +1,150 @@
+"""Python view of the C ABI (include/asteroid.h) over ctypes.
+
+`Context` mirrors the boundary one call per method; errors raise
+`AsteroidError` with the ABI's code and message (the reference printed and
+exited, cuda_errors.h:4-14).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+
+class AsteroidError(RuntimeError):
+    def __init__(self, code, where):
+        lib = abi.load()
+        msg = lib.ast_strerror(code).decode()
+        hip = lib.ast_last_hip_error().decode()
+        super().__init__(f"{where}: {abi.ERRORS.get(code, code)} ({msg}){' ' + hip if hip else ''}")
+        self.code = code
+
+
+def _chk(rc, where):
+    if rc != 0:
+        raise AsteroidError(rc, where)
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    """One fitting context on one GPU (ast_ctx)."""
+
+    def __init__(self, variant, device=0):
+        self.lib = abi.load()
+        self.variant = variant
+        self.ids = abi.type_ids(variant)
+        h = C.c_void_p()
+        _chk(self.lib.ast_ctx_create(C.byref(h), C.byref(variant), device), "ast_ctx_create")
+        self.h = h
+        self.space = None
+        self.curve = None
+
+    def close(self):
+        if self.h:
+            self.lib.ast_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_model_space(self, space):
+        prop = np.ascontiguousarray(space.property.ravel(), np.int32)
+        types = np.ascontiguousarray(space.types, np.int32)
+        lim = np.ascontiguousarray(space.limits, np.float64)
+        x2 = space.x2()
+        _chk(self.lib.ast_set_model_space(self.h, space.n_params, prop.ctypes.data_as(C.POINTER(C.c_int)),
+                                          space.n_types, types.ctypes.data_as(C.POINTER(C.c_int)), _dp(lim),
+                                          C.byref(x2)), "ast_set_model_space")
+        self.space = space
+
+    def upload_curve(self, curve):
+        obs = curve.obs_array()
+        eph = curve.ephem3()
+        _chk(self.lib.ast_upload_curve(self.h, obs, curve.n_data, curve.n_filters, C.byref(eph)),
+             "ast_upload_curve")
+        self.curve = curve
+
+    def rng_seed(self, seed):
+        _chk(self.lib.ast_rng_seed(self.h, int(seed)), "ast_rng_seed")
+
+    def reset_best(self):
+        _chk(self.lib.ast_reset_best(self.h), "ast_reset_best")
+
+    def set_start(self, params0):
+        p = np.ascontiguousarray(params0, np.float64)
+        _chk(self.lib.ast_set_start(self.h, _dp(p)), "ast_set_start")
+
+    def run(self, n_groups, group_size=256, nstages=1, n_steps=5000, eval_budget=0, size_min=0.0,
+            ticks_per_launch=0):
+        cfg = abi.RunCfg(n_groups, group_size, nstages, n_steps, eval_budget, size_min, ticks_per_launch)
+        _chk(self.lib.ast_run(self.h, C.byref(cfg)), "ast_run")
+        self._last_groups = n_groups
+        self._last_fits = n_groups * group_size
+
+    def fetch(self):
+        G, N, NF = self._last_groups, self.space.n_params, self.curve.n_filters
+        f = np.zeros(G)
+        p = np.zeros((G, N))
+        dv = np.zeros((G, NF))
+        _chk(self.lib.ast_fetch(self.h, _dp(f), _dp(p), _dp(dv)), "ast_fetch")
+        return f, p, dv
+
+    def fetch_lanes(self):
+        n = self._last_fits
+        f = np.zeros(n)
+        ne = np.zeros(n, np.int64)
+        ns = np.zeros(n, np.int64)
+        _chk(self.lib.ast_fetch_lanes(self.h, _dp(f), ne.ctypes.data_as(C.POINTER(C.c_int64)),
+                                      ns.ctypes.data_as(C.POINTER(C.c_int64))), "ast_fetch_lanes")
+        return f, ne, ns
+
+    def chi2_batch(self, params):
+        p = np.ascontiguousarray(np.atleast_2d(params), np.float64)
+        n = p.shape[0]
+        chi2 = np.zeros(n)
+        dv = np.zeros((n, self.curve.n_filters))
+        _chk(self.lib.ast_chi2_batch(self.h, _dp(p), n, _dp(chi2), _dp(dv)), "ast_chi2_batch")
+        return chi2, dv
+
+    def last_timing(self):
+        ms = C.c_double()
+        n = C.c_int()
+        _chk(self.lib.ast_last_timing(self.h, C.byref(ms), C.byref(n)), "ast_last_timing")
+        return ms.value, n.value
+
+    def last_counts(self):
+        e = C.c_int64()
+        p = C.c_int64()
+        _chk(self.lib.ast_last_counts(self.h, C.byref(e), C.byref(p)), "ast_last_counts")
+        return e.value, p.value
+
+
+def default_model_space_c(variant, ppsi_hours=(2.0, 4800.0)):
+    """The library's C++ copy of the reference tables (for cross-checking)."""
+    lib = abi.load()
+    n_params = C.c_int()
+    n_types = C.c_int()
+    prop = np.zeros(abi.MAX_PARAMS * abi.N_COLUMNS, np.int32)
+    types = np.zeros(abi.MAX_TYPES, np.int32)
+    lim = np.zeros(2 * abi.MAX_TYPES)
+    x2 = abi.X2()
+    pp = np.array(ppsi_hours, float)
+    _chk(lib.ast_default_model_space(C.byref(variant), _dp(pp), C.byref(n_params),
+                                     prop.ctypes.data_as(C.POINTER(C.c_int)), C.byref(n_types),
+                                     types.ctypes.data_as(C.POINTER(C.c_int)), _dp(lim), C.byref(x2)),
+         "ast_default_model_space")
+    n, nt = n_params.value, n_types.value
+    return dict(property=prop[:n * 6].reshape(n, 6), types=types[:nt], limits=lim[:2 * nt],
+                ppsi1=x2.ppsi1, ppsi2=x2.ppsi2)

@@ -1,0 +1,6 @@
+// Kernel instantiations of one model variant (TORQUE=false, BC=true, BW_BALL=false);
+// one translation unit per variant so the variants compile in parallel.
+#include "ast_kernels.h"
+namespace ast {
+template hipError_t launch_variant<false, true, false>(const LaunchArgs&);
+}

@@ -1,0 +1,6 @@
+// Kernel instantiations of one model variant (TORQUE=true, BC=true, BW_BALL=false);
+// one translation unit per variant so the variants compile in parallel.
+#include "ast_kernels.h"
+namespace ast {
+template hipError_t launch_variant<true, true, false>(const LaunchArgs&);
+}

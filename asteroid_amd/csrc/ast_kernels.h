@@ -1,0 +1,645 @@
+// ast_kernels.h -- the fitting kernels (template definitions; each variant
+// is instantiated in its own translation unit, ast_k_*.hip).
+//
+// The simplex of chi2_gpu (cuda.c:1424-1807) calls chi2one from five places
+// (initial vertices, reflection, expansion, contraction, shrink), so the
+// lanes of a wavefront diverge across the most expensive call in the
+// program.  Here the Nelder-Mead control flow is a per-lane state machine:
+// every "tick" each lane holds at most one pending point, all lanes
+// evaluate their points through ONE call of the evaluator (divergence-free,
+// since every model walks the same observation loop), and each lane then
+// advances its own state machine with the result until it needs the next
+// point.  The sequence of points a lane evaluates, the comparisons it makes
+// and every stored value are those of the reference control flow, so a
+// lane's trajectory is the oracle's trajectory.
+//
+// Simplex state lives in HBM in lane-fastest (structure-of-arrays) layout,
+// so every access of a wavefront is one contiguous, coalesced segment.  A
+// workgroup is one reseed group (the reference's block): after every tick
+// it votes on whether all of its simplexes finished the stage; if so the
+// block argmin, stage reseed or best-per-group write of cuda.c:1739-1823 run
+// in place.  Kernel launches are chunks of ticks; all state persists in
+// HBM between launches.
+#pragma once
+#include "ast_device.h"
+
+namespace ast {
+
+enum Phase : int {
+  PH_START = 0,     // begin a stage: draw the start point and the simplex
+  PH_INIT = 1,      // pending/complete: chi^2 of initial vertex j
+  PH_BEGIN = 2,     // internal: top of the simplex loop (sort, centroid, tests)
+  PH_REFLECT = 3,   // pending/complete: reflected point
+  PH_EXPAND = 4,    // pending/complete: expanded point
+  PH_CONTRACT = 5,  // pending/complete: contracted point
+  PH_SHRINK = 6,    // pending/complete: shrunk vertex ind[j]
+  PH_SHRINK_NEXT = 7,  // internal: shrink the next vertex
+  PH_STAGE_DONE = 8,
+  PH_FIT_DONE = 9
+};
+
+__device__ inline bool phase_pending(int ph) {
+  return ph == PH_INIT || ph == PH_REFLECT || ph == PH_EXPAND || ph == PH_CONTRACT || ph == PH_SHRINK;
+}
+
+// Device buffers of one fitting job (n_fits simplexes in n_groups groups).
+template <class CF>
+struct FitBufs {
+  CF* X;        // [(N+1)*N][n_fits]  simplex vertices
+  CF* F;        // [N+1][n_fits]      chi^2 at the vertices
+  uint8_t* IND; // [N+1][n_fits]      sorted vertex indices
+  CF* X0;       // [N][n_fits]        centroid
+  CF* XR;       // [N][n_fits]        reflected point
+  CF* XP;       // [N][n_fits]        expanded / contracted point
+  double* PRM;  // [N][n_fits]        the thread's params[] buffer (cuda.c:1344)
+  CF* DV;       // [MAXF][n_fits]     delta_V of the last evaluation
+  int32_t* ctl;       // phase | j<<4 | failed<<9 | bad<<10
+  int32_t* lstep;     // simplex steps in this stage (l, cuda.c:1436)
+  int32_t* nev;       // chi^2 evaluations in this stage
+  int32_t* nev_tot;   // evaluations, all stages of this job
+  int32_t* steps_tot; // steps, all stages of this job
+  uint32_t* rng;      // draws taken from the simplex's stream
+  CF* fr;             // f_r kept for the expansion test
+  CF* sf;             // stage-final best chi^2 (s_f, cuda.c:1739-1742)
+  int32_t* g_reopt;   // per group: s_x2_params.reopt
+  int32_t* g_stage;   // per group: current stage
+  int32_t* g_done;    // per group: all stages finished
+  CF* g_best_f;       // d_f
+  double* g_best_params;  // d_params [n_groups][N]
+  double* g_best_dV;      // d_dV [n_groups][n_filters]
+  int32_t* active_groups; // counter: groups still running after a launch
+};
+
+template <class CF>
+struct FitCfg {
+  int n_fits;
+  int nstages;
+  int n_steps;
+  int eval_budget;   // 0: none
+  CF size2_min;
+  uint64_t seed;
+  int ticks;
+  int n_data, n_filters, dof;
+  int reopt;         // job-level reopt (init kernel)
+};
+
+template <class CF>
+struct LaneRegs {
+  int phase, j, failed, bad, l, nev, nev_tot, steps_tot;
+  uint32_t rng;
+  CF fr, sf;
+};
+
+template <class CF, bool TQ, bool BC, bool BW>
+struct Machine {
+  using P = Pos<TQ, BC, BW>;
+  static constexpr int N = P::N;
+  const Space<CF>& sp;
+  const FitBufs<CF>& B;
+  const FitCfg<CF>& C;
+  size_t fit, nf;
+  int reopt;
+
+  __device__ CF& X(int v, int i) const { return B.X[((size_t)v * N + i) * nf + fit]; }
+  __device__ CF& F(int v) const { return B.F[(size_t)v * nf + fit]; }
+  __device__ uint8_t& IND(int v) const { return B.IND[(size_t)v * nf + fit]; }
+  __device__ CF& X0(int i) const { return B.X0[(size_t)i * nf + fit]; }
+  __device__ CF& XR(int i) const { return B.XR[(size_t)i * nf + fit]; }
+  __device__ CF& XP(int i) const { return B.XP[(size_t)i * nf + fit]; }
+  __device__ double& PRM(int i) const { return B.PRM[(size_t)i * nf + fit]; }
+
+  // x2params on a point, through the persistent params[] buffer.
+  __device__ bool propose(const CF* xv) const {
+    double prm[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) prm[i] = PRM(i);
+    int rc = x2params<CF, TQ, BC, BW>(xv, prm, sp, reopt);
+#pragma unroll
+    for (int i = 0; i < N; ++i) PRM(i) = prm[i];
+    return rc == 0;
+  }
+  __device__ bool propose_row(int v) const {
+    CF xv[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) xv[i] = X(v, i);
+    return propose(xv);
+  }
+  __device__ bool propose_xr() const {
+    CF xv[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) xv[i] = XR(i);
+    return propose(xv);
+  }
+  __device__ bool propose_xp() const {
+    CF xv[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) xv[i] = XP(i);
+    return propose(xv);
+  }
+
+  __device__ float rnd(LaneRegs<CF>& R) const { return astrng::uniform01(C.seed, fit, R.rng++); }
+
+  // Stage start (cuda.c:1456-1546): x[0] random or around the reoptimised
+  // point, then the N displaced vertices.
+  __device__ void start(LaneRegs<CF>& R) const {
+    const CF DX_INI = (CF)0.001, D2X_INI = (CF)-1.2, DX_RAND = (CF)0.0001;  // asteroid.h:205-209
+    int LAM = 0;
+    CF x0v[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) x0v[i] = X(0, i);
+    for (int i = 0; i < N; ++i) {
+      float r = rnd(R);
+      if (BC && !reopt) {
+        if (i == P::c) { x0v[i] = x0v[P::c_tumb]; continue; }
+        if (i == P::b) { x0v[i] = x0v[P::b_tumb]; continue; }
+      }
+      int per = sp.periodic[i];
+      CF xi;
+      if (!reopt || (reopt && sp.frozen[i] == -1)) {
+        xi = DX_INI + SMALL + r * (1.0 - 2 * (SMALL + DX_INI));
+      } else {
+        CF xmin = x0v[i] - DX_RAND;
+        CF xmax = x0v[i] + DX_RAND;
+        if (xmin < SMALL && hard_left(per, LAM)) xmin = SMALL;
+        if (xmax > 1.0 - SMALL && hard_right(per, LAM)) xmax = 1.0 - SMALL;
+        xi = xmin + r * (xmax - xmin);
+      }
+      x0v[i] = xi;
+      if (i == P::Es) LAM = xi >= 0.5;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) X(0, i) = x0v[i];
+    for (int jj = 1; jj < N + 1; ++jj) {
+      for (int i = 0; i < N; ++i) {
+        if (i == jj - 1) {
+          int per = sp.periodic[i];
+          float d2x = rnd(R);
+          CF dx = DX_INI * cf_exp((CF)(D2X_INI * d2x));
+          if (rnd(R) < 0.5) {
+            dx = -dx;
+            if (x0v[i] + dx < SMALL && hard_left(per, LAM)) dx = -dx;
+          } else {
+            if (x0v[i] + dx > 1.0 - SMALL && hard_right(per, LAM)) dx = -dx;
+          }
+          X(jj, i) = x0v[i] + dx;
+        } else {
+          X(jj, i) = x0v[i];
+        }
+      }
+    }
+  }
+
+  __device__ void finish(LaneRegs<CF>& R) const {
+    R.sf = R.failed ? (CF)1e30 : F(IND(0));
+    R.steps_tot += R.l;
+    R.phase = PH_STAGE_DONE;
+  }
+
+  __device__ void replace_worst_with_xr(CF fv) const {
+    const int w = IND(N);
+#pragma unroll
+    for (int i = 0; i < N; ++i) X(w, i) = XR(i);
+    F(w) = fv;
+  }
+  __device__ void replace_worst_with_xp(CF fv) const {
+    const int w = IND(N);
+#pragma unroll
+    for (int i = 0; i < N; ++i) X(w, i) = XP(i);
+    F(w) = fv;
+  }
+
+  // Advance the lane until it has a pending point or its stage is over.
+  // `val` is the chi^2 of the pending point when has_result.
+  __device__ __attribute__((noinline)) void advance(LaneRegs<CF>& R, CF val, bool has_result) const {
+    const CF ALPHA = 1.0, GAMMA = 2.0, RHO = 0.5, SIGMA = 0.5;  // asteroid.h:239-242
+    if (!has_result && R.phase != PH_START) return;
+    for (;;) {
+      switch (R.phase) {
+        case PH_START: {
+          start(R);
+          R.l = 0;
+          R.nev = 0;
+          R.failed = 0;
+          R.bad = 0;
+          R.j = 0;
+          R.phase = PH_INIT;
+          if (propose_row(0)) return;
+          R.failed = 1;
+          R.phase = PH_BEGIN;
+          break;
+        }
+        case PH_INIT: {  // cuda.c:1550-1559
+          F(R.j) = val;
+          R.j++;
+          if (R.j <= N) {
+            if (propose_row(R.j)) return;
+            R.failed = 1;
+          }
+          R.phase = PH_BEGIN;
+          break;
+        }
+        case PH_BEGIN: {  // cuda.c:1568-1645
+          if (R.failed) { finish(R); return; }
+          R.l++;
+          CF f[N + 1];
+          int ind[N + 1];
+#pragma unroll
+          for (int v = 0; v <= N; ++v) { f[v] = F(v); ind[v] = IND(v); }
+          bool taken[N + 1];
+#pragma unroll
+          for (int v = 0; v <= N; ++v) taken[v] = false;
+          for (int v = 0; v <= N; ++v) {
+            CF fmin = 1e30;
+            int jmin = -1;
+#pragma unroll
+            for (int v2 = 0; v2 <= N; ++v2)
+              if (!taken[v2] && f[v2] <= fmin) { fmin = f[v2]; jmin = v2; }
+            if (jmin < 0) {  // NaN values left (cuda.c:1592-1598)
+              ind[0] = 0;
+              F(0) = 1e30;
+              break;
+            }
+            ind[v] = jmin;
+#pragma unroll
+            for (int v2 = 0; v2 <= N; ++v2)
+              if (v2 == jmin) taken[v2] = true;
+          }
+#pragma unroll
+          for (int v = 0; v <= N; ++v) IND(v) = (uint8_t)ind[v];
+          CF x0[N];
+          for (int i = 0; i < N; ++i) {
+            CF sum = 0.0;
+            for (int v = 0; v < N + 1; ++v) sum = sum + X(v, i);
+            x0[i] = sum / (N + 1);
+          }
+          CF size2 = 0.0;
+          for (int v = 0; v < N + 1; ++v) {
+            CF sum = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+              CF dx = X(v, i) - x0[i];
+              sum = sum + dx * dx;
+            }
+            size2 = size2 + sum;
+          }
+          size2 = size2 / N;
+          if (size2 < C.size2_min || R.l > C.n_steps || (C.eval_budget > 0 && R.nev >= C.eval_budget)) {
+            finish(R);
+            return;
+          }
+          const int w = ind[N];
+#pragma unroll
+          for (int i = 0; i < N; ++i) {
+            X0(i) = x0[i];
+            CF xw = X(w, i);
+            XR(i) = (sp.frozen[i] != 1) ? (CF)(x0[i] + ALPHA * (x0[i] - xw)) : xw;
+          }
+          R.phase = PH_REFLECT;
+          if (propose_xr()) return;
+          val = 1e30;
+          break;
+        }
+        case PH_REFLECT: {  // cuda.c:1646-1712 (decisions after the reflection)
+          R.fr = val;
+          const CF fb = F(IND(0));
+          if (val >= fb && val < F(IND(N - 1))) {
+            replace_worst_with_xr(val);
+            R.phase = PH_BEGIN;
+            break;
+          }
+          if (val < fb) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+              CF xr = XR(i);
+              XP(i) = (sp.frozen[i] != 1) ? (CF)(X0(i) + GAMMA * (xr - X0(i))) : xr;
+            }
+            R.phase = PH_EXPAND;
+            if (propose_xp()) return;
+            val = 1e30;
+            break;
+          }
+          const int w = IND(N);
+#pragma unroll
+          for (int i = 0; i < N; ++i) {
+            CF xw = X(w, i);
+            XP(i) = (sp.frozen[i] != 1) ? (CF)(X0(i) + RHO * (xw - X0(i))) : xw;
+          }
+          R.phase = PH_CONTRACT;
+          if (propose_xp()) return;
+          val = 1e30;
+          break;
+        }
+        case PH_EXPAND: {  // cuda.c:1671-1689
+          if (val < R.fr) replace_worst_with_xp(val);
+          else replace_worst_with_xr(R.fr);
+          R.phase = PH_BEGIN;
+          break;
+        }
+        case PH_CONTRACT: {  // cuda.c:1703-1712
+          if (val < F(IND(N))) {
+            replace_worst_with_xp(val);
+            R.phase = PH_BEGIN;
+            break;
+          }
+          R.bad = 0;
+          R.j = 1;
+          R.phase = PH_SHRINK_NEXT;
+          break;
+        }
+        case PH_SHRINK: {  // cuda.c:1726
+          F(IND(R.j)) = val;
+          R.j++;
+          R.phase = PH_SHRINK_NEXT;
+          break;
+        }
+        case PH_SHRINK_NEXT: {  // cuda.c:1716-1733
+          while (R.j <= N) {
+            const int v = IND(R.j), b = IND(0);
+            CF xv[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+              CF xi = X(v, i);
+              if (sp.frozen[i] != 1) xi = X(b, i) + SIGMA * (xi - X(b, i));
+              xv[i] = xi;
+              X(v, i) = xi;
+            }
+            if (propose(xv)) { R.phase = PH_SHRINK; return; }
+            R.bad = 1;
+            R.j++;
+          }
+          if (R.bad) R.failed = 1;
+          R.phase = PH_BEGIN;
+          break;
+        }
+        default:
+          return;
+      }
+    }
+  }
+};
+
+// --------------------------------------------------------------- init kernel
+// Job start: identity sort order, stage-0 start point (params2x of the
+// reoptimisation model, cuda.c:1409-1416), counters, per-group flags.
+template <class CF, bool TQ, bool BC, bool BW>
+__global__ void fit_init_kernel(Space<CF> sp, FitBufs<CF> B, FitCfg<CF> C, const double* params0, int n_groups,
+                                int group_size) {
+  using P = Pos<TQ, BC, BW>;
+  constexpr int N = P::N;
+  const size_t nf = C.n_fits;
+  const size_t fit = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (fit < nf) {
+    for (int v = 0; v <= N; ++v) B.IND[v * nf + fit] = (uint8_t)v;
+    double prm[N];
+    for (int i = 0; i < N; ++i) prm[i] = C.reopt ? params0[i] : 0.0;
+    for (int i = 0; i < N; ++i) B.PRM[i * nf + fit] = prm[i];
+    if (C.reopt) {
+      CF x[N];
+      params2x<CF, TQ, BC, BW>(x, prm, sp);
+      for (int i = 0; i < N; ++i) B.X[(size_t)i * nf + fit] = x[i];
+    }
+    for (int m = 0; m < MAXF; ++m) B.DV[m * nf + fit] = 0;
+    B.ctl[fit] = PH_START;
+    B.lstep[fit] = 0;
+    B.nev[fit] = 0;
+    B.nev_tot[fit] = 0;
+    B.steps_tot[fit] = 0;
+    B.fr[fit] = 0;
+    B.sf[fit] = (CF)1e30;
+  }
+  if (fit < (size_t)n_groups) {
+    B.g_reopt[fit] = C.reopt;
+    B.g_stage[fit] = 0;
+    B.g_done[fit] = 0;
+  }
+}
+
+// One lane's evaluation inside the fitting kernel: parameters from the
+// lane's params[] buffer, delta_V to its last-dV slot.  Kept out of line so
+// the evaluator's register allocation does not compete with the simplex
+// bookkeeping live across it.
+template <class CF, bool TQ, bool BC, bool BW, int NF>
+__device__ __attribute__((noinline)) CF eval_lane(const double* PRM, CF* DV, size_t nf, size_t fit,
+                                                   const PointRec* pts_g, int n_data, int n_filters, int dof,
+                                                   const double* sum_w) {
+  constexpr int N = Pos<TQ, BC, BW>::N;
+  // Arguments of a device function arrive in vector registers; the table
+  // pointer and the sizes are wave-uniform, so move them to scalar registers
+  // to keep the observation loop on scalar loads and scalar branches.
+  const uint64_t pu = (uint64_t)(uintptr_t)pts_g;
+  const uint64_t plo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pu);
+  const uint64_t phi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(pu >> 32));
+  PointPtr pts = (PointPtr)(uintptr_t)((phi << 32) | plo);
+  n_data = __builtin_amdgcn_readfirstlane(n_data);
+  n_filters = __builtin_amdgcn_readfirstlane(n_filters);
+  dof = __builtin_amdgcn_readfirstlane(dof);
+  double prm[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) prm[i] = PRM[(size_t)i * nf + fit];
+  CF dv[NF];
+  CF fv = chi2_model<CF, TQ, BC, BW, NF>(prm, pts, n_data, n_filters, dof, sum_w, dv);
+#pragma unroll
+  for (int m = 0; m < NF; ++m)
+    if (m < n_filters) DV[(size_t)m * nf + fit] = dv[m];
+  return fv;
+}
+
+// ---------------------------------------------------------------- fit kernel
+template <class CF, bool TQ, bool BC, bool BW, int NF>
+__global__ __launch_bounds__(1024) void fit_kernel(Space<CF> sp, const PointRec* pts_g, const double* sum_w,
+                                                   FitBufs<CF> B, FitCfg<CF> C) {
+  using P = Pos<TQ, BC, BW>;
+  constexpr int N = P::N;
+  __shared__ CF s_f[1024];
+  __shared__ CF s_x0[MAXP];
+  __shared__ int s_tm;
+  __shared__ CF s_smin;
+  const int g = blockIdx.x;
+  const int t = threadIdx.x;
+  const int G = blockDim.x;
+  const size_t nf = C.n_fits;
+  const size_t fit = (size_t)g * G + t;
+  if (B.g_done[g]) return;  // same value for every thread of the group
+  int reopt = B.g_reopt[g];
+  int stage = B.g_stage[g];
+  LaneRegs<CF> R;
+  const int ctl = B.ctl[fit];
+  R.phase = ctl & 15;
+  R.j = (ctl >> 4) & 31;
+  R.failed = (ctl >> 9) & 1;
+  R.bad = (ctl >> 10) & 1;
+  R.l = B.lstep[fit];
+  R.nev = B.nev[fit];
+  R.nev_tot = B.nev_tot[fit];
+  R.steps_tot = B.steps_tot[fit];
+  R.rng = B.rng[fit];
+  R.fr = B.fr[fit];
+  R.sf = B.sf[fit];
+  bool done = false;
+  {
+    Machine<CF, TQ, BC, BW> M{sp, B, C, fit, nf, reopt};
+    M.advance(R, (CF)0, false);  // a lane at a stage start draws its simplex first
+  }
+  for (int tick = 0; tick < C.ticks && !done; ++tick) {
+    Machine<CF, TQ, BC, BW> M{sp, B, C, fit, nf, reopt};
+    if (phase_pending(R.phase)) {
+      CF fv = eval_lane<CF, TQ, BC, BW, NF>(B.PRM, B.DV, nf, fit, pts_g, C.n_data, C.n_filters, C.dof, sum_w);
+      R.nev++;
+      R.nev_tot++;
+      M.advance(R, fv, true);
+    }
+    const int running = __syncthreads_count(phase_pending(R.phase));
+    if (running) continue;
+    // Every simplex of the group finished the stage (cuda.c:1739-1823).
+    s_f[t] = R.sf;
+    __syncthreads();
+    if (t == 0) {
+      int tm = 0;
+      CF smin = (CF)1e30;
+      for (int k = 0; k < G; ++k)
+        if (s_f[k] < smin) { smin = s_f[k]; tm = k; }
+      s_tm = tm;
+      s_smin = smin;
+    }
+    __syncthreads();
+    const int tm = s_tm;
+    const CF smin = s_smin;
+    if (C.nstages > 1 && stage < C.nstages - 1) {
+      if (t == tm) {
+        CF xb[N];
+        const int b = M.IND(0);
+#pragma unroll
+        for (int i = 0; i < N; ++i) xb[i] = M.X(b, i);
+        if (sp.p_psi && reopt == 0) {  // cuda.c:1791-1798
+          double prm[N];
+#pragma unroll
+          for (int i = 0; i < N; ++i) prm[i] = M.PRM(i);
+          x2params<CF, TQ, BC, BW>(xb, prm, sp, reopt);
+          params2x<CF, TQ, BC, BW>(xb, prm, sp);
+#pragma unroll
+          for (int i = 0; i < N; ++i) { M.PRM(i) = prm[i]; M.X(b, i) = xb[i]; }
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) s_x0[i] = xb[i];
+      }
+      __syncthreads();
+      reopt = 1;
+      stage++;
+#pragma unroll
+      for (int i = 0; i < N; ++i) M.X(0, i) = s_x0[i];
+      R.phase = PH_START;
+      Machine<CF, TQ, BC, BW> M2{sp, B, C, fit, nf, reopt};
+      M2.advance(R, (CF)0, false);
+    } else {
+      if (t == tm && smin < B.g_best_f[g]) {  // cuda.c:1810-1823
+        B.g_best_f[g] = smin;
+        CF xb[N];
+        double prm[N];
+        const int b = M.IND(0);
+#pragma unroll
+        for (int i = 0; i < N; ++i) { xb[i] = M.X(b, i); prm[i] = M.PRM(i); }
+        x2params<CF, TQ, BC, BW>(xb, prm, sp, reopt);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          M.PRM(i) = prm[i];
+          B.g_best_params[(size_t)g * N + i] = prm[i];
+        }
+        for (int m = 0; m < C.n_filters; ++m)
+          B.g_best_dV[(size_t)g * C.n_filters + m] = B.DV[(size_t)m * nf + fit];
+      }
+      stage++;
+      R.phase = PH_FIT_DONE;
+      done = true;
+    }
+  }
+  B.ctl[fit] = R.phase | (R.j << 4) | (R.failed << 9) | (R.bad << 10);
+  B.lstep[fit] = R.l;
+  B.nev[fit] = R.nev;
+  B.nev_tot[fit] = R.nev_tot;
+  B.steps_tot[fit] = R.steps_tot;
+  B.rng[fit] = R.rng;
+  B.fr[fit] = R.fr;
+  B.sf[fit] = R.sf;
+  if (t == 0) {
+    B.g_reopt[g] = reopt;
+    B.g_stage[g] = stage;
+    B.g_done[g] = done ? 1 : 0;
+    if (!done) atomicAdd(B.active_groups, 1);
+  }
+}
+
+// ------------------------------------------------------------- chi2 batch
+template <class CF, bool TQ, bool BC, bool BW, int NF>
+__global__ __launch_bounds__(256) void chi2_batch_kernel(const PointRec* pts_g, const double* sum_w, int n_data,
+                                                        int n_filters, int dof, const double* params, int n,
+                                                        double* chi2, double* dV) {
+  using P = Pos<TQ, BC, BW>;
+  constexpr int N = P::N;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  PointPtr pts = (PointPtr)(uintptr_t)pts_g;
+  double prm[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) prm[i] = params[(size_t)k * N + i];
+  CF dv[NF];
+  CF c = chi2_model<CF, TQ, BC, BW, NF>(prm, pts, n_data, n_filters, dof, sum_w, dv);
+  chi2[k] = c;
+  for (int m = 0; m < n_filters; ++m) dV[(size_t)k * n_filters + m] = dv[m];
+}
+
+// --------------------------------------------------------- launch interface
+struct LaunchArgs {
+  int op;  // 0 init, 1 fit, 2 chi2 batch
+  int precision;
+  hipStream_t stream;
+  // model space (CF-typed copies for float and double simplexes)
+  const Space<double>* sp_d;
+  const Space<float>* sp_f;
+  const PointRec* pts;
+  const double* sum_w;
+  int n_data, n_filters, dof;
+  // fit
+  const void* bufs;   // FitBufs<CF>*
+  const void* cfg;    // FitCfg<CF>*
+  const double* params0;
+  int n_groups, group_size;
+  // batch
+  const double* params;
+  int n;
+  double* chi2;
+  double* dV;
+};
+
+template <class CF, bool TQ, bool BC, bool BW, int NF>
+hipError_t launch_nf(const LaunchArgs& a, const Space<CF>& sp) {
+  if (a.op == 0) {
+    const FitCfg<CF>& C = *(const FitCfg<CF>*)a.cfg;
+    int nthreads = C.n_fits > a.n_groups ? C.n_fits : a.n_groups;
+    int blocks = (nthreads + 255) / 256;
+    hipLaunchKernelGGL((fit_init_kernel<CF, TQ, BC, BW>), dim3(blocks), dim3(256), 0, a.stream, sp,
+                       *(const FitBufs<CF>*)a.bufs, C, a.params0, a.n_groups, a.group_size);
+  } else if (a.op == 1) {
+    hipLaunchKernelGGL((fit_kernel<CF, TQ, BC, BW, NF>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream, sp,
+                       a.pts, a.sum_w, *(const FitBufs<CF>*)a.bufs, *(const FitCfg<CF>*)a.cfg);
+  } else {
+    int blocks = (a.n + 255) / 256;
+    hipLaunchKernelGGL((chi2_batch_kernel<CF, TQ, BC, BW, NF>), dim3(blocks), dim3(256), 0, a.stream, a.pts,
+                       a.sum_w, a.n_data, a.n_filters, a.dof, a.params, a.n, a.chi2, a.dV);
+  }
+  return hipGetLastError();
+}
+
+template <class CF, bool TQ, bool BC, bool BW>
+hipError_t launch_cf(const LaunchArgs& a, const Space<CF>& sp) {
+  if (a.n_filters <= 1) return launch_nf<CF, TQ, BC, BW, 1>(a, sp);
+  if (a.n_filters <= 4) return launch_nf<CF, TQ, BC, BW, 4>(a, sp);
+  return launch_nf<CF, TQ, BC, BW, MAXF>(a, sp);
+}
+
+template <bool TQ, bool BC, bool BW>
+hipError_t launch_variant(const LaunchArgs& a) {
+  if (a.precision == AST_PREC_FP64) return launch_cf<double, TQ, BC, BW>(a, *a.sp_d);
+  return launch_cf<float, TQ, BC, BW>(a, *a.sp_f);
+}
+
+}  // namespace ast

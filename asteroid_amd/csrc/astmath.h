@@ -1,0 +1,219 @@
+// astmath.h -- the project's own FP64 elementary functions.
+//
+// One header, compiled unchanged by hipcc for gfx950 device code, by hipcc's
+// host pass and by g++.  Every function is a fixed sequence of IEEE-754
+// binary64 operations (+ - * /, sqrt, fma, rint, floor, comparisons and
+// exponent-field bit operations), all correctly rounded on both sides, so the
+// host and the GPU produce bitwise-identical results as long as the including
+// translation unit is compiled with -ffp-contract=off (fma is always spelled
+// out).  No vendor math library is called.
+//
+// Methods (coefficients from gen_math_coeffs.py, Chebyshev fits at 60 digits):
+//   sincos: Cody-Waite reduction by pi/2 (3-part constant, fma), odd sine and
+//           even cosine kernels on |r| <= pi/4, quadrant swap/sign.
+//   atan2 : ratio t = min/max in [0,1]; for t > tan(pi/8) the identity
+//           atan t = pi/4 + atan((t-1)/(t+1)); odd kernel on |u| <= tan(pi/8);
+//           octant/quadrant fix-up with two-part pi constants.
+//   log   : x = m 2^e, m in [sqrt(1/2), sqrt(2)); s = f/(2+f), f = m-1;
+//           log(1+f) = f - (f^2/2 - s (f^2/2 + z R1(z))), z = s^2; e ln2 added
+//           in two parts.
+//   log10 : log(x) times 1/ln10 in two parts.
+//   exp   : k = rint(x/ln2), r = x - k ln2 (two parts, fma), 1 + r + r^2 Q(r),
+//           scaled by 2^k through the exponent field.
+// Accuracy (tests/test_astmath.py against mpmath): about 1 ulp.
+//
+// The functions are templates over the scalar type R.  For double the
+// primitives below are used; the oracle's operation-counting scalar supplies
+// overloads of the same primitive names (found by argument-dependent lookup),
+// which is how the algorithmic FLOP count of an evaluation is measured.
+#pragma once
+#include <stdint.h>
+#include "astmath_coeffs.h"
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define AST_HD __host__ __device__ inline
+#else
+#define AST_HD inline
+#endif
+
+namespace astm {
+
+// ---------------------------------------------------------------- primitives
+AST_HD double p_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+AST_HD double p_sqrt(double a) { return __builtin_sqrt(a); }
+AST_HD double p_rint(double a) { return __builtin_rint(a); }
+AST_HD double p_floor(double a) { return __builtin_floor(a); }
+AST_HD double p_fabs(double a) { return __builtin_fabs(a); }
+AST_HD double p_copysign(double a, double b) { return __builtin_copysign(a, b); }
+// Raw value of a scalar, for comparisons and exponent-field work (not counted).
+AST_HD double p_val(double a) { return a; }
+
+AST_HD uint64_t bits_of(double x) { return __builtin_bit_cast(uint64_t, x); }
+AST_HD double of_bits(uint64_t b) { return __builtin_bit_cast(double, b); }
+
+// 2^k as a double, for -1022 <= k <= 1023.
+AST_HD double pow2i(int k) { return of_bits((uint64_t)(k + 1023) << 52); }
+
+// x * 2^k for |x| in [0.5, 2], any int k: two exact power-of-two factors, so
+// results in the subnormal range round once, at the last multiply.
+AST_HD double scale2(double x, int k) {
+  if (k > 2046) k = 2046;
+  if (k < -2100) k = -2100;
+  int k1 = k / 2;
+  int k2 = k - k1;
+  if (k1 < -1022) { k2 += k1 + 1022; k1 = -1022; }
+  if (k2 < -1022) k2 = -1022;
+  if (k1 > 1023) { k2 += k1 - 1023; k1 = 1023; }
+  if (k2 > 1023) k2 = 1023;
+  return (x * pow2i(k1)) * pow2i(k2);
+}
+
+// ------------------------------------------------------------------- kernels
+template <class R> AST_HD R k_sin(R z) {  // S(z), sin r = r + r^3 S(r^2)
+  R p = p_fma(z, R(AST_SIN_5), R(AST_SIN_4));
+  p = p_fma(z, p, R(AST_SIN_3));
+  p = p_fma(z, p, R(AST_SIN_2));
+  p = p_fma(z, p, R(AST_SIN_1));
+  return p_fma(z, p, R(AST_SIN_0));
+}
+template <class R> AST_HD R k_cos(R z) {  // C(z), cos r = 1 - r^2/2 + r^4 C(r^2)
+  R p = p_fma(z, R(AST_COS_5), R(AST_COS_4));
+  p = p_fma(z, p, R(AST_COS_3));
+  p = p_fma(z, p, R(AST_COS_2));
+  p = p_fma(z, p, R(AST_COS_1));
+  return p_fma(z, p, R(AST_COS_0));
+}
+template <class R> AST_HD R k_atan(R z) {  // A(z), atan u = u + u^3 A(u^2)
+  R p = p_fma(z, R(AST_ATAN_10), R(AST_ATAN_9));
+  p = p_fma(z, p, R(AST_ATAN_8));
+  p = p_fma(z, p, R(AST_ATAN_7));
+  p = p_fma(z, p, R(AST_ATAN_6));
+  p = p_fma(z, p, R(AST_ATAN_5));
+  p = p_fma(z, p, R(AST_ATAN_4));
+  p = p_fma(z, p, R(AST_ATAN_3));
+  p = p_fma(z, p, R(AST_ATAN_2));
+  p = p_fma(z, p, R(AST_ATAN_1));
+  return p_fma(z, p, R(AST_ATAN_0));
+}
+template <class R> AST_HD R k_log(R z) {  // R1(z)
+  R p = p_fma(z, R(AST_LOG_6), R(AST_LOG_5));
+  p = p_fma(z, p, R(AST_LOG_4));
+  p = p_fma(z, p, R(AST_LOG_3));
+  p = p_fma(z, p, R(AST_LOG_2));
+  p = p_fma(z, p, R(AST_LOG_1));
+  return p_fma(z, p, R(AST_LOG_0));
+}
+template <class R> AST_HD R k_exp(R r) {  // Q(r), exp r = 1 + r + r^2 Q(r)
+  R p = p_fma(r, R(AST_EXP_10), R(AST_EXP_9));
+  p = p_fma(r, p, R(AST_EXP_8));
+  p = p_fma(r, p, R(AST_EXP_7));
+  p = p_fma(r, p, R(AST_EXP_6));
+  p = p_fma(r, p, R(AST_EXP_5));
+  p = p_fma(r, p, R(AST_EXP_4));
+  p = p_fma(r, p, R(AST_EXP_3));
+  p = p_fma(r, p, R(AST_EXP_2));
+  p = p_fma(r, p, R(AST_EXP_1));
+  return p_fma(r, p, R(AST_EXP_0));
+}
+
+// ----------------------------------------------------------------- functions
+// sin and cos of x together.  Non-finite x gives NaN for both.
+template <class R> AST_HD void sincos(R x, R* s, R* c) {
+  R k = p_rint(x * R(AST_TWO_OVER_PI));
+  R r = p_fma(-k, R(AST_PIO2_1), x);
+  r = p_fma(-k, R(AST_PIO2_2), r);
+  r = p_fma(-k, R(AST_PIO2_3), r);
+  R z = r * r;
+  R sn = p_fma(r * z, k_sin(z), r);
+  R hz = R(0.5) * z;
+  R w = R(1.0) - hz;
+  R cs = w + p_fma(z * z, k_cos(z), (R(1.0) - w) - hz);
+  double kv = p_val(k);
+  int q = (p_fabs(kv) < 1e15) ? (int)((long long)kv & 3) : 0;
+  R s0 = (q & 1) ? cs : sn;
+  R c0 = (q & 1) ? sn : cs;
+  *s = (q & 2) ? -s0 : s0;
+  *c = ((q + 1) & 2) ? -c0 : c0;
+}
+
+template <class R> AST_HD R sin(R x) { R s, c; sincos(x, &s, &c); return s; }
+template <class R> AST_HD R cos(R x) { R s, c; sincos(x, &s, &c); return c; }
+template <class R> AST_HD R tan(R x) { R s, c; sincos(x, &s, &c); return s / c; }
+
+// atan2(y, x) for finite or NaN arguments; atan2(0, 0) = 0 with the sign of y
+// (x >= 0) or +-pi (x < 0).
+template <class R> AST_HD R atan2(R y, R x) {
+  R ax = p_fabs(x);
+  R ay = p_fabs(y);
+  bool sw = p_val(ay) > p_val(ax);
+  R num = sw ? ax : ay;
+  R den = sw ? ay : ax;
+  R t = (p_val(den) == 0.0) ? R(0.0) : num / den;
+  bool big = p_val(t) > AST_TAN_PI_8;
+  R u = t;
+  if (big) u = (t - R(1.0)) / (t + R(1.0));
+  R z = u * u;
+  R a = p_fma(u * z, k_atan(z), u);
+  if (big) a = R(AST_PIO4_HI) + (a + R(AST_PIO4_LO));
+  if (sw) a = (R(AST_PIO2_HI) - a) + R(AST_PIO2_LO);
+  if (p_val(x) < 0.0) a = (R(AST_PI_HI) - a) + R(AST_PI_LO);
+  return p_copysign(a, y);
+}
+
+template <class R> AST_HD R atan(R x) { return atan2(x, R(1.0)); }
+
+// asin(x) = atan2(x, sqrt((1-x)(1+x))); NaN for |x| > 1.
+template <class R> AST_HD R asin(R x) {
+  return atan2(x, p_sqrt((R(1.0) - x) * (R(1.0) + x)));
+}
+
+// Natural logarithm: -inf at 0, NaN below 0 or for NaN, +inf at +inf.
+template <class R> AST_HD R log(R x) {
+  double xv = p_val(x);
+  if (!(xv > 0.0)) return R(xv == 0.0 ? -__builtin_inf() : __builtin_nan(""));
+  if (xv == __builtin_inf()) return x;
+  uint64_t b = bits_of(xv);
+  int ex = (int)((b >> 52) & 0x7ff);
+  int sub = 0;
+  if (ex == 0) {  // subnormal: bring into the normal range first (exact)
+    b = bits_of(xv * 18014398509481984.0);  // 2^54
+    ex = (int)((b >> 52) & 0x7ff);
+    sub = 54;
+  }
+  int e = ex - 1023 - sub;
+  double mv = of_bits((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);  // [1,2)
+  if (mv > 1.4142135623730951) { mv *= 0.5; e += 1; }
+  R m = R(mv);
+  R f = m - R(1.0);
+  R s = f / (R(2.0) + f);
+  R z = s * s;
+  R rz = z * k_log(z);
+  R hfsq = R(0.5) * f * f;
+  R dk = R((double)e);
+  return dk * R(AST_LN2_HI) - ((hfsq - p_fma(s, hfsq + rz, dk * R(AST_LN2_LO))) - f);
+}
+
+template <class R> AST_HD R log10(R x) {
+  R y = log(x);
+  return p_fma(y, R(AST_INV_LN10_HI), y * R(AST_INV_LN10_LO));
+}
+
+// Exponential: +inf above ~709.78, +0 below ~-745.13, NaN for NaN.
+template <class R> AST_HD R exp(R x) {
+  double xv = p_val(x);
+  if (xv != xv) return x;
+  if (xv > 709.782712893384) return R(__builtin_inf());
+  if (xv < -745.1332191019412) return R(0.0);
+  R k = p_rint(x * R(AST_INV_LN2));
+  R r = p_fma(-k, R(AST_LN2_HI), x);
+  r = p_fma(-k, R(AST_LN2_LO), r);
+  R e = R(1.0) + p_fma(r * r, k_exp(r), r);
+  return R(scale2(p_val(e), (int)p_val(k)));
+}
+
+// Single-precision exponential of the simplex set-up (float arithmetic there):
+// the double result rounded to float.
+AST_HD float expf(float x) { return (float)exp<double>((double)x); }
+
+}  // namespace astm

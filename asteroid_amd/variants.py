@@ -1,0 +1,137 @@
+"""Model variants and their default parameter tables.
+
+Python mirror of what the reference's main() builds before the fit
+(asteroid.c:86-169 Property and Types, asteroid.c:473-601 hard-coded limits,
+-f/-l overrides and the P_psi mapping).  The C++ copy of the same tables is
+ast_default_model_space() in libasteroid_hip.so (used by the host driver); a
+test checks that the two agree.
+"""
+import math
+
+import numpy as np
+
+from . import abi
+
+PI = 3.141592653589793238          # asteroid.h:16
+RAD = 180.0 / PI                   # asteroid.h:17
+TMAX = 10.0                        # asteroid.c:507
+
+
+def make_variant(torque=True, bc=False, bw_ball=False, p_psi=True, interp=True,
+                 precision=abi.PREC_FP64, time_step=1e-2):
+    return abi.Variant(int(torque), int(bc), int(bw_ball), int(p_psi), int(interp), int(precision),
+                       float(time_step))
+
+
+# Named variants of the benchmark workloads.
+def ls_torque(precision=abi.PREC_FP64, time_step=1e-2, interp=True):
+    """Self-consistent LS ellipsoid with torque (-DP_PSI -DTORQUE), 11 parameters."""
+    return make_variant(True, False, False, True, interp, precision, time_step)
+
+
+def bc_free(precision=abi.PREC_FP64, time_step=5e-3, interp=True):
+    """Relaxed brightness ellipsoid without torque (-DBC), 10 parameters."""
+    return make_variant(False, True, False, True, interp, precision, time_step)
+
+
+def bw_ball_torque(precision=abi.PREC_FP64, time_step=1e-2, interp=True):
+    """Black-and-white ball with torque (-DBW_BALL -DTORQUE -DP_PSI), 14 parameters."""
+    return make_variant(True, False, True, True, interp, precision, time_step)
+
+
+class ModelSpace:
+    """Property / Types / Limits / x2 of one variant (the inputs of ast_set_model_space)."""
+
+    def __init__(self, variant, property_rows, types, limits_lo, limits_hi, reopt, ppsi1, ppsi2):
+        self.variant = variant
+        self.ids = abi.type_ids(variant)
+        self.property = np.array(property_rows, dtype=np.int32)
+        self.types = np.array(types, dtype=np.int32)
+        self.limits = np.concatenate([np.asarray(limits_lo, float), np.asarray(limits_hi, float)])
+        self.reopt = int(reopt)
+        self.ppsi1 = float(np.float32(ppsi1))
+        self.ppsi2 = float(np.float32(ppsi2))
+
+    @property
+    def n_params(self):
+        return self.property.shape[0]
+
+    @property
+    def n_types(self):
+        return len(self.types)
+
+    def x2(self):
+        return abi.X2(self.reopt, self.ppsi1, self.ppsi2)
+
+    def param_names(self):
+        inv = {v: k for k, v in self.ids.items() if k not in ("n_types", "n_params")}
+        return [inv[int(t)] for t in self.property[:, abi.P_TYPE]]
+
+    def with_reopt(self, reopt):
+        ms = ModelSpace(self.variant, self.property.copy(), self.types.copy(),
+                        self.limits[:self.n_types].copy(), self.limits[self.n_types:].copy(),
+                        reopt, self.ppsi1, self.ppsi2)
+        return ms
+
+
+def default_model_space(variant, ppsi_hours=(2.0, 4800.0), reopt=0, frozen=None, limits=None,
+                        free_random=()):
+    """The reference's tables for `variant`.
+
+    frozen: {type_id: value} as the -f option (asteroid.c:566-581);
+    limits: {type_id: (lo, hi)} as the -l option (asteroid.c:583-589);
+    free_random: parameter positions given as "v" in -m (P_frozen = -1, asteroid.c:308-313).
+    """
+    T = abi.type_ids(variant)
+    rows = [
+        (T["theta_M"], 1, 0, 0, 0, abi.HARD_BOTH),
+        (T["phi_M"], 1, 0, 0, 0, abi.PERIODIC),
+        (T["phi_0"], 1, 0, 0, 0, abi.PERIODIC),
+    ]
+    if variant.torque:
+        rows += [(T["Ti"], 1, 0, 0, 0, abi.SOFT_BOTH), (T["Ts"], 1, 0, 0, 0, abi.SOFT_BOTH),
+                 (T["Tl"], 1, 0, 0, 0, abi.SOFT_BOTH)]
+    rows += [
+        (T["c_tumb"], 1, 0, 0, 1, abi.HARD_RIGHT),
+        (T["b_tumb"], 0, 0, 0, 1, abi.HARD_BOTH),
+        (T["Es"], 0, 0, 0, 0, abi.HARD_BOTH),
+        (T["L"], 1, 0, 0, 0, abi.HARD_LEFT),
+        (T["psi_0"], 0, 0, 0, 0, abi.PERIODIC_LAM),
+    ]
+    if variant.bc:
+        rows += [(T["c"], 1, 0, 0, 1, abi.HARD_RIGHT), (T["b"], 0, 0, 0, 1, abi.HARD_BOTH)]
+    if variant.bw_ball:
+        rows += [(T["theta_R"], 1, 0, 0, 1, abi.HARD_BOTH), (T["phi_R"], 1, 0, 0, 1, abi.PERIODIC),
+                 (T["kappa"], 1, 0, 0, 1, abi.SOFT_BOTH)]
+    rows = [list(r) for r in rows]
+    nt = T["n_types"]
+    types = [-1] * nt
+    for i, r in enumerate(rows):
+        types[r[abi.P_TYPE]] = i
+    lo = [0.0] * nt
+    hi = [0.0] * nt
+    lo[T["theta_M"]], hi[T["theta_M"]] = 0.001 / RAD, 179.999 / RAD
+    lo[T["L"]], hi[T["L"]] = 48.0 * PI / 10, 48.0 * PI / 0.1
+    if variant.torque:
+        for n in ("Ti", "Ts", "Tl"):
+            lo[T[n]], hi[T[n]] = -TMAX, TMAX
+    lo[T["c_tumb"]], hi[T["c_tumb"]] = math.log(0.01), math.log(1.0)
+    lo[T["b_tumb"]], hi[T["b_tumb"]] = 0.0, 1.0
+    if variant.bc:
+        lo[T["c"]], hi[T["c"]] = lo[T["c_tumb"]], hi[T["c_tumb"]]
+        lo[T["b"]], hi[T["b"]] = 0.0, 1.0
+    if variant.bw_ball:
+        lo[T["theta_R"]], hi[T["theta_R"]] = 0.001 / RAD, 179.999 / RAD
+        lo[T["kappa"]], hi[T["kappa"]] = math.log(0.01), math.log(0.1)
+    for itype, value in (frozen or {}).items():
+        for r in rows:
+            if r[abi.P_TYPE] == itype:
+                r[abi.P_FROZEN] = 1
+                lo[itype] = hi[itype] = float(value)
+    for itype, (a, b) in (limits or {}).items():
+        lo[itype], hi[itype] = float(a), float(b)
+    for k in free_random:
+        rows[k][abi.P_FROZEN] = -1
+    ppsi1 = np.float32(24.0 / ppsi_hours[1])   # asteroid.c:599-600
+    ppsi2 = np.float32(24.0 / ppsi_hours[0])
+    return ModelSpace(variant, rows, types, lo, hi, reopt, ppsi1, ppsi2)

@@ -1,0 +1,237 @@
+/* asteroid.h -- C ABI of libasteroid_hip.so, the MI355X (gfx950) fitting core.
+ *
+ * The reference has no plugin or FFI API: its boundary is the CUDA launch of
+ * chi2_gpu plus the __device__ symbols the host fills with cudaMemcpyToSymbol.
+ * Each entry point below replaces one of those pieces; the reference
+ * file:line it stands in for is given with it.  Every function returns 0 on
+ * success or a negative AST_E* code (ast_strerror names it) and never exits
+ * the process (the reference's ERR macro printed and exited,
+ * cuda_errors.h:4-14; the host driver keeps that behaviour on top of these
+ * codes).  One context per host thread; calls on a context are synchronous
+ * unless stated otherwise.
+ */
+#ifndef ASTEROID_ABI_H
+#define ASTEROID_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ constants */
+/* Columns of the Property table (asteroid.h:115-123). */
+enum {
+  AST_P_TYPE = 0,
+  AST_P_INDEPENDENT = 1,
+  AST_P_FROZEN = 2,
+  AST_P_ISEG = 3,
+  AST_P_MULTI_SEGMENT = 4,
+  AST_P_PERIODIC = 5,
+  AST_N_COLUMNS = 6
+};
+/* Values of the P_periodic column (asteroid.h:126-131). */
+enum {
+  AST_PERIODIC = 1,
+  AST_PERIODIC_LAM = 0,
+  AST_SOFT_BOTH = -1,
+  AST_HARD_LEFT = -2,
+  AST_HARD_RIGHT = -3,
+  AST_HARD_BOTH = -4
+};
+enum { AST_MAX_PARAMS = 16, AST_MAX_TYPES = 16, AST_MAX_FILTERS = 8 };
+
+/* Error codes. */
+enum {
+  AST_OK = 0,
+  AST_E_ARG = -1,        /* bad argument (null pointer, size out of range) */
+  AST_E_STATE = -2,      /* call out of order (e.g. run before upload)    */
+  AST_E_HIP = -3,        /* HIP runtime error (ast_last_hip_error)         */
+  AST_E_NOMEM = -4,      /* device or host allocation failed              */
+  AST_E_DATA = -5,       /* input data rejected (unsorted times, too many filters, outside ephemeris span) */
+  AST_E_UNSUPPORTED = -6,/* variant combination not built                 */
+  AST_E_NODEVICE = -7    /* no HIP device                                 */
+};
+
+/* Arithmetic forms of the evaluator and simplex (CHI_FLOAT, asteroid.h:24-28). */
+enum {
+  AST_PREC_FP64 = 0,     /* -DACC: simplex and chi^2 in double, FP64 model      */
+  AST_PREC_F32SIMPLEX = 1,/* default build: simplex and chi^2 in float, FP64 model */
+  AST_PREC_MIXED = 2     /* float simplex; FP64 Euler-angle integration and chi^2 sums; FP32 geometry and brightness */
+};
+
+/* Model variant: the compile-time macros of the reference (makefile:3-36),
+ * chosen at run time here.  TREND, ROTATE, RECT, TORQUE2, SEGMENT, P_PHI,
+ * P_BOTH and NUDGE are not built. */
+typedef struct {
+  int torque;     /* -DTORQUE : 6-ODE constant-torque rotation                 */
+  int bc;         /* -DBC     : photometric b,c independent of kinematic ones  */
+  int bw_ball;    /* -DBW_BALL: black-and-white ball brightness                */
+  int p_psi;      /* -DP_PSI  : L generated from 1/P_psi in random search      */
+  int interp;     /* -DINTERP : E,S from a 3-point ephemeris                   */
+  int precision;  /* AST_PREC_*                                                */
+  double time_step; /* TIME_STEP, days (asteroid.h:202)                        */
+} ast_variant;
+
+/* Parameter type ids of a variant: the reference's __COUNTER__ order
+ * (asteroid.h:137-178); -1 where the variant has no such type.  The -f and
+ * -l options of the host driver take these ids. */
+typedef struct {
+  int theta_M, phi_M, phi_0, L, Ti, Ts, Tl, c_tumb, b_tumb, Es, psi_0;
+  int c, b, theta_R, phi_R, kappa;
+  int n_types;   /* N_TYPES  */
+  int n_params;  /* N_PARAMS (asteroid.h:110) */
+} ast_type_ids;
+
+static inline int ast_variant_type_ids(const ast_variant* v, ast_type_ids* t) {
+  int id = 0;
+  if (!v || !t) return AST_E_ARG;
+  t->theta_M = id++;
+  t->phi_M = id++;
+  t->phi_0 = id++;
+  t->L = id++;
+  t->Ti = t->Ts = t->Tl = -1;
+  if (v->torque) { t->Ti = id++; t->Ts = id++; t->Tl = id++; }
+  t->c_tumb = id++;
+  t->b_tumb = id++;
+  t->Es = id++;
+  t->psi_0 = id++;
+  t->c = t->b = -1;
+  if (v->bc) { t->c = id++; t->b = id++; }
+  t->theta_R = t->phi_R = t->kappa = -1;
+  if (v->bw_ball) { t->theta_R = id++; t->phi_R = id++; t->kappa = id++; }
+  t->n_types = id;
+  t->n_params = 8 + (v->bc ? 2 : 0) + (v->torque ? 3 : 0) + (v->bw_ball ? 3 : 0);
+  return AST_OK;
+}
+
+/* One observation after ingest (struct obs_data, asteroid.h:355-368):
+ * V absolute magnitude, w = 1/sgm^2, mjd asteroid time relative to the first
+ * observation, filter index.  E and S (unit asteroid->Earth and
+ * asteroid->Sun vectors) are read only when the variant is not INTERP. */
+typedef struct {
+  double mjd;
+  float V;
+  float w;
+  int filter;
+  int pad_;
+  double E[3];
+  double S[3];
+} ast_obs;
+
+/* The INTERP 3-point ephemeris (E_x0..S_z0, MJD0; asteroid.h:444,
+ * gpu_prepare.c:39-45): E[c][k] is component c (x,y,z) of the
+ * asteroid->Earth vector at ephemeris time mjd[k], times relative like
+ * ast_obs.mjd (read_data.c:298-302). */
+typedef struct {
+  double E[3][3];
+  double S[3][3];
+  double mjd[3];
+} ast_ephem3;
+
+/* struct x2_struct (asteroid.h:340-351).  ppsi1/ppsi2 are 1/P_psi bounds in
+ * 1/day, stored as float as the reference does (asteroid.c:599-600). */
+typedef struct {
+  int reopt;
+  float ppsi1;
+  float ppsi2;
+} ast_x2;
+
+/* One launch of the fitting kernel (chi2_gpu<<<N_BLOCKS,BSIZE>>>,
+ * asteroid.c:675): n_groups reseed groups (blocks) of group_size simplexes
+ * (threads). */
+typedef struct {
+  int n_groups;        /* N_BLOCKS                                             */
+  int group_size;      /* BSIZE: 64, 128, 256, 512 or 1024                     */
+  int nstages;         /* -Nstages                                             */
+  int n_steps;         /* N_STEPS simplex-step limit (asteroid.h:213)           */
+  int64_t eval_budget; /* chi^2 evaluations per simplex and stage; 0 = none    */
+  double size_min;     /* SIZE_MIN; 0 = the precision's default (asteroid.h:233-237) */
+  int ticks_per_launch;/* evaluation rounds per kernel launch; 0 = automatic   */
+} ast_run_cfg;
+
+typedef struct ast_ctx ast_ctx;
+
+/* Create a context on HIP device `device` for one variant (replaces the
+ * compile-time macro choice, makefile:38-53, and Is_GPU_present,
+ * cuda_errors.h:18-39). */
+int ast_ctx_create(ast_ctx** out, const ast_variant* variant, int device);
+int ast_ctx_destroy(ast_ctx* ctx);
+
+/* Parameter metadata: Property[N_PARAMS][6] (asteroid.c:86-125), Types
+ * (asteroid.c:152-169), Limits[2][N_TYPES] (asteroid.c:473-589, limits[t] is
+ * the lower and limits[n_types+t] the upper limit) and x2 (asteroid.c:591-601).
+ * Replaces the four cudaMemcpyToSymbol calls at asteroid.c:603-606.  The
+ * arrays are copied. */
+int ast_set_model_space(ast_ctx* ctx, int n_params, const int* property,
+                        int n_types, const int* types, const double* limits,
+                        const ast_x2* x2);
+
+/* Light curve: replaces gpu_prepare's dData upload (gpu_prepare.c:13-14) and
+ * the INTERP symbols (gpu_prepare.c:38-46).  `eph` is required for INTERP
+ * variants and ignored otherwise.  Times must increase (read_data.c:81-85);
+ * at most AST_MAX_FILTERS filters.  The data are copied. */
+int ast_upload_curve(ast_ctx* ctx, const ast_obs* obs, int n_data, int n_filters,
+                     const ast_ephem3* eph);
+
+/* Seed the per-simplex random streams and reset each group's best chi^2 to
+ * 1e30: setup_kernel(..., seed, d_f, 1) (cuda.c:2039-2054, asteroid.c:629-634).
+ * The project's counter-based generator replaces cuRAND. */
+int ast_rng_seed(ast_ctx* ctx, uint64_t seed);
+
+/* Reset each group's best chi^2 to 1e30 without touching the random
+ * streams: setup_kernel(..., 0, d_f, 0) (asteroid.c:816-818, -keep). */
+int ast_reset_best(ast_ctx* ctx);
+
+/* Starting model for reoptimisation, physical units: d_params0
+ * (asteroid.c:636-637, 766). */
+int ast_set_start(ast_ctx* ctx, const double* params0);
+
+/* Run one launch of the fitting kernel to completion (all simplexes of all
+ * groups finish their nstages stages): chi2_gpu (cuda.c:1325-1834) with the
+ * launch at asteroid.c:675 and the sync at asteroid.c:688. */
+int ast_run(ast_ctx* ctx, const ast_run_cfg* cfg);
+
+/* Per-group results of the runs since the last seed/reset: d_f, d_params,
+ * d_dV (asteroid.c:728-730).  f has n_groups entries, params
+ * n_groups*n_params, dV n_groups*n_filters (caller-owned). */
+int ast_fetch(ast_ctx* ctx, double* f, double* params, double* dV);
+
+/* Per-simplex diagnostics of the last ast_run (no counterpart in the
+ * reference, which keeps them in registers): the stage-final best chi^2 of
+ * each simplex (1e30 if it failed), its chi^2 evaluation count and simplex
+ * step count summed over stages.  Arrays of n_groups*group_size; any may be
+ * NULL. */
+int ast_fetch_lanes(ast_ctx* ctx, double* f_last, int64_t* n_evals, int64_t* n_steps);
+
+/* The evaluator alone (chi2one, cuda.c:81-1011): chi^2 and the per-filter
+ * delta_V of n parameter vectors (physical units, n*n_params doubles),
+ * computed one model per GPU lane.  chi2 has n entries, dV n*n_filters. */
+int ast_chi2_batch(ast_ctx* ctx, const double* params, int n, double* chi2, double* dV);
+
+/* Timing of the last ast_run / ast_chi2_batch, from HIP events on the
+ * context's stream: total kernel milliseconds and the number of launches. */
+int ast_last_timing(ast_ctx* ctx, double* kernel_ms, int* launches);
+
+/* Algorithmic counts of the last ast_run: chi^2 evaluations and
+ * model-point evaluations (evaluations times observations). */
+int ast_last_counts(ast_ctx* ctx, int64_t* evals, int64_t* model_points);
+
+/* Default Property/Types/Limits of a variant, as the reference's main()
+ * builds them (asteroid.c:86-169, 473-564), with the P_psi range
+ * ppsi_hours[0..1] mapped as at asteroid.c:599-600.  property needs
+ * AST_MAX_PARAMS*6 ints, types AST_MAX_TYPES ints, limits 2*AST_MAX_TYPES
+ * doubles (lower limits first, then upper, each n_types long). */
+int ast_default_model_space(const ast_variant* variant, const double* ppsi_hours,
+                            int* n_params, int* property, int* n_types, int* types,
+                            double* limits, ast_x2* x2);
+
+const char* ast_strerror(int code);
+/* Text of the last HIP error seen by this thread (for AST_E_HIP). */
+const char* ast_last_hip_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ASTEROID_ABI_H */

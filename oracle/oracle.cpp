@@ -1,0 +1,1048 @@
+// oracle.cpp -- CPU restatement of Asteroid's GPU fitting core.
+//
+// TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py load this library, and only to check or to
+// time beside the GPU path; the product (libasteroid_hip.so and the host
+// driver) never calls it.
+//
+// What it restates (reference = pulsar123/Asteroid):
+//   ode_*            ODE_func, cuda.c:13-77
+//   chi2_literal     chi2one, cuda.c:81-1011 (in-scope branches: TORQUE or the
+//                    3-ODE form, INTERP, LS ellipsoid cuda.c:663-690, BW_BALL
+//                    cuda.c:596-607, BC cuda.c:568-574, chi^2 cuda.c:884-910),
+//                    written with the reference's expression order and the C
+//                    library's elementary functions, no FMA.
+//   chi2_canon       the same model in the project's canonical evaluation
+//                    order: explicit FMA where a product feeds an add, the
+//                    project's own elementary functions (astmath.h), the
+//                    LS angle algebra done with trig identities instead of
+//                    atan2/tan/sincos, divisions by b^2, c^2 folded into scaled
+//                    axes.  This is the arithmetic the GPU kernels execute, so
+//                    the GPU FP64 path must match it bit for bit.
+//   x2params         cuda.c:1120-1319 (P_PSI branch; P_PHI/P_BOTH not built)
+//   params2x         cuda.c:1016-1115
+//   run_group        chi2_gpu, cuda.c:1325-1834: random or reoptimised start,
+//                    initial simplex, Nelder-Mead loop with its five chi2one
+//                    call sites, block argmin, stage reseed, best-per-block
+//                    write.  Random draws come from astrng.h instead of cuRAND.
+//   quadratic interpolation of E,S (INTERP)  cuda.c:522-545
+// Behaviour the reference leaves undefined is fixed here and in the kernels
+// the same way: coordinates of frozen parameters in reflected, expanded and
+// contracted points equal the worst vertex's (cuda.c:1638, 1663, 1696 leave
+// them unset); the sorted index array starts as the identity (cuda.c:1346);
+// params2x starts with LAM = 0 (cuda.c:1022); the last delta_V of a thread
+// starts at 0 (cuda.c:1345).
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../include/asteroid.h"
+#include "../asteroid_amd/csrc/astmath.h"
+#include "../asteroid_amd/csrc/astrng.h"
+
+namespace orc {
+
+const double PI = 3.141592653589793238;  // asteroid.h:16
+const double SMALL = 1e-8;               // cuda.c:1455
+const float BC_DEV_MAX = 100;            // asteroid.h:248
+
+// ----------------------------------------------------------------- problem
+struct Problem {
+  ast_variant v;
+  ast_type_ids T;
+  int n_params, n_types;
+  int prop[AST_MAX_PARAMS][AST_N_COLUMNS];
+  int types[AST_MAX_TYPES];
+  double lim[2][AST_MAX_TYPES];
+  ast_x2 x2;
+  int n_data, n_filters;
+  const ast_obs* obs;
+  ast_ephem3 eph;
+};
+
+// ------------------------------------------------------ elementary functions
+struct LibmMath {
+  static double sin(double x) { return std::sin(x); }
+  static double cos(double x) { return std::cos(x); }
+  static double tan(double x) { return std::tan(x); }
+  static double asin(double x) { return std::asin(x); }
+  static double atan(double x) { return std::atan(x); }
+  static double atan2(double y, double x) { return std::atan2(y, x); }
+  static double log(double x) { return std::log(x); }
+  static double log10(double x) { return std::log10(x); }
+  static double exp(double x) { return std::exp(x); }
+  static double sqrt(double x) { return std::sqrt(x); }
+  static float expf(float x) { return std::exp(x); }
+};
+struct AstMath {
+  static double sin(double x) { return astm::sin(x); }
+  static double cos(double x) { return astm::cos(x); }
+  static double tan(double x) { return astm::tan(x); }
+  static double asin(double x) { return astm::asin(x); }
+  static double atan(double x) { return astm::atan(x); }
+  static double atan2(double y, double x) { return astm::atan2(y, x); }
+  static double log(double x) { return astm::log(x); }
+  static double log10(double x) { return astm::log10(x); }
+  static double exp(double x) { return astm::exp(x); }
+  static double sqrt(double x) { return astm::p_sqrt(x); }
+  static float expf(float x) { return astm::expf(x); }
+};
+template <class M> inline double cf_exp(double x) { return M::exp(x); }
+template <class M> inline float cf_exp(float x) { return M::expf(x); }
+inline double cf_floor(double x) { return std::floor(x); }
+inline float cf_floor(float x) { return std::floor(x); }
+
+// ------------------------------------------------------ INTERP (cuda.c:528-545)
+// Literal form: the reference's expressions.
+inline void interp_literal(const ast_ephem3& e, double t, double E[3], double S[3]) {
+  const double* M = e.mjd;
+  double rr0 = (t - M[1]) * (t - M[2]) / (M[0] - M[1]) / (M[0] - M[2]);
+  double rr1 = (t - M[0]) * (t - M[2]) / (M[1] - M[0]) / (M[1] - M[2]);
+  double rr2 = (t - M[0]) * (t - M[1]) / (M[2] - M[0]) / (M[2] - M[1]);
+  for (int c = 0; c < 3; ++c) {
+    E[c] = e.E[c][0] * rr0 + e.E[c][1] * rr1 + e.E[c][2] * rr2;
+    S[c] = e.S[c][0] * rr0 + e.S[c][1] * rr1 + e.S[c][2] * rr2;
+  }
+  double En = std::sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2]);
+  double Sn = std::sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+  for (int c = 0; c < 3; ++c) { E[c] = E[c] / En; S[c] = S[c] / Sn; }
+}
+// Canonical form (FMA chains left to right).
+inline void interp_canon(const ast_ephem3& e, double t, double E[3], double S[3]) {
+  using astm::p_fma;
+  const double* M = e.mjd;
+  double d0 = t - M[0], d1 = t - M[1], d2 = t - M[2];
+  double rr0 = (d1 * d2) / (M[0] - M[1]) / (M[0] - M[2]);
+  double rr1 = (d0 * d2) / (M[1] - M[0]) / (M[1] - M[2]);
+  double rr2 = (d0 * d1) / (M[2] - M[0]) / (M[2] - M[1]);
+  for (int c = 0; c < 3; ++c) {
+    E[c] = p_fma(e.E[c][2], rr2, p_fma(e.E[c][1], rr1, e.E[c][0] * rr0));
+    S[c] = p_fma(e.S[c][2], rr2, p_fma(e.S[c][1], rr1, e.S[c][0] * rr0));
+  }
+  double En = astm::p_sqrt(p_fma(E[2], E[2], p_fma(E[1], E[1], E[0] * E[0])));
+  double Sn = astm::p_sqrt(p_fma(S[2], S[2], p_fma(S[1], S[1], S[0] * S[0])));
+  for (int c = 0; c < 3; ++c) { E[c] = E[c] / En; S[c] = S[c] / Sn; }
+}
+
+// RK4 step count and step of the interval t1 -> t2 (cuda.c:321-323).
+inline void rk4_interval(double t1, double t2, double time_step, int* nsteps, double* h) {
+  *nsteps = (int)((t2 - t1) / time_step + 1);
+  *h = (t2 - t1) / *nsteps;
+}
+
+// ============================================================ literal chi2one
+struct ModelOut {   // optional per-point outputs
+  double* vmod;     // n_data
+  double* traj;     // n_data*6: phi, theta, psi, Omega_i, Omega_s, Omega_l
+};
+
+inline void ode_literal(bool torque, const double* y, double* f, const double* mu) {
+  if (torque) {  // cuda.c:46-58
+    f[0] = mu[0] * y[1] * y[2] + mu[3];
+    f[1] = mu[1] * y[2] * y[0] + mu[4];
+    f[2] = mu[2] * y[0] * y[1] + mu[5];
+    f[3] = (y[0] * std::sin(y[5]) + y[1] * std::cos(y[5])) / std::sin(y[4]);
+    f[4] = y[0] * std::cos(y[5]) - y[1] * std::sin(y[5]);
+    f[5] = y[2] - f[3] * std::cos(y[4]);
+  } else {       // cuda.c:69-75
+    f[0] = mu[0] * (mu[1] - mu[2] * std::cos(2.0 * y[2]));
+    f[1] = mu[0] * mu[2] * std::sin(y[1]) * std::sin(2.0 * y[2]);
+    f[2] = std::cos(y[1]) * (mu[0] - f[0]);
+  }
+}
+
+// Per-filter sums of one model (cuda.c:195-715).
+inline void model_literal(const Problem& P, const double* params, double* sy2, double* sy,
+                          double* sw, ModelOut* out) {
+  const ast_type_ids& T = P.T;
+  const bool torque = P.v.torque;
+  for (int m = 0; m < P.n_filters; ++m) sy2[m] = sy[m] = sw[m] = 0.0;
+  double tM = params[P.types[T.theta_M]], pM = params[P.types[T.phi_M]];
+  double M_x = std::sin(tM) * std::cos(pM);
+  double M_y = std::sin(tM) * std::sin(pM);
+  double M_z = std::cos(tM);
+  double XM = std::sqrt(M_z * M_z + M_x * M_x);
+  double XM_x = M_z / XM;
+  double XM_z = -M_x / XM;
+  double YM_x = M_y * XM_z;
+  double YM_y = M_z * XM_x - M_x * XM_z;
+  double YM_z = -M_y * XM_x;
+  double bt = params[P.types[T.b_tumb]], ct = params[P.types[T.c_tumb]];
+  double Is = (1.0 + bt * bt) / (bt * bt + ct * ct);
+  double Ii = (1.0 + ct * ct) / (bt * bt + ct * ct);
+  double Is_inv = 1.0 / Is;
+  double Ii_inv = 1.0 / Ii;
+  double Es = params[P.types[T.Es]], psi0 = params[P.types[T.psi_0]], L = params[P.types[T.L]];
+  double phi = params[P.types[T.phi_0]];
+  double theta = std::asin(std::sqrt((Es - 1.0) / (std::sin(psi0) * std::sin(psi0) * (Ii_inv - Is_inv) + Is_inv - 1.0)));
+  double psi = psi0;
+  double mu[6];
+  double Om_i = 0, Om_s = 0, Om_l = 0;
+  if (torque) {
+    mu[0] = (Is - 1.0) * Ii_inv;
+    mu[1] = (1.0 - Ii) * Is_inv;
+    mu[2] = Ii - Is;
+    mu[3] = params[P.types[T.Ti]];
+    mu[4] = params[P.types[T.Ts]];
+    mu[5] = params[P.types[T.Tl]];
+    Om_i = L * Ii_inv * std::sin(theta) * std::sin(psi);
+    Om_s = L * Is_inv * std::sin(theta) * std::cos(psi);
+    Om_l = L * std::cos(theta);
+  } else {
+    double Ip = 0.5 * (Ii_inv + Is_inv);
+    double Im = 0.5 * (Ii_inv - Is_inv);
+    mu[0] = L;
+    mu[1] = Ip;
+    mu[2] = Im;
+  }
+  const int n_ode = torque ? 6 : 3;
+  for (int i = 0; i < P.n_data; ++i) {
+    if (i > 0) {
+      int N_steps;
+      double h;
+      rk4_interval(P.obs[i - 1].mjd, P.obs[i].mjd, P.v.time_step, &N_steps, &h);
+      double y[6];
+      if (torque) { y[0] = Om_i; y[1] = Om_s; y[2] = Om_l; y[3] = phi; y[4] = theta; y[5] = psi; }
+      else { y[0] = phi; y[1] = theta; y[2] = psi; }
+      for (int l = 0; l < N_steps; ++l) {  // cuda.c:348-372
+        double f[6], K1[6], K2[6], K3[6], K4[6];
+        ode_literal(torque, y, K1, mu);
+        for (int j = 0; j < n_ode; ++j) f[j] = y[j] + 0.5 * h * K1[j];
+        ode_literal(torque, f, K2, mu);
+        for (int j = 0; j < n_ode; ++j) f[j] = y[j] + 0.5 * h * K2[j];
+        ode_literal(torque, f, K3, mu);
+        for (int j = 0; j < n_ode; ++j) f[j] = y[j] + h * K3[j];
+        ode_literal(torque, f, K4, mu);
+        for (int j = 0; j < n_ode; ++j) y[j] = y[j] + 1 / 6.0 * h * (K1[j] + 2 * K2[j] + 2 * K3[j] + K4[j]);
+      }
+      if (torque) { Om_i = y[0]; Om_s = y[1]; Om_l = y[2]; phi = y[3]; theta = y[4]; psi = y[5]; }
+      else { phi = y[0]; theta = y[1]; psi = y[2]; }
+    }
+    if (out && out->traj) {
+      double* t6 = out->traj + 6 * i;
+      t6[0] = phi; t6[1] = theta; t6[2] = psi; t6[3] = Om_i; t6[4] = Om_s; t6[5] = Om_l;
+    }
+    // Body axes (cuda.c:432-473).
+    double cos_phi = std::cos(phi), sin_phi = std::sin(phi);
+    double N_x = XM_x * cos_phi + YM_x * sin_phi;
+    double N_y = YM_y * sin_phi;
+    double N_z = XM_z * cos_phi + YM_z * sin_phi;
+    double p_x = N_y * M_z - N_z * M_y;
+    double p_y = N_z * M_x - N_x * M_z;
+    double p_z = N_x * M_y - N_y * M_x;
+    double cos_theta = std::cos(theta), sin_theta = std::sin(theta);
+    double a_x = M_x * cos_theta + p_x * sin_theta;
+    double a_y = M_y * cos_theta + p_y * sin_theta;
+    double a_z = M_z * cos_theta + p_z * sin_theta;
+    double w_x = a_y * N_z - a_z * N_y;
+    double w_y = a_z * N_x - a_x * N_z;
+    double w_z = a_x * N_y - a_y * N_x;
+    double sin_psi = std::sin(psi), cos_psi = std::cos(psi);
+    double b_x = N_x * cos_psi + w_x * sin_psi;
+    double b_y = N_y * cos_psi + w_y * sin_psi;
+    double b_z = N_z * cos_psi + w_z * sin_psi;
+    double c_x = a_y * b_z - a_z * b_y;
+    double c_y = a_z * b_x - a_x * b_z;
+    double c_z = a_x * b_y - a_y * b_x;
+    if (P.v.bw_ball) {  // cuda.c:475-497
+      double cp = std::cos(params[P.types[T.phi_R]]), sp = std::sin(params[P.types[T.phi_R]]);
+      N_x = b_x * cp + c_x * sp;
+      N_y = b_y * cp + c_y * sp;
+      N_z = b_z * cp + c_z * sp;
+      p_x = N_y * a_z - N_z * a_y;
+      p_y = N_z * a_x - N_x * a_z;
+      p_z = N_x * a_y - N_y * a_x;
+      double ctR = std::cos(params[P.types[T.theta_R]]), stR = std::sin(params[P.types[T.theta_R]]);
+      a_x = a_x * ctR + p_x * stR;
+      a_y = a_y * ctR + p_y * stR;
+      a_z = a_z * ctR + p_z * stR;
+    }
+    double E[3], S[3];
+    if (P.v.interp) interp_literal(P.eph, P.obs[i].mjd, E, S);
+    else for (int c = 0; c < 3; ++c) { E[c] = P.obs[i].E[c]; S[c] = P.obs[i].S[c]; }
+    double Ep_b = b_x * E[0] + b_y * E[1] + b_z * E[2];
+    double Ep_c = c_x * E[0] + c_y * E[1] + c_z * E[2];
+    double Ep_a = a_x * E[0] + a_y * E[1] + a_z * E[2];
+    double Sp_b = b_x * S[0] + b_y * S[1] + b_z * S[2];
+    double Sp_c = c_x * S[0] + c_y * S[1] + c_z * S[2];
+    double Sp_a = a_x * S[0] + a_y * S[1] + a_z * S[2];
+    double b = P.v.bc ? params[P.types[T.b]] : bt;
+    double c = P.v.bc ? params[P.types[T.c]] : ct;
+    double Vmod;
+    if (P.v.bw_ball) {  // cuda.c:603-607
+      double cos_zeta = Ep_a;
+      double kappa = params[P.types[T.kappa]];
+      Vmod = -2.5 * std::log10(0.5 * (kappa * (1 + cos_zeta) + 0.5 * (1 - cos_zeta)));
+    } else {            // cuda.c:666-689
+      double scalar_Sun = std::sqrt(Sp_b * Sp_b / (b * b) + Sp_c * Sp_c / (c * c) + Sp_a * Sp_a);
+      double scalar_Earth = std::sqrt(Ep_b * Ep_b / (b * b) + Ep_c * Ep_c / (c * c) + Ep_a * Ep_a);
+      double cos_alpha_p = (Sp_b * Ep_b / (b * b) + Sp_c * Ep_c / (c * c) + Sp_a * Ep_a) / (scalar_Sun * scalar_Earth);
+      double sin_alpha_p = std::sqrt(1.0 - cos_alpha_p * cos_alpha_p);
+      double alpha_p = std::atan2(sin_alpha_p, cos_alpha_p);
+      double scalar = std::sqrt(scalar_Sun * scalar_Sun + scalar_Earth * scalar_Earth + 2 * scalar_Sun * scalar_Earth * cos_alpha_p);
+      double cos_lambda_p = (scalar_Sun + scalar_Earth * cos_alpha_p) / scalar;
+      double sin_lambda_p = scalar_Earth * sin_alpha_p / scalar;
+      double lambda_p = std::atan2(sin_lambda_p, cos_lambda_p);
+      Vmod = -2.5 * std::log10(b * c * scalar_Sun * scalar_Earth / scalar *
+                               (std::cos(lambda_p - alpha_p) + cos_lambda_p +
+                                sin_lambda_p * std::sin(lambda_p - alpha_p) *
+                                    std::log(1.0 / std::tan(0.5 * lambda_p) / std::tan(0.5 * (alpha_p - lambda_p)))));
+    }
+    if (out && out->vmod) out->vmod[i] = Vmod;
+    int m = P.obs[i].filter;  // cuda.c:708-714
+    double y = P.obs[i].V - Vmod;
+    sy2[m] = sy2[m] + y * y * P.obs[i].w;
+    sy[m] = sy[m] + y * P.obs[i].w;
+    sw[m] = sw[m] + P.obs[i].w;
+  }
+}
+
+// ========================================================== canonical chi2one
+// Templated over the scalar R so an operation-counting type can run it.
+template <class R>
+inline void ode_canon(bool torque, const R* y, R* f, const R* mu, R mu02) {
+  using astm::p_fma;
+  if (torque) {
+    R s5, c5, s4, c4;
+    astm::sincos(y[5], &s5, &c5);
+    astm::sincos(y[4], &s4, &c4);
+    f[0] = p_fma(mu[0] * y[1], y[2], mu[3]);
+    f[1] = p_fma(mu[1] * y[2], y[0], mu[4]);
+    f[2] = p_fma(mu[2] * y[0], y[1], mu[5]);
+    f[3] = p_fma(y[1], c5, y[0] * s5) / s4;
+    f[4] = p_fma(-y[1], s5, y[0] * c5);
+    f[5] = p_fma(-f[3], c4, y[2]);
+  } else {
+    R s2, c2, s1, c1;
+    astm::sincos(R(2.0) * y[2], &s2, &c2);
+    astm::sincos(y[1], &s1, &c1);
+    f[0] = mu[0] * p_fma(-mu[2], c2, mu[1]);
+    f[1] = (mu02 * s1) * s2;
+    f[2] = c1 * (mu[0] - f[0]);
+  }
+}
+
+template <class R> inline R dot3(R ax, R ay, R az, R bx, R by, R bz) {
+  using astm::p_fma;
+  return p_fma(az, bz, p_fma(ay, by, ax * bx));
+}
+
+template <class R>
+inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R* sw, ModelOut* out) {
+  using astm::p_fma;
+  using astm::p_sqrt;
+  const ast_type_ids& T = P.T;
+  const bool torque = P.v.torque;
+  for (int m = 0; m < P.n_filters; ++m) sy2[m] = sy[m] = sw[m] = R(0.0);
+  R sTM, cTM, sPM, cPM;
+  astm::sincos(R(params[P.types[T.theta_M]]), &sTM, &cTM);
+  astm::sincos(R(params[P.types[T.phi_M]]), &sPM, &cPM);
+  R M_x = sTM * cPM, M_y = sTM * sPM, M_z = cTM;
+  R XM = p_sqrt(p_fma(M_x, M_x, M_z * M_z));
+  R XM_x = M_z / XM;
+  R XM_z = -M_x / XM;
+  R YM_x = M_y * XM_z;
+  R YM_y = p_fma(-M_x, XM_z, M_z * XM_x);
+  R YM_z = -(M_y * XM_x);
+  R bt = R(params[P.types[T.b_tumb]]), ct = R(params[P.types[T.c_tumb]]);
+  R den = p_fma(ct, ct, bt * bt);
+  R Is = p_fma(bt, bt, R(1.0)) / den;
+  R Ii = p_fma(ct, ct, R(1.0)) / den;
+  R Is_inv = R(1.0) / Is;
+  R Ii_inv = R(1.0) / Ii;
+  R Es = R(params[P.types[T.Es]]), psi0 = R(params[P.types[T.psi_0]]), L = R(params[P.types[T.L]]);
+  R sP0, cP0;
+  astm::sincos(psi0, &sP0, &cP0);
+  R theta = astm::asin(p_sqrt((Es - R(1.0)) / (p_fma(sP0 * sP0, Ii_inv - Is_inv, Is_inv) - R(1.0))));
+  R phi = R(params[P.types[T.phi_0]]);
+  R psi = psi0;
+  R mu[6];
+  R mu02 = R(0.0);
+  R Om_i = R(0.0), Om_s = R(0.0), Om_l = R(0.0);
+  if (torque) {
+    mu[0] = (Is - R(1.0)) * Ii_inv;
+    mu[1] = (R(1.0) - Ii) * Is_inv;
+    mu[2] = Ii - Is;
+    mu[3] = R(params[P.types[T.Ti]]);
+    mu[4] = R(params[P.types[T.Ts]]);
+    mu[5] = R(params[P.types[T.Tl]]);
+    R sth, cth;
+    astm::sincos(theta, &sth, &cth);
+    Om_i = ((L * Ii_inv) * sth) * sP0;
+    Om_s = ((L * Is_inv) * sth) * cP0;
+    Om_l = L * cth;
+  } else {
+    mu[0] = L;
+    mu[1] = R(0.5) * (Ii_inv + Is_inv);
+    mu[2] = R(0.5) * (Ii_inv - Is_inv);
+    mu02 = mu[0] * mu[2];
+  }
+  // Photometric shape (LS) or ball constants (BW_BALL).
+  R ib = R(0.0), ic = R(0.0), bcp = R(0.0), kappa = R(0.0);
+  R sR = R(0.0), cR = R(0.0), stR = R(0.0), ctR = R(0.0);
+  if (P.v.bw_ball) {
+    astm::sincos(R(params[P.types[T.phi_R]]), &sR, &cR);
+    astm::sincos(R(params[P.types[T.theta_R]]), &stR, &ctR);
+    kappa = R(params[P.types[T.kappa]]);
+  } else {
+    R b = P.v.bc ? R(params[P.types[T.b]]) : bt;
+    R c = P.v.bc ? R(params[P.types[T.c]]) : ct;
+    ib = R(1.0) / b;
+    ic = R(1.0) / c;
+    bcp = b * c;
+  }
+  const int n_ode = torque ? 6 : 3;
+  for (int i = 0; i < P.n_data; ++i) {
+    if (i > 0) {
+      int N_steps;
+      double hd;
+      rk4_interval(P.obs[i - 1].mjd, P.obs[i].mjd, P.v.time_step, &N_steps, &hd);
+      R h = R(hd), hh = R(0.5 * hd), h6 = R(1 / 6.0 * hd);
+      R y[6];
+      if (torque) { y[0] = Om_i; y[1] = Om_s; y[2] = Om_l; y[3] = phi; y[4] = theta; y[5] = psi; }
+      else { y[0] = phi; y[1] = theta; y[2] = psi; }
+      for (int l = 0; l < N_steps; ++l) {
+        R f[6], K[6], s[6];
+        ode_canon(torque, y, K, mu, mu02);
+        for (int j = 0; j < n_ode; ++j) { s[j] = K[j]; f[j] = p_fma(hh, K[j], y[j]); }
+        ode_canon(torque, f, K, mu, mu02);
+        for (int j = 0; j < n_ode; ++j) { s[j] = p_fma(R(2.0), K[j], s[j]); f[j] = p_fma(hh, K[j], y[j]); }
+        ode_canon(torque, f, K, mu, mu02);
+        for (int j = 0; j < n_ode; ++j) { s[j] = p_fma(R(2.0), K[j], s[j]); f[j] = p_fma(h, K[j], y[j]); }
+        ode_canon(torque, f, K, mu, mu02);
+        for (int j = 0; j < n_ode; ++j) y[j] = p_fma(h6, s[j] + K[j], y[j]);
+      }
+      if (torque) { Om_i = y[0]; Om_s = y[1]; Om_l = y[2]; phi = y[3]; theta = y[4]; psi = y[5]; }
+      else { phi = y[0]; theta = y[1]; psi = y[2]; }
+    }
+    if (out && out->traj) {
+      double* t6 = out->traj + 6 * i;
+      t6[0] = astm::p_val(phi); t6[1] = astm::p_val(theta); t6[2] = astm::p_val(psi);
+      t6[3] = astm::p_val(Om_i); t6[4] = astm::p_val(Om_s); t6[5] = astm::p_val(Om_l);
+    }
+    R sf, cf, st, ctt, sp, cp;
+    astm::sincos(phi, &sf, &cf);
+    astm::sincos(theta, &st, &ctt);
+    astm::sincos(psi, &sp, &cp);
+    R N_x = p_fma(YM_x, sf, XM_x * cf);
+    R N_y = YM_y * sf;
+    R N_z = p_fma(YM_z, sf, XM_z * cf);
+    R p_x = p_fma(-N_z, M_y, N_y * M_z);
+    R p_y = p_fma(-N_x, M_z, N_z * M_x);
+    R p_z = p_fma(-N_y, M_x, N_x * M_y);
+    R a_x = p_fma(p_x, st, M_x * ctt);
+    R a_y = p_fma(p_y, st, M_y * ctt);
+    R a_z = p_fma(p_z, st, M_z * ctt);
+    R w_x = p_fma(-a_z, N_y, a_y * N_z);
+    R w_y = p_fma(-a_x, N_z, a_z * N_x);
+    R w_z = p_fma(-a_y, N_x, a_x * N_y);
+    R b_x = p_fma(w_x, sp, N_x * cp);
+    R b_y = p_fma(w_y, sp, N_y * cp);
+    R b_z = p_fma(w_z, sp, N_z * cp);
+    R c_x = p_fma(-a_z, b_y, a_y * b_z);
+    R c_y = p_fma(-a_x, b_z, a_z * b_x);
+    R c_z = p_fma(-a_y, b_x, a_x * b_y);
+    double Ed[3], Sd[3];
+    if (P.v.interp) interp_canon(P.eph, P.obs[i].mjd, Ed, Sd);
+    else for (int c = 0; c < 3; ++c) { Ed[c] = P.obs[i].E[c]; Sd[c] = P.obs[i].S[c]; }
+    R E0 = R(Ed[0]), E1 = R(Ed[1]), E2 = R(Ed[2]);
+    R Vmod;
+    if (P.v.bw_ball) {
+      R n_x = p_fma(c_x, sR, b_x * cR);
+      R n_y = p_fma(c_y, sR, b_y * cR);
+      R n_z = p_fma(c_z, sR, b_z * cR);
+      R q_x = p_fma(-n_z, a_y, n_y * a_z);
+      R q_y = p_fma(-n_x, a_z, n_z * a_x);
+      R q_z = p_fma(-n_y, a_x, n_x * a_y);
+      R r_x = p_fma(q_x, stR, a_x * ctR);
+      R r_y = p_fma(q_y, stR, a_y * ctR);
+      R r_z = p_fma(q_z, stR, a_z * ctR);
+      R cz = dot3(r_x, r_y, r_z, E0, E1, E2);
+      Vmod = R(-2.5) * astm::log10(R(0.5) * p_fma(kappa, R(1.0) + cz, R(0.5) * (R(1.0) - cz)));
+    } else {
+      R S0 = R(Sd[0]), S1 = R(Sd[1]), S2 = R(Sd[2]);
+      R bs_x = b_x * ib, bs_y = b_y * ib, bs_z = b_z * ib;
+      R cs_x = c_x * ic, cs_y = c_y * ic, cs_z = c_z * ic;
+      R uSb = dot3(bs_x, bs_y, bs_z, S0, S1, S2);
+      R uSc = dot3(cs_x, cs_y, cs_z, S0, S1, S2);
+      R uSa = dot3(a_x, a_y, a_z, S0, S1, S2);
+      R uEb = dot3(bs_x, bs_y, bs_z, E0, E1, E2);
+      R uEc = dot3(cs_x, cs_y, cs_z, E0, E1, E2);
+      R uEa = dot3(a_x, a_y, a_z, E0, E1, E2);
+      R sS2 = dot3(uSb, uSc, uSa, uSb, uSc, uSa);
+      R sE2 = dot3(uEb, uEc, uEa, uEb, uEc, uEa);
+      R sS = p_sqrt(sS2);
+      R sE = p_sqrt(sE2);
+      R ca = dot3(uSb, uSc, uSa, uEb, uEc, uEa) / (sS * sE);
+      R sa = p_sqrt(p_fma(-ca, ca, R(1.0)));
+      R sc = p_sqrt(p_fma((R(2.0) * sS) * sE, ca, sS2 + sE2));
+      R isc = R(1.0) / sc;
+      R cl = p_fma(sE, ca, sS) * isc;
+      R sl = (sE * sa) * isc;
+      R cd = p_fma(cl, ca, sl * sa);     // cos(lambda' - alpha')
+      R sd = p_fma(-cl, sa, sl * ca);    // sin(lambda' - alpha')
+      R lg = astm::log(((R(1.0) + cl) * (R(1.0) + cd)) / (sl * (-sd)));
+      R inner = p_fma(sl * sd, lg, cd + cl);
+      Vmod = R(-2.5) * astm::log10((((bcp * sS) * sE) * isc) * inner);
+    }
+    if (out && out->vmod) out->vmod[i] = astm::p_val(Vmod);
+    int m = P.obs[i].filter;
+    R w = R((double)P.obs[i].w);
+    R y = R((double)P.obs[i].V) - Vmod;
+    sy2[m] = p_fma(y * y, w, sy2[m]);
+    sy[m] = p_fma(y, w, sy[m]);
+    sw[m] = sw[m] + w;
+  }
+}
+
+// chi^2 and delta_V from the per-filter sums (cuda.c:884-910).
+template <class CF>
+inline CF chi2_from_sums(const Problem& P, const double* sy2, const double* sy, const double* sw, CF* dV) {
+  CF chi2a = 0.0;
+  for (int m = 0; m < P.n_filters; ++m) {
+    CF chi2m = sy2[m] - sy[m] * sy[m] / sw[m];
+    chi2a = chi2a + chi2m;
+    dV[m] = sy[m] / sw[m];
+  }
+  chi2a = chi2a / (P.n_data - P.n_params - P.n_filters);
+  return chi2a;
+}
+
+template <class CF>
+inline CF chi2_eval(const Problem& P, int form, const double* params, CF* dV, ModelOut* out = nullptr) {
+  double sy2[AST_MAX_FILTERS], sy[AST_MAX_FILTERS], sw[AST_MAX_FILTERS];
+  if (form == 0) model_literal(P, params, sy2, sy, sw, out);
+  else model_canon<double>(P, params, sy2, sy, sw, out);
+  return chi2_from_sums<CF>(P, sy2, sy, sw, dV);
+}
+
+// ================================================================ x2params
+inline bool hard_left(int per, int LAM) { return per == AST_HARD_BOTH || per == AST_HARD_LEFT || (LAM == 0 && per == AST_PERIODIC_LAM); }
+inline bool hard_right(int per, int LAM) { return per == AST_HARD_BOTH || per == AST_HARD_RIGHT || (LAM == 0 && per == AST_PERIODIC_LAM); }
+
+template <class CF, class M>
+int x2params(const Problem& P, const CF* x, double* params, int reopt) {
+  const ast_type_ids& T = P.T;
+  const int N = P.n_params;
+  int LAM = 0;
+  for (int i = 0; i < N; ++i) {  // cuda.c:1128-1147
+    if (P.prop[i][AST_P_TYPE] == T.Es) LAM = x[i] >= 0.5;
+    int per = P.prop[i][AST_P_PERIODIC];
+    if ((x[i] < 0.0 && hard_left(per, LAM)) || (x[i] > 1.0 && hard_right(per, LAM))) return 1;
+  }
+  double log_c_tumb = 0, log_b_tumb = 0, Is = 0, Ii = 0, psi_min = 0, psi_max = 0, log_c = 0;
+  for (int i = 0; i < N; ++i) {  // cuda.c:1155-1314
+    int t = P.prop[i][AST_P_TYPE];
+    CF L0 = (CF)P.lim[0][t], L1 = (CF)P.lim[1][t];
+    if (t == T.b_tumb) {
+      log_b_tumb = log_c_tumb * (x[i] * (L1 - L0) + L0);
+      params[i] = M::exp(log_b_tumb);
+      double b_tumb = params[i];
+      double c_tumb = params[P.types[T.c_tumb]];
+      Is = (1.0 + b_tumb * b_tumb) / (b_tumb * b_tumb + c_tumb * c_tumb);
+      Ii = (1.0 + c_tumb * c_tumb) / (b_tumb * b_tumb + c_tumb * c_tumb);
+    } else if (t == T.Es) {
+      if (LAM) {
+        params[i] = 2.0 * (x[i] - 0.5) * (1.0 - 1.0 / Ii) + 1.0 / Ii;
+        psi_min = 0.0;
+        psi_max = 2.0 * PI;
+      } else {
+        params[i] = 2.0 * x[i] * (1.0 / Ii - 1.0 / Is) + 1.0 / Is;
+        psi_max = M::atan(M::sqrt(Ii * (Is - 1.0 / params[i]) / Is / (1.0 / params[i] - Ii)));
+        psi_min = -psi_max;
+      }
+    } else if (t == T.psi_0) {
+      params[i] = x[i] * (psi_max - psi_min) + psi_min;
+    } else if (P.v.bc && t == T.b) {
+      double log_b = log_c * (x[i] * (L1 - L0) + L0);
+      if (std::fabs(log_b - log_b_tumb) > BC_DEV_MAX) return 1;
+      params[i] = M::exp(log_b);
+    } else if (P.prop[i][AST_P_PERIODIC] == AST_PERIODIC) {
+      params[i] = x[i] * 2.0 * PI;
+    } else if (P.prop[i][AST_P_INDEPENDENT] == 1) {
+      if (!P.v.p_psi || t != T.L || reopt) params[i] = x[i] * (L1 - L0) + L0;
+      if (t == T.c_tumb) {
+        log_c_tumb = params[i];
+        params[i] = M::exp(log_c_tumb);
+      } else if (P.v.bc && t == T.c) {
+        log_c = params[i];
+        if (std::fabs(log_c - log_c_tumb) > BC_DEV_MAX) return 1;
+        params[i] = M::exp(log_c);
+      } else if (P.v.bw_ball && t == T.kappa) {
+        params[i] = M::exp(params[i]);
+      } else if (t == T.L && !reopt && P.v.p_psi) {
+        float dP = P.x2.ppsi2 - P.x2.ppsi1;
+        params[i] = x[i] * (CF)dP + (CF)P.x2.ppsi1;
+        double EsV = params[P.types[T.Es]];
+        double Einv = 1.0 / EsV;
+        double k2;
+        if (LAM) k2 = (Is - Ii) * (Einv - 1.0) / ((Ii - 1.0) * (Is - Einv));
+        else k2 = (Ii - 1.0) * (Is - Einv) / ((Is - Ii) * (Einv - 1.0));
+        double a = 1.0, g = M::sqrt(1.0 - k2), a1, g1;
+        for (int ii = 0; ii < 5; ++ii) {
+          a1 = 0.5 * (a + g);
+          g1 = M::sqrt(a * g);
+          a = a1;
+          g = g1;
+        }
+        if (LAM) params[i] = 4.0 * params[i] * PI / (a + g) * M::sqrt(Ii * Is / (EsV * (Ii - 1.0) * (Is - Einv)));
+        else params[i] = 4.0 * params[i] * PI / (a + g) * M::sqrt(Ii * Is / (EsV * (Is - Ii) * (Einv - 1.0)));
+      }
+    }
+  }
+  return 0;
+}
+
+// ================================================================ params2x
+template <class CF, class M>
+void params2x(const Problem& P, CF* x, const double* params) {
+  const ast_type_ids& T = P.T;
+  const int N = P.n_params;
+  int LAM = 0, i_Es = 0;
+  double b_tumb = params[P.types[T.b_tumb]];
+  double c_tumb = params[P.types[T.c_tumb]];
+  double Is = (1.0 + b_tumb * b_tumb) / (b_tumb * b_tumb + c_tumb * c_tumb);
+  double Ii = (1.0 + c_tumb * c_tumb) / (b_tumb * b_tumb + c_tumb * c_tumb);
+  for (int i = 0; i < N; ++i) {
+    int t = P.prop[i][AST_P_TYPE];
+    if (P.prop[i][AST_P_FROZEN] != 0) { x[i] = 0; continue; }
+    CF L0 = (CF)P.lim[0][t], L1 = (CF)P.lim[1][t];
+    if (P.prop[i][AST_P_INDEPENDENT] == 1) {
+      double par = params[i];
+      if (P.prop[i][AST_P_PERIODIC] == 1) {
+        x[i] = par / (2 * PI);
+        x[i] = x[i] - cf_floor(x[i]);
+      } else {
+        if (P.v.bc && t == T.c) {
+          CF C0 = (CF)P.lim[0][T.c_tumb], C1 = (CF)P.lim[1][T.c_tumb];
+          x[i] = (M::log(params[i]) - C0) / (C1 - C0);
+          continue;
+        }
+        if (t == T.c_tumb) par = M::log(par);
+        if (P.v.bw_ball && t == T.kappa) par = M::log(par);
+        x[i] = (par - L0) / (L1 - L0);
+      }
+    } else {
+      if (t == T.b_tumb) {
+        double par = M::log(b_tumb) / M::log(c_tumb);
+        x[i] = (par - L0) / (L1 - L0);
+      } else if (t == T.Es) {
+        i_Es = i;
+        LAM = params[i] > 1.0 / Ii;
+        if (LAM) x[i] = 0.5 * ((params[i] - 1.0 / Ii) / (1.0 - 1.0 / Ii) + 1.0);
+        else x[i] = 0.5 * (params[i] - 1.0 / Is) / (1.0 / Ii - 1.0 / Is);
+      } else if (t == T.psi_0) {
+        double psi_min, psi_max;
+        if (LAM) { psi_min = 0.0; psi_max = 2.0 * PI; }
+        else {
+          psi_max = M::atan(M::sqrt(Ii * (Is - 1.0 / params[i_Es]) / Is / (1.0 / params[i_Es] - Ii)));
+          psi_min = -psi_max;
+        }
+        x[i] = (params[i] - psi_min) / (psi_max - psi_min);
+      } else if (P.v.bc && t == T.b) {
+        double par = M::log(params[i]) / M::log(params[P.types[T.c]]);
+        x[i] = (par - L0) / (L1 - L0);
+      }
+    }
+  }
+}
+
+// ================================================================ simplex
+struct RunCfg {
+  int n_groups, group_size, nstages, n_steps;
+  int64_t eval_budget;
+  double size_min;
+  uint64_t seed;
+};
+
+template <class CF> struct SimplexConst {
+  static CF DX_INI() { return (CF)0.001; }   // asteroid.h:205
+  static CF D2X_INI() { return (CF)-1.2; }   // asteroid.h:206
+  static CF DX_RAND() { return (CF)0.0001; } // asteroid.h:209
+};
+
+template <class CF> struct Lane {
+  CF x[AST_MAX_PARAMS + 1][AST_MAX_PARAMS];
+  CF f[AST_MAX_PARAMS + 1];
+  int ind[AST_MAX_PARAMS + 1];
+  CF dV[AST_MAX_FILTERS];
+  double params[AST_MAX_PARAMS];  // the thread's params[] buffer (cuda.c:1344), last x2params output
+  uint32_t rng;      // draws taken
+  int64_t nev, steps;
+  CF s_f;
+};
+
+// One stage of one simplex (cuda.c:1435-1742).  Returns s_f.
+template <class CF, class M>
+CF run_stage(const Problem& P, int form, const RunCfg& C, int reopt, uint64_t fit, Lane<CF>& S) {
+  const ast_type_ids& T = P.T;
+  const int N = P.n_params;
+  const CF DX_INI = SimplexConst<CF>::DX_INI(), D2X_INI = SimplexConst<CF>::D2X_INI(), DX_RAND = SimplexConst<CF>::DX_RAND();
+  const CF ALPHA = 1.0, GAMMA = 2.0, RHO = 0.5, SIGMA = 0.5;  // asteroid.h:239-242
+  const CF SIZE2_MIN = (CF)C.size_min * (CF)C.size_min;      // asteroid.h:245
+  double* params = S.params;
+  const int64_t nev0 = S.nev;
+  auto rnd = [&]() { return astrng::uniform01(C.seed, fit, S.rng++); };
+  auto eval = [&](const CF* xv, CF* fv) -> bool {  // x2params + chi2one
+    if (x2params<CF, M>(P, xv, params, reopt)) return false;
+    *fv = chi2_eval<CF>(P, form, params, S.dV);
+    S.nev++;
+    return true;
+  };
+  int l = 0;
+  int LAM = 0;
+  for (int i = 0; i < N; ++i) {  // cuda.c:1459-1508
+    float r = rnd();
+    int t = P.prop[i][AST_P_TYPE];
+    if (P.v.bc && !reopt) {
+      if (t == T.c) { S.x[0][i] = S.x[0][P.types[T.c_tumb]]; continue; }
+      if (t == T.b) { S.x[0][i] = S.x[0][P.types[T.b_tumb]]; continue; }
+    }
+    int per = P.prop[i][AST_P_PERIODIC];
+    if (!reopt || (reopt && P.prop[i][AST_P_FROZEN] == -1)) {
+      S.x[0][i] = DX_INI + SMALL + r * (1.0 - 2 * (SMALL + DX_INI));
+    } else {
+      CF xmin = S.x[0][i] - DX_RAND;
+      CF xmax = S.x[0][i] + DX_RAND;
+      if (xmin < SMALL && hard_left(per, LAM)) xmin = SMALL;
+      if (xmax > 1.0 - SMALL && hard_right(per, LAM)) xmax = 1.0 - SMALL;
+      S.x[0][i] = xmin + r * (xmax - xmin);
+    }
+    if (t == T.Es) LAM = S.x[0][i] >= 0.5;
+  }
+  for (int j = 1; j < N + 1; ++j) {  // cuda.c:1512-1545
+    for (int i = 0; i < N; ++i) {
+      if (i == j - 1) {
+        int per = P.prop[i][AST_P_PERIODIC];
+        float d2x = rnd();
+        CF dx = DX_INI * cf_exp<M>((CF)(D2X_INI * d2x));
+        if (rnd() < 0.5) {
+          dx = -dx;
+          if (S.x[0][i] + dx < SMALL && hard_left(per, LAM)) dx = -dx;
+        } else {
+          if (S.x[0][i] + dx > 1.0 - SMALL && hard_right(per, LAM)) dx = -dx;
+        }
+        S.x[j][i] = S.x[0][i] + dx;
+      } else {
+        S.x[j][i] = S.x[0][i];
+      }
+    }
+  }
+  bool failed = false;
+  for (int j = 0; j < N + 1; ++j) {  // cuda.c:1549-1559
+    if (!eval(S.x[j], &S.f[j])) { failed = true; break; }
+  }
+  while (true) {  // cuda.c:1568-1735
+    if (failed) break;
+    l++;
+    bool ind2[AST_MAX_PARAMS + 1];
+    for (int j = 0; j < N + 1; ++j) ind2[j] = false;
+    for (int j = 0; j < N + 1; ++j) {
+      CF fmin = 1e30;
+      int jmin = -1;
+      for (int j2 = 0; j2 < N + 1; ++j2)
+        if (!ind2[j2] && S.f[j2] <= fmin) { fmin = S.f[j2]; jmin = j2; }
+      if (jmin < 0) { S.ind[0] = 0; S.f[S.ind[0]] = 1e30; break; }
+      S.ind[j] = jmin;
+      ind2[jmin] = true;
+    }
+    CF x0[AST_MAX_PARAMS];
+    for (int i = 0; i < N; ++i) {
+      CF sum = 0.0;
+      for (int j = 0; j < N + 1; ++j) sum = sum + S.x[j][i];
+      x0[i] = sum / (N + 1);
+    }
+    CF size2 = 0.0;
+    for (int j = 0; j < N + 1; ++j) {
+      CF sum = 0.0;
+      for (int i = 0; i < N; ++i) {
+        CF dx = S.x[j][i] - x0[i];
+        sum = sum + dx * dx;
+      }
+      size2 = size2 + sum;
+    }
+    size2 = size2 / N;
+    if (size2 < SIZE2_MIN) break;
+    if (l > C.n_steps) break;
+    if (C.eval_budget > 0 && S.nev - nev0 >= C.eval_budget) break;
+    const int w = S.ind[N];
+    CF x_r[AST_MAX_PARAMS];
+    for (int i = 0; i < N; ++i)
+      x_r[i] = (P.prop[i][AST_P_FROZEN] != 1) ? (CF)(x0[i] + ALPHA * (x0[i] - S.x[w][i])) : S.x[w][i];
+    CF f_r;
+    if (!eval(x_r, &f_r)) f_r = 1e30;
+    if (f_r >= S.f[S.ind[0]] && f_r < S.f[S.ind[N - 1]]) {
+      for (int i = 0; i < N; ++i) S.x[w][i] = x_r[i];
+      S.f[w] = f_r;
+      continue;
+    }
+    if (f_r < S.f[S.ind[0]]) {
+      CF x_e[AST_MAX_PARAMS];
+      for (int i = 0; i < N; ++i)
+        x_e[i] = (P.prop[i][AST_P_FROZEN] != 1) ? (CF)(x0[i] + GAMMA * (x_r[i] - x0[i])) : x_r[i];
+      CF f_e;
+      if (!eval(x_e, &f_e)) f_e = 1e30;
+      if (f_e < f_r) {
+        for (int i = 0; i < N; ++i) S.x[w][i] = x_e[i];
+        S.f[w] = f_e;
+      } else {
+        for (int i = 0; i < N; ++i) S.x[w][i] = x_r[i];
+        S.f[w] = f_r;
+      }
+      continue;
+    }
+    for (int i = 0; i < N; ++i)
+      x_r[i] = (P.prop[i][AST_P_FROZEN] != 1) ? (CF)(x0[i] + RHO * (S.x[w][i] - x0[i])) : S.x[w][i];
+    if (!eval(x_r, &f_r)) f_r = 1e30;
+    if (f_r < S.f[w]) {
+      for (int i = 0; i < N; ++i) S.x[w][i] = x_r[i];
+      S.f[w] = f_r;
+      continue;
+    }
+    bool bad = false;
+    for (int j = 1; j < N + 1; ++j) {
+      const int v = S.ind[j], b = S.ind[0];
+      for (int i = 0; i < N; ++i)
+        if (P.prop[i][AST_P_FROZEN] != 1) S.x[v][i] = S.x[b][i] + SIGMA * (S.x[v][i] - S.x[b][i]);
+      CF fv;
+      if (!eval(S.x[v], &fv)) bad = true;
+      else S.f[v] = fv;
+    }
+    if (bad) { failed = true; break; }
+  }
+  S.steps += l;
+  S.s_f = failed ? (CF)1e30 : S.f[S.ind[0]];
+  return S.s_f;
+}
+
+// One block of chi2_gpu: group g of C.group_size simplexes.
+template <class CF, class M>
+void run_group(const Problem& P, int form, const RunCfg& C, int reopt_in, const double* params0, int g,
+               uint32_t* rng, double* d_f, double* d_params, double* d_dV,
+               double* lane_f, int64_t* lane_nev, int64_t* lane_steps) {
+  const int G = C.group_size, N = P.n_params;
+  std::vector<Lane<CF>> lanes(G);
+  int reopt = reopt_in;
+  for (int t = 0; t < G; ++t) {
+    Lane<CF>& S = lanes[t];
+    std::memset(&S, 0, sizeof(S));
+    for (int j = 0; j <= N; ++j) S.ind[j] = j;
+    S.rng = rng[(size_t)g * G + t];
+    if (reopt) {  // cuda.c:1409-1416
+      for (int i = 0; i < N; ++i) S.params[i] = params0[i];
+      params2x<CF, M>(P, S.x[0], S.params);
+    }
+  }
+  CF s_x0[AST_MAX_PARAMS];
+  int thread_min = 0;
+  CF smin = (CF)1e30;
+  for (int istage = 0; istage < C.nstages; ++istage) {
+    if (C.nstages > 1 && istage > 0)
+      for (int t = 0; t < G; ++t)
+        for (int i = 0; i < N; ++i) lanes[t].x[0][i] = s_x0[i];
+    for (int t = 0; t < G; ++t) run_stage<CF, M>(P, form, C, reopt, (uint64_t)g * G + t, lanes[t]);
+    thread_min = 0;  // cuda.c:1770-1782
+    smin = (CF)1e30;
+    for (int j = 0; j < G; ++j)
+      if (lanes[j].s_f < smin) { smin = lanes[j].s_f; thread_min = j; }
+    if (C.nstages > 1 && istage < C.nstages - 1) {  // cuda.c:1785-1805
+      Lane<CF>& S = lanes[thread_min];
+      CF* xb = S.x[S.ind[0]];
+      if (P.v.p_psi && reopt == 0) {
+        x2params<CF, M>(P, xb, S.params, reopt);
+        params2x<CF, M>(P, xb, S.params);
+      }
+      for (int i = 0; i < N; ++i) s_x0[i] = xb[i];
+      reopt = 1;
+    }
+  }
+  if (smin < (CF)d_f[g]) {  // cuda.c:1810-1823
+    Lane<CF>& S = lanes[thread_min];
+    d_f[g] = smin;
+    x2params<CF, M>(P, S.x[S.ind[0]], S.params, reopt);
+    for (int i = 0; i < N; ++i) d_params[(size_t)g * N + i] = S.params[i];
+    for (int m = 0; m < P.n_filters; ++m) d_dV[(size_t)g * P.n_filters + m] = S.dV[m];
+  }
+  for (int t = 0; t < G; ++t) {
+    size_t id = (size_t)g * G + t;
+    rng[id] = lanes[t].rng;
+    if (lane_f) lane_f[id] = lanes[t].s_f;
+    if (lane_nev) lane_nev[id] = lanes[t].nev;
+    if (lane_steps) lane_steps[id] = lanes[t].steps;
+  }
+}
+
+}  // namespace orc
+
+// ===================================================================== C API
+extern "C" {
+
+// Problem description passed from Python (ctypes mirror in tests/oracle_lib.py).
+typedef struct {
+  ast_variant v;
+  int n_params;
+  int n_types;
+  int property[AST_MAX_PARAMS * AST_N_COLUMNS];
+  int types[AST_MAX_TYPES];
+  double limits[2 * AST_MAX_TYPES];  // lower [0,n_types), upper [n_types, 2 n_types)
+  ast_x2 x2;
+  int n_data;
+  int n_filters;
+  const ast_obs* obs;
+  ast_ephem3 eph;
+} orc_problem;
+
+static int load_problem(const orc_problem* in, orc::Problem* P) {
+  if (!in || !in->obs || in->n_params <= 0 || in->n_params > AST_MAX_PARAMS || in->n_types > AST_MAX_TYPES ||
+      in->n_filters <= 0 || in->n_filters > AST_MAX_FILTERS)
+    return AST_E_ARG;
+  P->v = in->v;
+  ast_variant_type_ids(&in->v, &P->T);
+  P->n_params = in->n_params;
+  P->n_types = in->n_types;
+  for (int i = 0; i < in->n_params; ++i)
+    for (int c = 0; c < AST_N_COLUMNS; ++c) P->prop[i][c] = in->property[i * AST_N_COLUMNS + c];
+  for (int t = 0; t < AST_MAX_TYPES; ++t) {
+    P->types[t] = t < in->n_types ? in->types[t] : -1;
+    P->lim[0][t] = t < in->n_types ? in->limits[t] : 0.0;
+    P->lim[1][t] = t < in->n_types ? in->limits[in->n_types + t] : 0.0;
+  }
+  P->x2 = in->x2;
+  P->n_data = in->n_data;
+  P->n_filters = in->n_filters;
+  P->obs = in->obs;
+  P->eph = in->eph;
+  return 0;
+}
+
+static bool simplex_float(const ast_variant& v) { return v.precision != AST_PREC_FP64; }
+
+// chi^2 and delta_V of n models; form 0 = literal, 1 = canonical.
+int orc_chi2(const orc_problem* pb, int form, const double* params, int n, double* chi2, double* dV,
+             double* vmod, double* traj) {
+  orc::Problem P;
+  int rc = load_problem(pb, &P);
+  if (rc) return rc;
+  for (int k = 0; k < n; ++k) {
+    orc::ModelOut out{vmod ? vmod + (size_t)k * P.n_data : nullptr, traj ? traj + (size_t)k * 6 * P.n_data : nullptr};
+    const double* p = params + (size_t)k * P.n_params;
+    if (simplex_float(P.v)) {
+      float d[AST_MAX_FILTERS];
+      chi2[k] = orc::chi2_eval<float>(P, form, p, d, &out);
+      for (int m = 0; m < P.n_filters; ++m) dV[(size_t)k * P.n_filters + m] = d[m];
+    } else {
+      double d[AST_MAX_FILTERS];
+      chi2[k] = orc::chi2_eval<double>(P, form, p, d, &out);
+      for (int m = 0; m < P.n_filters; ++m) dV[(size_t)k * P.n_filters + m] = d[m];
+    }
+  }
+  return 0;
+}
+
+// x2params on n points (x in double, converted to the simplex float type).
+int orc_x2params(const orc_problem* pb, int form, int reopt, const double* x, int n, double* params, int* rcs) {
+  orc::Problem P;
+  int rc = load_problem(pb, &P);
+  if (rc) return rc;
+  for (int k = 0; k < n; ++k) {
+    const double* xk = x + (size_t)k * P.n_params;
+    double* pk = params + (size_t)k * P.n_params;
+    if (simplex_float(P.v)) {
+      float xf[AST_MAX_PARAMS];
+      for (int i = 0; i < P.n_params; ++i) xf[i] = (float)xk[i];
+      rcs[k] = form ? orc::x2params<float, orc::AstMath>(P, xf, pk, reopt) : orc::x2params<float, orc::LibmMath>(P, xf, pk, reopt);
+    } else {
+      rcs[k] = form ? orc::x2params<double, orc::AstMath>(P, xk, pk, reopt) : orc::x2params<double, orc::LibmMath>(P, xk, pk, reopt);
+    }
+  }
+  return 0;
+}
+
+int orc_params2x(const orc_problem* pb, int form, const double* params, int n, double* x) {
+  orc::Problem P;
+  int rc = load_problem(pb, &P);
+  if (rc) return rc;
+  for (int k = 0; k < n; ++k) {
+    const double* pk = params + (size_t)k * P.n_params;
+    double* xk = x + (size_t)k * P.n_params;
+    if (simplex_float(P.v)) {
+      float xf[AST_MAX_PARAMS];
+      if (form) orc::params2x<float, orc::AstMath>(P, xf, pk);
+      else orc::params2x<float, orc::LibmMath>(P, xf, pk);
+      for (int i = 0; i < P.n_params; ++i) xk[i] = xf[i];
+    } else {
+      if (form) orc::params2x<double, orc::AstMath>(P, xk, pk);
+      else orc::params2x<double, orc::LibmMath>(P, xk, pk);
+    }
+  }
+  return 0;
+}
+
+// One chi2_gpu launch over all groups, threaded over groups.
+int orc_run(const orc_problem* pb, int form, int n_groups, int group_size, int nstages, int n_steps,
+            int64_t eval_budget, double size_min, uint64_t seed, int reopt, const double* params0,
+            uint32_t* rng, double* d_f, double* d_params, double* d_dV, double* lane_f, int64_t* lane_nev,
+            int64_t* lane_steps, int n_threads) {
+  orc::Problem P;
+  int rc = load_problem(pb, &P);
+  if (rc) return rc;
+  if (n_groups <= 0 || group_size <= 0 || nstages <= 0) return AST_E_ARG;
+  orc::RunCfg C{n_groups, group_size, nstages, n_steps, eval_budget, size_min, seed};
+  if (C.size_min <= 0) C.size_min = simplex_float(P.v) ? 1e-5 : 1e-10;
+  auto work = [&](int g) {
+    if (simplex_float(P.v)) {
+      if (form) orc::run_group<float, orc::AstMath>(P, form, C, reopt, params0, g, rng, d_f, d_params, d_dV, lane_f, lane_nev, lane_steps);
+      else orc::run_group<float, orc::LibmMath>(P, form, C, reopt, params0, g, rng, d_f, d_params, d_dV, lane_f, lane_nev, lane_steps);
+    } else {
+      if (form) orc::run_group<double, orc::AstMath>(P, form, C, reopt, params0, g, rng, d_f, d_params, d_dV, lane_f, lane_nev, lane_steps);
+      else orc::run_group<double, orc::LibmMath>(P, form, C, reopt, params0, g, rng, d_f, d_params, d_dV, lane_f, lane_nev, lane_steps);
+    }
+  };
+  if (n_threads <= 1) {
+    for (int g = 0; g < n_groups; ++g) work(g);
+  } else {
+    std::vector<std::thread> th;
+    for (int w = 0; w < n_threads; ++w)
+      th.emplace_back([&, w]() { for (int g = w; g < n_groups; g += n_threads) work(g); });
+    for (auto& t : th) t.join();
+  }
+  return 0;
+}
+
+// Elementary functions over arrays (fn: 0 sin, 1 cos, 2 tan, 3 atan2(x,y as y,x), 4 log,
+// 5 log10, 6 exp, 7 asin, 8 atan, 9 expf).  For atan2, x holds y and y holds x.
+int orc_math(int fn, const double* x, const double* y, double* out, int n) {
+  for (int k = 0; k < n; ++k) {
+    double a = x[k];
+    switch (fn) {
+      case 0: out[k] = astm::sin(a); break;
+      case 1: out[k] = astm::cos(a); break;
+      case 2: out[k] = astm::tan(a); break;
+      case 3: out[k] = astm::atan2(a, y[k]); break;
+      case 4: out[k] = astm::log(a); break;
+      case 5: out[k] = astm::log10(a); break;
+      case 6: out[k] = astm::exp(a); break;
+      case 7: out[k] = astm::asin(a); break;
+      case 8: out[k] = astm::atan(a); break;
+      case 9: out[k] = astm::expf((float)a); break;
+      default: return AST_E_ARG;
+    }
+  }
+  return 0;
+}
+
+int orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  astrng::philox4x32_10(c, key[0], key[1]);
+  for (int i = 0; i < 4; ++i) out[i] = c[i];
+  return 0;
+}
+
+int orc_uniform(uint64_t seed, uint64_t fit, uint32_t k0, int n, float* out) {
+  for (int k = 0; k < n; ++k) out[k] = astrng::uniform01(seed, fit, k0 + (uint32_t)k);
+  return 0;
+}
+
+}  // extern "C"
