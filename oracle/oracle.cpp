@@ -333,6 +333,7 @@ template <class R>
 inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R* sw, ModelOut* out) {
   using astm::p_fma;
   using astm::p_sqrt;
+  using astm::p_val;
   const ast_type_ids& T = P.T;
   const bool torque = P.v.torque;
   for (int m = 0; m < P.n_filters; ++m) sy2[m] = sy[m] = sw[m] = R(0.0);
@@ -419,8 +420,8 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
     }
     if (out && out->traj) {
       double* t6 = out->traj + 6 * i;
-      t6[0] = astm::p_val(phi); t6[1] = astm::p_val(theta); t6[2] = astm::p_val(psi);
-      t6[3] = astm::p_val(Om_i); t6[4] = astm::p_val(Om_s); t6[5] = astm::p_val(Om_l);
+      t6[0] = p_val(phi); t6[1] = p_val(theta); t6[2] = p_val(psi);
+      t6[3] = p_val(Om_i); t6[4] = p_val(Om_s); t6[5] = p_val(Om_l);
     }
     R sf, cf, st, ctt, sp, cp;
     astm::sincos(phi, &sf, &cf);
@@ -487,7 +488,7 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
       R inner = p_fma(sl * sd, lg, cd + cl);
       Vmod = R(-2.5) * astm::log10((((bcp * sS) * sE) * isc) * inner);
     }
-    if (out && out->vmod) out->vmod[i] = astm::p_val(Vmod);
+    if (out && out->vmod) out->vmod[i] = p_val(Vmod);
     int m = P.obs[i].filter;
     R w = R((double)P.obs[i].w);
     R y = R((double)P.obs[i].V) - Vmod;
@@ -874,6 +875,35 @@ void run_group(const Problem& P, int form, const RunCfg& C, int reopt_in, const 
   }
 }
 
+// ============================================================ op counting
+// A scalar that counts the floating-point operations performed on it.  The
+// canonical evaluator and the elementary functions are templates over the
+// scalar type, so running them on Cnt counts exactly the operations the GPU
+// executes for one evaluation (the algorithmic FLOP count of the roofline).
+namespace cnt {
+struct Counts {
+  double add, mul, fma, div, sqrt, rnd;
+};
+thread_local Counts K;
+struct R {
+  double v;
+  R() : v(0) {}
+  R(double x) : v(x) {}
+};
+inline R operator+(R a, R b) { K.add++; return R(a.v + b.v); }
+inline R operator-(R a, R b) { K.add++; return R(a.v - b.v); }
+inline R operator*(R a, R b) { K.mul++; return R(a.v * b.v); }
+inline R operator/(R a, R b) { K.div++; return R(a.v / b.v); }
+inline R operator-(R a) { return R(-a.v); }  // sign flip: an operand modifier, not an operation
+inline R p_fma(R a, R b, R c) { K.fma++; return R(__builtin_fma(a.v, b.v, c.v)); }
+inline R p_sqrt(R a) { K.sqrt++; return R(__builtin_sqrt(a.v)); }
+inline R p_rint(R a) { K.rnd++; return R(__builtin_rint(a.v)); }
+inline R p_floor(R a) { K.rnd++; return R(__builtin_floor(a.v)); }
+inline R p_fabs(R a) { return R(__builtin_fabs(a.v)); }
+inline R p_copysign(R a, R b) { return R(__builtin_copysign(a.v, b.v)); }
+inline double p_val(R a) { return a.v; }
+}  // namespace cnt
+
 }  // namespace orc
 
 // ===================================================================== C API
@@ -1030,6 +1060,20 @@ int orc_math(int fn, const double* x, const double* y, double* out, int n) {
       default: return AST_E_ARG;
     }
   }
+  return 0;
+}
+
+// Operation counts of one canonical evaluation of `params` on the problem's
+// curve: out[0..5] = add/sub, mul, fma, div, sqrt, rint/floor.
+int orc_count_ops(const orc_problem* pb, const double* params, double* out) {
+  orc::Problem P;
+  int rc = load_problem(pb, &P);
+  if (rc) return rc;
+  orc::cnt::K = orc::cnt::Counts{0, 0, 0, 0, 0, 0};
+  orc::cnt::R sy2[AST_MAX_FILTERS], sy[AST_MAX_FILTERS], sw[AST_MAX_FILTERS];
+  orc::model_canon<orc::cnt::R>(P, params, sy2, sy, sw, nullptr);
+  const orc::cnt::Counts& k = orc::cnt::K;
+  out[0] = k.add; out[1] = k.mul; out[2] = k.fma; out[3] = k.div; out[4] = k.sqrt; out[5] = k.rnd;
   return 0;
 }
 
