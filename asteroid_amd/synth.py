@@ -154,16 +154,8 @@ def truth_state(variant, params, ids, t, rtol=1e-12, atol=1e-12):
 
 
 def _param_index(ids, variant):
-    # Positions in the parameter vector follow the Property table order.
-    order = ["theta_M", "phi_M", "phi_0"]
-    if variant.torque:
-        order += ["Ti", "Ts", "Tl"]
-    order += ["c_tumb", "b_tumb", "Es", "L", "psi_0"]
-    if variant.bc:
-        order += ["c", "b"]
-    if variant.bw_ball:
-        order += ["theta_R", "phi_R", "kappa"]
-    return {n: i for i, n in enumerate(order)}
+    from .variants import param_positions
+    return param_positions(variant)
 
 
 def truth_vmod(variant, params, t, E, S):

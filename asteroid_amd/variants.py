@@ -23,6 +23,20 @@ def make_variant(torque=True, bc=False, bw_ball=False, p_psi=True, interp=True,
                        float(time_step))
 
 
+def param_positions(variant):
+    """Position of each named parameter in the parameter vector (the Property
+    order, asteroid.c:86-125); type ids (abi.type_ids) are a different order."""
+    order = ["theta_M", "phi_M", "phi_0"]
+    if variant.torque:
+        order += ["Ti", "Ts", "Tl"]
+    order += ["c_tumb", "b_tumb", "Es", "L", "psi_0"]
+    if variant.bc:
+        order += ["c", "b"]
+    if variant.bw_ball:
+        order += ["theta_R", "phi_R", "kappa"]
+    return {n: i for i, n in enumerate(order)}
+
+
 # Named variants of the benchmark workloads.
 def ls_torque(precision=abi.PREC_FP64, time_step=1e-2, interp=True):
     """Self-consistent LS ellipsoid with torque (-DP_PSI -DTORQUE), 11 parameters."""

@@ -74,3 +74,16 @@ def test_fit_trajectories_identical(name, prec, nstages, reopt):
     assert np.array_equal(f, want["f"])
     assert np.array_equal(p, want["params"])
     assert np.array_equal(dv, want["dV"])
+
+
+@pytest.mark.parametrize("name", ["ls_torque_fp64", "bc_fp64_3f", "bw_torque_f32", "ls_torque_noninterp_f32"])
+def test_gpu_matches_golden_vectors(name):
+    from tests.test_golden import G, _problem, fx
+    case = G[name]
+    pb, space = _problem(case)
+    params = fx(case["params"], (-1, space.n_params))
+    with api.Context(space.variant) as ctx:
+        ctx.set_model_space(space)
+        ctx.upload_curve(pb.curve)
+        c, dv = ctx.chi2_batch(params)
+        assert np.array_equal(c, fx(case["chi2_canon"])) and np.array_equal(dv.ravel(), fx(case["dV_canon"]))

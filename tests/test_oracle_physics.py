@@ -1,0 +1,240 @@
+"""Pins of the CPU oracle against what the mathematics fixes.
+
+Each test is chosen so that a plausible slip in the restatement of chi2one,
+ODE_func, x2params or the chi^2 reduction (a dropped term, a wrong sign or
+index, a transposed operand) fails it:
+
+* LS ellipsoid with b = c = 1 is the Lommel-Seeliger sphere, whose phase
+  law is closed form;
+* the LS brightness is symmetric under exchange of Sun and Earth
+  (reciprocity of the LS reflection law);
+* torque-free rotation conserves |L| and the energy Es, and the angular
+  velocity obeys I_i W_i = L sin(theta) sin(psi), I_s W_s = L sin(theta) cos(psi),
+  W_l = L cos(theta) at every observation;
+* the 3-ODE torque-free form and the 6-ODE form with zero torque agree;
+* RK4 converges with order 4 under step refinement;
+* an independent model (scipy DOP853 integration, z-x-z rotation matrices)
+  reproduces the oracle's magnitudes to the RK4 truncation error;
+* the P_psi mapping of x2params produces an L whose integrated psi motion
+  has exactly the requested period;
+* delta_V and chi^2 equal a brute-force weighted least-squares minimum;
+* the canonical (GPU) form equals the literal restatement to rounding.
+"""
+import numpy as np
+import pytest
+from scipy.integrate import solve_ivp
+from scipy.optimize import minimize_scalar
+
+import oracle
+from asteroid_amd import abi, synth, variants
+
+
+def _problem(v, n=120, span=2.0, nf=1, seed=5, truth=None, reopt=0, ppsi=(2.0, 4800.0)):
+    curve, truth = synth.make_curve(seed, v, n, span, n_filters=nf, truth=truth)
+    space = variants.default_model_space(v, ppsi_hours=ppsi, reopt=reopt)
+    return oracle.Problem(space, curve), curve, truth
+
+
+def _phase_angle(curve):
+    return np.arccos(np.clip(np.sum(curve.E * curve.S, axis=1), -1, 1))
+
+
+@pytest.mark.parametrize("form", [oracle.LITERAL, oracle.CANON])
+def test_ls_sphere_phase_law(form):
+    v = variants.bc_free()
+    pb, curve, truth = _problem(v, n=150, span=2.0)
+    ids = variants.param_positions(v)
+    p = truth.copy()
+    p[ids["c"]] = 1.0
+    p[ids["b"]] = 1.0
+    _, _, vm = pb.chi2(p, form=form, want_vmod=True)
+    a = _phase_angle(curve)
+    closed = -2.5 * np.log10(1 - np.sin(a / 2) * np.tan(a / 2) * np.log(1 / np.tan(a / 4)))
+    assert np.max(np.abs(vm[0] - closed)) < 1e-12
+
+
+@pytest.mark.parametrize("form", [oracle.LITERAL, oracle.CANON])
+def test_ls_sun_earth_reciprocity(form):
+    v = variants.ls_torque(interp=False)
+    pb, curve, truth = _problem(v)
+    _, _, vm1 = pb.chi2(truth, form=form, want_vmod=True)
+    swapped = synth.Curve(curve.mjd, curve.V, curve.w, curve.filter, curve.n_filters, curve.S, curve.E)
+    pb2 = oracle.Problem(pb.space, swapped)
+    _, _, vm2 = pb2.chi2(truth, form=form, want_vmod=True)
+    assert np.max(np.abs(vm1 - vm2)) < 1e-11
+
+
+def _torque_free_truth(seed):
+    v = variants.ls_torque()
+    rng = np.random.default_rng(seed)
+    truth = synth.random_truth(rng, v, ppsi_hours=(8.0, 12.0), torque_scale=0.0)
+    ids = variants.param_positions(v)
+    for k in ("Ti", "Ts", "Tl"):
+        truth[ids[k]] = 0.0
+    return v, truth, ids
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_torque_free_invariants(seed):
+    v, truth, ids = _torque_free_truth(seed)
+    pb, curve, _ = _problem(v, n=200, span=3.0, truth=truth)
+    _, _, tr = pb.chi2(truth, form=oracle.CANON, want_traj=True)
+    phi, th, psi, wi, ws, wl = tr[0].T
+    bt, ct = truth[ids["b_tumb"]], truth[ids["c_tumb"]]
+    Is = (1 + bt * bt) / (bt * bt + ct * ct)
+    Ii = (1 + ct * ct) / (bt * bt + ct * ct)
+    L, Es = truth[ids["L"]], truth[ids["Es"]]
+    L2 = (Ii * wi) ** 2 + (Is * ws) ** 2 + wl ** 2
+    assert np.max(np.abs(np.sqrt(L2) / L - 1)) < 2e-6
+    Es_t = 1 + np.sin(th) ** 2 * (np.sin(psi) ** 2 * (1 / Ii - 1 / Is) + 1 / Is - 1)
+    assert np.max(np.abs(Es_t - Es)) < 2e-6
+    assert np.max(np.abs(Ii * wi - L * np.sin(th) * np.sin(psi))) / L < 2e-6
+    assert np.max(np.abs(Is * ws - L * np.sin(th) * np.cos(psi))) / L < 2e-6
+    assert np.max(np.abs(wl - L * np.cos(th))) / L < 2e-6
+
+
+def test_three_ode_form_matches_six_ode_form():
+    v6, truth6, ids6 = _torque_free_truth(7)
+    v3 = variants.make_variant(False, False, False, True, True)
+    ids3 = variants.param_positions(v3)
+    truth3 = np.zeros(len(ids3))
+    for k in ("theta_M", "phi_M", "phi_0", "c_tumb", "b_tumb", "Es", "L", "psi_0"):
+        truth3[ids3[k]] = truth6[ids6[k]]
+    pb6, curve, _ = _problem(v6, n=150, span=2.0, truth=truth6)
+    pb3 = oracle.Problem(variants.default_model_space(v3), curve)
+    _, _, t6 = pb6.chi2(truth6, want_traj=True)
+    _, _, t3 = pb3.chi2(truth3, want_traj=True)
+    assert np.max(np.abs(t6[0, :, :3] - t3[0, :, :3])) < 1e-6
+
+
+def test_rk4_is_fourth_order():
+    v0, truth, ids = _torque_free_truth(11)
+    truth[ids["Ti"]], truth[ids["Ts"]], truth[ids["Tl"]] = 0.3, -0.2, 0.1
+    mjd = np.arange(0, 13) * 0.25
+    errs, hs = [], []
+    trajs = {}
+    for k in (4, 8, 16, 256):
+        ts = 0.25 / k
+        v = variants.ls_torque(time_step=ts)
+        curve, _ = synth.make_curve(3, v, len(mjd), 3.0, truth=truth)
+        curve = synth.Curve(mjd, curve.V[:len(mjd)], curve.w[:len(mjd)], curve.filter[:len(mjd)], 1,
+                            curve.E[:len(mjd)], curve.S[:len(mjd)], curve.eph)
+        pb = oracle.Problem(variants.default_model_space(v), curve)
+        trajs[k] = pb.chi2(truth, want_traj=True)[2][0]
+        hs.append(0.25 / (k + 1))
+    ref = trajs[256]
+    for k in (4, 8, 16):
+        errs.append(np.max(np.abs(trajs[k][:, :3] - ref[:, :3])))
+    p1 = np.log(errs[0] / errs[1]) / np.log(hs[0] / hs[1])
+    p2 = np.log(errs[1] / errs[2]) / np.log(hs[1] / hs[2])
+    assert 3.6 < p1 < 4.4 and 3.6 < p2 < 4.4, (p1, p2, errs)
+
+
+@pytest.mark.parametrize("name,v", [("ls_torque", variants.ls_torque(time_step=1e-3)),
+                                    ("bc", variants.bc_free(time_step=1e-3)),
+                                    ("bw_torque", variants.bw_ball_torque(time_step=1e-3))])
+@pytest.mark.parametrize("seed", [5, 6])
+def test_independent_numpy_model(name, v, seed):
+    # A fine TIME_STEP makes the RK4 truncation error negligible, so the
+    # comparison tests the geometry and brightness formulas themselves.
+    pb, curve, truth = _problem(v, n=200, span=3.0, nf=2, seed=seed)
+    _, _, vm = pb.chi2(truth, form=oracle.LITERAL, want_vmod=True)
+    want = synth.truth_vmod(v, truth, curve.mjd, curve.E, curve.S)
+    assert np.max(np.abs(vm[0] - want)) < 1e-7
+
+
+def _psi_period(Ii, Is, L, Es, psi0, P_guess):
+    theta0 = np.arcsin(np.sqrt((Es - 1) / (np.sin(psi0) ** 2 * (1 / Ii - 1 / Is) + 1 / Is - 1)))
+    y0 = [L / Ii * np.sin(theta0) * np.sin(psi0), L / Is * np.sin(theta0) * np.cos(psi0), L * np.cos(theta0),
+          0.0, theta0, psi0]
+    rhs = synth._euler_rhs(Ii, Is, [0.0, 0.0, 0.0])
+    sol = solve_ivp(rhs, (0, 2.5 * P_guess), y0, method="DOP853", rtol=1e-12, atol=1e-12, dense_output=True)
+    t = np.linspace(0, 2.5 * P_guess, 200001)
+    psi = sol.sol(t)[5]
+    lam = Es > 1 / Ii
+    if lam:  # psi circulates: time to advance by 2 pi
+        d = np.abs(psi - psi0) - 2 * np.pi
+        k = np.argmax(d >= 0)
+    else:    # psi librates: second return to psi0 moving the same way
+        s = np.sign(sol.sol(1e-6)[5] - psi0)
+        f = (psi - psi0) * s
+        cross = np.where((f[:-1] < 0) & (f[1:] >= 0))[0]
+        k = cross[0] + 1
+        d = f
+    t0, t1 = t[k - 1], t[k]
+    return t0 + (t1 - t0) * (-d[k - 1]) / (d[k] - d[k - 1])
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4, 5, 6])
+def test_ppsi_mapping_gives_requested_period(seed):
+    v = variants.ls_torque()
+    pb, curve, _ = _problem(v, n=40, span=1.0, ppsi=(4.0, 20.0))
+    ids = variants.param_positions(v)
+    rng = np.random.default_rng(seed)
+    for _ in range(100):
+        x = rng.uniform(0.05, 0.95, pb.space.n_params)
+        p, rc = pb.x2params(x, reopt=0, form=oracle.LITERAL)
+        if rc[0] == 0:
+            break
+    p = p[0]
+    ppsi1, ppsi2 = pb.space.ppsi1, pb.space.ppsi2
+    inv_p = np.float32(np.float32(x[ids["L"]]) * np.float32(ppsi2 - ppsi1) + np.float32(ppsi1)) \
+        if v.precision != abi.PREC_FP64 else x[ids["L"]] * np.float32(np.float32(ppsi2) - np.float32(ppsi1)) + np.float32(ppsi1)
+    P_days = 1.0 / float(inv_p)
+    bt, ct = p[ids["b_tumb"]], p[ids["c_tumb"]]
+    Is = (1 + bt * bt) / (bt * bt + ct * ct)
+    Ii = (1 + ct * ct) / (bt * bt + ct * ct)
+    measured = _psi_period(Ii, Is, p[ids["L"]], p[ids["Es"]], p[ids["psi_0"]], P_days)
+    assert abs(measured / P_days - 1) < 1e-8, (measured, P_days)
+
+
+def test_agm_elliptic_k_matches_scipy():
+    # The AGM of x2params (cuda.c:1270-1279) after 5 iterations equals K(k^2) to 1e-10.
+    from scipy.special import ellipk
+    for k2 in np.linspace(0.0, 0.9999, 200):
+        a, g = 1.0, np.sqrt(1 - k2)
+        for _ in range(5):
+            a, g = 0.5 * (a + g), np.sqrt(a * g)
+        assert abs(np.pi / (a + g) - ellipk(k2)) / ellipk(k2) < 1e-10
+
+
+@pytest.mark.parametrize("form", [oracle.LITERAL, oracle.CANON])
+def test_delta_v_and_chi2_are_weighted_least_squares(form):
+    v = variants.bc_free()
+    pb, curve, truth = _problem(v, n=120, span=2.0, nf=3)
+    chi2, dv, vm = pb.chi2(truth, form=form, want_vmod=True)
+    y = curve.V.astype(float) - vm[0]
+    w = curve.w.astype(float)
+    total = 0.0
+    for m in range(curve.n_filters):
+        s = curve.filter == m
+        r = minimize_scalar(lambda d: np.sum(w[s] * (y[s] - d) ** 2), bracket=(-5, 5), tol=1e-14)
+        assert abs(r.x - dv[0, m]) < 1e-7
+        total += np.sum(w[s] * (y[s] - dv[0, m]) ** 2)
+    dof = curve.n_data - pb.space.n_params - curve.n_filters
+    assert abs(total / dof - chi2[0]) / chi2[0] < 1e-10
+
+
+@pytest.mark.parametrize("name,v,nf", [("ls_torque", variants.ls_torque(), 1), ("bc", variants.bc_free(), 3),
+                                       ("bw_torque", variants.bw_ball_torque(), 2),
+                                       ("ls_plain", variants.make_variant(False, False, False, True, True), 1),
+                                       ("ls_torque_f32", variants.ls_torque(abi.PREC_F32SIMPLEX), 1)])
+def test_canonical_form_equals_literal(name, v, nf):
+    pb, curve, truth = _problem(v, n=150, span=2.0, nf=nf)
+    rng = np.random.default_rng(4)
+    x = rng.uniform(0.02, 0.98, (200, pb.space.n_params))
+    p, rc = pb.x2params(x, reopt=0, form=oracle.LITERAL)
+    p = np.vstack([truth, p[rc == 0]])
+    c_l, d_l, vm_l = pb.chi2(p, form=oracle.LITERAL, want_vmod=True)
+    c_c, d_c, vm_c = pb.chi2(p, form=oracle.CANON, want_vmod=True)
+    # Models whose rotation is resolved by the RK4 step (L * TIME_STEP < 2 rad
+    # per step).  Beyond that the integration is unresolved and amplifies any
+    # rounding difference, as it would between two builds of the reference.
+    L = p[:, variants.param_positions(v)["L"]]
+    resolved = L * v.time_step < 2.0
+    ok = np.isfinite(c_l) & np.isfinite(vm_l).all(axis=1) & resolved
+    assert ok.sum() > 20
+    assert np.array_equal(np.isfinite(c_l), np.isfinite(c_c))
+    assert np.max(np.abs(vm_l[ok] - vm_c[ok])) < 1e-9
+    rtol = 1e-6 if v.precision != abi.PREC_FP64 else 1e-10
+    assert np.max(np.abs(c_l[ok] - c_c[ok]) / np.abs(c_l[ok])) < rtol
