@@ -68,7 +68,7 @@ def build_driver():
     if not os.path.exists(src):
         return None
     exe = os.path.join(LIB, "asteroid")
-    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), src, "-o", exe,
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"), src, "-o", exe,
           "-L", LIB, "-lasteroid_hip", "-Wl,-rpath,$ORIGIN"])
     return exe
 
