@@ -212,12 +212,14 @@ __device__ inline double dot3(double ax, double ay, double az, double bx, double
   return __builtin_fma(az, bz, __builtin_fma(ay, by, ax * bx));
 }
 
-template <bool TQ>
-__device__ inline void ode(const double* y, double* f, const double* mu, double mu02) {
+// ODE right-hand side.  With FIRST the sin/cos of theta (and psi) come from
+// sc[] (those of the state, shared with the body axes), else they are computed.
+template <bool TQ, bool FIRST>
+__device__ inline void ode(const double* y, double* f, const double* mu, double mu02, const double* sc) {
   if (TQ) {  // cuda.c:46-58
     double s5, c5, s4, c4;
-    astm::sincos(y[5], &s5, &c5);
-    astm::sincos(y[4], &s4, &c4);
+    if (FIRST) { s4 = sc[0]; c4 = sc[1]; s5 = sc[2]; c5 = sc[3]; }
+    else { astm::sincos(y[5], &s5, &c5); astm::sincos(y[4], &s4, &c4); }
     f[0] = __builtin_fma(mu[0] * y[1], y[2], mu[3]);
     f[1] = __builtin_fma(mu[1] * y[2], y[0], mu[4]);
     f[2] = __builtin_fma(mu[2] * y[0], y[1], mu[5]);
@@ -227,7 +229,8 @@ __device__ inline void ode(const double* y, double* f, const double* mu, double 
   } else {   // cuda.c:69-75
     double s2, c2, s1, c1;
     astm::sincos(2.0 * y[2], &s2, &c2);
-    astm::sincos(y[1], &s1, &c1);
+    if (FIRST) { s1 = sc[0]; c1 = sc[1]; }
+    else astm::sincos(y[1], &s1, &c1);
     f[0] = mu[0] * __builtin_fma(-mu[2], c2, mu[1]);
     f[1] = (mu02 * s1) * s2;
     f[2] = c1 * (mu[0] - f[0]);
@@ -271,6 +274,12 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
   double mu[6];
   double mu02 = 0.0;
   double y[6];
+  // sin/cos of the current theta and psi, shared by the next RK4 step's
+  // first stage and the body axes at the observation.
+  double sc[4];
+  astm::sincos(theta, &sc[0], &sc[1]);
+  sc[2] = sP0;
+  sc[3] = cP0;
   if (TQ) {
     mu[0] = (Is - 1.0) * Ii_inv;
     mu[1] = (1.0 - Ii) * Is_inv;
@@ -278,11 +287,9 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
     mu[3] = params[P::Ti];
     mu[4] = params[P::Ts];
     mu[5] = params[P::Tl];
-    double sth, cth;
-    astm::sincos(theta, &sth, &cth);
-    y[0] = ((L * Ii_inv) * sth) * sP0;
-    y[1] = ((L * Is_inv) * sth) * cP0;
-    y[2] = L * cth;
+    y[0] = ((L * Ii_inv) * sc[0]) * sP0;
+    y[1] = ((L * Is_inv) * sc[0]) * cP0;
+    y[2] = L * sc[1];
     y[3] = phi;
     y[4] = theta;
     y[5] = psi;
@@ -313,25 +320,32 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
     const double h = pr.h, hh = pr.hh, h6 = pr.h6;
     for (int l = 0; l < nsteps; ++l) {  // RK4, cuda.c:348-372
       double f[6], K[6], s[6];
-      ode<TQ>(y, K, mu, mu02);
+      ode<TQ, true>(y, K, mu, mu02, sc);
 #pragma unroll
       for (int j = 0; j < NODE; ++j) { s[j] = K[j]; f[j] = __builtin_fma(hh, K[j], y[j]); }
-      ode<TQ>(f, K, mu, mu02);
+      ode<TQ, false>(f, K, mu, mu02, sc);
 #pragma unroll
       for (int j = 0; j < NODE; ++j) { s[j] = __builtin_fma(2.0, K[j], s[j]); f[j] = __builtin_fma(hh, K[j], y[j]); }
-      ode<TQ>(f, K, mu, mu02);
+      ode<TQ, false>(f, K, mu, mu02, sc);
 #pragma unroll
       for (int j = 0; j < NODE; ++j) { s[j] = __builtin_fma(2.0, K[j], s[j]); f[j] = __builtin_fma(h, K[j], y[j]); }
-      ode<TQ>(f, K, mu, mu02);
+      ode<TQ, false>(f, K, mu, mu02, sc);
 #pragma unroll
       for (int j = 0; j < NODE; ++j) y[j] = __builtin_fma(h6, s[j] + K[j], y[j]);
+      if (TQ) {
+        astm::sincos(y[4], &sc[0], &sc[1]);
+        astm::sincos(y[5], &sc[2], &sc[3]);
+      } else {
+        astm::sincos(y[1], &sc[0], &sc[1]);
+      }
     }
     if (TQ) { phi = y[3]; theta = y[4]; psi = y[5]; }
     else { phi = y[0]; theta = y[1]; psi = y[2]; }
-    double sf, cf, st, ctt, sp, cp;
+    double sf, cf, sp, cp;
     astm::sincos(phi, &sf, &cf);
-    astm::sincos(theta, &st, &ctt);
-    astm::sincos(psi, &sp, &cp);
+    const double st = sc[0], ctt = sc[1];
+    if (TQ) { sp = sc[2]; cp = sc[3]; }
+    else astm::sincos(psi, &sp, &cp);
     const double N_x = __builtin_fma(YM_x, sf, XM_x * cf);
     const double N_y = YM_y * sf;
     const double N_z = __builtin_fma(YM_z, sf, XM_z * cf);

@@ -51,6 +51,10 @@ AST_HD double p_val(double a) { return a; }
 
 AST_HD uint64_t bits_of(double x) { return __builtin_bit_cast(uint64_t, x); }
 AST_HD double of_bits(uint64_t b) { return __builtin_bit_cast(double, b); }
+// -x if flag is nonzero, else x: one xor of the sign bit (exact, not counted).
+AST_HD double p_flipsign(double x, int flag) {
+  return of_bits(bits_of(x) ^ ((uint64_t)(flag != 0) << 63));
+}
 
 // 2^k as a double, for -1022 <= k <= 1023.
 AST_HD double pow2i(int k) { return of_bits((uint64_t)(k + 1023) << 52); }
@@ -130,11 +134,11 @@ template <class R> AST_HD void sincos(R x, R* s, R* c) {
   R w = R(1.0) - hz;
   R cs = w + p_fma(z * z, k_cos(z), (R(1.0) - w) - hz);
   double kv = p_val(k);
-  int q = (p_fabs(kv) < 1e15) ? (int)((long long)kv & 3) : 0;
+  int q = (p_fabs(kv) < 1073741824.0) ? ((int)kv & 3) : 0;  // |k| < 2^30: exact 32-bit conversion
   R s0 = (q & 1) ? cs : sn;
   R c0 = (q & 1) ? sn : cs;
-  *s = (q & 2) ? -s0 : s0;
-  *c = ((q + 1) & 2) ? -c0 : c0;
+  *s = p_flipsign(s0, q & 2);
+  *c = p_flipsign(c0, (q + 1) & 2);
 }
 
 template <class R> AST_HD R sin(R x) { R s, c; sincos(x, &s, &c); return s; }

@@ -19,13 +19,13 @@ _W = np.array([1, 1, 2, 1, 1, 1], float)
 
 # key: (torque, brightness) with brightness "ls" (LS ellipsoid, BC or not) or "bw".
 TABLE = {
-    (True, "ls"): dict(per_model=(28, 41, 77, 11, 3, 4), per_point=(30, 78, 103, 5, 4, 3),
+    (True, "ls"): dict(per_model=(28, 41, 77, 11, 3, 4), per_point=(22, 68, 73, 5, 4, 1),
                        per_rk4=(38, 60, 180, 4, 0, 8)),
-    (False, "ls"): dict(per_model=(23, 30, 62, 11, 3, 3), per_point=(30, 78, 103, 5, 4, 3),
+    (False, "ls"): dict(per_model=(27, 35, 77, 11, 3, 4), per_point=(26, 73, 88, 5, 4, 2),
                         per_rk4=(39, 60, 142, 0, 0, 8)),
-    (True, "bw"): dict(per_model=(36, 50, 107, 9, 3, 6), per_point=(22, 54, 84, 1, 0, 3),
+    (True, "bw"): dict(per_model=(36, 50, 107, 9, 3, 6), per_point=(14, 44, 54, 1, 0, 1),
                        per_rk4=(38, 60, 180, 4, 0, 8)),
-    (False, "bw"): dict(per_model=(31, 39, 92, 9, 3, 5), per_point=(22, 54, 84, 1, 0, 3),
+    (False, "bw"): dict(per_model=(35, 44, 107, 9, 3, 6), per_point=(18, 49, 69, 1, 0, 2),
                         per_rk4=(39, 60, 142, 0, 0, 8)),
 }
 

@@ -301,13 +301,16 @@ inline void model_literal(const Problem& P, const double* params, double* sy2, d
 
 // ========================================================== canonical chi2one
 // Templated over the scalar R so an operation-counting type can run it.
+// ODE right-hand side.  sc = {sin y_theta, cos y_theta, sin y_psi, cos y_psi}
+// when the caller already has them (the first RK4 stage reuses those of the
+// state), else nullptr.
 template <class R>
-inline void ode_canon(bool torque, const R* y, R* f, const R* mu, R mu02) {
+inline void ode_canon(bool torque, const R* y, R* f, const R* mu, R mu02, const R* sc = nullptr) {
   using astm::p_fma;
   if (torque) {
     R s5, c5, s4, c4;
-    astm::sincos(y[5], &s5, &c5);
-    astm::sincos(y[4], &s4, &c4);
+    if (sc) { s4 = sc[0]; c4 = sc[1]; s5 = sc[2]; c5 = sc[3]; }
+    else { astm::sincos(y[5], &s5, &c5); astm::sincos(y[4], &s4, &c4); }
     f[0] = p_fma(mu[0] * y[1], y[2], mu[3]);
     f[1] = p_fma(mu[1] * y[2], y[0], mu[4]);
     f[2] = p_fma(mu[2] * y[0], y[1], mu[5]);
@@ -317,7 +320,8 @@ inline void ode_canon(bool torque, const R* y, R* f, const R* mu, R mu02) {
   } else {
     R s2, c2, s1, c1;
     astm::sincos(R(2.0) * y[2], &s2, &c2);
-    astm::sincos(y[1], &s1, &c1);
+    if (sc) { s1 = sc[0]; c1 = sc[1]; }
+    else astm::sincos(y[1], &s1, &c1);
     f[0] = mu[0] * p_fma(-mu[2], c2, mu[1]);
     f[1] = (mu02 * s1) * s2;
     f[2] = c1 * (mu[0] - f[0]);
@@ -362,6 +366,12 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
   R mu[6];
   R mu02 = R(0.0);
   R Om_i = R(0.0), Om_s = R(0.0), Om_l = R(0.0);
+  // sin/cos of the current theta and psi, shared by the first RK4 stage of
+  // the next step and by the body axes at the observation.
+  R sc[4];
+  astm::sincos(theta, &sc[0], &sc[1]);
+  sc[2] = sP0;
+  sc[3] = cP0;
   if (torque) {
     mu[0] = (Is - R(1.0)) * Ii_inv;
     mu[1] = (R(1.0) - Ii) * Is_inv;
@@ -369,11 +379,9 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
     mu[3] = R(params[P.types[T.Ti]]);
     mu[4] = R(params[P.types[T.Ts]]);
     mu[5] = R(params[P.types[T.Tl]]);
-    R sth, cth;
-    astm::sincos(theta, &sth, &cth);
-    Om_i = ((L * Ii_inv) * sth) * sP0;
-    Om_s = ((L * Is_inv) * sth) * cP0;
-    Om_l = L * cth;
+    Om_i = ((L * Ii_inv) * sc[0]) * sP0;
+    Om_s = ((L * Is_inv) * sc[0]) * cP0;
+    Om_l = L * sc[1];
   } else {
     mu[0] = L;
     mu[1] = R(0.5) * (Ii_inv + Is_inv);
@@ -406,7 +414,7 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
       else { y[0] = phi; y[1] = theta; y[2] = psi; }
       for (int l = 0; l < N_steps; ++l) {
         R f[6], K[6], s[6];
-        ode_canon(torque, y, K, mu, mu02);
+        ode_canon(torque, y, K, mu, mu02, sc);
         for (int j = 0; j < n_ode; ++j) { s[j] = K[j]; f[j] = p_fma(hh, K[j], y[j]); }
         ode_canon(torque, f, K, mu, mu02);
         for (int j = 0; j < n_ode; ++j) { s[j] = p_fma(R(2.0), K[j], s[j]); f[j] = p_fma(hh, K[j], y[j]); }
@@ -414,6 +422,12 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
         for (int j = 0; j < n_ode; ++j) { s[j] = p_fma(R(2.0), K[j], s[j]); f[j] = p_fma(h, K[j], y[j]); }
         ode_canon(torque, f, K, mu, mu02);
         for (int j = 0; j < n_ode; ++j) y[j] = p_fma(h6, s[j] + K[j], y[j]);
+        if (torque) {
+          astm::sincos(y[4], &sc[0], &sc[1]);
+          astm::sincos(y[5], &sc[2], &sc[3]);
+        } else {
+          astm::sincos(y[1], &sc[0], &sc[1]);
+        }
       }
       if (torque) { Om_i = y[0]; Om_s = y[1]; Om_l = y[2]; phi = y[3]; theta = y[4]; psi = y[5]; }
       else { phi = y[0]; theta = y[1]; psi = y[2]; }
@@ -423,10 +437,11 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
       t6[0] = p_val(phi); t6[1] = p_val(theta); t6[2] = p_val(psi);
       t6[3] = p_val(Om_i); t6[4] = p_val(Om_s); t6[5] = p_val(Om_l);
     }
-    R sf, cf, st, ctt, sp, cp;
+    R sf, cf, sp, cp;
     astm::sincos(phi, &sf, &cf);
-    astm::sincos(theta, &st, &ctt);
-    astm::sincos(psi, &sp, &cp);
+    const R st = sc[0], ctt = sc[1];
+    if (torque) { sp = sc[2]; cp = sc[3]; }
+    else astm::sincos(psi, &sp, &cp);
     R N_x = p_fma(YM_x, sf, XM_x * cf);
     R N_y = YM_y * sf;
     R N_z = p_fma(YM_z, sf, XM_z * cf);
@@ -901,6 +916,7 @@ inline R p_rint(R a) { K.rnd++; return R(__builtin_rint(a.v)); }
 inline R p_floor(R a) { K.rnd++; return R(__builtin_floor(a.v)); }
 inline R p_fabs(R a) { return R(__builtin_fabs(a.v)); }
 inline R p_copysign(R a, R b) { return R(__builtin_copysign(a.v, b.v)); }
+inline R p_flipsign(R a, int flag) { return R(flag ? -a.v : a.v); }
 inline double p_val(R a) { return a.v; }
 }  // namespace cnt
 
