@@ -20,8 +20,10 @@ PREC_FP64, PREC_F32SIMPLEX, PREC_MIXED = 0, 1, 2
 
 ERRORS = {
     0: "AST_OK", -1: "AST_E_ARG", -2: "AST_E_STATE", -3: "AST_E_HIP", -4: "AST_E_NOMEM",
-    -5: "AST_E_DATA", -6: "AST_E_UNSUPPORTED", -7: "AST_E_NODEVICE",
+    -5: "AST_E_DATA", -6: "AST_E_UNSUPPORTED", -7: "AST_E_NODEVICE", -8: "AST_E_CANCELLED",
+    -9: "AST_E_BUSY",
 }
+E_CANCELLED = -8
 
 
 class Variant(C.Structure):
@@ -52,6 +54,34 @@ class RunCfg(C.Structure):
     _fields_ = [("n_groups", C.c_int), ("group_size", C.c_int), ("nstages", C.c_int),
                 ("n_steps", C.c_int), ("eval_budget", C.c_int64), ("size_min", C.c_double),
                 ("ticks_per_launch", C.c_int)]
+
+
+class ServeCfg(C.Structure):
+    _fields_ = [("max_resident_fits", C.c_int64), ("group_size", C.c_int), ("hbm_budget_bytes", C.c_int64),
+                ("work_points_per_launch", C.c_double)]
+
+
+class Job(C.Structure):
+    _fields_ = [("variant", Variant), ("n_params", C.c_int), ("property", C.POINTER(C.c_int)),
+                ("n_types", C.c_int), ("types", C.POINTER(C.c_int)), ("limits", C.POINTER(C.c_double)),
+                ("x2", X2), ("obs", C.POINTER(Obs)), ("n_data", C.c_int), ("n_filters", C.c_int),
+                ("eph", C.POINTER(Ephem3)), ("n_fits", C.c_int), ("nstages", C.c_int), ("n_steps", C.c_int),
+                ("eval_budget", C.c_int64), ("size_min", C.c_double), ("seed", C.c_uint64),
+                ("params0", C.POINTER(C.c_double)), ("tag", C.c_uint64)]
+
+
+class Result(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("tag", C.c_uint64), ("status", C.c_int), ("n_params", C.c_int),
+                ("n_filters", C.c_int), ("n_groups", C.c_int), ("best_group", C.c_int), ("best_f", C.c_double),
+                ("best_params", C.c_double * MAX_PARAMS), ("best_dV", C.c_double * MAX_FILTERS),
+                ("n_evals", C.c_int64), ("model_points", C.c_int64), ("submit_to_done_ms", C.c_double),
+                ("queue_ms", C.c_double), ("run_ms", C.c_double)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("pending", C.c_int), ("resident", C.c_int), ("free_fits", C.c_int64),
+                ("resident_bytes", C.c_int64), ("launches", C.c_int64), ("evals", C.c_int64),
+                ("model_points", C.c_int64), ("kernel_ms", C.c_double)]
 
 
 assert C.sizeof(Obs) == 72
@@ -116,6 +146,12 @@ def load():
         "ast_last_timing": (I, [P, pD, pI]),
         "ast_last_counts": (I, [P, pI64, pI64]),
         "ast_default_model_space": (I, [C.POINTER(Variant), pD, pI, pI, pI, pI, pD, C.POINTER(X2)]),
+        "ast_serve_start": (I, [P, C.POINTER(ServeCfg)]),
+        "ast_submit": (I, [P, C.POINTER(Job), C.POINTER(C.c_uint64)]),
+        "ast_poll": (I, [P, C.POINTER(Result), I, pI, I]),
+        "ast_cancel": (I, [P, C.c_uint64]),
+        "ast_serve_stats": (I, [P, C.POINTER(ServeStats)]),
+        "ast_serve_stop": (I, [P]),
         "ast_strerror": (C.c_char_p, [I]),
         "ast_last_hip_error": (C.c_char_p, []),
     }
@@ -131,5 +167,6 @@ EXPORTED = (
     "ast_ctx_create", "ast_ctx_destroy", "ast_set_model_space", "ast_upload_curve", "ast_rng_seed",
     "ast_reset_best", "ast_set_start", "ast_run", "ast_fetch", "ast_fetch_lanes", "ast_chi2_batch",
     "ast_last_timing", "ast_last_counts", "ast_default_model_space", "ast_strerror",
-    "ast_last_hip_error",
+    "ast_last_hip_error", "ast_serve_start", "ast_submit", "ast_poll", "ast_cancel", "ast_serve_stats",
+    "ast_serve_stop",
 )

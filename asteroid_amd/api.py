@@ -131,6 +131,70 @@ class Context:
         return e.value, p.value
 
 
+    # ------------------------------------------------------------- serving
+    def serve_start(self, max_resident_fits, group_size=256, hbm_budget_bytes=0, work_points_per_launch=0.0):
+        cfg = abi.ServeCfg(int(max_resident_fits), int(group_size), int(hbm_budget_bytes),
+                           float(work_points_per_launch))
+        _chk(self.lib.ast_serve_start(self.h, C.byref(cfg)), "ast_serve_start")
+        self._keep = {}
+
+    def submit(self, space, curve, n_fits, nstages=1, n_steps=5000, eval_budget=0, size_min=0.0, seed=1,
+               params0=None, tag=0):
+        """Submit a fit job (its own variant, model space and curve); returns its id."""
+        prop = np.ascontiguousarray(space.property.ravel(), np.int32)
+        types = np.ascontiguousarray(space.types, np.int32)
+        lim = np.ascontiguousarray(space.limits, np.float64)
+        obs = curve.obs_array()
+        eph = curve.ephem3()
+        p0 = None if params0 is None else np.ascontiguousarray(params0, np.float64)
+        job = abi.Job(space.variant, space.n_params, prop.ctypes.data_as(C.POINTER(C.c_int)), space.n_types,
+                      types.ctypes.data_as(C.POINTER(C.c_int)), _dp(lim), space.x2(),
+                      C.cast(obs, C.POINTER(abi.Obs)), curve.n_data, curve.n_filters, C.pointer(eph),
+                      int(n_fits), int(nstages), int(n_steps), int(eval_budget), float(size_min), int(seed),
+                      _dp(p0) if p0 is not None else None, int(tag))
+        jid = C.c_uint64()
+        _chk(self.lib.ast_submit(self.h, C.byref(job), C.byref(jid)), "ast_submit")
+        return jid.value
+
+    def poll(self, max_results=64, timeout_ms=0):
+        out = (abi.Result * max_results)()
+        n = C.c_int()
+        _chk(self.lib.ast_poll(self.h, out, max_results, C.byref(n), int(timeout_ms)), "ast_poll")
+        res = []
+        for k in range(n.value):
+            r = out[k]
+            res.append(dict(id=r.id, tag=r.tag, status=r.status, best_f=r.best_f, best_group=r.best_group,
+                            best_params=np.array(r.best_params[:r.n_params]),
+                            best_dV=np.array(r.best_dV[:r.n_filters]), n_evals=r.n_evals,
+                            model_points=r.model_points, latency_ms=r.submit_to_done_ms, queue_ms=r.queue_ms,
+                            run_ms=r.run_ms))
+        return res
+
+    def cancel(self, jid):
+        _chk(self.lib.ast_cancel(self.h, int(jid)), "ast_cancel")
+
+    def serve_stats(self):
+        st = abi.ServeStats()
+        _chk(self.lib.ast_serve_stats(self.h, C.byref(st)), "ast_serve_stats")
+        return {f: getattr(st, f) for f, _ in abi.ServeStats._fields_}
+
+    def serve_stop(self):
+        _chk(self.lib.ast_serve_stop(self.h), "ast_serve_stop")
+
+    def wait_all(self, ids, timeout_s=600.0):
+        """Poll until every id in `ids` has a result; returns {id: result}."""
+        import time
+        want = set(ids)
+        got = {}
+        t0 = time.time()
+        while want - set(got):
+            for r in self.poll(256, timeout_ms=100):
+                got[r["id"]] = r
+            if time.time() - t0 > timeout_s:
+                raise TimeoutError(f"{len(want - set(got))} jobs unfinished")
+        return got
+
+
 def default_model_space_c(variant, ppsi_hours=(2.0, 4800.0)):
     """The library's C++ copy of the reference tables (for cross-checking)."""
     lib = abi.load()

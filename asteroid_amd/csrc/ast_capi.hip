@@ -1,16 +1,34 @@
 // ast_capi.hip -- host side of libasteroid_hip.so (the C ABI of include/asteroid.h).
 //
-// Owns the device buffers of one context, builds the per-observation table
-// the kernels read (INTERP ephemeris evaluated once per observation and the
-// RK4 interval constants, in the canonical FMA order), drives the chunked
-// fitting launches until every group has finished, and times every launch
-// with HIP events on the context's stream.
+// Pieces:
+//  * validation of model spaces and light curves, and the per-observation
+//    table the kernels read (INTERP ephemeris evaluated once per observation
+//    and the RK4 interval constants, in the canonical FMA order);
+//  * a slot pool: the device state of every resident simplex, lane-fastest;
+//  * device jobs: one curve + model space + random streams + per-group bests
+//    + one descriptor per group;
+//  * the cycle: one launch of the fitting kernel per kernel class (variant,
+//    simplex precision, filter capacity) over every running group, each class
+//    on its own stream, timed with HIP events;
+//  * ast_run (the reference's chi2_gpu launch) = one job run to completion;
+//  * the serving scheduler (ast_serve_*, ast_submit/poll/cancel): a thread
+//    that admits jobs into free slots while their HBM footprint fits, runs
+//    cycles over everything resident (continuous batching) and retires jobs
+//    as they finish.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "../../include/asteroid.h"
@@ -30,16 +48,29 @@ using namespace ast;
 
 static thread_local char g_hip_err[256] = "";
 
-#define HIPCHK(call)                                                                   \
-  do {                                                                                 \
-    hipError_t e_ = (call);                                                            \
-    if (e_ != hipSuccess) {                                                            \
+static void set_err(const char* msg) { snprintf(g_hip_err, sizeof g_hip_err, "%s", msg); }
+
+#define HIPCHK(call)                                                                                   \
+  do {                                                                                                 \
+    hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess) {                                                                            \
       snprintf(g_hip_err, sizeof g_hip_err, "%s (%s:%d)", hipGetErrorString(e_), __FILE__, __LINE__); \
-      return e_ == hipErrorOutOfMemory ? AST_E_NOMEM : AST_E_HIP;                      \
-    }                                                                                  \
+      return e_ == hipErrorOutOfMemory ? AST_E_NOMEM : AST_E_HIP;                                      \
+    }                                                                                                  \
+  } while (0)
+
+#define RETIF(x)            \
+  do {                      \
+    int r_ = (x);           \
+    if (r_) return r_;      \
   } while (0)
 
 namespace {
+
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
 
 template <class T>
 int dev_alloc(T** p, size_t count) {
@@ -54,60 +85,7 @@ void dev_free(T*& p) {
   p = nullptr;
 }
 
-struct Job {  // device buffers of the simplex state, typed by the simplex float
-  size_t fits_cap = 0, groups_cap = 0;
-  int N = 0;
-  void* X = nullptr; void* F = nullptr; uint8_t* IND = nullptr; void* X0 = nullptr; void* XR = nullptr;
-  void* XP = nullptr; double* PRM = nullptr; void* DV = nullptr;
-  int32_t *ctl = nullptr, *lstep = nullptr, *nev = nullptr, *nev_tot = nullptr, *steps_tot = nullptr;
-  void* fr = nullptr; void* sf = nullptr;
-  int32_t *g_reopt = nullptr, *g_stage = nullptr, *g_done = nullptr, *active = nullptr;
-};
-
-}  // namespace
-
-struct ast_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  ast_variant v{};
-  ast_type_ids T{};
-  int cf_size = 8;  // bytes of the simplex float
-  // model space
-  bool space_set = false;
-  int n_params = 0, n_types = 0;
-  int prop[AST_MAX_PARAMS][AST_N_COLUMNS]{};
-  int types[AST_MAX_TYPES]{};
-  double lim[2][AST_MAX_TYPES]{};
-  ast_x2 x2{};
-  Space<double> sp_d{};
-  Space<float> sp_f{};
-  // curve
-  bool curve_set = false;
-  int n_data = 0, n_filters = 0;
-  PointRec* pts = nullptr;
-  double* sum_w = nullptr;
-  // start model
-  double params0[AST_MAX_PARAMS]{};
-  double* params0_d = nullptr;
-  // random streams and per-group bests
-  uint64_t seed = 0;
-  uint32_t* rng = nullptr;
-  size_t rng_cap = 0;
-  void* best_f = nullptr;       // CF[groups]
-  double* best_params = nullptr;
-  double* best_dV = nullptr;
-  size_t best_cap = 0;
-  Job job;
-  int last_groups = 0, last_group_size = 0;
-  // timing / counts of the last run
-  double last_ms = 0;
-  int last_launches = 0;
-  int64_t last_evals = 0, last_points = 0;
-  std::vector<hipEvent_t> ev;
-};
-
-namespace {
-
+// ------------------------------------------------------------ validation
 bool variant_supported(const ast_variant& v) {
   if (v.bc && v.bw_ball) return false;
   if (v.precision != AST_PREC_FP64 && v.precision != AST_PREC_F32SIMPLEX) return false;
@@ -115,23 +93,65 @@ bool variant_supported(const ast_variant& v) {
   return true;
 }
 
-hipError_t dispatch(ast_ctx* c, LaunchArgs& a) {
-  a.precision = c->v.precision;
-  a.stream = c->stream;
-  a.sp_d = &c->sp_d;
-  a.sp_f = &c->sp_f;
-  a.pts = c->pts;
-  a.sum_w = c->sum_w;
-  a.n_data = c->n_data;
-  a.n_filters = c->n_filters;
-  a.dof = c->n_data - c->n_params - c->n_filters;
-  const bool tq = c->v.torque, bc = c->v.bc, bw = c->v.bw_ball;
-  if (!tq && !bc && !bw) return launch_variant<false, false, false>(a);
-  if (!tq && bc) return launch_variant<false, true, false>(a);
-  if (!tq && bw) return launch_variant<false, false, true>(a);
-  if (tq && !bc && !bw) return launch_variant<true, false, false>(a);
-  if (tq && bc) return launch_variant<true, true, false>(a);
-  return launch_variant<true, false, true>(a);
+struct SpaceH {  // a validated model space
+  ast_variant v{};
+  ast_type_ids T{};
+  int n_params = 0, n_types = 0;
+  int prop[AST_MAX_PARAMS][AST_N_COLUMNS]{};
+  double lim[2][AST_MAX_TYPES]{};
+  ast_x2 x2{};
+};
+
+int make_space(const ast_variant& v, int n_params, const int* property, int n_types, const int* types,
+               const double* limits, const ast_x2* x2, SpaceH* out) {
+  if (!property || !types || !limits || !x2) return AST_E_ARG;
+  if (!variant_supported(v)) return AST_E_UNSUPPORTED;
+  SpaceH S;
+  S.v = v;
+  ast_variant_type_ids(&v, &S.T);
+  if (n_params != S.T.n_params || n_types != S.T.n_types) return AST_E_ARG;
+  // The kernels fix each parameter's role by its position: the Property
+  // order of the reference (asteroid.c:86-125).
+  const int expect[] = {S.T.theta_M, S.T.phi_M, S.T.phi_0, S.T.Ti, S.T.Ts, S.T.Tl, S.T.c_tumb, S.T.b_tumb,
+                        S.T.Es, S.T.L, S.T.psi_0, S.T.c, S.T.b, S.T.theta_R, S.T.phi_R, S.T.kappa};
+  int pos = 0;
+  for (int k = 0; k < 16; ++k) {
+    if (expect[k] < 0) continue;
+    if (property[pos * AST_N_COLUMNS + AST_P_TYPE] != expect[k]) return AST_E_UNSUPPORTED;
+    if (types[expect[k]] != pos) return AST_E_UNSUPPORTED;
+    int fr = property[pos * AST_N_COLUMNS + AST_P_FROZEN];
+    if (fr < -1 || fr > 1) return AST_E_ARG;
+    pos++;
+  }
+  for (int i = 0; i < n_params; ++i)
+    for (int k = 0; k < AST_N_COLUMNS; ++k) S.prop[i][k] = property[i * AST_N_COLUMNS + k];
+  for (int t = 0; t < n_types; ++t) {
+    S.lim[0][t] = limits[t];
+    S.lim[1][t] = limits[n_types + t];
+  }
+  S.n_params = n_params;
+  S.n_types = n_types;
+  S.x2 = *x2;
+  *out = S;
+  return AST_OK;
+}
+
+template <class CF>
+Space<CF> space_table(const SpaceH& S) {
+  Space<CF> sp;
+  std::memset(&sp, 0, sizeof sp);
+  for (int i = 0; i < S.n_params; ++i) {
+    int t = S.prop[i][AST_P_TYPE];
+    sp.frozen[i] = (int8_t)S.prop[i][AST_P_FROZEN];
+    sp.periodic[i] = (int8_t)S.prop[i][AST_P_PERIODIC];
+    sp.independent[i] = (int8_t)S.prop[i][AST_P_INDEPENDENT];
+    sp.lim0[i] = (CF)S.lim[0][t];
+    sp.lim1[i] = (CF)S.lim[1][t];
+  }
+  sp.p_psi = S.v.p_psi;
+  sp.ppsi1 = S.x2.ppsi1;
+  sp.ppsi2 = S.x2.ppsi2;
+  return sp;
 }
 
 // Canonical INTERP evaluation at time t (the oracle's interp_canon).
@@ -150,96 +170,456 @@ void interp_canon(const ast_ephem3& e, double t, double E[3], double S[3]) {
   for (int k = 0; k < 3; ++k) { E[k] = E[k] / En; S[k] = S[k] / Sn; }
 }
 
-template <class CF>
-void build_space(const ast_ctx* c, Space<CF>* sp) {
-  std::memset(sp, 0, sizeof(*sp));
-  for (int i = 0; i < c->n_params; ++i) {
-    int t = c->prop[i][AST_P_TYPE];
-    sp->frozen[i] = (int8_t)c->prop[i][AST_P_FROZEN];
-    sp->periodic[i] = (int8_t)c->prop[i][AST_P_PERIODIC];
-    sp->independent[i] = (int8_t)c->prop[i][AST_P_INDEPENDENT];
-    sp->lim0[i] = (CF)c->lim[0][t];
-    sp->lim1[i] = (CF)c->lim[1][t];
+struct CurveH {  // a validated light curve as the kernels read it
+  std::vector<PointRec> pts;
+  double sum_w[AST_MAX_FILTERS] = {0};
+  int n_data = 0, n_filters = 0;
+};
+
+int make_curve(const ast_variant& v, const ast_obs* obs, int n_data, int n_filters, const ast_ephem3* eph,
+               CurveH* out) {
+  if (!obs || n_data <= 0 || n_filters <= 0) return AST_E_ARG;
+  if (n_filters > AST_MAX_FILTERS) return AST_E_DATA;
+  if (v.interp && !eph) return AST_E_ARG;
+  CurveH C;
+  C.pts.resize(n_data);
+  std::vector<int> seen(n_filters, 0);
+  for (int i = 0; i < n_data; ++i) {
+    const ast_obs& o = obs[i];
+    if (o.filter < 0 || o.filter >= n_filters) return AST_E_DATA;
+    if (i > 0 && !(o.mjd > obs[i - 1].mjd)) return AST_E_DATA;  // read_data.c:81-85
+    if (!std::isfinite(o.mjd) || !std::isfinite(o.V) || !(o.w > 0)) return AST_E_DATA;
+    PointRec& p = C.pts[i];
+    std::memset(&p, 0, sizeof p);
+    if (v.interp) interp_canon(*eph, o.mjd, p.E, p.S);
+    else for (int k = 0; k < 3; ++k) { p.E[k] = o.E[k]; p.S[k] = o.S[k]; }
+    if (i > 0) {  // cuda.c:321-323
+      double t1 = obs[i - 1].mjd, t2 = o.mjd;
+      int nsteps = (int)((t2 - t1) / v.time_step + 1);
+      double h = (t2 - t1) / nsteps;
+      p.nsteps = nsteps;
+      p.h = h;
+      p.hh = 0.5 * h;
+      p.h6 = 1 / 6.0 * h;
+    }
+    p.V = o.V;
+    p.w = o.w;
+    p.filter = o.filter;
+    C.sum_w[o.filter] = C.sum_w[o.filter] + (double)o.w;
+    seen[o.filter] = 1;
   }
-  sp->p_psi = c->v.p_psi;
-  sp->ppsi1 = c->x2.ppsi1;
-  sp->ppsi2 = c->x2.ppsi2;
+  for (int m = 0; m < n_filters; ++m)
+    if (!seen[m]) return AST_E_DATA;
+  C.n_data = n_data;
+  C.n_filters = n_filters;
+  *out = std::move(C);
+  return AST_OK;
 }
 
-void free_job(Job& j) {
-  dev_free(j.X); dev_free(j.F); dev_free(j.IND); dev_free(j.X0); dev_free(j.XR); dev_free(j.XP);
-  dev_free(j.PRM); dev_free(j.DV); dev_free(j.ctl); dev_free(j.lstep); dev_free(j.nev); dev_free(j.nev_tot);
-  dev_free(j.steps_tot); dev_free(j.fr); dev_free(j.sf); dev_free(j.g_reopt); dev_free(j.g_stage);
-  dev_free(j.g_done); dev_free(j.active);
-  j.fits_cap = j.groups_cap = 0;
+int nf_class(int n_filters) { return n_filters <= 1 ? 1 : (n_filters <= 4 ? 4 : 8); }
+
+hipError_t dispatch(const ast_variant& v, LaunchArgs& a) {
+  a.precision = v.precision;
+  const bool tq = v.torque, bc = v.bc, bw = v.bw_ball;
+  if (!tq && !bc && !bw) return launch_variant<false, false, false>(a);
+  if (!tq && bc) return launch_variant<false, true, false>(a);
+  if (!tq && bw) return launch_variant<false, false, true>(a);
+  if (tq && !bc && !bw) return launch_variant<true, false, false>(a);
+  if (tq && bc) return launch_variant<true, true, false>(a);
+  return launch_variant<true, false, true>(a);
 }
 
-int ensure_job(ast_ctx* c, size_t fits, size_t groups) {
-  Job& j = c->job;
-  const int N = c->n_params;
-  if (j.fits_cap >= fits && j.groups_cap >= groups && j.N == N) return 0;
-  free_job(j);
-  const size_t cf = (size_t)c->cf_size;
+// ------------------------------------------------------------- slot pool
+// Device state of up to `nf` resident simplexes.  Element storage is 8 bytes
+// wide so that float and double simplexes share the pool; arrays have room
+// for AST_MAX_PARAMS parameters.
+constexpr int MAX_CLASSES = 36;  // 6 variants x 3 filter capacities x 2 simplex precisions
+constexpr int NSTREAMS = 4;      // classes share streams round-robin
+constexpr size_t SLOT_BYTES = 8 * ((AST_MAX_PARAMS + 1) * AST_MAX_PARAMS + (AST_MAX_PARAMS + 1) +
+                                   4 * AST_MAX_PARAMS + AST_MAX_FILTERS + 2) +
+                              (AST_MAX_PARAMS + 1) + 4 * 5;
+
+struct Pool {
+  size_t nf = 0;        // fits
+  int G = 0;            // group size the slots are cut for
+  int n_slots = 0;      // nf / G
+  void *X = nullptr, *F = nullptr, *X0 = nullptr, *XR = nullptr, *XP = nullptr, *DV = nullptr;
+  void *fr = nullptr, *sf = nullptr;
+  uint8_t* IND = nullptr;
+  double* PRM = nullptr;
+  int32_t *ctl = nullptr, *lstep = nullptr, *nev = nullptr, *nev_tot = nullptr, *steps_tot = nullptr;
+  int32_t *g_reopt = nullptr, *g_stage = nullptr, *g_done = nullptr;
+  int32_t* running = nullptr;  // [MAX_CLASSES]
+  std::vector<int> free_slots;
+};
+
+void pool_free(Pool& p) {
+  dev_free(p.X); dev_free(p.F); dev_free(p.X0); dev_free(p.XR); dev_free(p.XP); dev_free(p.DV);
+  dev_free(p.fr); dev_free(p.sf); dev_free(p.IND); dev_free(p.PRM); dev_free(p.ctl); dev_free(p.lstep);
+  dev_free(p.nev); dev_free(p.nev_tot); dev_free(p.steps_tot); dev_free(p.g_reopt); dev_free(p.g_stage);
+  dev_free(p.g_done); dev_free(p.running);
+  p.nf = 0;
+  p.n_slots = 0;
+  p.free_slots.clear();
+}
+
+int pool_alloc(Pool& p, size_t nf, int G) {
+  pool_free(p);
+  const size_t P = AST_MAX_PARAMS;
   int rc = 0;
 #define ALLOC(ptr, type, count) if (!rc) rc = dev_alloc((type**)&(ptr), (count))
-  ALLOC(j.X, char, cf * (N + 1) * N * fits);
-  ALLOC(j.F, char, cf * (N + 1) * fits);
-  ALLOC(j.IND, uint8_t, (size_t)(N + 1) * fits);
-  ALLOC(j.X0, char, cf * N * fits);
-  ALLOC(j.XR, char, cf * N * fits);
-  ALLOC(j.XP, char, cf * N * fits);
-  ALLOC(j.PRM, double, (size_t)N * fits);
-  ALLOC(j.DV, char, cf * AST_MAX_FILTERS * fits);
-  ALLOC(j.ctl, int32_t, fits);
-  ALLOC(j.lstep, int32_t, fits);
-  ALLOC(j.nev, int32_t, fits);
-  ALLOC(j.nev_tot, int32_t, fits);
-  ALLOC(j.steps_tot, int32_t, fits);
-  ALLOC(j.fr, char, cf * fits);
-  ALLOC(j.sf, char, cf * fits);
-  ALLOC(j.g_reopt, int32_t, groups);
-  ALLOC(j.g_stage, int32_t, groups);
-  ALLOC(j.g_done, int32_t, groups);
-  ALLOC(j.active, int32_t, 1);
+  ALLOC(p.X, double, (P + 1) * P * nf);
+  ALLOC(p.F, double, (P + 1) * nf);
+  ALLOC(p.X0, double, P * nf);
+  ALLOC(p.XR, double, P * nf);
+  ALLOC(p.XP, double, P * nf);
+  ALLOC(p.DV, double, AST_MAX_FILTERS * nf);
+  ALLOC(p.fr, double, nf);
+  ALLOC(p.sf, double, nf);
+  ALLOC(p.IND, uint8_t, (P + 1) * nf);
+  ALLOC(p.PRM, double, P * nf);
+  ALLOC(p.ctl, int32_t, nf);
+  ALLOC(p.lstep, int32_t, nf);
+  ALLOC(p.nev, int32_t, nf);
+  ALLOC(p.nev_tot, int32_t, nf);
+  ALLOC(p.steps_tot, int32_t, nf);
+  ALLOC(p.g_reopt, int32_t, nf / G);
+  ALLOC(p.g_stage, int32_t, nf / G);
+  ALLOC(p.g_done, int32_t, nf / G);
+  ALLOC(p.running, int32_t, MAX_CLASSES);
 #undef ALLOC
-  if (rc) { free_job(j); return rc; }
-  j.fits_cap = fits;
-  j.groups_cap = groups;
-  j.N = N;
+  if (rc) { pool_free(p); return rc; }
+  p.nf = nf;
+  p.G = G;
+  p.n_slots = (int)(nf / G);
+  p.free_slots.clear();
+  for (int s = p.n_slots - 1; s >= 0; --s) p.free_slots.push_back(s);
   return 0;
 }
 
-// Grow the random-stream counters and per-group bests, keeping old entries.
+template <class CF>
+SlotBufs<CF> pool_bufs(const Pool& p, int cls) {
+  SlotBufs<CF> B;
+  B.X = (CF*)p.X; B.F = (CF*)p.F; B.IND = p.IND; B.X0 = (CF*)p.X0; B.XR = (CF*)p.XR; B.XP = (CF*)p.XP;
+  B.PRM = p.PRM; B.DV = (CF*)p.DV; B.ctl = p.ctl; B.lstep = p.lstep; B.nev = p.nev; B.nev_tot = p.nev_tot;
+  B.steps_tot = p.steps_tot; B.fr = (CF*)p.fr; B.sf = (CF*)p.sf; B.g_reopt = p.g_reopt; B.g_stage = p.g_stage;
+  B.g_done = p.g_done; B.running = p.running + cls; B.nf = p.nf;
+  return B;
+}
+
+// --------------------------------------------------------------- jobs
+struct RunParams {
+  int n_groups = 0, group_size = 0, nstages = 1, n_steps = 0;
+  int64_t eval_budget = 0;
+  double size_min = 0;
+  uint64_t seed = 0;
+};
+
+struct DevJob {
+  uint64_t id = 0, tag = 0;
+  SpaceH space;
+  RunParams rp;
+  int n_data = 0, n_filters = 0;
+  int cls = 0;                 // kernel class index (for streams and counters)
+  int cf = 8;                  // simplex float bytes
+  // device memory (owned unless borrowed)
+  PointRec* pts = nullptr;
+  double* sum_w = nullptr;
+  void* sp = nullptr;          // Space<CF>
+  double* params0 = nullptr;
+  uint32_t* rng = nullptr;
+  void* best_f = nullptr;
+  double* best_params = nullptr;
+  double* best_dV = nullptr;
+  int32_t* done = nullptr;     // finished-group counter
+  void* descs = nullptr;       // GroupDesc<CF>[n_groups]
+  bool borrowed = false;       // rng and best arrays belong to the context
+  std::vector<int> slots;
+  size_t bytes = 0;
+  int groups_done = 0;
+  int status = AST_OK;
+  bool cancelled = false;
+  Clock::time_point t_submit, t_admit;
+  double run_ms = 0;
+};
+
+void job_free(DevJob& j) {
+  dev_free(j.pts); dev_free(j.sum_w); dev_free(j.sp); dev_free(j.params0); dev_free(j.done);
+  dev_free(j.descs);
+  if (!j.borrowed) { dev_free(j.rng); dev_free(j.best_f); dev_free(j.best_params); dev_free(j.best_dV); }
+}
+
+int ticks_for(int n_data, double work_points) {
+  double t = work_points / (n_data > 0 ? n_data : 1);
+  return t < 1 ? 1 : (t > 4096 ? 4096 : (int)t);
+}
+
+// Device memory of one job (its own streams and bests unless borrowed).  The
+// clears run on `s`, the stream the job's kernels run on, so they are ordered
+// before its first launch.
+int job_upload(DevJob& j, const CurveH& C, const double* params0, hipStream_t s) {
+  const SpaceH& S = j.space;
+  j.cf = S.v.precision == AST_PREC_FP64 ? 8 : 4;
+  j.n_data = C.n_data;
+  j.n_filters = C.n_filters;
+  const int N = S.n_params, G = j.rp.n_groups;
+  const size_t fits = (size_t)G * j.rp.group_size;
+  RETIF(dev_alloc(&j.pts, C.pts.size()));
+  RETIF(dev_alloc(&j.sum_w, (size_t)AST_MAX_FILTERS));
+  RETIF(dev_alloc((char**)&j.sp, sizeof(Space<double>)));
+  RETIF(dev_alloc(&j.params0, (size_t)AST_MAX_PARAMS));
+  RETIF(dev_alloc(&j.done, (size_t)1));
+  HIPCHK(hipMemcpy(j.pts, C.pts.data(), sizeof(PointRec) * C.pts.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(j.sum_w, C.sum_w, sizeof(double) * AST_MAX_FILTERS, hipMemcpyHostToDevice));
+  if (j.cf == 8) {
+    Space<double> sp = space_table<double>(S);
+    HIPCHK(hipMemcpy(j.sp, &sp, sizeof sp, hipMemcpyHostToDevice));
+  } else {
+    Space<float> sp = space_table<float>(S);
+    HIPCHK(hipMemcpy(j.sp, &sp, sizeof sp, hipMemcpyHostToDevice));
+  }
+  double p0[AST_MAX_PARAMS] = {0};
+  if (params0)
+    for (int i = 0; i < N; ++i) p0[i] = params0[i];
+  HIPCHK(hipMemcpy(j.params0, p0, sizeof p0, hipMemcpyHostToDevice));
+  HIPCHK(hipMemsetAsync(j.done, 0, sizeof(int32_t), s));
+  if (!j.borrowed) {
+    RETIF(dev_alloc(&j.rng, fits));
+    RETIF(dev_alloc((char**)&j.best_f, (size_t)j.cf * G));
+    RETIF(dev_alloc(&j.best_params, (size_t)G * AST_MAX_PARAMS));
+    RETIF(dev_alloc(&j.best_dV, (size_t)G * AST_MAX_FILTERS));
+    std::vector<char> hf((size_t)j.cf * G);
+    for (int g = 0; g < G; ++g) {
+      if (j.cf == 8) { double v = 1e30; std::memcpy(&hf[(size_t)g * 8], &v, 8); }
+      else { float v = 1e30f; std::memcpy(&hf[(size_t)g * 4], &v, 4); }
+    }
+    HIPCHK(hipMemcpy(j.best_f, hf.data(), hf.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(j.rng, 0, fits * sizeof(uint32_t), s));
+    HIPCHK(hipMemsetAsync(j.best_params, 0, sizeof(double) * G * AST_MAX_PARAMS, s));
+    HIPCHK(hipMemsetAsync(j.best_dV, 0, sizeof(double) * G * AST_MAX_FILTERS, s));
+  }
+  j.bytes = sizeof(PointRec) * C.pts.size() + fits * (SLOT_BYTES + 4) + (size_t)G * 256;
+  return 0;
+}
+
+template <class CF>
+int job_descs_typed(DevJob& j, double work_points) {
+  const int G = j.rp.n_groups;
+  std::vector<GroupDesc<CF>> d(G);
+  double sm = j.rp.size_min > 0 ? j.rp.size_min : (j.cf == 8 ? 1e-10 : 1e-5);  // asteroid.h:233-237
+  CF smc = (CF)sm;
+  for (int g = 0; g < G; ++g) {
+    GroupDesc<CF>& D = d[g];
+    std::memset(&D, 0, sizeof D);
+    D.pts = j.pts;
+    D.sum_w = j.sum_w;
+    D.sp = (const Space<CF>*)j.sp;
+    D.params0 = j.params0;
+    D.rng = j.rng;
+    D.best_f = (CF*)j.best_f;
+    D.best_params = j.best_params;
+    D.best_dV = j.best_dV;
+    D.job_done = j.done;
+    D.seed = j.rp.seed;
+    D.size2_min = smc * smc;  // asteroid.h:245
+    D.n_data = j.n_data;
+    D.n_filters = j.n_filters;
+    D.dof = j.n_data - j.space.n_params - j.n_filters;
+    D.nstages = j.rp.nstages;
+    D.n_steps = j.rp.n_steps;
+    D.eval_budget = (int)(j.rp.eval_budget > 0x7fffffff ? 0x7fffffff : j.rp.eval_budget);
+    D.reopt = j.space.x2.reopt;
+    D.slot = j.slots[g];
+    D.group_in_job = g;
+    D.ticks = ticks_for(j.n_data, work_points);
+  }
+  if (!j.descs) RETIF(dev_alloc((char**)&j.descs, sizeof(GroupDesc<CF>) * G));
+  HIPCHK(hipMemcpy(j.descs, d.data(), sizeof(GroupDesc<CF>) * G, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int job_descs(DevJob& j, double work_points) {
+  return j.cf == 8 ? job_descs_typed<double>(j, work_points) : job_descs_typed<float>(j, work_points);
+}
+
+// The shapes the kernels assume: one workgroup of pool.G threads per group,
+// every group on its own slot of the pool, the job's arrays sized for its
+// groups.  Checked on the host before every launch.
+bool job_fits_pool(const DevJob& j, const Pool& pool) {
+  if (j.rp.group_size != pool.G || j.rp.n_groups <= 0) return false;
+  if ((int)j.slots.size() != j.rp.n_groups || !j.descs) return false;
+  if (j.space.n_params > AST_MAX_PARAMS || j.n_filters > AST_MAX_FILTERS || j.n_filters <= 0) return false;
+  for (int s : j.slots)
+    if (s < 0 || s >= pool.n_slots) return false;
+  return (size_t)pool.n_slots * (size_t)pool.G <= pool.nf;
+}
+
+// Launch the init or fit kernel over all groups of one job.
+hipError_t job_launch(DevJob& j, const Pool& pool, int op, hipStream_t s) {
+  if (!job_fits_pool(j, pool)) {
+    set_err("job does not fit the slot pool");
+    return hipErrorInvalidValue;
+  }
+  LaunchArgs a{};
+  a.op = op;
+  a.stream = s;
+  a.descs = j.descs;
+  SlotBufs<double> bd;
+  SlotBufs<float> bf;
+  if (j.cf == 8) { bd = pool_bufs<double>(pool, j.cls); a.bufs = &bd; }
+  else { bf = pool_bufs<float>(pool, j.cls); a.bufs = &bf; }
+  a.n_groups = j.rp.n_groups;
+  a.group_size = j.rp.group_size;
+  a.n_filters = j.n_filters;
+  return dispatch(j.space.v, a);
+}
+
+// Kernel class of a job: one fit-kernel instantiation (variant, filter
+// capacity, simplex precision).  BC and BW_BALL are exclusive, so k < 6.
+int class_of(const ast_variant& v, int n_filters) {
+  int k = (v.torque ? 1 : 0) + (v.bc ? 2 : 0) + (v.bw_ball ? 4 : 0);
+  int f = n_filters <= 1 ? 0 : (n_filters <= 4 ? 1 : 2);
+  int p = v.precision == AST_PREC_FP64 ? 0 : 1;
+  return (k * 3 + f) * 2 + p;
+}
+int stream_of(int cls) { return cls % NSTREAMS; }
+
+// --------------------------------------------------------------- serving
+struct Serve {
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv_submit, cv_done;
+  bool stop = false;
+  std::deque<std::unique_ptr<DevJob>> pending;  // submitted, not admitted
+  std::vector<std::unique_ptr<DevJob>> resident;
+  std::deque<ast_result> done;
+  std::vector<uint64_t> cancels;
+  std::map<uint64_t, std::pair<CurveH, std::vector<double>>> inputs;  // curve + params0 until admission
+  uint64_t next_id = 1;
+  ast_serve_cfg cfg{};
+  int64_t resident_bytes = 0;
+  int64_t launches = 0;
+  double kernel_ms = 0;
+  int64_t evals = 0, points = 0;
+  int error = 0;
+};
+
+}  // namespace
+
+struct ast_ctx {
+  int device = 0;
+  hipStream_t streams[NSTREAMS] = {nullptr};
+  hipEvent_t ev0[NSTREAMS] = {nullptr}, ev1[NSTREAMS] = {nullptr};
+  ast_variant v{};
+  // reference path state
+  bool space_set = false;
+  SpaceH space;
+  bool curve_set = false;
+  CurveH curve;
+  PointRec* pts = nullptr;   // for ast_chi2_batch
+  double* sum_w = nullptr;
+  double params0[AST_MAX_PARAMS]{};
+  uint64_t seed = 0;
+  uint32_t* rng = nullptr;
+  size_t rng_cap = 0;
+  void* best_f = nullptr;
+  double* best_params = nullptr;
+  double* best_dV = nullptr;
+  size_t best_cap = 0;
+  int best_cf = 8;
+  Pool pool;
+  DevJob* last_job = nullptr;  // lanes of the last ast_run (slot rows 0..fits-1)
+  std::unique_ptr<DevJob> ref_job;
+  int last_groups = 0, last_group_size = 0;
+  double last_ms = 0;
+  int last_launches = 0;
+  int64_t last_evals = 0, last_points = 0;
+  std::unique_ptr<Serve> serve;
+};
+
+namespace {
+
+int set_device(ast_ctx* c) {
+  HIPCHK(hipSetDevice(c->device));
+  return 0;
+}
+
+// One cycle: one fit-kernel launch per job over all of its groups, every
+// stream drained at the end; adds the cycle's kernel time.
+int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, int* launches);
+
+// A failed cycle may leave kernels in flight: drain the device before the
+// caller frees anything they use.
+int run_cycle(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, int* launches) {
+  int rc = run_cycle_once(c, jobs, kernel_ms, launches);
+  if (rc) (void)hipDeviceSynchronize();
+  return rc;
+}
+
+int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, int* launches) {
+  bool used_cls[MAX_CLASSES] = {false}, used_s[NSTREAMS] = {false};
+  for (DevJob* j : jobs) {
+    used_cls[j->cls] = true;
+    used_s[stream_of(j->cls)] = true;
+  }
+  for (int k = 0; k < NSTREAMS; ++k)
+    if (used_s[k]) HIPCHK(hipEventRecord(c->ev0[k], c->streams[k]));
+  for (int k = 0; k < MAX_CLASSES; ++k)
+    if (used_cls[k]) HIPCHK(hipMemsetAsync(c->pool.running + k, 0, sizeof(int32_t), c->streams[stream_of(k)]));
+  for (DevJob* j : jobs) {
+    if (j->groups_done >= j->rp.n_groups) continue;
+    HIPCHK(job_launch(*j, c->pool, 1, c->streams[stream_of(j->cls)]));
+    (*launches)++;
+  }
+  for (int k = 0; k < NSTREAMS; ++k)
+    if (used_s[k]) HIPCHK(hipEventRecord(c->ev1[k], c->streams[k]));
+  // Per-job finished-group counters.
+  std::vector<int32_t> done(jobs.size());
+  for (size_t i = 0; i < jobs.size(); ++i)
+    HIPCHK(hipMemcpyAsync(&done[i], jobs[i]->done, sizeof(int32_t), hipMemcpyDeviceToHost,
+                          c->streams[stream_of(jobs[i]->cls)]));
+  double ms_max = 0;
+  for (int k = 0; k < NSTREAMS; ++k)
+    if (used_s[k]) {
+      HIPCHK(hipStreamSynchronize(c->streams[k]));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev0[k], c->ev1[k]));
+      if (ms > ms_max) ms_max = ms;
+    }
+  *kernel_ms += ms_max;
+  for (size_t i = 0; i < jobs.size(); ++i) jobs[i]->groups_done = done[i];
+  return 0;
+}
+
+// Grow the reference path's random-stream counters and per-group bests.
 int ensure_streams(ast_ctx* c, size_t fits, size_t groups) {
   if (c->rng_cap < fits) {
     uint32_t* nr = nullptr;
-    int rc = dev_alloc(&nr, fits);
-    if (rc) return rc;
-    HIPCHK(hipMemsetAsync(nr, 0, fits * sizeof(uint32_t), c->stream));
-    if (c->rng) HIPCHK(hipMemcpyAsync(nr, c->rng, c->rng_cap * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    RETIF(dev_alloc(&nr, fits));
+    HIPCHK(hipMemset(nr, 0, fits * sizeof(uint32_t)));
+    if (c->rng) HIPCHK(hipMemcpy(nr, c->rng, c->rng_cap * sizeof(uint32_t), hipMemcpyDeviceToDevice));
     dev_free(c->rng);
     c->rng = nr;
     c->rng_cap = fits;
   }
+  const int cf = c->v.precision == AST_PREC_FP64 ? 8 : 4;
   if (c->best_cap < groups || !c->best_f) {
-    const size_t cf = (size_t)c->cf_size;
     void* nf = nullptr;
     double *np = nullptr, *nd = nullptr;
-    int rc = dev_alloc((char**)&nf, cf * groups);
+    int rc = dev_alloc((char**)&nf, (size_t)cf * groups);
     if (!rc) rc = dev_alloc(&np, groups * AST_MAX_PARAMS);
     if (!rc) rc = dev_alloc(&nd, groups * AST_MAX_FILTERS);
     if (rc) { dev_free(nf); dev_free(np); dev_free(nd); return rc; }
-    std::vector<char> hf(cf * groups);
+    std::vector<char> hf((size_t)cf * groups);
     for (size_t g = 0; g < groups; ++g) {
       if (cf == 8) { double v = 1e30; std::memcpy(&hf[g * 8], &v, 8); }
       else { float v = 1e30f; std::memcpy(&hf[g * 4], &v, 4); }
     }
-    HIPCHK(hipMemcpy(nf, hf.data(), cf * groups, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(nf, hf.data(), hf.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(np, 0, groups * AST_MAX_PARAMS * sizeof(double)));
     HIPCHK(hipMemset(nd, 0, groups * AST_MAX_FILTERS * sizeof(double)));
     if (c->best_f) {
-      HIPCHK(hipMemcpy(nf, c->best_f, cf * c->best_cap, hipMemcpyDeviceToDevice));
+      HIPCHK(hipMemcpy(nf, c->best_f, (size_t)cf * c->best_cap, hipMemcpyDeviceToDevice));
       HIPCHK(hipMemcpy(np, c->best_params, c->best_cap * AST_MAX_PARAMS * sizeof(double), hipMemcpyDeviceToDevice));
       HIPCHK(hipMemcpy(nd, c->best_dV, c->best_cap * AST_MAX_FILTERS * sizeof(double), hipMemcpyDeviceToDevice));
     }
@@ -250,13 +630,14 @@ int ensure_streams(ast_ctx* c, size_t fits, size_t groups) {
     c->best_params = np;
     c->best_dV = nd;
     c->best_cap = groups;
+    c->best_cf = cf;
   }
   return 0;
 }
 
 int reset_best_f(ast_ctx* c) {
   if (!c->best_f) return 0;
-  const size_t cf = (size_t)c->cf_size, n = c->best_cap;
+  const size_t cf = (size_t)c->best_cf, n = c->best_cap;
   std::vector<char> hf(cf * n);
   for (size_t g = 0; g < n; ++g) {
     if (cf == 8) { double v = 1e30; std::memcpy(&hf[g * 8], &v, 8); }
@@ -266,98 +647,196 @@ int reset_best_f(ast_ctx* c) {
   return 0;
 }
 
-template <class CF>
-void fill_bufs(ast_ctx* c, FitBufs<CF>* B) {
-  Job& j = c->job;
-  B->X = (CF*)j.X; B->F = (CF*)j.F; B->IND = j.IND; B->X0 = (CF*)j.X0; B->XR = (CF*)j.XR; B->XP = (CF*)j.XP;
-  B->PRM = j.PRM; B->DV = (CF*)j.DV; B->ctl = j.ctl; B->lstep = j.lstep; B->nev = j.nev; B->nev_tot = j.nev_tot;
-  B->steps_tot = j.steps_tot; B->rng = c->rng; B->fr = (CF*)j.fr; B->sf = (CF*)j.sf; B->g_reopt = j.g_reopt;
-  B->g_stage = j.g_stage; B->g_done = j.g_done; B->g_best_f = (CF*)c->best_f; B->g_best_params = c->best_params;
-  B->g_best_dV = c->best_dV; B->active_groups = j.active;
+// Finished job -> result record (host).
+int job_result(DevJob& j, ast_result* r) {
+  std::memset(r, 0, sizeof *r);
+  r->id = j.id;
+  r->tag = j.tag;
+  r->status = j.cancelled ? AST_E_CANCELLED : j.status;
+  r->n_params = j.space.n_params;
+  r->n_filters = j.n_filters;
+  r->n_groups = j.rp.n_groups;
+  const int G = j.rp.n_groups, N = j.space.n_params, NF = j.n_filters;
+  std::vector<double> f(G), p((size_t)G * N), dv((size_t)G * NF);
+  if (!j.cancelled) {
+    if (j.cf == 8) {
+      HIPCHK(hipMemcpy(f.data(), j.best_f, sizeof(double) * G, hipMemcpyDeviceToHost));
+    } else {
+      std::vector<float> tmp(G);
+      HIPCHK(hipMemcpy(tmp.data(), j.best_f, sizeof(float) * G, hipMemcpyDeviceToHost));
+      for (int g = 0; g < G; ++g) f[g] = tmp[g];
+    }
+    HIPCHK(hipMemcpy(p.data(), j.best_params, sizeof(double) * G * N, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(dv.data(), j.best_dV, sizeof(double) * G * NF, hipMemcpyDeviceToHost));
+  }
+  // Best of the job's groups (strict <, first wins: asteroid.c:741-749).
+  int gb = 0;
+  double fb = 1e32;
+  for (int g = 0; g < G; ++g)
+    if (f[g] < fb) { fb = f[g]; gb = g; }
+  r->best_f = j.cancelled ? 1e30 : fb;
+  r->best_group = gb;
+  for (int i = 0; i < N; ++i) r->best_params[i] = p.empty() ? 0 : p[(size_t)gb * N + i];
+  for (int m = 0; m < NF; ++m) r->best_dV[m] = dv.empty() ? 0 : dv[(size_t)gb * NF + m];
+  return 0;
 }
 
-template <class CF>
-int run_typed(ast_ctx* c, const ast_run_cfg* rc) {
-  const size_t fits = (size_t)rc->n_groups * rc->group_size;
-  int e = ensure_streams(c, fits, rc->n_groups);
-  if (e) return e;
-  e = ensure_job(c, fits, rc->n_groups);
-  if (e) return e;
-  FitBufs<CF> B;
-  fill_bufs<CF>(c, &B);
-  FitCfg<CF> C;
-  C.n_fits = (int)fits;
-  C.nstages = rc->nstages;
-  C.n_steps = rc->n_steps;
-  C.eval_budget = (int)(rc->eval_budget > 0x7fffffff ? 0x7fffffff : rc->eval_budget);
-  double smin = rc->size_min > 0 ? rc->size_min : (c->v.precision == AST_PREC_FP64 ? 1e-10 : 1e-5);
-  CF sm = (CF)smin;
-  C.size2_min = sm * sm;
-  C.seed = c->seed;
-  C.n_data = c->n_data;
-  C.n_filters = c->n_filters;
-  C.dof = c->n_data - c->n_params - c->n_filters;
-  C.reopt = c->x2.reopt;
-  int ticks = rc->ticks_per_launch;
-  if (ticks <= 0) {
-    double work = (double)fits * (double)c->n_data;
-    double tt = 3.0e9 / (work > 1 ? work : 1);
-    ticks = tt < 16 ? 16 : (tt > 4096 ? 4096 : (int)tt);
-  }
-  C.ticks = ticks;
-  HIPCHK(hipMemcpy(c->params0_d, c->params0, sizeof(c->params0), hipMemcpyHostToDevice));
-  LaunchArgs a{};
-  a.op = 0;
-  a.bufs = &B;
-  a.cfg = &C;
-  a.params0 = c->params0_d;
-  a.n_groups = rc->n_groups;
-  a.group_size = rc->group_size;
-  HIPCHK(dispatch(c, a));
-  a.op = 1;
-  c->last_ms = 0;
-  c->last_launches = 0;
-  // Upper bound on launches: every tick completes one evaluation of each
-  // running simplex; a stage needs at most (N+1) + n_steps*(N+2) of them.
-  const double per_stage = (double)(c->n_params + 1) + (double)rc->n_steps * (c->n_params + 2);
-  const double max_ticks = per_stage * rc->nstages + 16.0;
-  const long max_launches = (long)(max_ticks / ticks) + 4;
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  int result = 0;
-  for (long L = 0;; ++L) {
-    if (L > max_launches) { result = AST_E_STATE; snprintf(g_hip_err, sizeof g_hip_err, "fit did not terminate"); break; }
-    hipError_t he = hipMemsetAsync(c->job.active, 0, sizeof(int32_t), c->stream);
-    if (he == hipSuccess) he = hipEventRecord(e0, c->stream);
-    if (he == hipSuccess) he = dispatch(c, a);
-    if (he == hipSuccess) he = hipEventRecord(e1, c->stream);
-    int32_t active = 0;
-    if (he == hipSuccess) he = hipMemcpyAsync(&active, c->job.active, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
-    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
-    float ms = 0;
-    if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
-    if (he != hipSuccess) {
-      snprintf(g_hip_err, sizeof g_hip_err, "%s (fit launch %ld)", hipGetErrorString(he), L);
-      result = AST_E_HIP;
-      break;
-    }
-    c->last_ms += ms;
-    c->last_launches++;
-    if (active == 0) break;
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (result) return result;
-  std::vector<int32_t> nev(fits);
-  HIPCHK(hipMemcpy(nev.data(), c->job.nev_tot, fits * sizeof(int32_t), hipMemcpyDeviceToHost));
+// Slot-state counters of a job's groups -> evaluation count.
+int job_evals(const ast_ctx* c, const DevJob& j, int64_t* evals) {
+  const int G = j.rp.group_size;
+  std::vector<int32_t> tmp(G);
   int64_t s = 0;
-  for (size_t k = 0; k < fits; ++k) s += nev[k];
-  c->last_evals = s;
-  c->last_points = s * c->n_data;
-  c->last_groups = rc->n_groups;
-  c->last_group_size = rc->group_size;
+  for (int slot : j.slots) {
+    HIPCHK(hipMemcpy(tmp.data(), c->pool.nev_tot + (size_t)slot * G, sizeof(int32_t) * G, hipMemcpyDeviceToHost));
+    for (int k = 0; k < G; ++k) s += tmp[k];
+  }
+  *evals = s;
   return 0;
+}
+
+// ------------------------------------------------------- serving thread
+// Upload an admitted job and start its groups.  Its slots were reserved
+// under the scheduler lock when it was admitted.
+int serve_admit(ast_ctx* c, Serve& S, DevJob& j, const CurveH& curve, const std::vector<double>& p0) {
+  const hipStream_t s = c->streams[stream_of(j.cls)];
+  RETIF(job_upload(j, curve, p0.empty() ? nullptr : p0.data(), s));
+  RETIF(job_descs(j, S.cfg.work_points_per_launch));
+  HIPCHK(job_launch(j, c->pool, 0, s));
+  return 0;
+}
+
+void serve_loop(ast_ctx* c) {
+  Serve& S = *c->serve;
+  (void)hipSetDevice(c->device);
+  for (;;) {
+    std::vector<std::unique_ptr<DevJob>> admitted;
+    std::vector<std::pair<CurveH, std::vector<double>>> admitted_in;
+    {
+      std::unique_lock<std::mutex> lk(S.mu);
+      if (S.resident.empty() && S.pending.empty())
+        S.cv_submit.wait_for(lk, std::chrono::milliseconds(20));
+      if (S.stop && S.pending.empty() && S.resident.empty()) break;
+      if (S.stop) {  // cancel everything left
+        for (auto& j : S.pending) S.cancels.push_back(j->id);
+        for (auto& j : S.resident) S.cancels.push_back(j->id);
+      }
+      // Cancellations of pending jobs complete at once.
+      for (uint64_t id : S.cancels) {
+        for (auto it = S.pending.begin(); it != S.pending.end(); ++it)
+          if ((*it)->id == id) {
+            ast_result r;
+            std::memset(&r, 0, sizeof r);
+            r.id = id;
+            r.tag = (*it)->tag;
+            r.status = AST_E_CANCELLED;
+            r.best_f = 1e30;
+            S.done.push_back(r);
+            S.inputs.erase(id);
+            S.pending.erase(it);
+            S.cv_done.notify_all();
+            break;
+          }
+        for (auto& j : S.resident)
+          if (j->id == id) j->cancelled = true;
+      }
+      S.cancels.clear();
+      // Admission: FIFO, while the job's groups fit the free slots and its
+      // footprint fits the HBM budget.  The slots are taken off the free
+      // list here, so later jobs of the same pass see what is left.
+      while (!S.pending.empty()) {
+        DevJob& j = *S.pending.front();
+        auto in = S.inputs.find(j.id);
+        const int64_t need = (int64_t)j.rp.n_groups;
+        const int64_t bytes = (int64_t)(j.rp.n_groups * (int64_t)j.rp.group_size) * (int64_t)(SLOT_BYTES + 4) +
+                              (int64_t)j.n_data * (int64_t)sizeof(PointRec);
+        if ((int64_t)c->pool.free_slots.size() < need) break;
+        if (S.cfg.hbm_budget_bytes > 0 && S.resident_bytes + bytes > S.cfg.hbm_budget_bytes &&
+            !(S.resident.empty() && admitted.empty()))
+          break;
+        j.slots.clear();
+        for (int64_t g = 0; g < need; ++g) {
+          j.slots.push_back(c->pool.free_slots.back());
+          c->pool.free_slots.pop_back();
+        }
+        S.resident_bytes += bytes;
+        admitted_in.push_back(std::move(in->second));
+        S.inputs.erase(in);
+        admitted.push_back(std::move(S.pending.front()));
+        S.pending.pop_front();
+      }
+    }
+    for (size_t k = 0; k < admitted.size(); ++k) {
+      DevJob& j = *admitted[k];
+      int rc = serve_admit(c, S, j, admitted_in[k].first, admitted_in[k].second);
+      if (rc) j.status = rc;
+      j.t_admit = Clock::now();
+      std::lock_guard<std::mutex> lk(S.mu);
+      S.resident.push_back(std::move(admitted[k]));
+    }
+    std::vector<DevJob*> run;
+    {
+      std::lock_guard<std::mutex> lk(S.mu);
+      for (auto& j : S.resident)
+        if (!j->cancelled && j->status == AST_OK) run.push_back(j.get());
+    }
+    if (!run.empty()) {
+      double ms = 0;
+      int nl = 0;
+      auto t0 = Clock::now();
+      int rc = run_cycle(c, run, &ms, &nl);
+      double wall = ms_since(t0);
+      std::lock_guard<std::mutex> lk(S.mu);
+      if (rc) {
+        S.error = rc;
+        for (DevJob* j : run) j->status = rc;
+      }
+      S.kernel_ms += ms;
+      S.launches += nl;
+      for (DevJob* j : run) j->run_ms += wall;
+    }
+    // Retire finished, failed and cancelled jobs.
+    std::vector<std::unique_ptr<DevJob>> finished;
+    {
+      std::lock_guard<std::mutex> lk(S.mu);
+      for (auto it = S.resident.begin(); it != S.resident.end();) {
+        DevJob& j = **it;
+        if (j.cancelled || j.status != AST_OK || j.groups_done >= j.rp.n_groups) {
+          finished.push_back(std::move(*it));
+          it = S.resident.erase(it);
+        } else {
+          ++it;
+        }
+      }
+    }
+    for (auto& jp : finished) {
+      DevJob& j = *jp;
+      ast_result r;
+      int rc = j.status == AST_OK ? job_result(j, &r) : 0;
+      if (j.status != AST_OK || rc) {
+        std::memset(&r, 0, sizeof r);
+        r.id = j.id;
+        r.tag = j.tag;
+        r.status = rc ? rc : j.status;
+        r.best_f = 1e30;
+      }
+      int64_t ev = 0;
+      if (!j.cancelled && j.status == AST_OK) job_evals(c, j, &ev);
+      r.n_evals = ev;
+      r.model_points = ev * j.n_data;
+      r.submit_to_done_ms = ms_since(j.t_submit);
+      r.queue_ms = std::chrono::duration<double, std::milli>(j.t_admit - j.t_submit).count();
+      r.run_ms = j.run_ms;
+      std::lock_guard<std::mutex> lk(S.mu);
+      for (int s : j.slots) c->pool.free_slots.push_back(s);
+      S.resident_bytes -= (int64_t)(j.rp.n_groups * (int64_t)j.rp.group_size) * (int64_t)(SLOT_BYTES + 4) +
+                          (int64_t)j.n_data * (int64_t)sizeof(PointRec);
+      S.evals += ev;
+      S.points += ev * j.n_data;
+      job_free(j);
+      S.done.push_back(r);
+      S.cv_done.notify_all();
+    }
+  }
 }
 
 }  // namespace
@@ -374,6 +853,8 @@ const char* ast_strerror(int code) {
     case AST_E_DATA: return "input data rejected";
     case AST_E_UNSUPPORTED: return "variant or model space not supported";
     case AST_E_NODEVICE: return "no HIP device";
+    case AST_E_CANCELLED: return "job cancelled";
+    case AST_E_BUSY: return "context is serving";
     default: return "unknown error";
   }
 }
@@ -391,14 +872,15 @@ int ast_ctx_create(ast_ctx** out, const ast_variant* v, int device) {
   if (!c) return AST_E_NOMEM;
   c->device = device;
   c->v = *v;
-  ast_variant_type_ids(v, &c->T);
-  c->cf_size = v->precision == AST_PREC_FP64 ? 8 : 4;
   hipError_t e = hipSetDevice(device);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipMalloc((void**)&c->params0_d, sizeof(c->params0));
+  for (int k = 0; k < NSTREAMS && e == hipSuccess; ++k) {
+    e = hipStreamCreateWithFlags(&c->streams[k], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0[k]);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1[k]);
+  }
   if (e != hipSuccess) {
-    snprintf(g_hip_err, sizeof g_hip_err, "%s", hipGetErrorString(e));
-    delete c;
+    set_err(hipGetErrorString(e));
+    ast_ctx_destroy(c);
     return AST_E_HIP;
   }
   *out = c;
@@ -407,103 +889,54 @@ int ast_ctx_create(ast_ctx** out, const ast_variant* v, int device) {
 
 int ast_ctx_destroy(ast_ctx* c) {
   if (!c) return AST_E_ARG;
+  if (c->serve) ast_serve_stop(c);
   (void)hipSetDevice(c->device);
-  free_job(c->job);
+  if (c->ref_job) job_free(*c->ref_job);
+  pool_free(c->pool);
   dev_free(c->pts);
   dev_free(c->sum_w);
-  dev_free(c->params0_d);
   dev_free(c->rng);
   dev_free(c->best_f);
   dev_free(c->best_params);
   dev_free(c->best_dV);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  for (int k = 0; k < NSTREAMS; ++k) {
+    if (c->streams[k]) (void)hipStreamDestroy(c->streams[k]);
+    if (c->ev0[k]) (void)hipEventDestroy(c->ev0[k]);
+    if (c->ev1[k]) (void)hipEventDestroy(c->ev1[k]);
+  }
   delete c;
   return AST_OK;
 }
 
 int ast_set_model_space(ast_ctx* c, int n_params, const int* property, int n_types, const int* types,
                         const double* limits, const ast_x2* x2) {
-  if (!c || !property || !types || !limits || !x2) return AST_E_ARG;
-  if (n_params != c->T.n_params || n_types != c->T.n_types) return AST_E_ARG;
-  // The kernels fix each parameter's role by its position: the Property
-  // order of the reference (asteroid.c:86-125).
-  const int expect[] = {c->T.theta_M, c->T.phi_M, c->T.phi_0, c->T.Ti, c->T.Ts, c->T.Tl, c->T.c_tumb, c->T.b_tumb,
-                        c->T.Es, c->T.L, c->T.psi_0, c->T.c, c->T.b, c->T.theta_R, c->T.phi_R, c->T.kappa};
-  int pos = 0;
-  for (int k = 0; k < 16; ++k) {
-    if (expect[k] < 0) continue;
-    if (property[pos * AST_N_COLUMNS + AST_P_TYPE] != expect[k]) return AST_E_UNSUPPORTED;
-    if (types[expect[k]] != pos) return AST_E_UNSUPPORTED;
-    int fr = property[pos * AST_N_COLUMNS + AST_P_FROZEN];
-    if (fr < -1 || fr > 1) return AST_E_ARG;
-    pos++;
-  }
-  for (int i = 0; i < n_params; ++i)
-    for (int k = 0; k < AST_N_COLUMNS; ++k) c->prop[i][k] = property[i * AST_N_COLUMNS + k];
-  for (int t = 0; t < n_types; ++t) {
-    c->types[t] = types[t];
-    c->lim[0][t] = limits[t];
-    c->lim[1][t] = limits[n_types + t];
-  }
-  c->n_params = n_params;
-  c->n_types = n_types;
-  c->x2 = *x2;
-  build_space<double>(c, &c->sp_d);
-  build_space<float>(c, &c->sp_f);
+  if (!c) return AST_E_ARG;
+  SpaceH S;
+  RETIF(make_space(c->v, n_params, property, n_types, types, limits, x2, &S));
+  c->space = S;
   c->space_set = true;
   return AST_OK;
 }
 
 int ast_upload_curve(ast_ctx* c, const ast_obs* obs, int n_data, int n_filters, const ast_ephem3* eph) {
-  if (!c || !obs || n_data <= 0 || n_filters <= 0) return AST_E_ARG;
-  if (n_filters > AST_MAX_FILTERS) return AST_E_DATA;
-  if (c->v.interp && !eph) return AST_E_ARG;
-  std::vector<PointRec> h(n_data);
-  std::vector<double> sw(AST_MAX_FILTERS, 0.0);
-  std::vector<int> seen(n_filters, 0);
-  for (int i = 0; i < n_data; ++i) {
-    const ast_obs& o = obs[i];
-    if (o.filter < 0 || o.filter >= n_filters) return AST_E_DATA;
-    if (i > 0 && !(o.mjd > obs[i - 1].mjd)) return AST_E_DATA;  // read_data.c:81-85
-    if (!std::isfinite(o.mjd) || !std::isfinite(o.V) || !(o.w > 0)) return AST_E_DATA;
-    PointRec& p = h[i];
-    std::memset(&p, 0, sizeof p);
-    if (c->v.interp) interp_canon(*eph, o.mjd, p.E, p.S);
-    else for (int k = 0; k < 3; ++k) { p.E[k] = o.E[k]; p.S[k] = o.S[k]; }
-    if (i > 0) {  // cuda.c:321-323
-      double t1 = obs[i - 1].mjd, t2 = o.mjd;
-      int nsteps = (int)((t2 - t1) / c->v.time_step + 1);
-      double hh = (t2 - t1) / nsteps;
-      p.nsteps = nsteps;
-      p.h = hh;
-      p.hh = 0.5 * hh;
-      p.h6 = 1 / 6.0 * hh;
-    }
-    p.V = o.V;
-    p.w = o.w;
-    p.filter = o.filter;
-    sw[o.filter] = sw[o.filter] + (double)o.w;
-    seen[o.filter] = 1;
-  }
-  for (int m = 0; m < n_filters; ++m)
-    if (!seen[m]) return AST_E_DATA;
-  (void)hipSetDevice(c->device);
+  if (!c) return AST_E_ARG;
+  CurveH C;
+  RETIF(make_curve(c->v, obs, n_data, n_filters, eph, &C));
+  RETIF(set_device(c));
   dev_free(c->pts);
   dev_free(c->sum_w);
-  int rc = dev_alloc(&c->pts, (size_t)n_data);
-  if (!rc) rc = dev_alloc(&c->sum_w, (size_t)AST_MAX_FILTERS);
-  if (rc) return rc;
-  HIPCHK(hipMemcpy(c->pts, h.data(), sizeof(PointRec) * n_data, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->sum_w, sw.data(), sizeof(double) * AST_MAX_FILTERS, hipMemcpyHostToDevice));
-  c->n_data = n_data;
-  c->n_filters = n_filters;
+  RETIF(dev_alloc(&c->pts, (size_t)n_data));
+  RETIF(dev_alloc(&c->sum_w, (size_t)AST_MAX_FILTERS));
+  HIPCHK(hipMemcpy(c->pts, C.pts.data(), sizeof(PointRec) * n_data, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->sum_w, C.sum_w, sizeof(double) * AST_MAX_FILTERS, hipMemcpyHostToDevice));
+  c->curve = std::move(C);
   c->curve_set = true;
   return AST_OK;
 }
 
 int ast_rng_seed(ast_ctx* c, uint64_t seed) {
   if (!c) return AST_E_ARG;
-  (void)hipSetDevice(c->device);
+  RETIF(set_device(c));
   c->seed = seed;
   if (c->rng) HIPCHK(hipMemset(c->rng, 0, c->rng_cap * sizeof(uint32_t)));
   return reset_best_f(c);
@@ -511,35 +944,96 @@ int ast_rng_seed(ast_ctx* c, uint64_t seed) {
 
 int ast_reset_best(ast_ctx* c) {
   if (!c) return AST_E_ARG;
-  (void)hipSetDevice(c->device);
+  RETIF(set_device(c));
   return reset_best_f(c);
 }
 
 int ast_set_start(ast_ctx* c, const double* params0) {
-  if (!c || !params0 || !c->space_set) return c && !c->space_set ? AST_E_STATE : AST_E_ARG;
-  for (int i = 0; i < c->n_params; ++i) c->params0[i] = params0[i];
+  if (!c || !params0) return AST_E_ARG;
+  if (!c->space_set) return AST_E_STATE;
+  for (int i = 0; i < c->space.n_params; ++i) c->params0[i] = params0[i];
   return AST_OK;
 }
 
 int ast_run(ast_ctx* c, const ast_run_cfg* rc) {
   if (!c || !rc) return AST_E_ARG;
+  if (c->serve) return AST_E_BUSY;
   if (!c->space_set || !c->curve_set) return AST_E_STATE;
   const int G = rc->group_size;
   if (rc->n_groups <= 0 || G < 64 || G > 1024 || (G & 63) || rc->nstages <= 0 || rc->n_steps < 0 ||
       rc->eval_budget < 0)
     return AST_E_ARG;
-  if (c->n_data - c->n_params - c->n_filters <= 0) return AST_E_DATA;
-  (void)hipSetDevice(c->device);
-  return c->v.precision == AST_PREC_FP64 ? run_typed<double>(c, rc) : run_typed<float>(c, rc);
+  if (c->curve.n_data - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
+  RETIF(set_device(c));
+  const size_t fits = (size_t)rc->n_groups * G;
+  RETIF(ensure_streams(c, fits, rc->n_groups));
+  if (c->pool.nf < fits || c->pool.G != G) RETIF(pool_alloc(c->pool, fits, G));
+  // The reference path is one job that borrows the context's streams and bests.
+  if (c->ref_job) job_free(*c->ref_job);
+  c->ref_job.reset(new DevJob());
+  DevJob& j = *c->ref_job;
+  j.space = c->space;
+  j.rp.n_groups = rc->n_groups;
+  j.rp.group_size = G;
+  j.rp.nstages = rc->nstages;
+  j.rp.n_steps = rc->n_steps;
+  j.rp.eval_budget = rc->eval_budget;
+  j.rp.size_min = rc->size_min;
+  j.rp.seed = c->seed;
+  j.borrowed = true;
+  j.rng = c->rng;
+  j.best_f = c->best_f;
+  j.best_params = c->best_params;
+  j.best_dV = c->best_dV;
+  j.cls = class_of(c->v, c->curve.n_filters);
+  const hipStream_t s = c->streams[stream_of(j.cls)];
+  RETIF(job_upload(j, c->curve, c->params0, s));
+  // The context's streams and bests were cleared by synchronous-API calls on
+  // the null stream, which the job's non-blocking stream does not wait for.
+  HIPCHK(hipDeviceSynchronize());
+  for (int g = 0; g < rc->n_groups; ++g) j.slots.push_back(g);
+  double work = 0;
+  if (rc->ticks_per_launch > 0) {
+    work = (double)rc->ticks_per_launch * c->curve.n_data;
+  } else {  // about 100 ms of kernel per launch when the GPU is full
+    double tot = (double)fits * c->curve.n_data;
+    double ticks = 3.0e9 / (tot > 1 ? tot : 1);
+    ticks = ticks < 16 ? 16 : (ticks > 4096 ? 4096 : ticks);
+    work = ticks * c->curve.n_data;
+  }
+  RETIF(job_descs(j, work));
+  HIPCHK(job_launch(j, c->pool, 0, s));
+  c->last_ms = 0;
+  c->last_launches = 0;
+  // Every tick completes one evaluation of each running simplex; a stage
+  // needs at most (N+1) + n_steps*(N+2) of them.
+  const double per_stage = (double)(c->space.n_params + 1) + (double)rc->n_steps * (c->space.n_params + 2);
+  const double ticks_per_launch = (double)ticks_for(c->curve.n_data, work);
+  const long max_cycles = (long)((per_stage * rc->nstages + 16.0) / ticks_per_launch) + 4;
+  std::vector<DevJob*> jobs{&j};
+  for (long L = 0; j.groups_done < rc->n_groups; ++L) {
+    if (L > max_cycles) {
+      set_err("fit did not terminate");
+      return AST_E_STATE;
+    }
+    RETIF(run_cycle(c, jobs, &c->last_ms, &c->last_launches));
+  }
+  int64_t ev = 0;
+  RETIF(job_evals(c, j, &ev));
+  c->last_evals = ev;
+  c->last_points = ev * c->curve.n_data;
+  c->last_groups = rc->n_groups;
+  c->last_group_size = G;
+  return AST_OK;
 }
 
 int ast_fetch(ast_ctx* c, double* f, double* params, double* dV) {
   if (!c) return AST_E_ARG;
   if (!c->last_groups) return AST_E_STATE;
-  (void)hipSetDevice(c->device);
-  const int G = c->last_groups, N = c->n_params, NF = c->n_filters;
+  RETIF(set_device(c));
+  const int G = c->last_groups, N = c->space.n_params, NF = c->curve.n_filters;
   if (f) {
-    if (c->cf_size == 8) {
+    if (c->best_cf == 8) {
       HIPCHK(hipMemcpy(f, c->best_f, sizeof(double) * G, hipMemcpyDeviceToHost));
     } else {
       std::vector<float> tmp(G);
@@ -554,25 +1048,27 @@ int ast_fetch(ast_ctx* c, double* f, double* params, double* dV) {
 
 int ast_fetch_lanes(ast_ctx* c, double* f_last, int64_t* n_evals, int64_t* n_steps) {
   if (!c) return AST_E_ARG;
+  if (c->serve) return AST_E_BUSY;  // the slot pool belongs to the scheduler
   if (!c->last_groups) return AST_E_STATE;
-  (void)hipSetDevice(c->device);
+  RETIF(set_device(c));
   const size_t fits = (size_t)c->last_groups * c->last_group_size;
-  if (f_last) {
-    if (c->cf_size == 8) {
-      HIPCHK(hipMemcpy(f_last, c->job.sf, sizeof(double) * fits, hipMemcpyDeviceToHost));
+  if (f_last) {  // 8-byte cells; a float group fills the first half of its slot's cells
+    if (c->best_cf == 8) {
+      HIPCHK(hipMemcpy(f_last, c->pool.sf, sizeof(double) * fits, hipMemcpyDeviceToHost));
     } else {
-      std::vector<float> tmp(fits);
-      HIPCHK(hipMemcpy(tmp.data(), c->job.sf, sizeof(float) * fits, hipMemcpyDeviceToHost));
-      for (size_t k = 0; k < fits; ++k) f_last[k] = tmp[k];
+      const size_t G = (size_t)c->last_group_size;
+      std::vector<float> tmp(2 * fits);
+      HIPCHK(hipMemcpy(tmp.data(), c->pool.sf, sizeof(float) * 2 * fits, hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < fits; ++k) f_last[k] = tmp[2 * G * (k / G) + k % G];
     }
   }
   std::vector<int32_t> tmp(fits);
   if (n_evals) {
-    HIPCHK(hipMemcpy(tmp.data(), c->job.nev_tot, sizeof(int32_t) * fits, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(tmp.data(), c->pool.nev_tot, sizeof(int32_t) * fits, hipMemcpyDeviceToHost));
     for (size_t k = 0; k < fits; ++k) n_evals[k] = tmp[k];
   }
   if (n_steps) {
-    HIPCHK(hipMemcpy(tmp.data(), c->job.steps_tot, sizeof(int32_t) * fits, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(tmp.data(), c->pool.steps_tot, sizeof(int32_t) * fits, hipMemcpyDeviceToHost));
     for (size_t k = 0; k < fits; ++k) n_steps[k] = tmp[k];
   }
   return AST_OK;
@@ -580,33 +1076,38 @@ int ast_fetch_lanes(ast_ctx* c, double* f_last, int64_t* n_evals, int64_t* n_ste
 
 int ast_chi2_batch(ast_ctx* c, const double* params, int n, double* chi2, double* dV) {
   if (!c || !params || n <= 0 || !chi2 || !dV) return AST_E_ARG;
+  if (c->serve) return AST_E_BUSY;  // its streams and events belong to the scheduler
   if (!c->space_set || !c->curve_set) return AST_E_STATE;
-  if (c->n_data - c->n_params - c->n_filters <= 0) return AST_E_DATA;
-  (void)hipSetDevice(c->device);
+  if (c->curve.n_data - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
+  RETIF(set_device(c));
   double *pd = nullptr, *cd = nullptr, *dd = nullptr;
-  int rc = dev_alloc(&pd, (size_t)n * c->n_params);
+  int rc = dev_alloc(&pd, (size_t)n * c->space.n_params);
   if (!rc) rc = dev_alloc(&cd, (size_t)n);
-  if (!rc) rc = dev_alloc(&dd, (size_t)n * c->n_filters);
+  if (!rc) rc = dev_alloc(&dd, (size_t)n * c->curve.n_filters);
   if (rc) { dev_free(pd); dev_free(cd); dev_free(dd); return rc; }
   int result = AST_OK;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  hipError_t he = hipMemcpy(pd, params, sizeof(double) * n * c->n_params, hipMemcpyHostToDevice);
-  if (he == hipSuccess) he = hipEventCreate(&e0);
-  if (he == hipSuccess) he = hipEventCreate(&e1);
+  hipStream_t s = c->streams[0];
+  hipError_t he = hipMemcpy(pd, params, sizeof(double) * n * c->space.n_params, hipMemcpyHostToDevice);
   LaunchArgs a{};
   a.op = 2;
+  a.stream = s;
+  a.pts = c->pts;
+  a.sum_w = c->sum_w;
+  a.n_data = c->curve.n_data;
+  a.n_filters = c->curve.n_filters;
+  a.dof = c->curve.n_data - c->space.n_params - c->curve.n_filters;
   a.params = pd;
   a.n = n;
   a.chi2 = cd;
   a.dV = dd;
-  if (he == hipSuccess) he = hipEventRecord(e0, c->stream);
-  if (he == hipSuccess) he = dispatch(c, a);
-  if (he == hipSuccess) he = hipEventRecord(e1, c->stream);
-  if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+  if (he == hipSuccess) he = hipEventRecord(c->ev0[0], s);
+  if (he == hipSuccess) he = dispatch(c->v, a);
+  if (he == hipSuccess) he = hipEventRecord(c->ev1[0], s);
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
   float ms = 0;
-  if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+  if (he == hipSuccess) he = hipEventElapsedTime(&ms, c->ev0[0], c->ev1[0]);
   if (he == hipSuccess) he = hipMemcpy(chi2, cd, sizeof(double) * n, hipMemcpyDeviceToHost);
-  if (he == hipSuccess) he = hipMemcpy(dV, dd, sizeof(double) * n * c->n_filters, hipMemcpyDeviceToHost);
+  if (he == hipSuccess) he = hipMemcpy(dV, dd, sizeof(double) * n * c->curve.n_filters, hipMemcpyDeviceToHost);
   if (he != hipSuccess) {
     snprintf(g_hip_err, sizeof g_hip_err, "%s (chi2 batch)", hipGetErrorString(he));
     result = AST_E_HIP;
@@ -614,10 +1115,8 @@ int ast_chi2_batch(ast_ctx* c, const double* params, int n, double* chi2, double
     c->last_ms = ms;
     c->last_launches = 1;
     c->last_evals = n;
-    c->last_points = (int64_t)n * c->n_data;
+    c->last_points = (int64_t)n * c->curve.n_data;
   }
-  if (e0) (void)hipEventDestroy(e0);
-  if (e1) (void)hipEventDestroy(e1);
   dev_free(pd);
   dev_free(cd);
   dev_free(dd);
@@ -635,6 +1134,114 @@ int ast_last_counts(ast_ctx* c, int64_t* evals, int64_t* model_points) {
   if (!c) return AST_E_ARG;
   if (evals) *evals = c->last_evals;
   if (model_points) *model_points = c->last_points;
+  return AST_OK;
+}
+
+// ------------------------------------------------------------------ serving
+int ast_serve_start(ast_ctx* c, const ast_serve_cfg* cfg) {
+  if (!c || !cfg) return AST_E_ARG;
+  if (c->serve) return AST_E_STATE;
+  const int G = cfg->group_size ? cfg->group_size : 256;
+  if (G < 64 || G > 1024 || (G & 63) || cfg->max_resident_fits < G) return AST_E_ARG;
+  RETIF(set_device(c));
+  size_t fits = (size_t)(cfg->max_resident_fits / G) * G;
+  RETIF(pool_alloc(c->pool, fits, G));
+  c->serve.reset(new Serve());
+  c->serve->cfg = *cfg;
+  c->serve->cfg.group_size = G;
+  if (c->serve->cfg.work_points_per_launch <= 0) c->serve->cfg.work_points_per_launch = 8000;
+  c->serve->th = std::thread(serve_loop, c);
+  return AST_OK;
+}
+
+int ast_submit(ast_ctx* c, const ast_job* job, uint64_t* id) {
+  if (!c || !job || !id) return AST_E_ARG;
+  if (!c->serve) return AST_E_STATE;
+  Serve& S = *c->serve;
+  const int G = S.cfg.group_size;
+  if (job->n_fits <= 0 || job->n_fits % G || job->nstages <= 0 || job->n_steps < 0 || job->eval_budget < 0)
+    return AST_E_ARG;
+  if (job->n_fits / G > c->pool.n_slots) return AST_E_NOMEM;
+  std::unique_ptr<DevJob> j(new DevJob());
+  RETIF(make_space(job->variant, job->n_params, job->property, job->n_types, job->types, job->limits, &job->x2,
+                   &j->space));
+  CurveH C;
+  RETIF(make_curve(job->variant, job->obs, job->n_data, job->n_filters, job->eph, &C));
+  if (C.n_data - j->space.n_params - C.n_filters <= 0) return AST_E_DATA;
+  j->rp.n_groups = job->n_fits / G;
+  j->rp.group_size = G;
+  j->rp.nstages = job->nstages;
+  j->rp.n_steps = job->n_steps;
+  j->rp.eval_budget = job->eval_budget;
+  j->rp.size_min = job->size_min;
+  j->rp.seed = job->seed;
+  j->tag = job->tag;
+  j->cls = class_of(job->variant, job->n_filters);
+  j->n_data = C.n_data;
+  j->n_filters = C.n_filters;
+  j->t_submit = Clock::now();
+  std::vector<double> p0;
+  if (job->params0) p0.assign(job->params0, job->params0 + j->space.n_params);
+  std::lock_guard<std::mutex> lk(S.mu);
+  j->id = S.next_id++;
+  *id = j->id;
+  S.inputs[j->id] = std::make_pair(std::move(C), std::move(p0));
+  S.pending.push_back(std::move(j));
+  S.cv_submit.notify_all();
+  return AST_OK;
+}
+
+int ast_poll(ast_ctx* c, ast_result* out, int max, int* n_out, int timeout_ms) {
+  if (!c || !out || max <= 0 || !n_out) return AST_E_ARG;
+  if (!c->serve) return AST_E_STATE;
+  Serve& S = *c->serve;
+  std::unique_lock<std::mutex> lk(S.mu);
+  if (S.done.empty() && timeout_ms > 0)
+    S.cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return !S.done.empty(); });
+  int n = 0;
+  while (n < max && !S.done.empty()) {
+    out[n++] = S.done.front();
+    S.done.pop_front();
+  }
+  *n_out = n;
+  return S.error ? S.error : AST_OK;
+}
+
+int ast_cancel(ast_ctx* c, uint64_t id) {
+  if (!c) return AST_E_ARG;
+  if (!c->serve) return AST_E_STATE;
+  std::lock_guard<std::mutex> lk(c->serve->mu);
+  c->serve->cancels.push_back(id);
+  c->serve->cv_submit.notify_all();
+  return AST_OK;
+}
+
+int ast_serve_stats(ast_ctx* c, ast_serve_stats_t* st) {
+  if (!c || !st) return AST_E_ARG;
+  if (!c->serve) return AST_E_STATE;
+  Serve& S = *c->serve;
+  std::lock_guard<std::mutex> lk(S.mu);
+  st->pending = (int)S.pending.size();
+  st->resident = (int)S.resident.size();
+  st->free_fits = (int64_t)c->pool.free_slots.size() * S.cfg.group_size;
+  st->resident_bytes = S.resident_bytes;
+  st->launches = S.launches;
+  st->kernel_ms = S.kernel_ms;
+  st->evals = S.evals;
+  st->model_points = S.points;
+  return AST_OK;
+}
+
+int ast_serve_stop(ast_ctx* c) {
+  if (!c) return AST_E_ARG;
+  if (!c->serve) return AST_E_STATE;
+  {
+    std::lock_guard<std::mutex> lk(c->serve->mu);
+    c->serve->stop = true;
+    c->serve->cv_submit.notify_all();
+  }
+  if (c->serve->th.joinable()) c->serve->th.join();
+  c->serve.reset();
   return AST_OK;
 }
 

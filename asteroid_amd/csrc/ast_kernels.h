@@ -42,45 +42,61 @@ __device__ inline bool phase_pending(int ph) {
   return ph == PH_INIT || ph == PH_REFLECT || ph == PH_EXPAND || ph == PH_CONTRACT || ph == PH_SHRINK;
 }
 
-// Device buffers of one fitting job (n_fits simplexes in n_groups groups).
+// Simplex state of the resident simplexes ("slots"), lane-fastest.  A group
+// of G simplexes owns slot fits [slot*G, slot*G + G) while it runs; nf is the
+// slot capacity.  Element (row, fit) of an array of fixed type (IND, PRM and
+// the int32 counters) lives at row * nf + fit.  The arrays of simplex floats
+// (CF) are made of 8-byte cells so that float and double groups can share the
+// pool side by side: with KC = 8 / sizeof(CF), element (row, fit = slot*G + t)
+// lives at CF index row * (KC*nf) + KC*slot*G + t, i.e. a float group packs
+// its G values into the first half of its slot's cells of the row.  Every
+// access of a wavefront stays one contiguous segment.
 template <class CF>
-struct FitBufs {
-  CF* X;        // [(N+1)*N][n_fits]  simplex vertices
-  CF* F;        // [N+1][n_fits]      chi^2 at the vertices
-  uint8_t* IND; // [N+1][n_fits]      sorted vertex indices
-  CF* X0;       // [N][n_fits]        centroid
-  CF* XR;       // [N][n_fits]        reflected point
-  CF* XP;       // [N][n_fits]        expanded / contracted point
-  double* PRM;  // [N][n_fits]        the thread's params[] buffer (cuda.c:1344)
-  CF* DV;       // [MAXF][n_fits]     delta_V of the last evaluation
+struct SlotBufs {
+  CF* X;        // [(N+1)*N]  simplex vertices
+  CF* F;        // [N+1]      chi^2 at the vertices
+  uint8_t* IND; // [N+1]      sorted vertex indices
+  CF* X0;       // [N]        centroid
+  CF* XR;       // [N]        reflected point
+  CF* XP;       // [N]        expanded / contracted point
+  double* PRM;  // [N]        the thread's params[] buffer (cuda.c:1344)
+  CF* DV;       // [MAXF]     delta_V of the last evaluation
   int32_t* ctl;       // phase | j<<4 | failed<<9 | bad<<10
   int32_t* lstep;     // simplex steps in this stage (l, cuda.c:1436)
   int32_t* nev;       // chi^2 evaluations in this stage
   int32_t* nev_tot;   // evaluations, all stages of this job
   int32_t* steps_tot; // steps, all stages of this job
-  uint32_t* rng;      // draws taken from the simplex's stream
   CF* fr;             // f_r kept for the expansion test
   CF* sf;             // stage-final best chi^2 (s_f, cuda.c:1739-1742)
-  int32_t* g_reopt;   // per group: s_x2_params.reopt
-  int32_t* g_stage;   // per group: current stage
-  int32_t* g_done;    // per group: all stages finished
-  CF* g_best_f;       // d_f
-  double* g_best_params;  // d_params [n_groups][N]
-  double* g_best_dV;      // d_dV [n_groups][n_filters]
-  int32_t* active_groups; // counter: groups still running after a launch
+  int32_t* g_reopt;   // per group slot: s_x2_params.reopt
+  int32_t* g_stage;   // per group slot: current stage
+  int32_t* g_done;    // per group slot: all stages finished
+  int32_t* running;   // counter: groups still running after a launch
+  size_t nf;          // slot capacity (fits)
 };
 
+// One group of one job, as a launch sees it (read once per workgroup,
+// wave-uniform).  The job's curve, model space, start model, random streams
+// and per-group best arrays are the job's own, so where a group runs (its
+// slot, its neighbours in the launch) does not change its results.
 template <class CF>
-struct FitCfg {
-  int n_fits;
-  int nstages;
-  int n_steps;
-  int eval_budget;   // 0: none
-  CF size2_min;
+struct GroupDesc {
+  const PointRec* pts;
+  const double* sum_w;
+  const Space<CF>* sp;
+  const double* params0;
+  uint32_t* rng;          // the job's stream counters, by job fit index
+  CF* best_f;             // d_f of the job, by group
+  double* best_params;    // d_params, [group][N]
+  double* best_dV;        // d_dV, [group][n_filters]
+  int32_t* job_done;      // groups of the job that finished
   uint64_t seed;
-  int ticks;
+  CF size2_min;
   int n_data, n_filters, dof;
-  int reopt;         // job-level reopt (init kernel)
+  int nstages, n_steps, eval_budget, reopt;
+  int slot;               // group slot
+  int group_in_job;
+  int ticks;              // evaluation rounds in this launch
 };
 
 template <class CF>
@@ -90,22 +106,32 @@ struct LaneRegs {
   CF fr, sf;
 };
 
+// Per-stage settings a lane needs while advancing (copied out of the descriptor).
+template <class CF>
+struct StageCfg {
+  uint64_t seed;
+  CF size2_min;
+  int n_steps, eval_budget;
+};
+
 template <class CF, bool TQ, bool BC, bool BW>
 struct Machine {
   using P = Pos<TQ, BC, BW>;
   static constexpr int N = P::N;
   const Space<CF>& sp;
-  const FitBufs<CF>& B;
-  const FitCfg<CF>& C;
-  size_t fit, nf;
+  const SlotBufs<CF>& B;
+  const StageCfg<CF>& C;
+  size_t fit, nf;   // slot fit index and slot capacity (fixed-type arrays)
+  size_t fo, rs;    // the same element in the CF arrays: offset and row stride
+  uint64_t jfit;    // fit index within the job (random stream)
   int reopt;
 
-  __device__ CF& X(int v, int i) const { return B.X[((size_t)v * N + i) * nf + fit]; }
-  __device__ CF& F(int v) const { return B.F[(size_t)v * nf + fit]; }
+  __device__ CF& X(int v, int i) const { return B.X[((size_t)v * N + i) * rs + fo]; }
+  __device__ CF& F(int v) const { return B.F[(size_t)v * rs + fo]; }
   __device__ uint8_t& IND(int v) const { return B.IND[(size_t)v * nf + fit]; }
-  __device__ CF& X0(int i) const { return B.X0[(size_t)i * nf + fit]; }
-  __device__ CF& XR(int i) const { return B.XR[(size_t)i * nf + fit]; }
-  __device__ CF& XP(int i) const { return B.XP[(size_t)i * nf + fit]; }
+  __device__ CF& X0(int i) const { return B.X0[(size_t)i * rs + fo]; }
+  __device__ CF& XR(int i) const { return B.XR[(size_t)i * rs + fo]; }
+  __device__ CF& XP(int i) const { return B.XP[(size_t)i * rs + fo]; }
   __device__ double& PRM(int i) const { return B.PRM[(size_t)i * nf + fit]; }
 
   // x2params on a point, through the persistent params[] buffer.
@@ -137,7 +163,7 @@ struct Machine {
     return propose(xv);
   }
 
-  __device__ float rnd(LaneRegs<CF>& R) const { return astrng::uniform01(C.seed, fit, R.rng++); }
+  __device__ float rnd(LaneRegs<CF>& R) const { return astrng::uniform01(C.seed, jfit, R.rng++); }
 
   // Stage start (cuda.c:1456-1546): x[0] random or around the reoptimised
   // point, then the N displaced vertices.
@@ -379,38 +405,39 @@ struct Machine {
 };
 
 // --------------------------------------------------------------- init kernel
-// Job start: identity sort order, stage-0 start point (params2x of the
-// reoptimisation model, cuda.c:1409-1416), counters, per-group flags.
+// A group's start (one workgroup of G threads per new group): identity sort
+// order, stage-0 start point (params2x of the reoptimisation model,
+// cuda.c:1409-1416), counters, group flags.
 template <class CF, bool TQ, bool BC, bool BW>
-__global__ void fit_init_kernel(Space<CF> sp, FitBufs<CF> B, FitCfg<CF> C, const double* params0, int n_groups,
-                                int group_size) {
+__global__ void fit_init_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
   using P = Pos<TQ, BC, BW>;
   constexpr int N = P::N;
-  const size_t nf = C.n_fits;
-  const size_t fit = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (fit < nf) {
-    for (int v = 0; v <= N; ++v) B.IND[v * nf + fit] = (uint8_t)v;
-    double prm[N];
-    for (int i = 0; i < N; ++i) prm[i] = C.reopt ? params0[i] : 0.0;
-    for (int i = 0; i < N; ++i) B.PRM[i * nf + fit] = prm[i];
-    if (C.reopt) {
-      CF x[N];
-      params2x<CF, TQ, BC, BW>(x, prm, sp);
-      for (int i = 0; i < N; ++i) B.X[(size_t)i * nf + fit] = x[i];
-    }
-    for (int m = 0; m < MAXF; ++m) B.DV[m * nf + fit] = 0;
-    B.ctl[fit] = PH_START;
-    B.lstep[fit] = 0;
-    B.nev[fit] = 0;
-    B.nev_tot[fit] = 0;
-    B.steps_tot[fit] = 0;
-    B.fr[fit] = 0;
-    B.sf[fit] = (CF)1e30;
-  }
-  if (fit < (size_t)n_groups) {
-    B.g_reopt[fit] = C.reopt;
-    B.g_stage[fit] = 0;
-    B.g_done[fit] = 0;
+  constexpr int KC = 8 / sizeof(CF);
+  const GroupDesc<CF>& D = descs[blockIdx.x];
+  const size_t nf = B.nf, rs = KC * nf;
+  const size_t fit = (size_t)D.slot * blockDim.x + threadIdx.x;
+  const size_t fo = (size_t)D.slot * blockDim.x * KC + threadIdx.x;
+  for (int v = 0; v <= N; ++v) B.IND[v * nf + fit] = (uint8_t)v;
+  double prm[N];
+  for (int i = 0; i < N; ++i) prm[i] = D.reopt ? D.params0[i] : 0.0;
+  for (int i = 0; i < N; ++i) B.PRM[i * nf + fit] = prm[i];
+  CF x[N];
+  if (D.reopt) params2x<CF, TQ, BC, BW>(x, prm, *D.sp);
+  else
+    for (int i = 0; i < N; ++i) x[i] = 0;
+  for (int i = 0; i < N; ++i) B.X[(size_t)i * rs + fo] = x[i];
+  for (int m = 0; m < MAXF; ++m) B.DV[m * rs + fo] = 0;
+  B.ctl[fit] = PH_START;
+  B.lstep[fit] = 0;
+  B.nev[fit] = 0;
+  B.nev_tot[fit] = 0;
+  B.steps_tot[fit] = 0;
+  B.fr[fo] = 0;
+  B.sf[fo] = (CF)1e30;
+  if (threadIdx.x == 0) {
+    B.g_reopt[D.slot] = D.reopt;
+    B.g_stage[D.slot] = 0;
+    B.g_done[D.slot] = 0;
   }
 }
 
@@ -419,8 +446,8 @@ __global__ void fit_init_kernel(Space<CF> sp, FitBufs<CF> B, FitCfg<CF> C, const
 // the evaluator's register allocation does not compete with the simplex
 // bookkeeping live across it.
 template <class CF, bool TQ, bool BC, bool BW, int NF>
-__device__ __attribute__((noinline)) CF eval_lane(const double* PRM, CF* DV, size_t nf, size_t fit,
-                                                   const PointRec* pts_g, int n_data, int n_filters, int dof,
+__device__ __attribute__((noinline)) CF eval_lane(const double* PRM, CF* DV, size_t nf, size_t fit, size_t rs,
+                                                   size_t fo, const PointRec* pts_g, int n_data, int n_filters, int dof,
                                                    const double* sum_w) {
   constexpr int N = Pos<TQ, BC, BW>::N;
   // Arguments of a device function arrive in vector registers; the table
@@ -440,28 +467,36 @@ __device__ __attribute__((noinline)) CF eval_lane(const double* PRM, CF* DV, siz
   CF fv = chi2_model<CF, TQ, BC, BW, NF>(prm, pts, n_data, n_filters, dof, sum_w, dv);
 #pragma unroll
   for (int m = 0; m < NF; ++m)
-    if (m < n_filters) DV[(size_t)m * nf + fit] = dv[m];
+    if (m < n_filters) DV[(size_t)m * rs + fo] = dv[m];
   return fv;
 }
 
 // ---------------------------------------------------------------- fit kernel
+// One workgroup = one group (the reference's block); it runs D.ticks rounds
+// of "evaluate the pending points, advance every lane", then saves its state.
 template <class CF, bool TQ, bool BC, bool BW, int NF>
-__global__ __launch_bounds__(1024) void fit_kernel(Space<CF> sp, const PointRec* pts_g, const double* sum_w,
-                                                   FitBufs<CF> B, FitCfg<CF> C) {
+__global__ __launch_bounds__(1024) void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
   using P = Pos<TQ, BC, BW>;
   constexpr int N = P::N;
   __shared__ CF s_f[1024];
   __shared__ CF s_x0[MAXP];
   __shared__ int s_tm;
   __shared__ CF s_smin;
-  const int g = blockIdx.x;
+  constexpr int KC = 8 / sizeof(CF);
+  const GroupDesc<CF>& D = descs[blockIdx.x];
   const int t = threadIdx.x;
   const int G = blockDim.x;
-  const size_t nf = C.n_fits;
-  const size_t fit = (size_t)g * G + t;
-  if (B.g_done[g]) return;  // same value for every thread of the group
-  int reopt = B.g_reopt[g];
-  int stage = B.g_stage[g];
+  const int slot = D.slot;
+  const size_t nf = B.nf, rs = KC * nf;
+  const size_t fit = (size_t)slot * G + t;
+  const size_t fo = (size_t)slot * G * KC + t;
+  const uint64_t jfit = (uint64_t)D.group_in_job * G + t;
+  if (B.g_done[slot]) return;  // same value for every thread of the group
+  const Space<CF>& sp = *D.sp;
+  const StageCfg<CF> SC{D.seed, D.size2_min, D.n_steps, D.eval_budget};
+  const int nstages = D.nstages, ticks = D.ticks;
+  int reopt = B.g_reopt[slot];
+  int stage = B.g_stage[slot];
   LaneRegs<CF> R;
   const int ctl = B.ctl[fit];
   R.phase = ctl & 15;
@@ -472,18 +507,19 @@ __global__ __launch_bounds__(1024) void fit_kernel(Space<CF> sp, const PointRec*
   R.nev = B.nev[fit];
   R.nev_tot = B.nev_tot[fit];
   R.steps_tot = B.steps_tot[fit];
-  R.rng = B.rng[fit];
-  R.fr = B.fr[fit];
-  R.sf = B.sf[fit];
+  R.rng = D.rng[jfit];
+  R.fr = B.fr[fo];
+  R.sf = B.sf[fo];
   bool done = false;
   {
-    Machine<CF, TQ, BC, BW> M{sp, B, C, fit, nf, reopt};
+    Machine<CF, TQ, BC, BW> M{sp, B, SC, fit, nf, fo, rs, jfit, reopt};
     M.advance(R, (CF)0, false);  // a lane at a stage start draws its simplex first
   }
-  for (int tick = 0; tick < C.ticks && !done; ++tick) {
-    Machine<CF, TQ, BC, BW> M{sp, B, C, fit, nf, reopt};
+  for (int tick = 0; tick < ticks && !done; ++tick) {
+    Machine<CF, TQ, BC, BW> M{sp, B, SC, fit, nf, fo, rs, jfit, reopt};
     if (phase_pending(R.phase)) {
-      CF fv = eval_lane<CF, TQ, BC, BW, NF>(B.PRM, B.DV, nf, fit, pts_g, C.n_data, C.n_filters, C.dof, sum_w);
+      CF fv = eval_lane<CF, TQ, BC, BW, NF>(B.PRM, B.DV, nf, fit, rs, fo, D.pts, D.n_data, D.n_filters, D.dof,
+                                            D.sum_w);
       R.nev++;
       R.nev_tot++;
       M.advance(R, fv, true);
@@ -504,7 +540,7 @@ __global__ __launch_bounds__(1024) void fit_kernel(Space<CF> sp, const PointRec*
     __syncthreads();
     const int tm = s_tm;
     const CF smin = s_smin;
-    if (C.nstages > 1 && stage < C.nstages - 1) {
+    if (nstages > 1 && stage < nstages - 1) {
       if (t == tm) {
         CF xb[N];
         const int b = M.IND(0);
@@ -528,11 +564,12 @@ __global__ __launch_bounds__(1024) void fit_kernel(Space<CF> sp, const PointRec*
 #pragma unroll
       for (int i = 0; i < N; ++i) M.X(0, i) = s_x0[i];
       R.phase = PH_START;
-      Machine<CF, TQ, BC, BW> M2{sp, B, C, fit, nf, reopt};
+      Machine<CF, TQ, BC, BW> M2{sp, B, SC, fit, nf, fo, rs, jfit, reopt};
       M2.advance(R, (CF)0, false);
     } else {
-      if (t == tm && smin < B.g_best_f[g]) {  // cuda.c:1810-1823
-        B.g_best_f[g] = smin;
+      const int g = D.group_in_job;
+      if (t == tm && smin < D.best_f[g]) {  // cuda.c:1810-1823
+        D.best_f[g] = smin;
         CF xb[N];
         double prm[N];
         const int b = M.IND(0);
@@ -542,10 +579,10 @@ __global__ __launch_bounds__(1024) void fit_kernel(Space<CF> sp, const PointRec*
 #pragma unroll
         for (int i = 0; i < N; ++i) {
           M.PRM(i) = prm[i];
-          B.g_best_params[(size_t)g * N + i] = prm[i];
+          D.best_params[(size_t)g * N + i] = prm[i];
         }
-        for (int m = 0; m < C.n_filters; ++m)
-          B.g_best_dV[(size_t)g * C.n_filters + m] = B.DV[(size_t)m * nf + fit];
+        for (int m = 0; m < D.n_filters; ++m)
+          D.best_dV[(size_t)g * D.n_filters + m] = B.DV[(size_t)m * rs + fo];
       }
       stage++;
       R.phase = PH_FIT_DONE;
@@ -557,14 +594,15 @@ __global__ __launch_bounds__(1024) void fit_kernel(Space<CF> sp, const PointRec*
   B.nev[fit] = R.nev;
   B.nev_tot[fit] = R.nev_tot;
   B.steps_tot[fit] = R.steps_tot;
-  B.rng[fit] = R.rng;
-  B.fr[fit] = R.fr;
-  B.sf[fit] = R.sf;
+  D.rng[jfit] = R.rng;
+  B.fr[fo] = R.fr;
+  B.sf[fo] = R.sf;
   if (t == 0) {
-    B.g_reopt[g] = reopt;
-    B.g_stage[g] = stage;
-    B.g_done[g] = done ? 1 : 0;
-    if (!done) atomicAdd(B.active_groups, 1);
+    B.g_reopt[slot] = reopt;
+    B.g_stage[slot] = stage;
+    B.g_done[slot] = done ? 1 : 0;
+    if (done) atomicAdd(D.job_done, 1);
+    else atomicAdd(B.running, 1);
   }
 }
 
@@ -589,21 +627,17 @@ __global__ __launch_bounds__(256) void chi2_batch_kernel(const PointRec* pts_g, 
 
 // --------------------------------------------------------- launch interface
 struct LaunchArgs {
-  int op;  // 0 init, 1 fit, 2 chi2 batch
+  int op;  // 0 init groups, 1 fit ticks, 2 chi2 batch
   int precision;
   hipStream_t stream;
-  // model space (CF-typed copies for float and double simplexes)
-  const Space<double>* sp_d;
-  const Space<float>* sp_f;
+  // fit: descriptors of the groups of this launch (device), their count, G
+  const void* descs;   // GroupDesc<CF>*
+  const void* bufs;    // SlotBufs<CF>*
+  int n_groups, group_size, n_filters;
+  // batch
   const PointRec* pts;
   const double* sum_w;
-  int n_data, n_filters, dof;
-  // fit
-  const void* bufs;   // FitBufs<CF>*
-  const void* cfg;    // FitCfg<CF>*
-  const double* params0;
-  int n_groups, group_size;
-  // batch
+  int n_data, dof;
   const double* params;
   int n;
   double* chi2;
@@ -611,16 +645,13 @@ struct LaunchArgs {
 };
 
 template <class CF, bool TQ, bool BC, bool BW, int NF>
-hipError_t launch_nf(const LaunchArgs& a, const Space<CF>& sp) {
+hipError_t launch_nf(const LaunchArgs& a) {
   if (a.op == 0) {
-    const FitCfg<CF>& C = *(const FitCfg<CF>*)a.cfg;
-    int nthreads = C.n_fits > a.n_groups ? C.n_fits : a.n_groups;
-    int blocks = (nthreads + 255) / 256;
-    hipLaunchKernelGGL((fit_init_kernel<CF, TQ, BC, BW>), dim3(blocks), dim3(256), 0, a.stream, sp,
-                       *(const FitBufs<CF>*)a.bufs, C, a.params0, a.n_groups, a.group_size);
+    hipLaunchKernelGGL((fit_init_kernel<CF, TQ, BC, BW>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
+                       (const GroupDesc<CF>*)a.descs, *(const SlotBufs<CF>*)a.bufs);
   } else if (a.op == 1) {
-    hipLaunchKernelGGL((fit_kernel<CF, TQ, BC, BW, NF>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream, sp,
-                       a.pts, a.sum_w, *(const FitBufs<CF>*)a.bufs, *(const FitCfg<CF>*)a.cfg);
+    hipLaunchKernelGGL((fit_kernel<CF, TQ, BC, BW, NF>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
+                       (const GroupDesc<CF>*)a.descs, *(const SlotBufs<CF>*)a.bufs);
   } else {
     int blocks = (a.n + 255) / 256;
     hipLaunchKernelGGL((chi2_batch_kernel<CF, TQ, BC, BW, NF>), dim3(blocks), dim3(256), 0, a.stream, a.pts,
@@ -630,16 +661,16 @@ hipError_t launch_nf(const LaunchArgs& a, const Space<CF>& sp) {
 }
 
 template <class CF, bool TQ, bool BC, bool BW>
-hipError_t launch_cf(const LaunchArgs& a, const Space<CF>& sp) {
-  if (a.n_filters <= 1) return launch_nf<CF, TQ, BC, BW, 1>(a, sp);
-  if (a.n_filters <= 4) return launch_nf<CF, TQ, BC, BW, 4>(a, sp);
-  return launch_nf<CF, TQ, BC, BW, MAXF>(a, sp);
+hipError_t launch_cf(const LaunchArgs& a) {
+  if (a.n_filters <= 1) return launch_nf<CF, TQ, BC, BW, 1>(a);
+  if (a.n_filters <= 4) return launch_nf<CF, TQ, BC, BW, 4>(a);
+  return launch_nf<CF, TQ, BC, BW, MAXF>(a);
 }
 
 template <bool TQ, bool BC, bool BW>
 hipError_t launch_variant(const LaunchArgs& a) {
-  if (a.precision == AST_PREC_FP64) return launch_cf<double, TQ, BC, BW>(a, *a.sp_d);
-  return launch_cf<float, TQ, BC, BW>(a, *a.sp_f);
+  if (a.precision == AST_PREC_FP64) return launch_cf<double, TQ, BC, BW>(a);
+  return launch_cf<float, TQ, BC, BW>(a);
 }
 
 }  // namespace ast

@@ -51,7 +51,9 @@ enum {
   AST_E_NOMEM = -4,      /* device or host allocation failed              */
   AST_E_DATA = -5,       /* input data rejected (unsorted times, too many filters, outside ephemeris span) */
   AST_E_UNSUPPORTED = -6,/* variant combination not built                 */
-  AST_E_NODEVICE = -7    /* no HIP device                                 */
+  AST_E_NODEVICE = -7,   /* no HIP device                                 */
+  AST_E_CANCELLED = -8,  /* job cancelled before it finished              */
+  AST_E_BUSY = -9        /* context is serving: ast_run, ast_chi2_batch and ast_fetch_lanes are refused until ast_serve_stop */
 };
 
 /* Arithmetic forms of the evaluator and simplex (CHI_FLOAT, asteroid.h:24-28). */
@@ -226,6 +228,78 @@ int ast_last_counts(ast_ctx* ctx, int64_t* evals, int64_t* model_points);
 int ast_default_model_space(const ast_variant* variant, const double* ppsi_hours,
                             int* n_params, int* property, int* n_types, int* types,
                             double* limits, ast_x2* x2);
+
+/* ------------------------------------------------------------ serving
+ * No counterpart in the reference, whose host loop runs one kernel at a time
+ * (asteroid.c:655-825).  A serving context admits fit jobs as they arrive,
+ * each with its own light curve, model space, variant, seed and number of
+ * simplex starts, while earlier jobs are still on the GPU: a scheduler thread
+ * places each job's groups into free simplex slots when its HBM footprint
+ * fits, runs one launch per kernel class over everything resident (continuous
+ * batching) and retires jobs as their groups finish.  A job's results depend
+ * only on the job (its own random streams, indexed by its own simplex
+ * numbers), not on what runs beside it.  Submission, polling and
+ * cancellation are thread-safe. */
+typedef struct {
+  int64_t max_resident_fits;     /* simplex slots on the device                  */
+  int group_size;                /* simplexes per group (workgroup); 0 = 256     */
+  int64_t hbm_budget_bytes;      /* admission limit on resident job bytes; 0 = slots only */
+  double work_points_per_launch; /* model-point evaluations per simplex per launch; 0 = 8000 */
+} ast_serve_cfg;
+
+typedef struct {
+  ast_variant variant;
+  int n_params;
+  const int* property;      /* as ast_set_model_space */
+  int n_types;
+  const int* types;
+  const double* limits;
+  ast_x2 x2;                /* x2.reopt = 1 starts from params0 */
+  const ast_obs* obs;       /* as ast_upload_curve */
+  int n_data;
+  int n_filters;
+  const ast_ephem3* eph;
+  int n_fits;               /* simplex starts, a multiple of the group size */
+  int nstages;
+  int n_steps;
+  int64_t eval_budget;
+  double size_min;
+  uint64_t seed;
+  const double* params0;    /* start model (reoptimisation), or NULL */
+  uint64_t tag;             /* returned with the result */
+} ast_job;
+
+typedef struct {
+  uint64_t id;
+  uint64_t tag;
+  int status;               /* AST_OK, AST_E_CANCELLED or an error */
+  int n_params, n_filters, n_groups;
+  int best_group;
+  double best_f;            /* best chi^2 over the job's groups */
+  double best_params[AST_MAX_PARAMS];
+  double best_dV[AST_MAX_FILTERS];
+  int64_t n_evals;          /* chi^2 evaluations */
+  int64_t model_points;     /* evaluations times observations */
+  double submit_to_done_ms; /* latency */
+  double queue_ms;          /* submit to admission */
+  double run_ms;            /* wall time of the cycles the job took part in */
+} ast_result;
+
+typedef struct {
+  int pending, resident;
+  int64_t free_fits, resident_bytes, launches, evals, model_points;
+  double kernel_ms;
+} ast_serve_stats_t;
+
+int ast_serve_start(ast_ctx* ctx, const ast_serve_cfg* cfg);
+/* Copies the job's inputs; returns its id. */
+int ast_submit(ast_ctx* ctx, const ast_job* job, uint64_t* id);
+/* Up to `max` finished jobs; waits up to timeout_ms for the first one. */
+int ast_poll(ast_ctx* ctx, ast_result* out, int max, int* n_out, int timeout_ms);
+int ast_cancel(ast_ctx* ctx, uint64_t id);
+int ast_serve_stats(ast_ctx* ctx, ast_serve_stats_t* stats);
+/* Cancels what is left and joins the scheduler. */
+int ast_serve_stop(ast_ctx* ctx);
 
 const char* ast_strerror(int code);
 /* Text of the last HIP error seen by this thread (for AST_E_HIP). */
