@@ -88,7 +88,8 @@ void dev_free(T*& p) {
 // ------------------------------------------------------------ validation
 bool variant_supported(const ast_variant& v) {
   if (v.bc && v.bw_ball) return false;
-  if (v.precision != AST_PREC_FP64 && v.precision != AST_PREC_F32SIMPLEX) return false;
+  if (v.precision != AST_PREC_FP64 && v.precision != AST_PREC_F32SIMPLEX && v.precision != AST_PREC_MIXED)
+    return false;
   if (!(v.time_step > 0)) return false;
   return true;
 }
@@ -233,7 +234,7 @@ hipError_t dispatch(const ast_variant& v, LaunchArgs& a) {
 // Device state of up to `nf` resident simplexes.  Element storage is 8 bytes
 // wide so that float and double simplexes share the pool; arrays have room
 // for AST_MAX_PARAMS parameters.
-constexpr int MAX_CLASSES = 36;  // 6 variants x 3 filter capacities x 2 simplex precisions
+constexpr int MAX_CLASSES = 54;  // 6 variants x 3 filter capacities x 3 precision forms
 constexpr int NSTREAMS = 4;      // classes share streams round-robin
 constexpr size_t SLOT_BYTES = 8 * ((AST_MAX_PARAMS + 1) * AST_MAX_PARAMS + (AST_MAX_PARAMS + 1) +
                                    4 * AST_MAX_PARAMS + AST_MAX_FILTERS + 2) +
@@ -475,12 +476,12 @@ hipError_t job_launch(DevJob& j, const Pool& pool, int op, hipStream_t s) {
 }
 
 // Kernel class of a job: one fit-kernel instantiation (variant, filter
-// capacity, simplex precision).  BC and BW_BALL are exclusive, so k < 6.
+// capacity, precision form).  BC and BW_BALL are exclusive, so k < 6.
 int class_of(const ast_variant& v, int n_filters) {
   int k = (v.torque ? 1 : 0) + (v.bc ? 2 : 0) + (v.bw_ball ? 4 : 0);
   int f = n_filters <= 1 ? 0 : (n_filters <= 4 ? 1 : 2);
-  int p = v.precision == AST_PREC_FP64 ? 0 : 1;
-  return (k * 3 + f) * 2 + p;
+  int p = v.precision == AST_PREC_FP64 ? 0 : (v.precision == AST_PREC_F32SIMPLEX ? 1 : 2);
+  return (k * 3 + f) * 3 + p;
 }
 int stream_of(int cls) { return cls % NSTREAMS; }
 

@@ -208,8 +208,44 @@ __device__ void params2x(CF* x, const double* params, const Space<CF>& sp) {
 }
 
 // ----------------------------------------------------------------- evaluator
-__device__ inline double dot3(double ax, double ay, double az, double bx, double by, double bz) {
-  return __builtin_fma(az, bz, __builtin_fma(ay, by, ax * bx));
+__device__ inline double fmx(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ inline float fmx(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+template <class T>
+__device__ inline T dot3(T ax, T ay, T az, T bx, T by, T bz) {
+  return fmx(az, bz, fmx(ay, by, ax * bx));
+}
+
+// Spin frame of the model (cuda.c:190-200): the angular-momentum axis M and
+// the two axes XM, YM perpendicular to it (XM_y = 0).
+template <class T>
+struct SpinFrame {
+  T M_x, M_y, M_z, XM_x, XM_z, YM_x, YM_y, YM_z;
+};
+
+// Body axes a, b, c at one observation from the Euler angles' sines and
+// cosines (cuda.c:432-473): N in the XM-YM plane at phi, p = M x N, a at
+// theta from M towards p, w = a x N, b at psi from N towards w, c = a x b.
+template <class T>
+__device__ inline void body_axes(const SpinFrame<T>& F, T sf, T cf, T st, T ctt, T sp, T cp, T* a, T* b, T* c) {
+  const T N_x = fmx(F.YM_x, sf, F.XM_x * cf);
+  const T N_y = F.YM_y * sf;
+  const T N_z = fmx(F.YM_z, sf, F.XM_z * cf);
+  const T p_x = fmx(-N_z, F.M_y, N_y * F.M_z);
+  const T p_y = fmx(-N_x, F.M_z, N_z * F.M_x);
+  const T p_z = fmx(-N_y, F.M_x, N_x * F.M_y);
+  a[0] = fmx(p_x, st, F.M_x * ctt);
+  a[1] = fmx(p_y, st, F.M_y * ctt);
+  a[2] = fmx(p_z, st, F.M_z * ctt);
+  const T w_x = fmx(-a[2], N_y, a[1] * N_z);
+  const T w_y = fmx(-a[0], N_z, a[2] * N_x);
+  const T w_z = fmx(-a[1], N_x, a[0] * N_y);
+  b[0] = fmx(w_x, sp, N_x * cp);
+  b[1] = fmx(w_y, sp, N_y * cp);
+  b[2] = fmx(w_z, sp, N_z * cp);
+  c[0] = fmx(-a[2], b[1], a[1] * b[2]);
+  c[1] = fmx(-a[0], b[2], a[2] * b[0]);
+  c[2] = fmx(-a[1], b[0], a[0] * b[1]);
 }
 
 // ODE right-hand side.  With FIRST the sin/cos of theta (and psi) come from
@@ -239,9 +275,145 @@ __device__ inline void ode(const double* y, double* f, const double* mu, double 
 
 typedef __attribute__((address_space(4))) const PointRec* PointPtr;
 
+// Per-model constants of the brightness.
+struct ModelConsts {
+  double ib, ic, bcp, kappa, sR, cR, stR, ctR;
+};
+
+#define AST_AI __device__ __attribute__((always_inline)) inline
+
+// Body axes at one observation (FP64) from the spin frame, phi and the sines
+// and cosines of theta (sc[0..1]) and, with torque, of psi (sc[2..3]).
+template <bool TQ>
+AST_AI void axes_at(const SpinFrame<double>& Fd, double phi, double psi, const double* sc, double* a, double* b,
+                    double* c) {
+  double sf, cf, sp, cp;
+  astm::sincos(phi, &sf, &cf);
+  const double st = sc[0], ctt = sc[1];
+  if (TQ) { sp = sc[2]; cp = sc[3]; }
+  else astm::sincos(psi, &sp, &cp);
+  body_axes<double>(Fd, sf, cf, st, ctt, sp, cp, a, b, c);
+}
+
+// Sun and Earth directions in the body frame scaled by the ellipsoid's axes
+// (cuda.c:666-671 in the canonical form): u_S = (b.S/b, c.S/c, a.S) and u_E.
+struct ScaledDirs {
+  double uSb, uSc, uSa, uEb, uEc, uEa;
+};
+AST_AI ScaledDirs scaled_dirs(const double* a, const double* b, const double* c, const double* E,
+                              const double* S, const ModelConsts& K) {
+  const double bs_x = b[0] * K.ib, bs_y = b[1] * K.ib, bs_z = b[2] * K.ib;
+  const double cs_x = c[0] * K.ic, cs_y = c[1] * K.ic, cs_z = c[2] * K.ic;
+  ScaledDirs u;
+  u.uSb = dot3(bs_x, bs_y, bs_z, S[0], S[1], S[2]);
+  u.uSc = dot3(cs_x, cs_y, cs_z, S[0], S[1], S[2]);
+  u.uSa = dot3(a[0], a[1], a[2], S[0], S[1], S[2]);
+  u.uEb = dot3(bs_x, bs_y, bs_z, E[0], E[1], E[2]);
+  u.uEc = dot3(cs_x, cs_y, cs_z, E[0], E[1], E[2]);
+  u.uEa = dot3(a[0], a[1], a[2], E[0], E[1], E[2]);
+  return u;
+}
+
+// Lommel-Seeliger magnitude from the scaled directions (cuda.c:672-689), FP64
+// canonical form.
+AST_AI double ls_tail_f64(const ScaledDirs& u, double bcp) {
+  const double sS2 = dot3(u.uSb, u.uSc, u.uSa, u.uSb, u.uSc, u.uSa);
+  const double sE2 = dot3(u.uEb, u.uEc, u.uEa, u.uEb, u.uEc, u.uEa);
+  const double sS = __builtin_sqrt(sS2);
+  const double sE = __builtin_sqrt(sE2);
+  const double ca = dot3(u.uSb, u.uSc, u.uSa, u.uEb, u.uEc, u.uEa) / (sS * sE);
+  const double sa = __builtin_sqrt(__builtin_fma(-ca, ca, 1.0));
+  const double scn = __builtin_sqrt(__builtin_fma((2.0 * sS) * sE, ca, sS2 + sE2));
+  const double isc = 1.0 / scn;
+  const double cl = __builtin_fma(sE, ca, sS) * isc;
+  const double sl = (sE * sa) * isc;
+  const double cd = __builtin_fma(cl, ca, sl * sa);
+  const double sd = __builtin_fma(-cl, sa, sl * ca);
+  const double lg = astm::log(((1.0 + cl) * (1.0 + cd)) / (sl * (-sd)));
+  const double inner = __builtin_fma(sl * sd, lg, cd + cl);
+  return -2.5 * astm::log10((((bcp * sS) * sE) * isc) * inner);
+}
+
+// Lambert + ball magnitude (cuda.c:475-497, 603-607): the reflecting a axis
+// turned by theta_R, phi_R, and cos zeta = r.E.  Returns the argument of the
+// final log10.
+AST_AI double bw_arg_f64(const double* a, const double* b, const double* c, const double* E, const ModelConsts& K) {
+  const double n_x = __builtin_fma(c[0], K.sR, b[0] * K.cR);
+  const double n_y = __builtin_fma(c[1], K.sR, b[1] * K.cR);
+  const double n_z = __builtin_fma(c[2], K.sR, b[2] * K.cR);
+  const double q_x = __builtin_fma(-n_z, a[1], n_y * a[2]);
+  const double q_y = __builtin_fma(-n_x, a[2], n_z * a[0]);
+  const double q_z = __builtin_fma(-n_y, a[0], n_x * a[1]);
+  const double r_x = __builtin_fma(q_x, K.stR, a[0] * K.ctR);
+  const double r_y = __builtin_fma(q_y, K.stR, a[1] * K.ctR);
+  const double r_z = __builtin_fma(q_z, K.stR, a[2] * K.ctR);
+  const double cz = dot3(r_x, r_y, r_z, E[0], E[1], E[2]);
+  return 0.5 * __builtin_fma(K.kappa, 1.0 + cz, 0.5 * (1.0 - cz));
+}
+
+// ------------------------------------------------- mixed-precision brightness
+// The mixed form keeps the Euler-angle integration, the body axes, the
+// scaled Sun and Earth directions and the chi^2 sums in FP64, and evaluates
+// the brightness functions (square roots, divisions, logarithms) in FP32
+// with the hardware reciprocal square root, reciprocal and log2 (about 1 ulp
+// each).  The axes stay FP64 because the c-axis projections are scaled by
+// 1/c (up to ~100 for flat shapes), which would magnify FP32 rounding of the
+// axes to ~1e-5 mag.
+
+// log2 of a float with an absolute error of about 2^-24: the exponent is
+// taken out exactly and the hardware log2 sees the mantissa in [0.5, 1).
+__device__ inline float log2_f32(float x) {
+  return (float)__builtin_amdgcn_frexp_expf(x) + __builtin_amdgcn_logf(__builtin_amdgcn_frexp_mantf(x));
+}
+// The same, summed in FP64 (for the magnitude itself).
+__device__ inline double log2_f32x(float x) {
+  return (double)__builtin_amdgcn_frexp_expf(x) + (double)__builtin_amdgcn_logf(__builtin_amdgcn_frexp_mantf(x));
+}
+
+constexpr float LN2_F = 0.693147180559945309f;
+constexpr double M2P5_LOG10_2 = -0.75257498915995295;  // -2.5 log10(2): magnitude from log2
+
+// Lommel-Seeliger magnitude from the scaled directions in FP32.  sin(alpha')
+// comes from |u_S x u_E| and the bisector normaliser from |u_S + u_E|, so
+// nothing cancels at small phase angles.  ca_out = cos(alpha').
+__device__ inline double ls_tail_f32(const ScaledDirs& ud, float bcp, float* ca_out) {
+  const float uSb = (float)ud.uSb, uSc = (float)ud.uSc, uSa = (float)ud.uSa;
+  const float uEb = (float)ud.uEb, uEc = (float)ud.uEc, uEa = (float)ud.uEa;
+  const float sS2 = dot3(uSb, uSc, uSa, uSb, uSc, uSa);
+  const float sE2 = dot3(uEb, uEc, uEa, uEb, uEc, uEa);
+  const float rS = __builtin_amdgcn_rsqf(sS2), rE = __builtin_amdgcn_rsqf(sE2);
+  const float sS = sS2 * rS, sE = sE2 * rE;
+  const float k = rS * rE;  // 1 / (sS sE)
+  const float ca = dot3(uSb, uSc, uSa, uEb, uEc, uEa) * k;
+  *ca_out = ca;
+  const float x0 = fmx(-uSa, uEc, uSc * uEa);
+  const float x1 = fmx(-uSb, uEa, uSa * uEb);
+  const float x2 = fmx(-uSc, uEb, uSb * uEc);
+  const float cr2 = dot3(x0, x1, x2, x0, x1, x2);
+  const float sa = (cr2 * __builtin_amdgcn_rsqf(cr2)) * k;
+  const float t0 = uSb + uEb, t1 = uSc + uEc, t2 = uSa + uEa;
+  const float isc = __builtin_amdgcn_rsqf(dot3(t0, t1, t2, t0, t1, t2));
+  const float cl = fmx(sE, ca, sS) * isc;
+  const float sl = (sE * sa) * isc;
+  const float cd = fmx(cl, ca, sl * sa);
+  const float sd = fmx(-cl, sa, sl * ca);
+  const float lg = LN2_F * log2_f32(((1.0f + cl) * (1.0f + cd)) * __builtin_amdgcn_rcpf(sl * (-sd)));
+  const float inner = fmx(sl * sd, lg, cd + cl);
+  return M2P5_LOG10_2 * log2_f32x((((bcp * sS) * sE) * isc) * inner);
+}
+
+// The mixed form redoes the Lommel-Seeliger tail in FP64 when the scaled
+// Sun and Earth directions are more than about 143 degrees apart
+// (cos < -0.8).  There the sum cancels to a small fraction of its terms (flat
+// shapes seen from the unlit side) and FP32 would lose several digits.
+constexpr float MX_BACKLIT_COS = -0.8f;
+
 // chi^2 of one model (chi2one, cuda.c:81-1011, in-scope branches), summed
-// per filter; NF is the compile-time filter capacity (>= n_filters).
-template <class CF, bool TQ, bool BC, bool BW, int NF>
+// per filter; NF is the compile-time filter capacity (>= n_filters).  With MX
+// (the mixed-precision form) the Euler angles and angular velocity are
+// integrated exactly as in the FP64 form, and each observation's geometry
+// and brightness run in FP32; the chi^2 sums stay FP64.
+template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX = false>
 __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_filters, int dof,
                          const double* sum_w, CF* dV) {
   using P = Pos<TQ, BC, BW>;
@@ -251,13 +423,16 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
   double sTM, cTM, sPM, cPM;
   astm::sincos(params[P::theta_M], &sTM, &cTM);
   astm::sincos(params[P::phi_M], &sPM, &cPM);
-  const double M_x = sTM * cPM, M_y = sTM * sPM, M_z = cTM;
-  const double XM = __builtin_sqrt(__builtin_fma(M_x, M_x, M_z * M_z));
-  const double XM_x = M_z / XM;
-  const double XM_z = -M_x / XM;
-  const double YM_x = M_y * XM_z;
-  const double YM_y = __builtin_fma(-M_x, XM_z, M_z * XM_x);
-  const double YM_z = -(M_y * XM_x);
+  SpinFrame<double> Fd;
+  Fd.M_x = sTM * cPM;
+  Fd.M_y = sTM * sPM;
+  Fd.M_z = cTM;
+  const double XM = __builtin_sqrt(__builtin_fma(Fd.M_x, Fd.M_x, Fd.M_z * Fd.M_z));
+  Fd.XM_x = Fd.M_z / XM;
+  Fd.XM_z = -Fd.M_x / XM;
+  Fd.YM_x = Fd.M_y * Fd.XM_z;
+  Fd.YM_y = __builtin_fma(-Fd.M_x, Fd.XM_z, Fd.M_z * Fd.XM_x);
+  Fd.YM_z = -(Fd.M_y * Fd.XM_x);
   const double bt = params[P::b_tumb], ct = params[P::c_tumb];
   const double den = __builtin_fma(ct, ct, bt * bt);
   const double Is = __builtin_fma(bt, bt, 1.0) / den;
@@ -302,36 +477,37 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
     y[1] = theta;
     y[2] = psi;
   }
-  double ib = 0, ic = 0, bcp = 0, kappa = 0, sR = 0, cR = 0, stR = 0, ctR = 0;
+  ModelConsts K{0, 0, 0, 0, 0, 0, 0, 0};
   if (BW) {
-    astm::sincos(params[P::phi_R], &sR, &cR);
-    astm::sincos(params[P::theta_R], &stR, &ctR);
-    kappa = params[P::kappa];
+    astm::sincos(params[P::phi_R], &K.sR, &K.cR);
+    astm::sincos(params[P::theta_R], &K.stR, &K.ctR);
+    K.kappa = params[P::kappa];
   } else {
     const double b = BC ? params[P::b] : bt;
     const double c = BC ? params[P::c] : ct;
-    ib = 1.0 / b;
-    ic = 1.0 / c;
-    bcp = b * c;
+    K.ib = 1.0 / b;
+    K.ic = 1.0 / c;
+    K.bcp = b * c;
   }
+  const float bcpf = (float)K.bcp;  // mixed form
   for (int i = 0; i < n_data; ++i) {
     const auto& pr = pts[i];
     const int nsteps = pr.nsteps;  // 0 for the first observation
     const double h = pr.h, hh = pr.hh, h6 = pr.h6;
     for (int l = 0; l < nsteps; ++l) {  // RK4, cuda.c:348-372
-      double f[6], K[6], s[6];
-      ode<TQ, true>(y, K, mu, mu02, sc);
+      double f[6], K4[6], s[6];
+      ode<TQ, true>(y, K4, mu, mu02, sc);
 #pragma unroll
-      for (int j = 0; j < NODE; ++j) { s[j] = K[j]; f[j] = __builtin_fma(hh, K[j], y[j]); }
-      ode<TQ, false>(f, K, mu, mu02, sc);
+      for (int j = 0; j < NODE; ++j) { s[j] = K4[j]; f[j] = __builtin_fma(hh, K4[j], y[j]); }
+      ode<TQ, false>(f, K4, mu, mu02, sc);
 #pragma unroll
-      for (int j = 0; j < NODE; ++j) { s[j] = __builtin_fma(2.0, K[j], s[j]); f[j] = __builtin_fma(hh, K[j], y[j]); }
-      ode<TQ, false>(f, K, mu, mu02, sc);
+      for (int j = 0; j < NODE; ++j) { s[j] = __builtin_fma(2.0, K4[j], s[j]); f[j] = __builtin_fma(hh, K4[j], y[j]); }
+      ode<TQ, false>(f, K4, mu, mu02, sc);
 #pragma unroll
-      for (int j = 0; j < NODE; ++j) { s[j] = __builtin_fma(2.0, K[j], s[j]); f[j] = __builtin_fma(h, K[j], y[j]); }
-      ode<TQ, false>(f, K, mu, mu02, sc);
+      for (int j = 0; j < NODE; ++j) { s[j] = __builtin_fma(2.0, K4[j], s[j]); f[j] = __builtin_fma(h, K4[j], y[j]); }
+      ode<TQ, false>(f, K4, mu, mu02, sc);
 #pragma unroll
-      for (int j = 0; j < NODE; ++j) y[j] = __builtin_fma(h6, s[j] + K[j], y[j]);
+      for (int j = 0; j < NODE; ++j) y[j] = __builtin_fma(h6, s[j] + K4[j], y[j]);
       if (TQ) {
         astm::sincos(y[4], &sc[0], &sc[1]);
         astm::sincos(y[5], &sc[2], &sc[3]);
@@ -341,68 +517,23 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
     }
     if (TQ) { phi = y[3]; theta = y[4]; psi = y[5]; }
     else { phi = y[0]; theta = y[1]; psi = y[2]; }
-    double sf, cf, sp, cp;
-    astm::sincos(phi, &sf, &cf);
-    const double st = sc[0], ctt = sc[1];
-    if (TQ) { sp = sc[2]; cp = sc[3]; }
-    else astm::sincos(psi, &sp, &cp);
-    const double N_x = __builtin_fma(YM_x, sf, XM_x * cf);
-    const double N_y = YM_y * sf;
-    const double N_z = __builtin_fma(YM_z, sf, XM_z * cf);
-    const double p_x = __builtin_fma(-N_z, M_y, N_y * M_z);
-    const double p_y = __builtin_fma(-N_x, M_z, N_z * M_x);
-    const double p_z = __builtin_fma(-N_y, M_x, N_x * M_y);
-    const double a_x = __builtin_fma(p_x, st, M_x * ctt);
-    const double a_y = __builtin_fma(p_y, st, M_y * ctt);
-    const double a_z = __builtin_fma(p_z, st, M_z * ctt);
-    const double w_x = __builtin_fma(-a_z, N_y, a_y * N_z);
-    const double w_y = __builtin_fma(-a_x, N_z, a_z * N_x);
-    const double w_z = __builtin_fma(-a_y, N_x, a_x * N_y);
-    const double b_x = __builtin_fma(w_x, sp, N_x * cp);
-    const double b_y = __builtin_fma(w_y, sp, N_y * cp);
-    const double b_z = __builtin_fma(w_z, sp, N_z * cp);
-    const double c_x = __builtin_fma(-a_z, b_y, a_y * b_z);
-    const double c_y = __builtin_fma(-a_x, b_z, a_z * b_x);
-    const double c_z = __builtin_fma(-a_y, b_x, a_x * b_y);
-    const double E0 = pr.E[0], E1 = pr.E[1], E2 = pr.E[2];
+    const double E[3] = {pr.E[0], pr.E[1], pr.E[2]};
+    double a[3], b[3], c[3];
+    axes_at<TQ>(Fd, phi, psi, sc, a, b, c);
     double Vmod;
-    if (BW) {  // cuda.c:475-497, 603-607
-      const double n_x = __builtin_fma(c_x, sR, b_x * cR);
-      const double n_y = __builtin_fma(c_y, sR, b_y * cR);
-      const double n_z = __builtin_fma(c_z, sR, b_z * cR);
-      const double q_x = __builtin_fma(-n_z, a_y, n_y * a_z);
-      const double q_y = __builtin_fma(-n_x, a_z, n_z * a_x);
-      const double q_z = __builtin_fma(-n_y, a_x, n_x * a_y);
-      const double r_x = __builtin_fma(q_x, stR, a_x * ctR);
-      const double r_y = __builtin_fma(q_y, stR, a_y * ctR);
-      const double r_z = __builtin_fma(q_z, stR, a_z * ctR);
-      const double cz = dot3(r_x, r_y, r_z, E0, E1, E2);
-      Vmod = -2.5 * astm::log10(0.5 * __builtin_fma(kappa, 1.0 + cz, 0.5 * (1.0 - cz)));
-    } else {   // cuda.c:666-689 in the canonical form
-      const double S0 = pr.S[0], S1 = pr.S[1], S2 = pr.S[2];
-      const double bs_x = b_x * ib, bs_y = b_y * ib, bs_z = b_z * ib;
-      const double cs_x = c_x * ic, cs_y = c_y * ic, cs_z = c_z * ic;
-      const double uSb = dot3(bs_x, bs_y, bs_z, S0, S1, S2);
-      const double uSc = dot3(cs_x, cs_y, cs_z, S0, S1, S2);
-      const double uSa = dot3(a_x, a_y, a_z, S0, S1, S2);
-      const double uEb = dot3(bs_x, bs_y, bs_z, E0, E1, E2);
-      const double uEc = dot3(cs_x, cs_y, cs_z, E0, E1, E2);
-      const double uEa = dot3(a_x, a_y, a_z, E0, E1, E2);
-      const double sS2 = dot3(uSb, uSc, uSa, uSb, uSc, uSa);
-      const double sE2 = dot3(uEb, uEc, uEa, uEb, uEc, uEa);
-      const double sS = __builtin_sqrt(sS2);
-      const double sE = __builtin_sqrt(sE2);
-      const double ca = dot3(uSb, uSc, uSa, uEb, uEc, uEa) / (sS * sE);
-      const double sa = __builtin_sqrt(__builtin_fma(-ca, ca, 1.0));
-      const double sc = __builtin_sqrt(__builtin_fma((2.0 * sS) * sE, ca, sS2 + sE2));
-      const double isc = 1.0 / sc;
-      const double cl = __builtin_fma(sE, ca, sS) * isc;
-      const double sl = (sE * sa) * isc;
-      const double cd = __builtin_fma(cl, ca, sl * sa);
-      const double sd = __builtin_fma(-cl, sa, sl * ca);
-      const double lg = astm::log(((1.0 + cl) * (1.0 + cd)) / (sl * (-sd)));
-      const double inner = __builtin_fma(sl * sd, lg, cd + cl);
-      Vmod = -2.5 * astm::log10((((bcp * sS) * sE) * isc) * inner);
+    if (BW) {
+      const double arg = bw_arg_f64(a, b, c, E, K);
+      Vmod = MX ? M2P5_LOG10_2 * log2_f32x((float)arg) : -2.5 * astm::log10(arg);
+    } else {
+      const double S[3] = {pr.S[0], pr.S[1], pr.S[2]};
+      const ScaledDirs u = scaled_dirs(a, b, c, E, S, K);
+      if (MX) {
+        float ca;
+        Vmod = ls_tail_f32(u, bcpf, &ca);
+        if (!(ca >= MX_BACKLIT_COS)) Vmod = ls_tail_f64(u, K.bcp);  // ill-conditioned (or NaN): FP64
+      } else {
+        Vmod = ls_tail_f64(u, K.bcp);
+      }
     }
     const double w = (double)pr.w;
     const double yv = (double)pr.V - Vmod;

@@ -445,7 +445,7 @@ __global__ void fit_init_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
 // lane's params[] buffer, delta_V to its last-dV slot.  Kept out of line so
 // the evaluator's register allocation does not compete with the simplex
 // bookkeeping live across it.
-template <class CF, bool TQ, bool BC, bool BW, int NF>
+template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX>
 __device__ __attribute__((noinline)) CF eval_lane(const double* PRM, CF* DV, size_t nf, size_t fit, size_t rs,
                                                    size_t fo, const PointRec* pts_g, int n_data, int n_filters, int dof,
                                                    const double* sum_w) {
@@ -464,7 +464,7 @@ __device__ __attribute__((noinline)) CF eval_lane(const double* PRM, CF* DV, siz
 #pragma unroll
   for (int i = 0; i < N; ++i) prm[i] = PRM[(size_t)i * nf + fit];
   CF dv[NF];
-  CF fv = chi2_model<CF, TQ, BC, BW, NF>(prm, pts, n_data, n_filters, dof, sum_w, dv);
+  CF fv = chi2_model<CF, TQ, BC, BW, NF, MX>(prm, pts, n_data, n_filters, dof, sum_w, dv);
 #pragma unroll
   for (int m = 0; m < NF; ++m)
     if (m < n_filters) DV[(size_t)m * rs + fo] = dv[m];
@@ -474,7 +474,7 @@ __device__ __attribute__((noinline)) CF eval_lane(const double* PRM, CF* DV, siz
 // ---------------------------------------------------------------- fit kernel
 // One workgroup = one group (the reference's block); it runs D.ticks rounds
 // of "evaluate the pending points, advance every lane", then saves its state.
-template <class CF, bool TQ, bool BC, bool BW, int NF>
+template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX>
 __global__ __launch_bounds__(1024) void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
   using P = Pos<TQ, BC, BW>;
   constexpr int N = P::N;
@@ -518,7 +518,7 @@ __global__ __launch_bounds__(1024) void fit_kernel(const GroupDesc<CF>* descs, S
   for (int tick = 0; tick < ticks && !done; ++tick) {
     Machine<CF, TQ, BC, BW> M{sp, B, SC, fit, nf, fo, rs, jfit, reopt};
     if (phase_pending(R.phase)) {
-      CF fv = eval_lane<CF, TQ, BC, BW, NF>(B.PRM, B.DV, nf, fit, rs, fo, D.pts, D.n_data, D.n_filters, D.dof,
+      CF fv = eval_lane<CF, TQ, BC, BW, NF, MX>(B.PRM, B.DV, nf, fit, rs, fo, D.pts, D.n_data, D.n_filters, D.dof,
                                             D.sum_w);
       R.nev++;
       R.nev_tot++;
@@ -607,7 +607,7 @@ __global__ __launch_bounds__(1024) void fit_kernel(const GroupDesc<CF>* descs, S
 }
 
 // ------------------------------------------------------------- chi2 batch
-template <class CF, bool TQ, bool BC, bool BW, int NF>
+template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX>
 __global__ __launch_bounds__(256) void chi2_batch_kernel(const PointRec* pts_g, const double* sum_w, int n_data,
                                                         int n_filters, int dof, const double* params, int n,
                                                         double* chi2, double* dV) {
@@ -620,7 +620,7 @@ __global__ __launch_bounds__(256) void chi2_batch_kernel(const PointRec* pts_g, 
 #pragma unroll
   for (int i = 0; i < N; ++i) prm[i] = params[(size_t)k * N + i];
   CF dv[NF];
-  CF c = chi2_model<CF, TQ, BC, BW, NF>(prm, pts, n_data, n_filters, dof, sum_w, dv);
+  CF c = chi2_model<CF, TQ, BC, BW, NF, MX>(prm, pts, n_data, n_filters, dof, sum_w, dv);
   chi2[k] = c;
   for (int m = 0; m < n_filters; ++m) dV[(size_t)k * n_filters + m] = dv[m];
 }
@@ -644,33 +644,37 @@ struct LaunchArgs {
   double* dV;
 };
 
-template <class CF, bool TQ, bool BC, bool BW, int NF>
+template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX>
 hipError_t launch_nf(const LaunchArgs& a) {
   if (a.op == 0) {
     hipLaunchKernelGGL((fit_init_kernel<CF, TQ, BC, BW>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
                        (const GroupDesc<CF>*)a.descs, *(const SlotBufs<CF>*)a.bufs);
   } else if (a.op == 1) {
-    hipLaunchKernelGGL((fit_kernel<CF, TQ, BC, BW, NF>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
+    hipLaunchKernelGGL((fit_kernel<CF, TQ, BC, BW, NF, MX>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
                        (const GroupDesc<CF>*)a.descs, *(const SlotBufs<CF>*)a.bufs);
   } else {
     int blocks = (a.n + 255) / 256;
-    hipLaunchKernelGGL((chi2_batch_kernel<CF, TQ, BC, BW, NF>), dim3(blocks), dim3(256), 0, a.stream, a.pts,
+    hipLaunchKernelGGL((chi2_batch_kernel<CF, TQ, BC, BW, NF, MX>), dim3(blocks), dim3(256), 0, a.stream, a.pts,
                        a.sum_w, a.n_data, a.n_filters, a.dof, a.params, a.n, a.chi2, a.dV);
   }
   return hipGetLastError();
 }
 
-template <class CF, bool TQ, bool BC, bool BW>
+template <class CF, bool TQ, bool BC, bool BW, bool MX>
 hipError_t launch_cf(const LaunchArgs& a) {
-  if (a.n_filters <= 1) return launch_nf<CF, TQ, BC, BW, 1>(a);
-  if (a.n_filters <= 4) return launch_nf<CF, TQ, BC, BW, 4>(a);
-  return launch_nf<CF, TQ, BC, BW, MAXF>(a);
+  if (a.n_filters <= 1) return launch_nf<CF, TQ, BC, BW, 1, MX>(a);
+  if (a.n_filters <= 4) return launch_nf<CF, TQ, BC, BW, 4, MX>(a);
+  return launch_nf<CF, TQ, BC, BW, MAXF, MX>(a);
 }
 
+// Simplex float and evaluator form by precision: FP64 (double simplex, FP64
+// evaluator), F32SIMPLEX (float simplex, FP64 evaluator: the reference's
+// default CHI_FLOAT), MIXED (float simplex, mixed-precision evaluator).
 template <bool TQ, bool BC, bool BW>
 hipError_t launch_variant(const LaunchArgs& a) {
-  if (a.precision == AST_PREC_FP64) return launch_cf<double, TQ, BC, BW>(a);
-  return launch_cf<float, TQ, BC, BW>(a);
+  if (a.precision == AST_PREC_FP64) return launch_cf<double, TQ, BC, BW, false>(a);
+  if (a.precision == AST_PREC_MIXED) return launch_cf<float, TQ, BC, BW, true>(a);
+  return launch_cf<float, TQ, BC, BW, false>(a);
 }
 
 }  // namespace ast

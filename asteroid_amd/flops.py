@@ -43,8 +43,41 @@ def eval_ops(variant, mjd):
 
 
 def eval_flops(variant, mjd):
+    """FLOPs of one evaluation; for the mixed form, its FP64 plus FP32 FLOPs."""
+    if variant.precision == MIXED:
+        f64, f32 = eval_flops_mixed(variant, mjd)
+        return f64 + f32
     return float(eval_ops(variant, mjd) @ _W)
 
 
-def flops_per_model_point(variant, mjd):
-    return eval_flops(variant, mjd) / len(mjd)
+# ------------------------------------------------------------ mixed form
+# The mixed-precision evaluator runs the per-model set-up, the RK4
+# integration, the body axes, the scaled Sun/Earth directions and the chi^2
+# sums exactly as the FP64 form (the rows above), and replaces the FP64
+# brightness tail by an FP32 one.  Counted from ast_device.h:
+#   FP64 Lommel-Seeliger tail (ls_tail_f64, from |u_S|^2 to the magnitude,
+#     with the log and log10 of astmath.h): 16 add, 32 mul, 27 fma, 5 div,
+#     4 sqrt; the Lambert + ball tail (log10 and the -2.5 factor): 6 add,
+#     8 mul, 8 fma, 1 div.
+#   FP32 Lommel-Seeliger tail (ls_tail_f32): 7 add, 28 mul, 17 fma and 7
+#     "special" instructions (hardware rsq, rcp or log2, 1 FLOP each); the
+#     Lambert + ball tail: 1 log2.  Both add 1 FP64 add and 1 FP64 mul (the
+#     magnitude from the exponent and the log2 of the mantissa).
+# Observations the mixed form redoes in FP64 (backlit scaled geometry) are
+# extra work and not counted.
+MIXED = 2
+_TAIL64 = {"ls": np.array([16, 32, 27, 5, 4, 0], float), "bw": np.array([6, 8, 8, 1, 0, 0], float)}
+_TAIL32 = {"ls": np.array([7, 28, 17, 7], float), "bw": np.array([0, 0, 0, 1], float)}  # add mul fma special
+_MX_F64_EXTRA = np.array([1, 1, 0, 0, 0, 0], float)
+_W4 = np.array([1, 1, 2, 1], float)
+
+
+def eval_flops_mixed(variant, mjd):
+    """(FP64 FLOPs, FP32 FLOPs) of one mixed-form evaluation."""
+    kind = "bw" if variant.bw_ball else "ls"
+    t = TABLE[(bool(variant.torque), kind)]
+    n = len(mjd)
+    f64_pt = np.array(t["per_point"], float) - _TAIL64[kind] + _MX_F64_EXTRA
+    f64 = (np.array(t["per_model"], float) @ _W + rk4_steps(mjd, variant.time_step) * (np.array(t["per_rk4"], float) @ _W)
+           + n * (f64_pt @ _W))
+    return float(f64), float(n * (_TAIL32[kind] @ _W4))

@@ -10,6 +10,12 @@ stopped after 4800 chi^2 evaluations (or size convergence).  One step is one
 launch cycle of the reference's host loop over all 2^18 simplexes
 (chi2_gpu + sync, asteroid.c:675-688) with inputs already resident in HBM.
 
+Both precision forms run the same workload: the mixed form (FP64 Euler-angle
+integration and chi^2 sums, FP32 geometry, brightness and simplex) is the
+headline, the plain FP64 form is reported beside it.  The mixed run's
+per-group best models are re-evaluated with the FP64 form to state its
+relative chi^2 error against the 1e-6 gate.
+
 Prints one JSON line: model-point evaluations per second (evaluations times
 observations) over the timed steps, fits/s beside it, the roofline of the
 fitting kernel (algorithmic FLOPs from asteroid_amd/flops.py over its
@@ -32,6 +38,11 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
 
 N_OBS, SPAN_DAYS, CURVE_SEED = 400, 3.0, 2024
 PPSI_HOURS = (2.0, 4800.0)
+# Both forms stop on the evaluation budget: the simplex-size test uses the
+# FP64 threshold (SIZE_MIN of the ACC build, asteroid.h:233-237) for both.
+SIZE_MIN = 1e-10
+MIXED = 2
+PREC_NAMES = {0: "fp64", 1: "f32simplex", 2: "mixed"}
 
 
 def log(msg):
@@ -46,9 +57,9 @@ def make_workload(precision):
     return v, curve, truth, space
 
 
-def traffic_from_profile():
+def traffic_from_profile(form):
     """HBM bytes per launch of the fitting kernel from the committed PMC summary, if present."""
-    path = os.path.join(ROOT, "profiles", "fit_kernel_stage_one_2p18_traffic.txt")
+    path = os.path.join(ROOT, "profiles", f"fit_kernel_stage_one_2p18_{form}_traffic.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -85,25 +96,9 @@ def cpu_baseline(precision, budget, seconds):
             "fits_per_s": nfit / dt}
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", choices=["fp64", "f32simplex"], default="fp64")
-    ap.add_argument("--fits", type=int, default=1 << 18)
-    ap.add_argument("--budget", type=int, default=4800)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: the fits are independent and shard across GPUs "
-              "without communication (see DESIGN.md); run it with --gpus 1.", file=sys.stderr)
-        sys.exit(2)
-
-    from asteroid_amd import abi, api, flops
-    prec = {"fp64": abi.PREC_FP64, "f32simplex": abi.PREC_F32SIMPLEX}[args.precision]
+def run_form(prec, args):
+    """Warm up, then time args.steps Stage One steps of one precision form."""
+    from asteroid_amd import api, flops
     v, curve, truth, space = make_workload(prec)
     group = 256
     n_groups = args.fits // group
@@ -114,12 +109,13 @@ def main():
 
     def step():
         ctx.reset_best()
-        ctx.run(n_groups, group, nstages=1, n_steps=5000, eval_budget=args.budget)
+        ctx.run(n_groups, group, nstages=1, n_steps=5000, eval_budget=args.budget, size_min=SIZE_MIN)
 
+    name = PREC_NAMES[prec]
     for w in range(args.warmup):
         t = time.perf_counter()
         step()
-        log(f"warmup {w + 1}/{args.warmup} done in {time.perf_counter() - t:.2f} s")
+        log(f"{name}: warmup {w + 1}/{args.warmup} done in {time.perf_counter() - t:.2f} s")
     tot_points = tot_evals = 0
     kern_ms = 0.0
     launches = 0
@@ -132,46 +128,107 @@ def main():
         tot_points += p
         kern_ms += ms
         launches += nl
-        log(f"step {k + 1}/{args.steps}: {e} evals, kernel {ms / 1e3:.2f} s in {nl} launches")
+        log(f"{name}: step {k + 1}/{args.steps}: {e} evals, kernel {ms / 1e3:.2f} s in {nl} launches")
     wall = time.perf_counter() - t0
     f, params, dv = ctx.fetch()
     ctx.close()
-
-    value = tot_points / wall
     flop_per_eval = flops.eval_flops(v, curve.mjd)
-    flop_total = flop_per_eval * tot_evals
-    achieved = flop_total / (kern_ms / 1e3) / 1e12
-    traffic = traffic_from_profile()
-    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
-            "kernel": "fit_kernel<double,TORQUE,LS,NF=1>",
-            "flop_per_model_point": round(flop_per_eval / curve.n_data, 2),
+    achieved = flop_per_eval * tot_evals / (kern_ms / 1e3) / 1e12
+    if prec == MIXED:
+        f64, f32 = flops.eval_flops_mixed(v, curve.mjd)
+        # peak of this op mix: FP64 FLOPs at the FP64 rate, FP32 FLOPs at the FP32 rate
+        peak = (f64 + f32) / (f64 / FP64_PEAK_TFLOPS + f32 / FP32_PEAK_TFLOPS)
+        kernel = "fit_kernel<float,TORQUE,LS,NF=1,mixed>"
+        split = {"fp64_flop_per_model_point": round(f64 / curve.n_data, 2),
+                 "fp32_flop_per_model_point": round(f32 / curve.n_data, 2)}
+    else:
+        peak = FP64_PEAK_TFLOPS
+        kernel = "fit_kernel<double,TORQUE,LS,NF=1>"
+        split = {}
+    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 2), "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4), "traffic": traffic_from_profile(name),
+            "kernel": kernel, "flop_per_model_point": round(flop_per_eval / curve.n_data, 2), **split,
             "avg_launch_ms": round(kern_ms / max(launches, 1), 3), "launches": launches}
+    return {"value": tot_points / wall, "wall": wall, "ms_per_step": wall / args.steps * 1e3,
+            "fits_per_s": args.fits * args.steps / wall, "evals_per_s": tot_evals / wall,
+            "evals_per_fit": tot_evals / (args.fits * args.steps), "best_f": f, "best_params": params,
+            "roofline": roof}
+
+
+def gate_at_best_fits(res_mx):
+    """FP64 chi^2 (GPU FP64 form, bitwise the oracle's) of the mixed run's
+    per-group best models against the chi^2 the mixed run reported."""
+    from asteroid_amd import abi, api
+    v, curve, truth, space = make_workload(abi.PREC_FP64)
+    ok = res_mx["best_f"] < 1e29
+    with api.Context(v) as ctx:
+        ctx.set_model_space(space)
+        ctx.upload_curve(curve)
+        c64, _ = ctx.chi2_batch(res_mx["best_params"][ok])
+    rel = np.abs(res_mx["best_f"][ok] - c64) / c64
+    return {"max_rel_chi2_err": float(rel.max()), "median_rel_chi2_err": float(np.median(rel)),
+            "n_models": int(ok.sum()), "tolerance": 1e-6, "pass": bool(rel.max() <= 1e-6 + 2.0 ** -23)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", choices=["both", "mixed", "fp64", "f32simplex"], default="both")
+    ap.add_argument("--fits", type=int, default=1 << 18)
+    ap.add_argument("--budget", type=int, default=4800)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the fits are independent and shard across GPUs "
+              "without communication (see DESIGN.md); run it with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from asteroid_amd import abi
+    forms = {"both": [MIXED, abi.PREC_FP64], "mixed": [MIXED], "fp64": [abi.PREC_FP64],
+             "f32simplex": [abi.PREC_F32SIMPLEX]}[args.precision]
+    results = {prec: run_form(prec, args) for prec in forms}
+    head_prec = forms[0]
+    head = results[head_prec]
+    dtype = {abi.PREC_FP64: "f64", abi.PREC_F32SIMPLEX: "f64 model, f32 simplex",
+             MIXED: "f64 Euler-angle integration and chi^2 sums, f32 geometry and brightness, f32 simplex"}
     out = {
         "metric": "model-point evals/s, fits/s at 1e-6 rel chi^2 vs FP64; p50/p99 fit latency",
-        "value": value,
+        "value": head["value"],
         "unit": "model-point evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == abi.PREC_FP64 else "f64 model, f32 simplex",
+        "dtype": dtype[head_prec],
         "data": "synthetic",
         "config": {"workload": "stage-one random search (LS ellipsoid + torque, P_PSI, INTERP)",
                    "n_obs": N_OBS, "span_days": SPAN_DAYS, "filters": 1, "simplexes": args.fits,
-                   "evals_per_simplex": args.budget, "group_size": group, "time_step_days": 1e-2,
-                   "precision": args.precision},
-        "fits_per_s": args.fits * args.steps / wall,
-        "evals_per_s": tot_evals / wall,
-        "best_chi2": float(np.min(f)),
-        "roofline": roof,
+                   "evals_per_simplex": args.budget, "size_min": SIZE_MIN, "group_size": 256,
+                   "time_step_days": 1e-2, "precision": PREC_NAMES[head_prec]},
+        "fits_per_s": head["fits_per_s"],
+        "evals_per_s": head["evals_per_s"],
+        "evals_per_fit": round(head["evals_per_fit"], 1),
+        "best_chi2": float(np.min(head["best_f"])),
+        "roofline": head["roofline"],
     }
+    if MIXED in results:
+        out["precision_gate"] = gate_at_best_fits(results[MIXED])
+    for prec, r in results.items():
+        if prec == head_prec:
+            continue
+        out[PREC_NAMES[prec]] = {"value": r["value"], "ms_per_step": r["ms_per_step"],
+                                 "fits_per_s": r["fits_per_s"], "evals_per_fit": round(r["evals_per_fit"], 1),
+                                 "best_chi2": float(np.min(r["best_f"])), "roofline": r["roofline"]}
     if not args.no_cpu:
         log("timing the CPU oracle sample ...")
-        out["cpu_baseline"] = cpu_baseline(prec, args.budget, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(abi.PREC_FP64, args.budget, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

@@ -92,3 +92,4 @@ def test_coefficients_regenerate():
                          text=True, check=True).stdout
     with open(os.path.join(here, "astmath_coeffs.h")) as f:
         assert f.read().strip() == out.strip()
+

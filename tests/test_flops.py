@@ -31,3 +31,21 @@ def test_table_matches_counting_oracle(v, n, span):
     # up to two adds, one division and two adds more on some arguments.
     d = np.abs(got - table)
     assert np.all(d <= np.array([4, 0, 0, 1, 0, 0])), (got, table)
+
+
+def test_mixed_table_consistent():
+    """The FP64 brightness tail the mixed form replaces holds every division
+    and square root of an observation (the rest of the point is fma/mul/add
+    work), and the mixed counts are the FP64 counts minus that tail plus the
+    FP32 tail."""
+    from asteroid_amd import abi, synth, variants
+    for torque, kind in ((True, "ls"), (False, "ls"), (True, "bw")):
+        per_point = np.array(flops.TABLE[(torque, kind)]["per_point"], float)
+        assert np.array_equal(flops._TAIL64[kind][3:5], per_point[3:5])
+        assert np.all(per_point - flops._TAIL64[kind] >= 0)
+    v64, vmx = variants.ls_torque(abi.PREC_FP64), variants.ls_torque(abi.PREC_MIXED)
+    curve, _ = synth.make_curve(3, v64, 50, 1.0)
+    f64, f32 = flops.eval_flops_mixed(vmx, curve.mjd)
+    w = np.array([1, 1, 2, 1, 1, 1], float)
+    expect64 = flops.eval_flops(v64, curve.mjd) - 50 * (flops._TAIL64["ls"] - flops._MX_F64_EXTRA) @ w
+    assert f64 == expect64 and f32 == 50 * 76 and flops.eval_flops(vmx, curve.mjd) == f64 + f32
