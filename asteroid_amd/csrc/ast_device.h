@@ -248,6 +248,17 @@ __device__ inline void body_axes(const SpinFrame<T>& F, T sf, T cf, T st, T ctt,
   c[2] = fmx(-a[1], b[0], a[0] * b[1]);
 }
 
+// sin and cos of an Euler angle for the body axes at an observation: the
+// canonical FP64 functions, or in the mixed form the degree-4 ones (about
+// 2e-14 relative).  The integration itself always uses the canonical ones:
+// for models whose Euler angles pass near the singular theta = 0 or pi the
+// RK4 integration is chaotic and magnifies any change in its arithmetic.
+template <bool MX>
+__device__ inline void sincos_e(double x, double* s, double* c) {
+  if (MX) astm::sincos_mx(x, s, c);
+  else astm::sincos(x, s, c);
+}
+
 // ODE right-hand side.  With FIRST the sin/cos of theta (and psi) come from
 // sc[] (those of the state, shared with the body axes), else they are computed.
 template <bool TQ, bool FIRST>
@@ -284,14 +295,14 @@ struct ModelConsts {
 
 // Body axes at one observation (FP64) from the spin frame, phi and the sines
 // and cosines of theta (sc[0..1]) and, with torque, of psi (sc[2..3]).
-template <bool TQ>
+template <bool TQ, bool MX>
 AST_AI void axes_at(const SpinFrame<double>& Fd, double phi, double psi, const double* sc, double* a, double* b,
                     double* c) {
   double sf, cf, sp, cp;
-  astm::sincos(phi, &sf, &cf);
+  sincos_e<MX>(phi, &sf, &cf);
   const double st = sc[0], ctt = sc[1];
   if (TQ) { sp = sc[2]; cp = sc[3]; }
-  else astm::sincos(psi, &sp, &cp);
+  else sincos_e<MX>(psi, &sp, &cp);
   body_axes<double>(Fd, sf, cf, st, ctt, sp, cp, a, b, c);
 }
 
@@ -519,7 +530,7 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
     else { phi = y[0]; theta = y[1]; psi = y[2]; }
     const double E[3] = {pr.E[0], pr.E[1], pr.E[2]};
     double a[3], b[3], c[3];
-    axes_at<TQ>(Fd, phi, psi, sc, a, b, c);
+    axes_at<TQ, MX>(Fd, phi, psi, sc, a, b, c);
     double Vmod;
     if (BW) {
       const double arg = bw_arg_f64(a, b, c, E, K);

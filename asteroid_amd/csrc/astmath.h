@@ -51,9 +51,10 @@ AST_HD double p_val(double a) { return a; }
 
 AST_HD uint64_t bits_of(double x) { return __builtin_bit_cast(uint64_t, x); }
 AST_HD double of_bits(uint64_t b) { return __builtin_bit_cast(double, b); }
-// -x if flag is nonzero, else x: one xor of the sign bit (exact, not counted).
+// -x if bit 1 of flag is set, else x (flag is 0 or 2): the bit is shifted
+// into the sign position and xored in (exact, not counted).
 AST_HD double p_flipsign(double x, int flag) {
-  return of_bits(bits_of(x) ^ ((uint64_t)(flag != 0) << 63));
+  return of_bits(bits_of(x) ^ ((uint64_t)(uint32_t)(flag & 2) << 62));
 }
 
 // 2^k as a double, for -1022 <= k <= 1023.
@@ -135,6 +136,30 @@ template <class R> AST_HD void sincos(R x, R* s, R* c) {
   R cs = w + p_fma(z * z, k_cos(z), (R(1.0) - w) - hz);
   double kv = p_val(k);
   int q = (p_fabs(kv) < 1073741824.0) ? ((int)kv & 3) : 0;  // |k| < 2^30: exact 32-bit conversion
+  R s0 = (q & 1) ? cs : sn;
+  R c0 = (q & 1) ? sn : cs;
+  *s = p_flipsign(s0, q & 2);
+  *c = p_flipsign(c0, (q + 1) & 2);
+}
+
+// sin and cos for the mixed-precision form's body axes (relative error about
+// 2e-14): two-part pi/2 (exact enough for |x| < 2^20) and degree-4 kernels;
+// same quadrant logic.
+template <class R> AST_HD void sincos_mx(R x, R* s, R* c) {
+  R k = p_rint(x * R(AST_TWO_OVER_PI));
+  R r = p_fma(-k, R(AST_PIO2_1), x);
+  r = p_fma(-k, R(AST_PIO2_2), r);
+  R z = r * r;
+  R ps = p_fma(z, p_fma(z, p_fma(z, p_fma(z, R(AST_SINM_4), R(AST_SINM_3)), R(AST_SINM_2)), R(AST_SINM_1)),
+               R(AST_SINM_0));
+  R sn = p_fma(r * z, ps, r);
+  R pc = p_fma(z, p_fma(z, p_fma(z, p_fma(z, R(AST_COSM_4), R(AST_COSM_3)), R(AST_COSM_2)), R(AST_COSM_1)),
+               R(AST_COSM_0));
+  R hz = R(0.5) * z;
+  R w = R(1.0) - hz;
+  R cs = w + p_fma(z * z, pc, (R(1.0) - w) - hz);
+  double kv = p_val(k);
+  int q = (p_fabs(kv) < 1073741824.0) ? ((int)kv & 3) : 0;
   R s0 = (q & 1) ? cs : sn;
   R c0 = (q & 1) ? sn : cs;
   *s = p_flipsign(s0, q & 2);

@@ -141,7 +141,7 @@ class Problem:
 
 
 MATH = {"sin": 0, "cos": 1, "tan": 2, "atan2": 3, "log": 4, "log10": 5, "exp": 6, "asin": 7,
-        "atan": 8, "expf": 9}
+        "atan": 8, "expf": 9, "sin_mx": 10, "cos_mx": 11}
 
 
 def math(fn, x, y=None):

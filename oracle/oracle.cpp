@@ -1073,6 +1073,8 @@ int orc_math(int fn, const double* x, const double* y, double* out, int n) {
       case 7: out[k] = astm::asin(a); break;
       case 8: out[k] = astm::atan(a); break;
       case 9: out[k] = astm::expf((float)a); break;
+      case 10: { double sn, cs; astm::sincos_mx(a, &sn, &cs); out[k] = sn; break; }
+      case 11: { double sn, cs; astm::sincos_mx(a, &sn, &cs); out[k] = cs; break; }
       default: return AST_E_ARG;
     }
   }

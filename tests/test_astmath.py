@@ -93,3 +93,16 @@ def test_coefficients_regenerate():
     with open(os.path.join(here, "astmath_coeffs.h")) as f:
         assert f.read().strip() == out.strip()
 
+
+
+@pytest.mark.parametrize("name,ref", [("sin_mx", mp.sin), ("cos_mx", mp.cos)])
+def test_mixed_form_sincos(name, ref):
+    """The mixed form's sin/cos (two-part pi/2, degree-4 kernels) on the
+    Euler angles' range (up to ~1e4 rad after days of fast rotation):
+    absolute error below 1e-13."""
+    rng = np.random.default_rng(21)
+    x = np.concatenate([rng.uniform(-4, 4, 3000), rng.uniform(-2e4, 2e4, 3000)])
+    got = oracle.math(name, x)
+    want = np.array([float(ref(mp.mpf(v))) for v in x])
+    err = np.abs(got - want) / np.maximum(np.abs(want), 1.0)
+    assert err.max() < 1e-13
