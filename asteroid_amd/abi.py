@@ -78,6 +78,10 @@ class Result(C.Structure):
                 ("queue_ms", C.c_double), ("run_ms", C.c_double)]
 
 
+class BatchCfg(C.Structure):
+    _fields_ = [("group_size", C.c_int), ("work_points_per_launch", C.c_double)]
+
+
 class ServeStats(C.Structure):
     _fields_ = [("pending", C.c_int), ("resident", C.c_int), ("free_fits", C.c_int64),
                 ("resident_bytes", C.c_int64), ("launches", C.c_int64), ("evals", C.c_int64),
@@ -146,6 +150,7 @@ def load():
         "ast_last_timing": (I, [P, pD, pI]),
         "ast_last_counts": (I, [P, pI64, pI64]),
         "ast_default_model_space": (I, [C.POINTER(Variant), pD, pI, pI, pI, pI, pD, C.POINTER(X2)]),
+        "ast_run_jobs": (I, [P, C.POINTER(Job), I, C.POINTER(BatchCfg), C.POINTER(Result)]),
         "ast_serve_start": (I, [P, C.POINTER(ServeCfg)]),
         "ast_submit": (I, [P, C.POINTER(Job), C.POINTER(C.c_uint64)]),
         "ast_poll": (I, [P, C.POINTER(Result), I, pI, I]),
@@ -168,5 +173,5 @@ EXPORTED = (
     "ast_reset_best", "ast_set_start", "ast_run", "ast_fetch", "ast_fetch_lanes", "ast_chi2_batch",
     "ast_last_timing", "ast_last_counts", "ast_default_model_space", "ast_strerror",
     "ast_last_hip_error", "ast_serve_start", "ast_submit", "ast_poll", "ast_cancel", "ast_serve_stats",
-    "ast_serve_stop",
+    "ast_serve_stop", "ast_run_jobs",
 )

@@ -131,6 +131,40 @@ class Context:
         return e.value, p.value
 
 
+    # ------------------------------------------------------------- batches
+    @staticmethod
+    def _job(space, curve, n_fits, nstages=1, n_steps=5000, eval_budget=0, size_min=0.0, seed=1, params0=None,
+             tag=0):
+        """A ctypes ast_job plus the arrays it points into (kept alive by the caller)."""
+        keep = dict(prop=np.ascontiguousarray(space.property.ravel(), np.int32),
+                    types=np.ascontiguousarray(space.types, np.int32),
+                    lim=np.ascontiguousarray(space.limits, np.float64),
+                    obs=curve.obs_array(), eph=curve.ephem3(),
+                    p0=None if params0 is None else np.ascontiguousarray(params0, np.float64))
+        job = abi.Job(space.variant, space.n_params, keep["prop"].ctypes.data_as(C.POINTER(C.c_int)),
+                      space.n_types, keep["types"].ctypes.data_as(C.POINTER(C.c_int)), _dp(keep["lim"]),
+                      space.x2(), C.cast(keep["obs"], C.POINTER(abi.Obs)), curve.n_data, curve.n_filters,
+                      C.pointer(keep["eph"]), int(n_fits), int(nstages), int(n_steps), int(eval_budget),
+                      float(size_min), int(seed), _dp(keep["p0"]) if keep["p0"] is not None else None, int(tag))
+        return job, keep
+
+    @staticmethod
+    def _result(r):
+        return dict(id=r.id, tag=r.tag, status=r.status, best_f=r.best_f, best_group=r.best_group,
+                    best_params=np.array(r.best_params[:r.n_params]), best_dV=np.array(r.best_dV[:r.n_filters]),
+                    n_evals=r.n_evals, model_points=r.model_points, latency_ms=r.submit_to_done_ms,
+                    queue_ms=r.queue_ms, run_ms=r.run_ms)
+
+    def run_jobs(self, jobs, group_size=256, work_points_per_launch=0.0):
+        """Run a batch of jobs (dicts of Context.submit's arguments) to completion
+        in shared launch cycles; returns their results in order."""
+        built = [self._job(**j) for j in jobs]
+        arr = (abi.Job * len(built))(*[b[0] for b in built])
+        out = (abi.Result * len(built))()
+        cfg = abi.BatchCfg(int(group_size), float(work_points_per_launch))
+        _chk(self.lib.ast_run_jobs(self.h, arr, len(built), C.byref(cfg), out), "ast_run_jobs")
+        return [self._result(out[k]) for k in range(len(built))]
+
     # ------------------------------------------------------------- serving
     def serve_start(self, max_resident_fits, group_size=256, hbm_budget_bytes=0, work_points_per_launch=0.0):
         cfg = abi.ServeCfg(int(max_resident_fits), int(group_size), int(hbm_budget_bytes),
@@ -141,17 +175,7 @@ class Context:
     def submit(self, space, curve, n_fits, nstages=1, n_steps=5000, eval_budget=0, size_min=0.0, seed=1,
                params0=None, tag=0):
         """Submit a fit job (its own variant, model space and curve); returns its id."""
-        prop = np.ascontiguousarray(space.property.ravel(), np.int32)
-        types = np.ascontiguousarray(space.types, np.int32)
-        lim = np.ascontiguousarray(space.limits, np.float64)
-        obs = curve.obs_array()
-        eph = curve.ephem3()
-        p0 = None if params0 is None else np.ascontiguousarray(params0, np.float64)
-        job = abi.Job(space.variant, space.n_params, prop.ctypes.data_as(C.POINTER(C.c_int)), space.n_types,
-                      types.ctypes.data_as(C.POINTER(C.c_int)), _dp(lim), space.x2(),
-                      C.cast(obs, C.POINTER(abi.Obs)), curve.n_data, curve.n_filters, C.pointer(eph),
-                      int(n_fits), int(nstages), int(n_steps), int(eval_budget), float(size_min), int(seed),
-                      _dp(p0) if p0 is not None else None, int(tag))
+        job, keep = self._job(space, curve, n_fits, nstages, n_steps, eval_budget, size_min, seed, params0, tag)
         jid = C.c_uint64()
         _chk(self.lib.ast_submit(self.h, C.byref(job), C.byref(jid)), "ast_submit")
         return jid.value
@@ -160,15 +184,7 @@ class Context:
         out = (abi.Result * max_results)()
         n = C.c_int()
         _chk(self.lib.ast_poll(self.h, out, max_results, C.byref(n), int(timeout_ms)), "ast_poll")
-        res = []
-        for k in range(n.value):
-            r = out[k]
-            res.append(dict(id=r.id, tag=r.tag, status=r.status, best_f=r.best_f, best_group=r.best_group,
-                            best_params=np.array(r.best_params[:r.n_params]),
-                            best_dV=np.array(r.best_dV[:r.n_filters]), n_evals=r.n_evals,
-                            model_points=r.model_points, latency_ms=r.submit_to_done_ms, queue_ms=r.queue_ms,
-                            run_ms=r.run_ms))
-        return res
+        return [self._result(out[k]) for k in range(n.value)]
 
     def cancel(self, jid):
         _chk(self.lib.ast_cancel(self.h, int(jid)), "ast_cancel")

@@ -318,6 +318,7 @@ struct RunParams {
 
 struct DevJob {
   uint64_t id = 0, tag = 0;
+  uint64_t uid = 0;            // unique within the context (launch-batch membership)
   SpaceH space;
   RunParams rp;
   int n_data = 0, n_filters = 0;
@@ -333,14 +334,15 @@ struct DevJob {
   double* best_params = nullptr;
   double* best_dV = nullptr;
   int32_t* done = nullptr;     // finished-group counter
-  void* descs = nullptr;       // GroupDesc<CF>[n_groups]
+  void* descs = nullptr;       // GroupDesc<CF>[n_groups] (device, for the init kernel)
+  std::vector<unsigned char> desc_host;  // the same descriptors on the host
   bool borrowed = false;       // rng and best arrays belong to the context
   std::vector<int> slots;
   size_t bytes = 0;
   int groups_done = 0;
   int status = AST_OK;
   bool cancelled = false;
-  Clock::time_point t_submit, t_admit;
+  Clock::time_point t_submit, t_admit, t_done;
   double run_ms = 0;
 };
 
@@ -436,6 +438,8 @@ int job_descs_typed(DevJob& j, double work_points) {
   }
   if (!j.descs) RETIF(dev_alloc((char**)&j.descs, sizeof(GroupDesc<CF>) * G));
   HIPCHK(hipMemcpy(j.descs, d.data(), sizeof(GroupDesc<CF>) * G, hipMemcpyHostToDevice));
+  j.desc_host.resize(sizeof(GroupDesc<CF>) * G);
+  std::memcpy(j.desc_host.data(), d.data(), j.desc_host.size());
   return 0;
 }
 
@@ -484,6 +488,17 @@ int class_of(const ast_variant& v, int n_filters) {
   return (k * 3 + f) * 3 + p;
 }
 int stream_of(int cls) { return cls % NSTREAMS; }
+
+// One launch per kernel class and cycle: the descriptors of every running
+// group of the class's jobs, concatenated in device memory.  Rebuilt when
+// the set of running jobs of the class changes (between cycles, with no
+// kernel in flight).
+struct ClassBatch {
+  std::vector<uint64_t> members;  // job uids, in descriptor order
+  void* d_descs = nullptr;
+  size_t cap = 0;                 // bytes
+  int n_groups = 0;
+};
 
 // --------------------------------------------------------------- serving
 struct Serve {
@@ -536,6 +551,8 @@ struct ast_ctx {
   int last_launches = 0;
   int64_t last_evals = 0, last_points = 0;
   std::unique_ptr<Serve> serve;
+  ClassBatch batch[MAX_CLASSES];
+  std::atomic<uint64_t> next_uid{1};
 };
 
 namespace {
@@ -545,8 +562,9 @@ int set_device(ast_ctx* c) {
   return 0;
 }
 
-// One cycle: one fit-kernel launch per job over all of its groups, every
-// stream drained at the end; adds the cycle's kernel time.
+// One cycle: one fit-kernel launch per kernel class over every running
+// group of every job of that class, every stream drained at the end; adds
+// the cycle's kernel time (the longest stream).
 int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, int* launches);
 
 // A failed cycle may leave kernels in flight: drain the device before the
@@ -557,19 +575,69 @@ int run_cycle(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, int* la
   return rc;
 }
 
-int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, int* launches) {
-  bool used_cls[MAX_CLASSES] = {false}, used_s[NSTREAMS] = {false};
-  for (DevJob* j : jobs) {
-    used_cls[j->cls] = true;
-    used_s[stream_of(j->cls)] = true;
+// Bring the class's concatenated descriptors up to date with `members`.
+int class_batch_update(ast_ctx* c, int k, const std::vector<DevJob*>& members) {
+  ClassBatch& cb = c->batch[k];
+  std::vector<uint64_t> ids;
+  for (DevJob* j : members) ids.push_back(j->uid);
+  if (ids == cb.members && cb.d_descs) return 0;
+  size_t bytes = 0;
+  int groups = 0;
+  for (DevJob* j : members) {
+    bytes += j->desc_host.size();
+    groups += j->rp.n_groups;
   }
-  for (int k = 0; k < NSTREAMS; ++k)
-    if (used_s[k]) HIPCHK(hipEventRecord(c->ev0[k], c->streams[k]));
-  for (int k = 0; k < MAX_CLASSES; ++k)
-    if (used_cls[k]) HIPCHK(hipMemsetAsync(c->pool.running + k, 0, sizeof(int32_t), c->streams[stream_of(k)]));
+  if (bytes > cb.cap) {
+    dev_free(cb.d_descs);
+    cb.cap = 0;
+    RETIF(dev_alloc((char**)&cb.d_descs, bytes));
+    cb.cap = bytes;
+  }
+  std::vector<unsigned char> h(bytes);
+  size_t off = 0;
+  for (DevJob* j : members) {
+    std::memcpy(h.data() + off, j->desc_host.data(), j->desc_host.size());
+    off += j->desc_host.size();
+  }
+  HIPCHK(hipMemcpy(cb.d_descs, h.data(), bytes, hipMemcpyHostToDevice));
+  cb.members = ids;
+  cb.n_groups = groups;
+  return 0;
+}
+
+int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, int* launches) {
+  std::vector<DevJob*> members[MAX_CLASSES];
+  bool used_s[NSTREAMS] = {false};
   for (DevJob* j : jobs) {
     if (j->groups_done >= j->rp.n_groups) continue;
-    HIPCHK(job_launch(*j, c->pool, 1, c->streams[stream_of(j->cls)]));
+    if (!job_fits_pool(*j, c->pool)) {
+      set_err("job does not fit the slot pool");
+      return AST_E_STATE;
+    }
+    members[j->cls].push_back(j);
+    used_s[stream_of(j->cls)] = true;
+  }
+  for (int k = 0; k < MAX_CLASSES; ++k)
+    if (!members[k].empty()) RETIF(class_batch_update(c, k, members[k]));
+  for (int k = 0; k < NSTREAMS; ++k)
+    if (used_s[k]) HIPCHK(hipEventRecord(c->ev0[k], c->streams[k]));
+  for (int k = 0; k < MAX_CLASSES; ++k) {
+    if (members[k].empty()) continue;
+    const DevJob& j0 = *members[k][0];
+    const hipStream_t s = c->streams[stream_of(k)];
+    HIPCHK(hipMemsetAsync(c->pool.running + k, 0, sizeof(int32_t), s));
+    LaunchArgs a{};
+    a.op = 1;
+    a.stream = s;
+    a.descs = c->batch[k].d_descs;
+    SlotBufs<double> bd;
+    SlotBufs<float> bf;
+    if (j0.cf == 8) { bd = pool_bufs<double>(c->pool, k); a.bufs = &bd; }
+    else { bf = pool_bufs<float>(c->pool, k); a.bufs = &bf; }
+    a.n_groups = c->batch[k].n_groups;
+    a.group_size = c->pool.G;
+    a.n_filters = j0.n_filters;
+    HIPCHK(dispatch(j0.space.v, a));
     (*launches)++;
   }
   for (int k = 0; k < NSTREAMS; ++k)
@@ -580,15 +648,23 @@ int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, in
     HIPCHK(hipMemcpyAsync(&done[i], jobs[i]->done, sizeof(int32_t), hipMemcpyDeviceToHost,
                           c->streams[stream_of(jobs[i]->cls)]));
   double ms_max = 0;
-  for (int k = 0; k < NSTREAMS; ++k)
+  for (int k = 0; k < NSTREAMS; ++k) {
+    bool any = used_s[k];
+    for (size_t i = 0; i < jobs.size() && !any; ++i) any = stream_of(jobs[i]->cls) == k;
+    if (!any) continue;
+    HIPCHK(hipStreamSynchronize(c->streams[k]));
     if (used_s[k]) {
-      HIPCHK(hipStreamSynchronize(c->streams[k]));
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, c->ev0[k], c->ev1[k]));
       if (ms > ms_max) ms_max = ms;
     }
+  }
   *kernel_ms += ms_max;
-  for (size_t i = 0; i < jobs.size(); ++i) jobs[i]->groups_done = done[i];
+  const Clock::time_point now = Clock::now();
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    if (jobs[i]->groups_done < jobs[i]->rp.n_groups && done[i] >= jobs[i]->rp.n_groups) jobs[i]->t_done = now;
+    jobs[i]->groups_done = done[i];
+  }
   return 0;
 }
 
@@ -693,6 +769,39 @@ int job_evals(const ast_ctx* c, const DevJob& j, int64_t* evals) {
   }
   *evals = s;
   return 0;
+}
+
+// A job from its C description: validated model space and light curve,
+// run parameters and kernel class; the curve and start model are returned
+// for the upload.
+int make_job(ast_ctx* c, const ast_job* job, int G, std::unique_ptr<DevJob>* out, CurveH* curve,
+             std::vector<double>* p0) {
+  if (job->n_fits <= 0 || job->n_fits % G || job->nstages <= 0 || job->n_steps < 0 || job->eval_budget < 0)
+    return AST_E_ARG;
+  std::unique_ptr<DevJob> j(new DevJob());
+  j->uid = c->next_uid++;
+  RETIF(make_space(job->variant, job->n_params, job->property, job->n_types, job->types, job->limits, &job->x2,
+                   &j->space));
+  CurveH C;
+  RETIF(make_curve(job->variant, job->obs, job->n_data, job->n_filters, job->eph, &C));
+  if (C.n_data - j->space.n_params - C.n_filters <= 0) return AST_E_DATA;
+  j->rp.n_groups = job->n_fits / G;
+  j->rp.group_size = G;
+  j->rp.nstages = job->nstages;
+  j->rp.n_steps = job->n_steps;
+  j->rp.eval_budget = job->eval_budget;
+  j->rp.size_min = job->size_min;
+  j->rp.seed = job->seed;
+  j->tag = job->tag;
+  j->cls = class_of(job->variant, job->n_filters);
+  j->n_data = C.n_data;
+  j->n_filters = C.n_filters;
+  j->t_submit = Clock::now();
+  p0->clear();
+  if (job->params0) p0->assign(job->params0, job->params0 + j->space.n_params);
+  *curve = std::move(C);
+  *out = std::move(j);
+  return AST_OK;
 }
 
 // ------------------------------------------------------- serving thread
@@ -894,6 +1003,7 @@ int ast_ctx_destroy(ast_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->ref_job) job_free(*c->ref_job);
   pool_free(c->pool);
+  for (int k = 0; k < MAX_CLASSES; ++k) dev_free(c->batch[k].d_descs);
   dev_free(c->pts);
   dev_free(c->sum_w);
   dev_free(c->rng);
@@ -973,6 +1083,7 @@ int ast_run(ast_ctx* c, const ast_run_cfg* rc) {
   if (c->ref_job) job_free(*c->ref_job);
   c->ref_job.reset(new DevJob());
   DevJob& j = *c->ref_job;
+  j.uid = c->next_uid++;
   j.space = c->space;
   j.rp.n_groups = rc->n_groups;
   j.rp.group_size = G;
@@ -1159,30 +1270,11 @@ int ast_submit(ast_ctx* c, const ast_job* job, uint64_t* id) {
   if (!c || !job || !id) return AST_E_ARG;
   if (!c->serve) return AST_E_STATE;
   Serve& S = *c->serve;
-  const int G = S.cfg.group_size;
-  if (job->n_fits <= 0 || job->n_fits % G || job->nstages <= 0 || job->n_steps < 0 || job->eval_budget < 0)
-    return AST_E_ARG;
-  if (job->n_fits / G > c->pool.n_slots) return AST_E_NOMEM;
-  std::unique_ptr<DevJob> j(new DevJob());
-  RETIF(make_space(job->variant, job->n_params, job->property, job->n_types, job->types, job->limits, &job->x2,
-                   &j->space));
+  std::unique_ptr<DevJob> j;
   CurveH C;
-  RETIF(make_curve(job->variant, job->obs, job->n_data, job->n_filters, job->eph, &C));
-  if (C.n_data - j->space.n_params - C.n_filters <= 0) return AST_E_DATA;
-  j->rp.n_groups = job->n_fits / G;
-  j->rp.group_size = G;
-  j->rp.nstages = job->nstages;
-  j->rp.n_steps = job->n_steps;
-  j->rp.eval_budget = job->eval_budget;
-  j->rp.size_min = job->size_min;
-  j->rp.seed = job->seed;
-  j->tag = job->tag;
-  j->cls = class_of(job->variant, job->n_filters);
-  j->n_data = C.n_data;
-  j->n_filters = C.n_filters;
-  j->t_submit = Clock::now();
   std::vector<double> p0;
-  if (job->params0) p0.assign(job->params0, job->params0 + j->space.n_params);
+  RETIF(make_job(c, job, S.cfg.group_size, &j, &C, &p0));
+  if (j->rp.n_groups > c->pool.n_slots) return AST_E_NOMEM;
   std::lock_guard<std::mutex> lk(S.mu);
   j->id = S.next_id++;
   *id = j->id;
@@ -1190,6 +1282,95 @@ int ast_submit(ast_ctx* c, const ast_job* job, uint64_t* id) {
   S.pending.push_back(std::move(j));
   S.cv_submit.notify_all();
   return AST_OK;
+}
+
+int ast_run_jobs(ast_ctx* c, const ast_job* jobs, int n_jobs, const ast_batch_cfg* cfg, ast_result* results) {
+  if (!c || !jobs || n_jobs <= 0 || !results) return AST_E_ARG;
+  if (c->serve) return AST_E_BUSY;
+  const int G = (cfg && cfg->group_size) ? cfg->group_size : 256;
+  if (G < 64 || G > 1024 || (G & 63)) return AST_E_ARG;
+  const double work = (cfg && cfg->work_points_per_launch > 0) ? cfg->work_points_per_launch : 8000.0;
+  std::vector<std::unique_ptr<DevJob>> J(n_jobs);
+  std::vector<CurveH> curves(n_jobs);
+  std::vector<std::vector<double>> p0(n_jobs);
+  size_t groups = 0;
+  for (int i = 0; i < n_jobs; ++i) {
+    RETIF(make_job(c, &jobs[i], G, &J[i], &curves[i], &p0[i]));
+    J[i]->id = (uint64_t)i;
+    groups += (size_t)J[i]->rp.n_groups;
+  }
+  RETIF(set_device(c));
+  if ((size_t)c->pool.n_slots < groups || c->pool.G != G) RETIF(pool_alloc(c->pool, groups * G, G));
+  c->last_groups = 0;  // the pool's rows no longer hold an ast_run's lanes
+  std::vector<DevJob*> run;
+  int slot = 0;
+  const Clock::time_point t0 = Clock::now();
+  int rc = 0;
+  for (int i = 0; i < n_jobs && !rc; ++i) {
+    DevJob& j = *J[i];
+    for (int g = 0; g < j.rp.n_groups; ++g) j.slots.push_back(slot++);
+    const hipStream_t s = c->streams[stream_of(j.cls)];
+    rc = job_upload(j, curves[i], p0[i].empty() ? nullptr : p0[i].data(), s);
+    if (!rc) rc = job_descs(j, work);
+    if (!rc) {
+      hipError_t e = job_launch(j, c->pool, 0, s);
+      if (e != hipSuccess) {
+        snprintf(g_hip_err, sizeof g_hip_err, "%s (init kernel)", hipGetErrorString(e));
+        rc = AST_E_HIP;
+      }
+    }
+    j.t_submit = j.t_admit = t0;
+    run.push_back(&j);
+  }
+  // Every tick completes one evaluation of each running simplex; a stage
+  // needs at most (N+1) + n_steps*(N+2) of them.
+  long max_cycles = 0;
+  for (DevJob* j : run) {
+    const double per_stage = (double)(j->space.n_params + 1) + (double)j->rp.n_steps * (j->space.n_params + 2);
+    const long mc = (long)((per_stage * j->rp.nstages + 16.0) / ticks_for(j->n_data, work)) + 4;
+    if (mc > max_cycles) max_cycles = mc;
+  }
+  c->last_ms = 0;
+  c->last_launches = 0;
+  for (long L = 0; !rc; ++L) {
+    bool all = true;
+    for (DevJob* j : run) all = all && j->groups_done >= j->rp.n_groups;
+    if (all) break;
+    if (L > max_cycles) {
+      set_err("fit did not terminate");
+      rc = AST_E_STATE;
+      break;
+    }
+    rc = run_cycle(c, run, &c->last_ms, &c->last_launches);
+  }
+  if (rc) (void)hipDeviceSynchronize();
+  int64_t ev_tot = 0, pts_tot = 0;
+  for (int i = 0; i < n_jobs; ++i) {
+    DevJob& j = *J[i];
+    ast_result& r = results[i];
+    std::memset(&r, 0, sizeof r);
+    if (!rc && i < (int)run.size()) {
+      int e = job_result(j, &r);
+      int64_t ev = 0;
+      if (!e) e = job_evals(c, j, &ev);
+      if (e && !rc) rc = e;
+      r.n_evals = ev;
+      r.model_points = ev * j.n_data;
+      r.submit_to_done_ms = std::chrono::duration<double, std::milli>(j.t_done - t0).count();
+      r.run_ms = r.submit_to_done_ms;
+      ev_tot += ev;
+      pts_tot += ev * j.n_data;
+    } else {
+      r.status = rc ? rc : AST_E_STATE;
+      r.best_f = 1e30;
+    }
+    r.id = (uint64_t)i;
+    r.tag = j.tag;
+    job_free(j);
+  }
+  c->last_evals = ev_tot;
+  c->last_points = pts_tot;
+  return rc;
 }
 
 int ast_poll(ast_ctx* c, ast_result* out, int max, int* n_out, int timeout_ms) {

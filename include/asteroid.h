@@ -291,6 +291,23 @@ typedef struct {
   double kernel_ms;
 } ast_serve_stats_t;
 
+/* ------------------------------------------------------------ batches
+ * No counterpart in the reference, which fits one light curve per process
+ * (README.md:80-86 runs many processes).  Runs n_jobs fit jobs (each with
+ * its own curve, variant, model space, seed and number of simplexes) to
+ * completion in shared launch cycles: every cycle is one launch of the
+ * fitting kernel per kernel class (variant, filter capacity, precision) over
+ * every running group of every job of that class.  results[i] belongs to
+ * jobs[i]; its latency fields are measured from the call's start.  Each
+ * job's results are those it gets alone (its random streams are indexed by
+ * its own simplex numbers).  Refused while the context is serving. */
+typedef struct {
+  int group_size;                /* simplexes per group; 0 = 256 */
+  double work_points_per_launch; /* model-point evaluations per simplex per launch; 0 = 8000 */
+} ast_batch_cfg;
+
+int ast_run_jobs(ast_ctx* ctx, const ast_job* jobs, int n_jobs, const ast_batch_cfg* cfg, ast_result* results);
+
 int ast_serve_start(ast_ctx* ctx, const ast_serve_cfg* cfg);
 /* Copies the job's inputs; returns its id. */
 int ast_submit(ast_ctx* ctx, const ast_job* job, uint64_t* id);

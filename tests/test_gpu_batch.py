@@ -1,0 +1,88 @@
+"""Batched jobs (ast_run_jobs): many light curves of uneven length, filter
+count, model variant and precision fitted in shared launch cycles (one
+launch per kernel class).  Each job's results equal the same job run alone,
+and for the FP64 and float-simplex forms they equal the oracle's run of that
+job bitwise (best chi^2, model and delta_V per group, evaluation counts)."""
+import numpy as np
+import pytest
+
+import oracle
+from asteroid_amd import abi, api, synth, variants
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs():
+    specs = [  # (variant, n_filters, n_obs, span, n_fits, nstages, n_steps, reopt)
+        (variants.ls_torque(abi.PREC_FP64), 1, 90, 1.5, 512, 2, 30, 0),
+        (variants.bc_free(abi.PREC_FP64), 3, 140, 2.0, 256, 1, 40, 0),
+        (variants.bw_ball_torque(abi.PREC_F32SIMPLEX), 2, 60, 1.0, 768, 1, 30, 0),
+        (variants.ls_torque(abi.PREC_MIXED), 1, 200, 3.0, 256, 1, 30, 0),
+        (variants.ls_torque(abi.PREC_FP64), 4, 50, 1.0, 256, 1, 25, 1),
+        (variants.bc_free(abi.PREC_MIXED), 2, 120, 2.5, 512, 1, 30, 0),
+    ]
+    jobs = []
+    for k, (v, nf, n_obs, span, n_fits, nst, nsteps, reopt) in enumerate(specs):
+        curve, truth = synth.make_curve(100 + k, v, n_obs, span, n_filters=nf)
+        space = variants.default_model_space(v, reopt=reopt)
+        jobs.append(dict(space=space, curve=curve, n_fits=n_fits, nstages=nst, n_steps=nsteps, seed=500 + k,
+                         params0=truth if reopt else None, tag=k))
+    return jobs
+
+
+def _same(a, b):
+    for key in ("status", "best_f", "best_group", "n_evals", "model_points", "tag"):
+        assert a[key] == b[key], key
+    assert np.array_equal(a["best_params"], b["best_params"])
+    assert np.array_equal(a["best_dV"], b["best_dV"])
+
+
+def test_batch_equals_each_job_alone():
+    jobs = _jobs()
+    with api.Context(variants.ls_torque()) as ctx:
+        batch = ctx.run_jobs(jobs, work_points_per_launch=3000)
+        alone = [ctx.run_jobs([j], work_points_per_launch=5000)[0] for j in jobs]
+    for k, (b, a) in enumerate(zip(batch, alone)):
+        print(f"job {k}: best {b['best_f']:.6g} evals {b['n_evals']} latency {b['latency_ms']:.1f} ms")
+        _same(b, a)
+    assert all(r["status"] == 0 for r in batch)
+
+
+def test_batch_matches_oracle():
+    jobs = _jobs()
+    with api.Context(variants.ls_torque()) as ctx:
+        res = ctx.run_jobs(jobs)
+    for k, (j, r) in enumerate(zip(jobs, res)):
+        v = j["space"].variant
+        if v.precision == abi.PREC_MIXED:
+            continue  # tolerance-checked in test_gpu_mixed
+        pb = oracle.Problem(j["space"], j["curve"])
+        G = j["n_fits"] // 256
+        want = pb.run(G, 256, nstages=j["nstages"], n_steps=j["n_steps"], seed=j["seed"],
+                      reopt=j["space"].x2().reopt, params0=j["params0"], n_threads=8)
+        g = int(np.argmin(want["f"]))
+        assert r["best_f"] == want["f"][g] and r["best_group"] == g, k
+        assert np.array_equal(r["best_params"], want["params"][g]), k
+        assert np.array_equal(r["best_dV"], want["dV"][g]), k
+        assert r["n_evals"] == want["lane_nev"].sum(), k
+
+
+def test_batch_one_launch_per_class():
+    """Eight jobs of one kernel class (different curves and seeds) share one
+    launch per cycle: the batch takes about as many launches as its slowest
+    job alone, not eight times as many."""
+    v = variants.ls_torque(abi.PREC_FP64)
+    space = variants.default_model_space(v)
+    jobs = []
+    for k in range(8):
+        curve, _ = synth.make_curve(300 + k, v, 80, 1.5)
+        jobs.append(dict(space=space, curve=curve, n_fits=256, n_steps=40, seed=k))
+    with api.Context(v) as ctx:
+        ctx.run_jobs(jobs, work_points_per_launch=2000)
+        _, launches = ctx.last_timing()
+        alone = []
+        for j in jobs:
+            ctx.run_jobs([j], work_points_per_launch=2000)
+            alone.append(ctx.last_timing()[1])
+    print("batch launches", launches, "alone", alone)
+    assert launches == max(alone)
