@@ -1,26 +1,31 @@
 #!/usr/bin/env python3
 """Benchmark of the fitting core on one MI355X.
 
-Workload (the Stage One random search at real scale): the self-consistent
-Lommel-Seeliger ellipsoid with torque (P_PSI, TORQUE, INTERP; 11 parameters),
-one synthetic 400-observation single-filter light curve over 3 days,
-TIME_STEP 1e-2 d, L limits 48pi/10..48pi/0.1 rad/day, Tmax 10, P_psi range
-2..4800 h, and 2^18 independent simplexes (1024 reseed groups of 256), each
-stopped after 4800 chi^2 evaluations (or size convergence).  One step is one
-launch cycle of the reference's host loop over all 2^18 simplexes
-(chi2_gpu + sync, asteroid.c:675-688) with inputs already resident in HBM.
+Default workload (the Stage One random search at real scale): the
+self-consistent Lommel-Seeliger ellipsoid with torque (P_PSI, TORQUE, INTERP;
+11 parameters), one synthetic 400-observation single-filter light curve over
+3 days, TIME_STEP 1e-2 d, L limits 48pi/10..48pi/0.1 rad/day, Tmax 10, P_psi
+range 2..4800 h, and 2^18 independent simplexes (1024 reseed groups of 256),
+each stopped after 4800 chi^2 evaluations.  One step is one launch cycle of
+the reference's host loop over all 2^18 simplexes (chi2_gpu + sync,
+asteroid.c:675-688) with inputs already resident in HBM.
 
 Both precision forms run the same workload: the mixed form (FP64 Euler-angle
-integration and chi^2 sums, FP32 geometry, brightness and simplex) is the
-headline, the plain FP64 form is reported beside it.  The mixed run's
+integration, axes and chi^2 sums, FP32 brightness and simplex) is the
+headline, the plain FP64 form is reported beside it, and the mixed run's
 per-group best models are re-evaluated with the FP64 form to state its
-relative chi^2 error against the 1e-6 gate.
+relative chi^2 error against the 1e-6 gate.  A short serving run (Poisson
+arrivals of batched-style fit requests, mixed form) adds the p50/p99 fit
+latency.
 
 Prints one JSON line: model-point evaluations per second (evaluations times
 observations) over the timed steps, fits/s beside it, the roofline of the
 fitting kernel (algorithmic FLOPs from asteroid_amd/flops.py over its
 HIP-event-timed launch durations) and the CPU oracle timed on a bounded
 sample of the same workload on this host's cores.
+
+--workload small|reopt|batched|serving measures the other configurations,
+one JSON line each (see DESIGN.md for what each one is).
 """
 import argparse
 import json
@@ -33,28 +38,28 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (and dense MFMA) peak
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector peak
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
-
-N_OBS, SPAN_DAYS, CURVE_SEED = 400, 3.0, 2024
-PPSI_HOURS = (2.0, 4800.0)
+METRIC = "model-point evals/s, fits/s at 1e-6 rel chi^2 vs FP64; p50/p99 fit latency"
 # Both forms stop on the evaluation budget: the simplex-size test uses the
 # FP64 threshold (SIZE_MIN of the ACC build, asteroid.h:233-237) for both.
 SIZE_MIN = 1e-10
-MIXED = 2
-PREC_NAMES = {0: "fp64", 1: "f32simplex", 2: "mixed"}
+FP64, F32SIMPLEX, MIXED = 0, 1, 2
+PREC_NAMES = {FP64: "fp64", F32SIMPLEX: "f32simplex", MIXED: "mixed"}
+DTYPE = {FP64: "f64", F32SIMPLEX: "f64 model, f32 simplex",
+         MIXED: "f64 Euler-angle integration, axes and chi^2 sums; f32 brightness and simplex"}
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_workload(precision):
-    from asteroid_amd import synth, variants
-    v = variants.ls_torque(precision, time_step=1e-2)
-    curve, truth = synth.make_curve(CURVE_SEED, v, N_OBS, SPAN_DAYS, n_filters=1, sigma_range=(0.05, 0.3))
-    space = variants.default_model_space(v, ppsi_hours=PPSI_HOURS, reopt=0)
-    return v, curve, truth, space
+def cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
 
 
 def traffic_from_profile(form):
@@ -69,37 +74,50 @@ def traffic_from_profile(form):
     return None
 
 
-def cpu_baseline(precision, budget, seconds):
+def peak_of(prec, variant, mjd):
+    """Vector peak of the form's op mix: FP64 FLOPs at the FP64 rate, FP32 FLOPs at the FP32 rate."""
+    from asteroid_amd import flops
+    if prec != MIXED:
+        return FP64_PEAK_TFLOPS, {}
+    f64, f32 = flops.eval_flops_mixed(variant, mjd)
+    return ((f64 + f32) / (f64 / FP64_PEAK_TFLOPS + f32 / FP32_PEAK_TFLOPS),
+            {"fp64_flop_per_model_point": round(f64 / len(mjd), 2),
+             "fp32_flop_per_model_point": round(f32 / len(mjd), 2)})
+
+
+def base_line(value, steps, warmup, ms_per_step, dtype, config):
+    return {"metric": METRIC, "value": value, "unit": "model-point evals/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+
+
+# ------------------------------------------------------------- stage one
+def cpu_baseline(budget, seconds):
     """The oracle (canonical form, C++) on this host's cores, bounded sample."""
     import oracle
-    v, curve, truth, space = make_workload(precision)
+    from asteroid_amd import workloads
+    v, curve, truth, space = workloads.stage_one(FP64)
     pb = oracle.Problem(space, curve)
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    n = cores()
     # calibrate with one simplex per core, then size the sample to ~`seconds`
     t0 = time.perf_counter()
-    r = pb.run(cores, 1, nstages=1, n_steps=5000, eval_budget=budget, seed=7, form=oracle.CANON, n_threads=cores)
+    pb.run(n, 1, nstages=1, n_steps=5000, eval_budget=budget, seed=7, form=oracle.CANON, n_threads=n)
     t1 = time.perf_counter() - t0
-    reps = max(1, int(round(seconds / max(t1, 1e-3))))
-    nfit = cores * reps
+    nfit = n * max(1, int(round(seconds / max(t1, 1e-3))))
     t0 = time.perf_counter()
-    r = pb.run(nfit, 1, nstages=1, n_steps=5000, eval_budget=budget, seed=11, form=oracle.CANON, n_threads=cores)
+    r = pb.run(nfit, 1, nstages=1, n_steps=5000, eval_budget=budget, seed=11, form=oracle.CANON, n_threads=n)
     dt = time.perf_counter() - t0
     mp = float(r["lane_nev"].sum()) * curve.n_data
-    return {"value": mp / dt, "unit": "model-point evals/s", "cores": cores, "kind": "port",
+    return {"value": mp / dt, "unit": "model-point evals/s", "cores": n, "kind": "port",
             "sample": f"{nfit} simplexes x {budget} chi2 evals on the same 400-point curve "
-                      f"({int(r['lane_nev'].sum())} evals, {dt:.1f} s wall, oracle canonical form, "
-                      f"{cores} threads)",
+                      f"({int(r['lane_nev'].sum())} evals, {dt:.1f} s wall, oracle canonical form, {n} threads)",
             "fits_per_s": nfit / dt}
 
 
-def run_form(prec, args):
+def run_stage_one(prec, args):
     """Warm up, then time args.steps Stage One steps of one precision form."""
-    from asteroid_amd import api, flops
-    v, curve, truth, space = make_workload(prec)
+    from asteroid_amd import api, flops, workloads
+    v, curve, truth, space = workloads.stage_one(prec)
     group = 256
     n_groups = args.fits // group
     ctx = api.Context(v)
@@ -134,20 +152,12 @@ def run_form(prec, args):
     ctx.close()
     flop_per_eval = flops.eval_flops(v, curve.mjd)
     achieved = flop_per_eval * tot_evals / (kern_ms / 1e3) / 1e12
-    if prec == MIXED:
-        f64, f32 = flops.eval_flops_mixed(v, curve.mjd)
-        # peak of this op mix: FP64 FLOPs at the FP64 rate, FP32 FLOPs at the FP32 rate
-        peak = (f64 + f32) / (f64 / FP64_PEAK_TFLOPS + f32 / FP32_PEAK_TFLOPS)
-        kernel = "fit_kernel<float,TORQUE,LS,NF=1,mixed>"
-        split = {"fp64_flop_per_model_point": round(f64 / curve.n_data, 2),
-                 "fp32_flop_per_model_point": round(f32 / curve.n_data, 2)}
-    else:
-        peak = FP64_PEAK_TFLOPS
-        kernel = "fit_kernel<double,TORQUE,LS,NF=1>"
-        split = {}
+    peak, split = peak_of(prec, v, curve.mjd)
+    kernel = {MIXED: "fit_kernel<float,TORQUE,LS,NF=1,mixed>", FP64: "fit_kernel<double,TORQUE,LS,NF=1>",
+              F32SIMPLEX: "fit_kernel<float,TORQUE,LS,NF=1>"}[prec]
     roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 2), "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 4), "traffic": traffic_from_profile(name),
-            "kernel": kernel, "flop_per_model_point": round(flop_per_eval / curve.n_data, 2), **split,
+            "frac": round(achieved / peak, 4), "traffic": traffic_from_profile(name), "kernel": kernel,
+            "flop_per_model_point": round(flop_per_eval / curve.n_data, 2), **split,
             "avg_launch_ms": round(kern_ms / max(launches, 1), 3), "launches": launches}
     return {"value": tot_points / wall, "wall": wall, "ms_per_step": wall / args.steps * 1e3,
             "fits_per_s": args.fits * args.steps / wall, "evals_per_s": tot_evals / wall,
@@ -155,11 +165,11 @@ def run_form(prec, args):
             "roofline": roof}
 
 
-def gate_at_best_fits(res_mx):
+def gate_stage_one(res_mx):
     """FP64 chi^2 (GPU FP64 form, bitwise the oracle's) of the mixed run's
     per-group best models against the chi^2 the mixed run reported."""
-    from asteroid_amd import abi, api
-    v, curve, truth, space = make_workload(abi.PREC_FP64)
+    from asteroid_amd import api, workloads
+    v, curve, truth, space = workloads.stage_one(FP64)
     ok = res_mx["best_f"] < 1e29
     with api.Context(v) as ctx:
         ctx.set_model_space(space)
@@ -170,66 +180,272 @@ def gate_at_best_fits(res_mx):
             "n_models": int(ok.sum()), "tolerance": 1e-6, "pass": bool(rel.max() <= 1e-6 + 2.0 ** -23)}
 
 
+def request_stream(prec, rate, duration, budget):
+    """A Poisson request stream.  Generated before the process touches the
+    GPU: the curve generator forks worker processes."""
+    from asteroid_amd import workloads
+    times, jobs, made = workloads.requests(prec, rate, duration, eval_budget=budget)
+    return rate, times, jobs
+
+
+def serving_leg(prec, stream, max_resident_fits=1 << 18):
+    """One Poisson request stream through the serving API; latency percentiles."""
+    from asteroid_amd import api, variants
+    rate, times, jobs = stream
+    ctx = api.Context(variants.ls_torque(prec))
+    ctx.serve_start(max_resident_fits=max_resident_fits)
+    ids = []
+    t0 = time.perf_counter()
+    for t, j in zip(times, jobs):
+        dt = t0 + t - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        ids.append(ctx.submit(**j))
+    res = ctx.wait_all(ids, timeout_s=900)
+    wall = time.perf_counter() - t0
+    st = ctx.serve_stats()
+    ctx.serve_stop()
+    ctx.close()
+    lat = np.array([res[i]["latency_ms"] for i in ids])
+    fits = sum(j["n_fits"] for j in jobs)
+    mp = sum(res[i]["model_points"] for i in ids)
+    return {"rate_req_per_s": rate, "requests": len(ids), "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "fits_per_s": fits / wall, "model_points_per_s": mp / wall,
+            "wall_s": wall, "launches": st["launches"], "ok": all(res[i]["status"] == 0 for i in ids),
+            "lat_ms": lat}, jobs, times
+
+
+def main_stage_one(args):
+    forms = {"both": [MIXED, FP64], "mixed": [MIXED], "fp64": [FP64], "f32simplex": [F32SIMPLEX]}[args.precision]
+    stream = None if args.no_serving else request_stream(forms[0], 30.0, 4.0, 200)
+    results = {prec: run_stage_one(prec, args) for prec in forms}
+    head_prec = forms[0]
+    head = results[head_prec]
+    out = base_line(head["value"], args.steps, args.warmup, head["ms_per_step"], DTYPE[head_prec],
+                    {"workload": "stage-one random search (LS ellipsoid + torque, P_PSI, INTERP)",
+                     "n_obs": 400, "span_days": 3.0, "filters": 1, "simplexes": args.fits,
+                     "evals_per_simplex": args.budget, "size_min": SIZE_MIN, "group_size": 256,
+                     "time_step_days": 1e-2, "precision": PREC_NAMES[head_prec]})
+    out.update({"fits_per_s": head["fits_per_s"], "evals_per_s": head["evals_per_s"],
+                "evals_per_fit": round(head["evals_per_fit"], 1), "best_chi2": float(np.min(head["best_f"])),
+                "roofline": head["roofline"]})
+    if MIXED in results:
+        out["precision_gate"] = gate_stage_one(results[MIXED])
+    for prec, r in results.items():
+        if prec != head_prec:
+            out[PREC_NAMES[prec]] = {"value": r["value"], "ms_per_step": r["ms_per_step"],
+                                     "fits_per_s": r["fits_per_s"], "evals_per_fit": round(r["evals_per_fit"], 1),
+                                     "best_chi2": float(np.min(r["best_f"])), "roofline": r["roofline"]}
+    if not args.no_serving:
+        log("serving leg ...")
+        lat, _, _ = serving_leg(head_prec, stream)
+        del lat["lat_ms"]
+        out["latency"] = dict(lat, workload="Poisson 30 req/s for 4 s, batched-style curves, "
+                                            "256-4096 starts x 200 chi2 evals each")
+    if not args.no_cpu:
+        log("timing the CPU oracle sample ...")
+        out["cpu_baseline"] = cpu_baseline(args.budget, args.cpu_seconds)
+    return out
+
+
+# ------------------------------------------------- other configurations
+def run_batch(jobs, prec, work_points=8000.0):
+    from asteroid_amd import api, flops, variants
+    with api.Context(variants.ls_torque(prec)) as ctx:
+        t0 = time.perf_counter()
+        res = ctx.run_jobs(jobs, work_points_per_launch=work_points)
+        wall = time.perf_counter() - t0
+        ms, launches = ctx.last_timing()
+    mp = sum(r["model_points"] for r in res)
+    fits = sum(j["n_fits"] for j in jobs)
+    # algorithmic FLOPs of the evaluations done, FP64 and FP32 apart
+    f64 = f32 = 0.0
+    for j, r in zip(jobs, res):
+        v, mjd = j["space"].variant, j["curve"].mjd
+        if prec == MIXED:
+            a, b = flops.eval_flops_mixed(v, mjd)
+        else:
+            a, b = flops.eval_flops(v, mjd), 0.0
+        f64 += a * r["n_evals"]
+        f32 += b * r["n_evals"]
+    achieved = (f64 + f32) / (ms / 1e3) / 1e12
+    peak = (f64 + f32) / (f64 / FP64_PEAK_TFLOPS + f32 / FP32_PEAK_TFLOPS)
+    return res, {"value": mp / wall, "fits_per_s": fits / wall, "wall_s": wall, "kernel_ms": ms,
+                 "launches": launches, "evals": int(sum(r["n_evals"] for r in res)),
+                 "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 2),
+                              "unit": "TFLOP/s", "frac": round(achieved / peak, 4)}}
+
+
+def gate_jobs(jobs, res_mx):
+    """FP64 chi^2 of each mixed job's reported best model."""
+    from asteroid_amd import api, workloads
+    jobs64 = workloads.with_precision(jobs, FP64)
+    rel = []
+    for j, r in zip(jobs64, res_mx):
+        if r["best_f"] >= 1e29:
+            continue
+        with api.Context(j["space"].variant) as ctx:
+            ctx.set_model_space(j["space"])
+            ctx.upload_curve(j["curve"])
+            c64, _ = ctx.chi2_batch(r["best_params"])
+        rel.append(abs(r["best_f"] - c64[0]) / c64[0])
+    rel = np.array(rel)
+    return {"max_rel_chi2_err": float(rel.max()), "median_rel_chi2_err": float(np.median(rel)),
+            "n_models": int(len(rel)), "tolerance": 1e-6, "pass": bool(rel.max() <= 1e-6 + 2.0 ** -23)}
+
+
+def main_small(args):
+    """4096 random starts x 200 simplex steps on the 400-point curve, FP64,
+    on the GPU and on the host CPU (the oracle, all cores)."""
+    import oracle
+    from asteroid_amd import api, workloads
+    v, curve, truth, space = workloads.stage_one(FP64)
+    with api.Context(v) as ctx:
+        ctx.set_model_space(space)
+        ctx.upload_curve(curve)
+        ctx.rng_seed(3)
+        ctx.run(16, 256, nstages=1, n_steps=200)  # warm-up
+        ctx.rng_seed(3)
+        ctx.reset_best()
+        t0 = time.perf_counter()
+        ctx.run(16, 256, nstages=1, n_steps=200)
+        wall = time.perf_counter() - t0
+        e, p = ctx.last_counts()
+        f, params, dv = ctx.fetch()
+    log(f"GPU: {e} evals in {wall:.3f} s")
+    pb = oracle.Problem(space, curve)
+    n = cores()
+    t0 = time.perf_counter()
+    want = pb.run(16, 256, nstages=1, n_steps=200, seed=3, form=oracle.CANON, n_threads=n)
+    cpu_wall = time.perf_counter() - t0
+    same = bool(np.array_equal(want["f"], f) and np.array_equal(want["params"], params))
+    out = base_line(p / wall, 1, 1, wall * 1e3, DTYPE[FP64],
+                    {"workload": "plain FP64 small: 4096 random starts x 200 simplex steps, 400-point curve",
+                     "n_obs": 400, "span_days": 3.0, "simplexes": 4096, "n_steps": 200})
+    out.update({"fits_per_s": 4096 / wall, "evals": e, "best_chi2": float(np.min(f)),
+                "identical_to_oracle": same,
+                "cpu_baseline": {"value": float(want["lane_nev"].sum()) * curve.n_data / cpu_wall,
+                                 "unit": "model-point evals/s", "cores": n, "kind": "port",
+                                 "sample": f"the whole run: 4096 simplexes x 200 steps ({cpu_wall:.1f} s wall)",
+                                 "fits_per_s": 4096 / cpu_wall}})
+    return out
+
+
+def main_reopt(args):
+    from asteroid_amd import workloads
+    jobs_mx, truth = workloads.reopt_bc(MIXED)
+    jobs64 = workloads.with_precision(jobs_mx, FP64)
+    res_mx, m_mx = run_batch(jobs_mx, MIXED)
+    log(f"mixed: {m_mx}")
+    res_64, m_64 = run_batch(jobs64, FP64)
+    log(f"fp64: {m_64}")
+    b_mx = min(res_mx, key=lambda r: r["best_f"])
+    b_64 = min(res_64, key=lambda r: r["best_f"])
+    dpar = np.abs(b_mx["best_params"] - b_64["best_params"]) / np.maximum(np.abs(b_64["best_params"]), 1e-300)
+    out = base_line(m_mx["value"], 1, 0, m_mx["wall_s"] * 1e3, DTYPE[MIXED],
+                    {"workload": "stage-two reoptimisation, BC no torque, 3 filters, 818 points over 5 days, "
+                                 "TIME_STEP 5e-3, 20 starts (truth +-5%) x 3328 simplexes", "simplexes": 66560,
+                     "n_steps": 5000})
+    out.update({"fits_per_s": m_mx["fits_per_s"], "roofline": m_mx["roofline"], "mixed": m_mx, "fp64": m_64,
+                "best_chi2_mixed": b_mx["best_f"], "best_chi2_fp64": b_64["best_f"],
+                "best_rel_chi2_diff": abs(b_mx["best_f"] - b_64["best_f"]) / b_64["best_f"],
+                "best_param_max_rel_diff": float(dpar.max()), "precision_gate": gate_jobs(jobs_mx, res_mx)})
+    return out
+
+
+def main_batched(args):
+    from asteroid_amd import workloads
+    t = time.perf_counter()
+    jobs_mx, made = workloads.batched(MIXED, eval_budget=args.batch_budget)
+    log(f"512 curves generated in {time.perf_counter() - t:.1f} s")
+    jobs64 = workloads.with_precision(jobs_mx, FP64)
+    res_mx, m_mx = run_batch(jobs_mx, MIXED)
+    log(f"mixed: {m_mx}")
+    res_64, m_64 = run_batch(jobs64, FP64)
+    log(f"fp64: {m_64}")
+    pts = sum(j["curve"].n_data for j in jobs_mx)
+    out = base_line(m_mx["value"], 1, 0, m_mx["wall_s"] * 1e3, DTYPE[MIXED],
+                    {"workload": "batched uneven: 512 curves, 50-2000 points (sum %d), 1-4 filters, 1-8 days, "
+                                 "60%% LS+torque / 30%% BC / 10%% BW_BALL+torque, 1024 simplexes each" % pts,
+                     "simplexes": 512 * 1024, "evals_per_simplex": args.batch_budget})
+    out.update({"fits_per_s": m_mx["fits_per_s"], "roofline": m_mx["roofline"], "mixed": m_mx, "fp64": m_64,
+                "precision_gate": gate_jobs(jobs_mx, res_mx)})
+    return out
+
+
+def fifo_latency(arrivals, service):
+    """Latencies (ms) of a single FIFO server with these arrival and service times (s)."""
+    free, lat = 0.0, []
+    for a, s in zip(arrivals, service):
+        start = max(a, free)
+        free = start + s
+        lat.append(free - a)
+    return np.array(lat) * 1e3
+
+
+def main_serving(args):
+    """Continuous batching (mixed form) over swept Poisson rates, beside the
+    plain FP64 form run one request at a time (its own launch cycles per
+    request, FIFO), the latter timed on each stream's first requests."""
+    from asteroid_amd import api, variants, workloads
+    streams = [request_stream(MIXED, rate, min(args.duration, args.max_requests / rate), args.serve_budget)
+               for rate in args.rates]
+    rows = []
+    for rate, stream in zip(args.rates, streams):
+        lat, jobs, times = serving_leg(MIXED, stream)
+        log(f"rate {rate}: {lat}")
+        k = min(args.baseline_requests, len(jobs))
+        jobs64 = workloads.with_precision(jobs[:k], FP64)
+        svc = []
+        with api.Context(variants.ls_torque(FP64)) as ctx:
+            for j in jobs64:
+                t0 = time.perf_counter()
+                ctx.run_jobs([j])
+                svc.append(time.perf_counter() - t0)
+        l64 = fifo_latency(times[:k], svc)
+        lat["first_k"] = {"k": k, "p50_ms": float(np.percentile(lat["lat_ms"][:k], 50)),
+                          "p99_ms": float(np.percentile(lat["lat_ms"][:k], 99))}
+        lat["fp64_one_at_a_time_first_k"] = {"p50_ms": float(np.percentile(l64, 50)),
+                                             "p99_ms": float(np.percentile(l64, 99)),
+                                             "mean_service_ms": float(np.mean(svc) * 1e3)}
+        del lat["lat_ms"]
+        rows.append(lat)
+    best = max(rows, key=lambda r: r["fits_per_s"])
+    out = base_line(best["model_points_per_s"], 1, 0, best["wall_s"] * 1e3, DTYPE[MIXED],
+                    {"workload": "serving: Poisson arrivals, batched-style curves, 256-4096 starts x "
+                                 f"{args.serve_budget} chi2 evals per request", "rates": args.rates})
+    out.update({"fits_per_s": best["fits_per_s"], "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"],
+                "sweep": rows})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["stage1", "small", "reopt", "batched", "serving"], default="stage1")
     ap.add_argument("--precision", choices=["both", "mixed", "fp64", "f32simplex"], default="both")
     ap.add_argument("--fits", type=int, default=1 << 18)
     ap.add_argument("--budget", type=int, default=4800)
+    ap.add_argument("--batch-budget", type=int, default=1000)
+    ap.add_argument("--serve-budget", type=int, default=200)
+    ap.add_argument("--rates", type=float, nargs="+", default=[10, 30, 100, 300, 1000, 2000])
+    ap.add_argument("--duration", type=float, default=6.0)
+    ap.add_argument("--max-requests", type=float, default=200.0)
+    ap.add_argument("--baseline-requests", type=int, default=40)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the fits are independent and shard across GPUs "
               "without communication (see DESIGN.md); run it with --gpus 1.", file=sys.stderr)
         sys.exit(2)
-
-    from asteroid_amd import abi
-    forms = {"both": [MIXED, abi.PREC_FP64], "mixed": [MIXED], "fp64": [abi.PREC_FP64],
-             "f32simplex": [abi.PREC_F32SIMPLEX]}[args.precision]
-    results = {prec: run_form(prec, args) for prec in forms}
-    head_prec = forms[0]
-    head = results[head_prec]
-    dtype = {abi.PREC_FP64: "f64", abi.PREC_F32SIMPLEX: "f64 model, f32 simplex",
-             MIXED: "f64 Euler-angle integration and chi^2 sums, f32 geometry and brightness, f32 simplex"}
-    out = {
-        "metric": "model-point evals/s, fits/s at 1e-6 rel chi^2 vs FP64; p50/p99 fit latency",
-        "value": head["value"],
-        "unit": "model-point evals/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": head["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": dtype[head_prec],
-        "data": "synthetic",
-        "config": {"workload": "stage-one random search (LS ellipsoid + torque, P_PSI, INTERP)",
-                   "n_obs": N_OBS, "span_days": SPAN_DAYS, "filters": 1, "simplexes": args.fits,
-                   "evals_per_simplex": args.budget, "size_min": SIZE_MIN, "group_size": 256,
-                   "time_step_days": 1e-2, "precision": PREC_NAMES[head_prec]},
-        "fits_per_s": head["fits_per_s"],
-        "evals_per_s": head["evals_per_s"],
-        "evals_per_fit": round(head["evals_per_fit"], 1),
-        "best_chi2": float(np.min(head["best_f"])),
-        "roofline": head["roofline"],
-    }
-    if MIXED in results:
-        out["precision_gate"] = gate_at_best_fits(results[MIXED])
-    for prec, r in results.items():
-        if prec == head_prec:
-            continue
-        out[PREC_NAMES[prec]] = {"value": r["value"], "ms_per_step": r["ms_per_step"],
-                                 "fits_per_s": r["fits_per_s"], "evals_per_fit": round(r["evals_per_fit"], 1),
-                                 "best_chi2": float(np.min(r["best_f"])), "roofline": r["roofline"]}
-    if not args.no_cpu:
-        log("timing the CPU oracle sample ...")
-        out["cpu_baseline"] = cpu_baseline(abi.PREC_FP64, args.budget, args.cpu_seconds)
-    print(json.dumps(out), flush=True)
+    fn = {"stage1": main_stage_one, "small": main_small, "reopt": main_reopt, "batched": main_batched,
+          "serving": main_serving}[args.workload]
+    print(json.dumps(fn(args)), flush=True)
 
 
 if __name__ == "__main__":
