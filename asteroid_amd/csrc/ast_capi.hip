@@ -175,6 +175,7 @@ struct CurveH {  // a validated light curve as the kernels read it
   std::vector<PointRec> pts;
   double sum_w[AST_MAX_FILTERS] = {0};
   int n_data = 0, n_filters = 0;
+  int64_t rk4_steps = 0;  // RK4 steps of one evaluation
 };
 
 int make_curve(const ast_variant& v, const ast_obs* obs, int n_data, int n_filters, const ast_ephem3* eph,
@@ -199,6 +200,7 @@ int make_curve(const ast_variant& v, const ast_obs* obs, int n_data, int n_filte
       int nsteps = (int)((t2 - t1) / v.time_step + 1);
       double h = (t2 - t1) / nsteps;
       p.nsteps = nsteps;
+      C.rk4_steps += nsteps;
       p.h = h;
       p.hh = 0.5 * h;
       p.h6 = 1 / 6.0 * h;
@@ -323,6 +325,7 @@ struct DevJob {
   RunParams rp;
   int n_data = 0, n_filters = 0;
   int cls = 0;                 // kernel class index (for streams and counters)
+  double cost = 1;             // one evaluation, in observation-equivalents (eval_cost)
   int cf = 8;                  // simplex float bytes
   // device memory (owned unless borrowed)
   PointRec* pts = nullptr;
@@ -352,9 +355,20 @@ void job_free(DevJob& j) {
   if (!j.borrowed) { dev_free(j.rng); dev_free(j.best_f); dev_free(j.best_params); dev_free(j.best_dV); }
 }
 
-int ticks_for(int n_data, double work_points) {
-  double t = work_points / (n_data > 0 ? n_data : 1);
+int ticks_for(double cost, double work) {
+  double t = work / (cost > 1 ? cost : 1);
   return t < 1 ? 1 : (t > 4096 ? 4096 : (int)t);
+}
+
+// Cost of one evaluation in observation-equivalents: the observations plus
+// the RK4 steps weighted by a step's FLOPs over an observation's (the
+// per_rk4 / per_point rows of the FLOP table: about 1.9 with torque, 1.4
+// without, 2.8 for the cheaper Lambert + ball brightness).  Launch sizes
+// follow it, so sparse curves (many RK4 steps per observation) and dense
+// ones do similar work per launch.
+double eval_cost(const ast_variant& v, const CurveH& C) {
+  const double w = v.torque ? (v.bw_ball ? 2.8 : 1.9) : 1.4;
+  return (double)C.n_data + w * (double)C.rk4_steps;
 }
 
 // Device memory of one job (its own streams and bests unless borrowed).  The
@@ -406,7 +420,7 @@ int job_upload(DevJob& j, const CurveH& C, const double* params0, hipStream_t s)
 }
 
 template <class CF>
-int job_descs_typed(DevJob& j, double work_points) {
+int job_descs_typed(DevJob& j, int ticks) {
   const int G = j.rp.n_groups;
   std::vector<GroupDesc<CF>> d(G);
   double sm = j.rp.size_min > 0 ? j.rp.size_min : (j.cf == 8 ? 1e-10 : 1e-5);  // asteroid.h:233-237
@@ -434,7 +448,7 @@ int job_descs_typed(DevJob& j, double work_points) {
     D.reopt = j.space.x2.reopt;
     D.slot = j.slots[g];
     D.group_in_job = g;
-    D.ticks = ticks_for(j.n_data, work_points);
+    D.ticks = ticks;
   }
   if (!j.descs) RETIF(dev_alloc((char**)&j.descs, sizeof(GroupDesc<CF>) * G));
   HIPCHK(hipMemcpy(j.descs, d.data(), sizeof(GroupDesc<CF>) * G, hipMemcpyHostToDevice));
@@ -443,8 +457,8 @@ int job_descs_typed(DevJob& j, double work_points) {
   return 0;
 }
 
-int job_descs(DevJob& j, double work_points) {
-  return j.cf == 8 ? job_descs_typed<double>(j, work_points) : job_descs_typed<float>(j, work_points);
+int job_descs(DevJob& j, int ticks) {
+  return j.cf == 8 ? job_descs_typed<double>(j, ticks) : job_descs_typed<float>(j, ticks);
 }
 
 // The shapes the kernels assume: one workgroup of pool.G threads per group,
@@ -796,6 +810,7 @@ int make_job(ast_ctx* c, const ast_job* job, int G, std::unique_ptr<DevJob>* out
   j->cls = class_of(job->variant, job->n_filters);
   j->n_data = C.n_data;
   j->n_filters = C.n_filters;
+  j->cost = eval_cost(job->variant, C);
   j->t_submit = Clock::now();
   p0->clear();
   if (job->params0) p0->assign(job->params0, job->params0 + j->space.n_params);
@@ -810,7 +825,7 @@ int make_job(ast_ctx* c, const ast_job* job, int G, std::unique_ptr<DevJob>* out
 int serve_admit(ast_ctx* c, Serve& S, DevJob& j, const CurveH& curve, const std::vector<double>& p0) {
   const hipStream_t s = c->streams[stream_of(j.cls)];
   RETIF(job_upload(j, curve, p0.empty() ? nullptr : p0.data(), s));
-  RETIF(job_descs(j, S.cfg.work_points_per_launch));
+  RETIF(job_descs(j, ticks_for(j.cost, S.cfg.work_points_per_launch)));
   HIPCHK(job_launch(j, c->pool, 0, s));
   return 0;
 }
@@ -1104,24 +1119,22 @@ int ast_run(ast_ctx* c, const ast_run_cfg* rc) {
   // the null stream, which the job's non-blocking stream does not wait for.
   HIPCHK(hipDeviceSynchronize());
   for (int g = 0; g < rc->n_groups; ++g) j.slots.push_back(g);
-  double work = 0;
+  int ticks;
   if (rc->ticks_per_launch > 0) {
-    work = (double)rc->ticks_per_launch * c->curve.n_data;
+    ticks = rc->ticks_per_launch > 4096 ? 4096 : rc->ticks_per_launch;
   } else {  // about 100 ms of kernel per launch when the GPU is full
     double tot = (double)fits * c->curve.n_data;
-    double ticks = 3.0e9 / (tot > 1 ? tot : 1);
-    ticks = ticks < 16 ? 16 : (ticks > 4096 ? 4096 : ticks);
-    work = ticks * c->curve.n_data;
+    double t = 3.0e9 / (tot > 1 ? tot : 1);
+    ticks = (int)(t < 16 ? 16 : (t > 4096 ? 4096 : t));
   }
-  RETIF(job_descs(j, work));
+  RETIF(job_descs(j, ticks));
   HIPCHK(job_launch(j, c->pool, 0, s));
   c->last_ms = 0;
   c->last_launches = 0;
   // Every tick completes one evaluation of each running simplex; a stage
   // needs at most (N+1) + n_steps*(N+2) of them.
   const double per_stage = (double)(c->space.n_params + 1) + (double)rc->n_steps * (c->space.n_params + 2);
-  const double ticks_per_launch = (double)ticks_for(c->curve.n_data, work);
-  const long max_cycles = (long)((per_stage * rc->nstages + 16.0) / ticks_per_launch) + 4;
+  const long max_cycles = (long)((per_stage * rc->nstages + 16.0) / ticks) + 4;
   std::vector<DevJob*> jobs{&j};
   for (long L = 0; j.groups_done < rc->n_groups; ++L) {
     if (L > max_cycles) {
@@ -1261,7 +1274,7 @@ int ast_serve_start(ast_ctx* c, const ast_serve_cfg* cfg) {
   c->serve.reset(new Serve());
   c->serve->cfg = *cfg;
   c->serve->cfg.group_size = G;
-  if (c->serve->cfg.work_points_per_launch <= 0) c->serve->cfg.work_points_per_launch = 8000;
+  if (c->serve->cfg.work_points_per_launch <= 0) c->serve->cfg.work_points_per_launch = 16000;
   c->serve->th = std::thread(serve_loop, c);
   return AST_OK;
 }
@@ -1289,7 +1302,7 @@ int ast_run_jobs(ast_ctx* c, const ast_job* jobs, int n_jobs, const ast_batch_cf
   if (c->serve) return AST_E_BUSY;
   const int G = (cfg && cfg->group_size) ? cfg->group_size : 256;
   if (G < 64 || G > 1024 || (G & 63)) return AST_E_ARG;
-  const double work = (cfg && cfg->work_points_per_launch > 0) ? cfg->work_points_per_launch : 8000.0;
+  const double work = (cfg && cfg->work_points_per_launch > 0) ? cfg->work_points_per_launch : 16000.0;
   std::vector<std::unique_ptr<DevJob>> J(n_jobs);
   std::vector<CurveH> curves(n_jobs);
   std::vector<std::vector<double>> p0(n_jobs);
@@ -1311,7 +1324,7 @@ int ast_run_jobs(ast_ctx* c, const ast_job* jobs, int n_jobs, const ast_batch_cf
     for (int g = 0; g < j.rp.n_groups; ++g) j.slots.push_back(slot++);
     const hipStream_t s = c->streams[stream_of(j.cls)];
     rc = job_upload(j, curves[i], p0[i].empty() ? nullptr : p0[i].data(), s);
-    if (!rc) rc = job_descs(j, work);
+    if (!rc) rc = job_descs(j, ticks_for(j.cost, work));
     if (!rc) {
       hipError_t e = job_launch(j, c->pool, 0, s);
       if (e != hipSuccess) {
@@ -1327,7 +1340,7 @@ int ast_run_jobs(ast_ctx* c, const ast_job* jobs, int n_jobs, const ast_batch_cf
   long max_cycles = 0;
   for (DevJob* j : run) {
     const double per_stage = (double)(j->space.n_params + 1) + (double)j->rp.n_steps * (j->space.n_params + 2);
-    const long mc = (long)((per_stage * j->rp.nstages + 16.0) / ticks_for(j->n_data, work)) + 4;
+    const long mc = (long)((per_stage * j->rp.nstages + 16.0) / ticks_for(j->cost, work)) + 4;
     if (mc > max_cycles) max_cycles = mc;
   }
   c->last_ms = 0;

@@ -249,7 +249,7 @@ def main_stage_one(args):
 
 
 # ------------------------------------------------- other configurations
-def run_batch(jobs, prec, work_points=8000.0):
+def run_batch(jobs, prec, work_points=16000.0):
     from asteroid_amd import api, flops, variants
     with api.Context(variants.ls_torque(prec)) as ctx:
         t0 = time.perf_counter()
@@ -393,7 +393,8 @@ def main_serving(args):
     rows = []
     for rate, stream in zip(args.rates, streams):
         lat, jobs, times = serving_leg(MIXED, stream)
-        log(f"rate {rate}: {lat}")
+        log(f"rate {rate}: {lat['requests']} requests, p50 {lat['p50_ms']:.0f} ms, p99 {lat['p99_ms']:.0f} ms, "
+            f"{lat['fits_per_s']:.0f} fits/s")
         k = min(args.baseline_requests, len(jobs))
         jobs64 = workloads.with_precision(jobs[:k], FP64)
         svc = []

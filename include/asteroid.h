@@ -244,7 +244,8 @@ typedef struct {
   int64_t max_resident_fits;     /* simplex slots on the device                  */
   int group_size;                /* simplexes per group (workgroup); 0 = 256     */
   int64_t hbm_budget_bytes;      /* admission limit on resident job bytes; 0 = slots only */
-  double work_points_per_launch; /* model-point evaluations per simplex per launch; 0 = 8000 */
+  double work_points_per_launch; /* evaluation work per simplex per launch, in observation-
+                                    equivalents (an RK4 step counts 1.4-2.8); 0 = 16000 */
 } ast_serve_cfg;
 
 typedef struct {
@@ -303,7 +304,7 @@ typedef struct {
  * its own simplex numbers).  Refused while the context is serving. */
 typedef struct {
   int group_size;                /* simplexes per group; 0 = 256 */
-  double work_points_per_launch; /* model-point evaluations per simplex per launch; 0 = 8000 */
+  double work_points_per_launch; /* as in ast_serve_cfg; 0 = 16000 */
 } ast_batch_cfg;
 
 int ast_run_jobs(ast_ctx* ctx, const ast_job* jobs, int n_jobs, const ast_batch_cfg* cfg, ast_result* results);
