@@ -6,7 +6,8 @@
 // stdout and result-file lines, on top of the C ABI of libasteroid_hip.so.
 // The reference chooses its model variant at compile time (makefile:3-36);
 // here the same macros are run-time flags (-DTORQUE, -DP_PSI, -DBC,
-// -DBW_BALL, -DINTERP, -DACC), defaulting to -DP_PSI -DTORQUE -DINTERP.
+// -DBW_BALL, -DINTERP, -DACC), defaulting to -DP_PSI -DTORQUE -DINTERP;
+// -DMIXED (no counterpart in the reference) selects the mixed-precision form.
 // A failing library call prints the error and exits with status 1 (the
 // reference's ERR macro, cuda_errors.h:4-14).
 #include <cmath>
@@ -207,7 +208,7 @@ void usage() {
   printf("-reopt : reoptimize the model provided with -m switch\n");
   printf("-seed SEED : use the SEED number to initialize the random number generator\n");
   printf("-t : travelling reoptimization\n");
-  printf("-D<MACRO> : model variant (TORQUE, P_PSI, BC, BW_BALL, INTERP, ACC); default -DP_PSI -DTORQUE -DINTERP\n");
+  printf("-D<MACRO> : model variant (TORQUE, P_PSI, BC, BW_BALL, INTERP, ACC, MIXED); default -DP_PSI -DTORQUE -DINTERP\n");
   printf("-blocks n, -bsize n : reseed groups per launch and simplexes per group (N_BLOCKS, BSIZE)\n");
   printf("-steps n : simplex steps per stage (N_STEPS); -budget n : chi^2 evaluations per stage\n");
   printf("-time_step d : ODE time step in days (TIME_STEP); -device n : HIP device\n");
@@ -252,6 +253,7 @@ int main(int argc, char** argv) {
       else if (mac == "BW_BALL") v.bw_ball = 1;
       else if (mac == "INTERP") v.interp = 1;
       else if (mac == "ACC") acc = 1;
+      else if (mac == "MIXED") acc = 2;
       else {
         printf("Unsupported macro -D%s\n", mac.c_str());
         return 1;
@@ -294,7 +296,7 @@ int main(int argc, char** argv) {
     else die("Bad arguments!");
   }
   if (!any_macro) { v.torque = 1; v.p_psi = 1; v.interp = 1; }
-  v.precision = acc ? AST_PREC_FP64 : AST_PREC_F32SIMPLEX;
+  v.precision = acc == 2 ? AST_PREC_MIXED : (acc ? AST_PREC_FP64 : AST_PREC_F32SIMPLEX);
   if (j_input == -1) die("-i parameter is missing!");
   if (plot) die("-plot is not built in this version (fitting only)");
   if (reopt && !model) die("-reopt and -plot switches require -m switch!");
