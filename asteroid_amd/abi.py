@@ -82,6 +82,11 @@ class BatchCfg(C.Structure):
     _fields_ = [("group_size", C.c_int), ("work_points_per_launch", C.c_double)]
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("torque", C.c_int), ("bc", C.c_int), ("bw_ball", C.c_int), ("precision", C.c_int),
+                ("filter_capacity", C.c_int), ("waves", C.c_int), ("kernel_ms", C.c_double * 3)]
+
+
 class ServeStats(C.Structure):
     _fields_ = [("pending", C.c_int), ("resident", C.c_int), ("free_fits", C.c_int64),
                 ("resident_bytes", C.c_int64), ("launches", C.c_int64), ("evals", C.c_int64),
@@ -151,6 +156,8 @@ def load():
         "ast_last_counts": (I, [P, pI64, pI64]),
         "ast_default_model_space": (I, [C.POINTER(Variant), pD, pI, pI, pI, pI, pD, C.POINTER(X2)]),
         "ast_run_jobs": (I, [P, C.POINTER(Job), I, C.POINTER(BatchCfg), C.POINTER(Result)]),
+        "ast_autotune": (I, [P, C.POINTER(Job), I, C.POINTER(BatchCfg), I, C.POINTER(TuneEntry), I, pI]),
+        "ast_set_waves": (I, [P, C.POINTER(Variant), I, I]),
         "ast_serve_start": (I, [P, C.POINTER(ServeCfg)]),
         "ast_submit": (I, [P, C.POINTER(Job), C.POINTER(C.c_uint64)]),
         "ast_poll": (I, [P, C.POINTER(Result), I, pI, I]),
@@ -173,5 +180,5 @@ EXPORTED = (
     "ast_reset_best", "ast_set_start", "ast_run", "ast_fetch", "ast_fetch_lanes", "ast_chi2_batch",
     "ast_last_timing", "ast_last_counts", "ast_default_model_space", "ast_strerror",
     "ast_last_hip_error", "ast_serve_start", "ast_submit", "ast_poll", "ast_cancel", "ast_serve_stats",
-    "ast_serve_stop", "ast_run_jobs",
+    "ast_serve_stop", "ast_run_jobs", "ast_autotune", "ast_set_waves",
 )

@@ -165,6 +165,22 @@ class Context:
         _chk(self.lib.ast_run_jobs(self.h, arr, len(built), C.byref(cfg), out), "ast_run_jobs")
         return [self._result(out[k]) for k in range(len(built))]
 
+    def autotune(self, jobs, group_size=256, work_points_per_launch=0.0, cycles=4):
+        """Pick each kernel class's register budget by timing `jobs`; returns the report."""
+        built = [self._job(**j) for j in jobs]
+        arr = (abi.Job * len(built))(*[b[0] for b in built])
+        out = (abi.TuneEntry * 64)()
+        n = C.c_int()
+        cfg = abi.BatchCfg(int(group_size), float(work_points_per_launch))
+        _chk(self.lib.ast_autotune(self.h, arr, len(built), C.byref(cfg), int(cycles), out, 64, C.byref(n)),
+             "ast_autotune")
+        return [dict(torque=e.torque, bc=e.bc, bw_ball=e.bw_ball, precision=e.precision,
+                     filter_capacity=e.filter_capacity, waves=e.waves, kernel_ms=list(e.kernel_ms))
+                for e in out[:n.value]]
+
+    def set_waves(self, variant, n_filters, waves):
+        _chk(self.lib.ast_set_waves(self.h, C.byref(variant), int(n_filters), int(waves)), "ast_set_waves")
+
     # ------------------------------------------------------------- serving
     def serve_start(self, max_resident_fits, group_size=256, hbm_budget_bytes=0, work_points_per_launch=0.0):
         cfg = abi.ServeCfg(int(max_resident_fits), int(group_size), int(hbm_budget_bytes),

@@ -252,7 +252,6 @@ struct Pool {
   double* PRM = nullptr;
   int32_t *ctl = nullptr, *lstep = nullptr, *nev = nullptr, *nev_tot = nullptr, *steps_tot = nullptr;
   int32_t *g_reopt = nullptr, *g_stage = nullptr, *g_done = nullptr;
-  int32_t* running = nullptr;  // [MAX_CLASSES]
   std::vector<int> free_slots;
 };
 
@@ -260,7 +259,7 @@ void pool_free(Pool& p) {
   dev_free(p.X); dev_free(p.F); dev_free(p.X0); dev_free(p.XR); dev_free(p.XP); dev_free(p.DV);
   dev_free(p.fr); dev_free(p.sf); dev_free(p.IND); dev_free(p.PRM); dev_free(p.ctl); dev_free(p.lstep);
   dev_free(p.nev); dev_free(p.nev_tot); dev_free(p.steps_tot); dev_free(p.g_reopt); dev_free(p.g_stage);
-  dev_free(p.g_done); dev_free(p.running);
+  dev_free(p.g_done);
   p.nf = 0;
   p.n_slots = 0;
   p.free_slots.clear();
@@ -289,7 +288,6 @@ int pool_alloc(Pool& p, size_t nf, int G) {
   ALLOC(p.g_reopt, int32_t, nf / G);
   ALLOC(p.g_stage, int32_t, nf / G);
   ALLOC(p.g_done, int32_t, nf / G);
-  ALLOC(p.running, int32_t, MAX_CLASSES);
 #undef ALLOC
   if (rc) { pool_free(p); return rc; }
   p.nf = nf;
@@ -306,7 +304,7 @@ SlotBufs<CF> pool_bufs(const Pool& p, int cls) {
   B.X = (CF*)p.X; B.F = (CF*)p.F; B.IND = p.IND; B.X0 = (CF*)p.X0; B.XR = (CF*)p.XR; B.XP = (CF*)p.XP;
   B.PRM = p.PRM; B.DV = (CF*)p.DV; B.ctl = p.ctl; B.lstep = p.lstep; B.nev = p.nev; B.nev_tot = p.nev_tot;
   B.steps_tot = p.steps_tot; B.fr = (CF*)p.fr; B.sf = (CF*)p.sf; B.g_reopt = p.g_reopt; B.g_stage = p.g_stage;
-  B.g_done = p.g_done; B.running = p.running + cls; B.nf = p.nf;
+  B.g_done = p.g_done; B.nf = p.nf;
   return B;
 }
 
@@ -567,6 +565,8 @@ struct ast_ctx {
   std::unique_ptr<Serve> serve;
   ClassBatch batch[MAX_CLASSES];
   std::atomic<uint64_t> next_uid{1};
+  int waves[MAX_CLASSES];  // fit-kernel register budget per class (4, 3 or 2 waves per SIMD)
+  ast_ctx() { for (int k = 0; k < MAX_CLASSES; ++k) waves[k] = 4; }
 };
 
 namespace {
@@ -639,7 +639,6 @@ int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, in
     if (members[k].empty()) continue;
     const DevJob& j0 = *members[k][0];
     const hipStream_t s = c->streams[stream_of(k)];
-    HIPCHK(hipMemsetAsync(c->pool.running + k, 0, sizeof(int32_t), s));
     LaunchArgs a{};
     a.op = 1;
     a.stream = s;
@@ -651,6 +650,7 @@ int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, in
     a.n_groups = c->batch[k].n_groups;
     a.group_size = c->pool.G;
     a.n_filters = j0.n_filters;
+    a.waves = c->waves[k];
     HIPCHK(dispatch(j0.space.v, a));
     (*launches)++;
   }
@@ -817,6 +817,97 @@ int make_job(ast_ctx* c, const ast_job* job, int G, std::unique_ptr<DevJob>* out
   *curve = std::move(C);
   *out = std::move(j);
   return AST_OK;
+}
+
+// Upload `jobs` into slots 0.., run them in shared cycles and collect their
+// results (results may be null).  cycle_limit > 0 stops after that many
+// cycles (the autotuner's fixed work); 0 runs every job to completion.
+int batch_run(ast_ctx* c, const std::vector<const ast_job*>& jobs, int G, double work, long cycle_limit,
+              ast_result* results, double* kernel_ms, int* launches) {
+  const int n_jobs = (int)jobs.size();
+  std::vector<std::unique_ptr<DevJob>> J(n_jobs);
+  std::vector<CurveH> curves(n_jobs);
+  std::vector<std::vector<double>> p0(n_jobs);
+  size_t groups = 0;
+  for (int i = 0; i < n_jobs; ++i) {
+    RETIF(make_job(c, jobs[i], G, &J[i], &curves[i], &p0[i]));
+    J[i]->id = (uint64_t)i;
+    groups += (size_t)J[i]->rp.n_groups;
+  }
+  RETIF(set_device(c));
+  if ((size_t)c->pool.n_slots < groups || c->pool.G != G) RETIF(pool_alloc(c->pool, groups * G, G));
+  c->last_groups = 0;  // the pool's rows no longer hold an ast_run's lanes
+  std::vector<DevJob*> run;
+  int slot = 0;
+  const Clock::time_point t0 = Clock::now();
+  int rc = 0;
+  for (int i = 0; i < n_jobs && !rc; ++i) {
+    DevJob& j = *J[i];
+    for (int g = 0; g < j.rp.n_groups; ++g) j.slots.push_back(slot++);
+    const hipStream_t s = c->streams[stream_of(j.cls)];
+    rc = job_upload(j, curves[i], p0[i].empty() ? nullptr : p0[i].data(), s);
+    if (!rc) rc = job_descs(j, ticks_for(j.cost, work));
+    if (!rc) {
+      hipError_t e = job_launch(j, c->pool, 0, s);
+      if (e != hipSuccess) {
+        snprintf(g_hip_err, sizeof g_hip_err, "%s (init kernel)", hipGetErrorString(e));
+        rc = AST_E_HIP;
+      }
+    }
+    j.t_submit = j.t_admit = t0;
+    run.push_back(&j);
+  }
+  // Every tick completes one evaluation of each running simplex; a stage
+  // needs at most (N+1) + n_steps*(N+2) of them.
+  long max_cycles = 0;
+  for (DevJob* j : run) {
+    const double per_stage = (double)(j->space.n_params + 1) + (double)j->rp.n_steps * (j->space.n_params + 2);
+    const long mc = (long)((per_stage * j->rp.nstages + 16.0) / ticks_for(j->cost, work)) + 4;
+    if (mc > max_cycles) max_cycles = mc;
+  }
+  for (long L = 0; !rc; ++L) {
+    bool all = true;
+    for (DevJob* j : run) all = all && j->groups_done >= j->rp.n_groups;
+    if (all || (cycle_limit > 0 && L >= cycle_limit)) break;
+    if (L > max_cycles) {
+      set_err("fit did not terminate");
+      rc = AST_E_STATE;
+      break;
+    }
+    rc = run_cycle(c, run, kernel_ms, launches);
+  }
+  if (rc) (void)hipDeviceSynchronize();
+  int64_t ev_tot = 0, pts_tot = 0;
+  for (int i = 0; i < n_jobs; ++i) {
+    DevJob& j = *J[i];
+    if (results) {
+      ast_result& r = results[i];
+      std::memset(&r, 0, sizeof r);
+      if (!rc && i < (int)run.size()) {
+        int e = job_result(j, &r);
+        int64_t ev = 0;
+        if (!e) e = job_evals(c, j, &ev);
+        if (e && !rc) rc = e;
+        r.n_evals = ev;
+        r.model_points = ev * j.n_data;
+        r.submit_to_done_ms = std::chrono::duration<double, std::milli>(j.t_done - t0).count();
+        r.run_ms = r.submit_to_done_ms;
+        ev_tot += ev;
+        pts_tot += ev * j.n_data;
+      } else {
+        r.status = rc ? rc : AST_E_STATE;
+        r.best_f = 1e30;
+      }
+      r.id = (uint64_t)i;
+      r.tag = j.tag;
+    }
+    job_free(j);
+  }
+  if (results) {
+    c->last_evals = ev_tot;
+    c->last_points = pts_tot;
+  }
+  return rc;
 }
 
 // ------------------------------------------------------- serving thread
@@ -1303,87 +1394,68 @@ int ast_run_jobs(ast_ctx* c, const ast_job* jobs, int n_jobs, const ast_batch_cf
   const int G = (cfg && cfg->group_size) ? cfg->group_size : 256;
   if (G < 64 || G > 1024 || (G & 63)) return AST_E_ARG;
   const double work = (cfg && cfg->work_points_per_launch > 0) ? cfg->work_points_per_launch : 16000.0;
-  std::vector<std::unique_ptr<DevJob>> J(n_jobs);
-  std::vector<CurveH> curves(n_jobs);
-  std::vector<std::vector<double>> p0(n_jobs);
-  size_t groups = 0;
-  for (int i = 0; i < n_jobs; ++i) {
-    RETIF(make_job(c, &jobs[i], G, &J[i], &curves[i], &p0[i]));
-    J[i]->id = (uint64_t)i;
-    groups += (size_t)J[i]->rp.n_groups;
-  }
-  RETIF(set_device(c));
-  if ((size_t)c->pool.n_slots < groups || c->pool.G != G) RETIF(pool_alloc(c->pool, groups * G, G));
-  c->last_groups = 0;  // the pool's rows no longer hold an ast_run's lanes
-  std::vector<DevJob*> run;
-  int slot = 0;
-  const Clock::time_point t0 = Clock::now();
-  int rc = 0;
-  for (int i = 0; i < n_jobs && !rc; ++i) {
-    DevJob& j = *J[i];
-    for (int g = 0; g < j.rp.n_groups; ++g) j.slots.push_back(slot++);
-    const hipStream_t s = c->streams[stream_of(j.cls)];
-    rc = job_upload(j, curves[i], p0[i].empty() ? nullptr : p0[i].data(), s);
-    if (!rc) rc = job_descs(j, ticks_for(j.cost, work));
-    if (!rc) {
-      hipError_t e = job_launch(j, c->pool, 0, s);
-      if (e != hipSuccess) {
-        snprintf(g_hip_err, sizeof g_hip_err, "%s (init kernel)", hipGetErrorString(e));
-        rc = AST_E_HIP;
-      }
-    }
-    j.t_submit = j.t_admit = t0;
-    run.push_back(&j);
-  }
-  // Every tick completes one evaluation of each running simplex; a stage
-  // needs at most (N+1) + n_steps*(N+2) of them.
-  long max_cycles = 0;
-  for (DevJob* j : run) {
-    const double per_stage = (double)(j->space.n_params + 1) + (double)j->rp.n_steps * (j->space.n_params + 2);
-    const long mc = (long)((per_stage * j->rp.nstages + 16.0) / ticks_for(j->cost, work)) + 4;
-    if (mc > max_cycles) max_cycles = mc;
-  }
+  std::vector<const ast_job*> js;
+  for (int i = 0; i < n_jobs; ++i) js.push_back(&jobs[i]);
   c->last_ms = 0;
   c->last_launches = 0;
-  for (long L = 0; !rc; ++L) {
-    bool all = true;
-    for (DevJob* j : run) all = all && j->groups_done >= j->rp.n_groups;
-    if (all) break;
-    if (L > max_cycles) {
-      set_err("fit did not terminate");
-      rc = AST_E_STATE;
-      break;
-    }
-    rc = run_cycle(c, run, &c->last_ms, &c->last_launches);
-  }
-  if (rc) (void)hipDeviceSynchronize();
-  int64_t ev_tot = 0, pts_tot = 0;
+  return batch_run(c, js, G, work, 0, results, &c->last_ms, &c->last_launches);
+}
+
+int ast_set_waves(ast_ctx* c, const ast_variant* v, int n_filters, int waves) {
+  if (!c || !v || n_filters <= 0 || n_filters > AST_MAX_FILTERS || (waves != 2 && waves != 3 && waves != 4))
+    return AST_E_ARG;
+  if (!variant_supported(*v)) return AST_E_UNSUPPORTED;
+  c->waves[class_of(*v, n_filters)] = waves;
+  return AST_OK;
+}
+
+int ast_autotune(ast_ctx* c, const ast_job* jobs, int n_jobs, const ast_batch_cfg* cfg, int cycles,
+                 ast_tune_entry* out, int max_out, int* n_out) {
+  if (!c || !jobs || n_jobs <= 0 || (max_out > 0 && !out) || !n_out) return AST_E_ARG;
+  if (c->serve) return AST_E_BUSY;
+  *n_out = 0;
+  const int G = (cfg && cfg->group_size) ? cfg->group_size : 256;
+  if (G < 64 || G > 1024 || (G & 63)) return AST_E_ARG;
+  const double work = (cfg && cfg->work_points_per_launch > 0) ? cfg->work_points_per_launch : 16000.0;
+  if (cycles <= 0) cycles = 4;
+  // the jobs of each kernel class
+  std::map<int, std::vector<const ast_job*>> by_class;
   for (int i = 0; i < n_jobs; ++i) {
-    DevJob& j = *J[i];
-    ast_result& r = results[i];
-    std::memset(&r, 0, sizeof r);
-    if (!rc && i < (int)run.size()) {
-      int e = job_result(j, &r);
-      int64_t ev = 0;
-      if (!e) e = job_evals(c, j, &ev);
-      if (e && !rc) rc = e;
-      r.n_evals = ev;
-      r.model_points = ev * j.n_data;
-      r.submit_to_done_ms = std::chrono::duration<double, std::milli>(j.t_done - t0).count();
-      r.run_ms = r.submit_to_done_ms;
-      ev_tot += ev;
-      pts_tot += ev * j.n_data;
-    } else {
-      r.status = rc ? rc : AST_E_STATE;
-      r.best_f = 1e30;
-    }
-    r.id = (uint64_t)i;
-    r.tag = j.tag;
-    job_free(j);
+    const ast_job& j = jobs[i];
+    if (j.n_filters <= 0 || j.n_filters > AST_MAX_FILTERS) return AST_E_DATA;
+    if (!variant_supported(j.variant)) return AST_E_UNSUPPORTED;
+    by_class[class_of(j.variant, j.n_filters)].push_back(&j);
   }
-  c->last_evals = ev_tot;
-  c->last_points = pts_tot;
-  return rc;
+  const int cand[3] = {4, 3, 2};
+  for (auto& kv : by_class) {
+    const int k = kv.first;
+    double ms[3] = {0, 0, 0};
+    int best = 4;
+    double best_ms = 0;
+    for (int w = 0; w < 3; ++w) {
+      if (cand[w] < 4 && G > 256) continue;
+      c->waves[k] = cand[w];
+      int launches = 0;
+      // the same jobs from the same start each time: identical work, so the
+      // kernel times compare the register budgets alone
+      RETIF(batch_run(c, kv.second, G, work, cycles, nullptr, &ms[w], &launches));
+      if (best_ms == 0 || ms[w] < best_ms) { best_ms = ms[w]; best = cand[w]; }
+    }
+    c->waves[k] = best;
+    if (*n_out < max_out) {
+      ast_tune_entry& e = out[(*n_out)++];
+      std::memset(&e, 0, sizeof e);
+      const ast_job& j = *kv.second[0];
+      e.torque = j.variant.torque;
+      e.bc = j.variant.bc;
+      e.bw_ball = j.variant.bw_ball;
+      e.precision = j.variant.precision;
+      e.filter_capacity = j.n_filters <= 1 ? 1 : (j.n_filters <= 4 ? 4 : AST_MAX_FILTERS);
+      e.waves = best;
+      for (int w = 0; w < 3; ++w) e.kernel_ms[w] = ms[w];
+    }
+  }
+  return AST_OK;
 }
 
 int ast_poll(ast_ctx* c, ast_result* out, int max, int* n_out, int timeout_ms) {

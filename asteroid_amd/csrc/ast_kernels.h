@@ -71,7 +71,6 @@ struct SlotBufs {
   int32_t* g_reopt;   // per group slot: s_x2_params.reopt
   int32_t* g_stage;   // per group slot: current stage
   int32_t* g_done;    // per group slot: all stages finished
-  int32_t* running;   // counter: groups still running after a launch
   size_t nf;          // slot capacity (fits)
 };
 
@@ -474,8 +473,13 @@ __device__ __attribute__((noinline)) CF eval_lane(const double* PRM, CF* DV, siz
 // ---------------------------------------------------------------- fit kernel
 // One workgroup = one group (the reference's block); it runs D.ticks rounds
 // of "evaluate the pending points, advance every lane", then saves its state.
-template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX>
-__global__ __launch_bounds__(1024) void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
+// W is the register budget as waves per SIMD: 4 (128 VGPRs, groups of up to
+// 1024 simplexes), 3 (168 VGPRs) or 2 (256 VGPRs), both for groups of up to
+// 256.  The arithmetic is the same for every W; only speed differs, and the
+// autotuner picks W per kernel class by measurement.
+template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX, int W>
+__global__ __launch_bounds__(W == 4 ? 1024 : 256) __attribute__((amdgpu_waves_per_eu(W, W)))
+void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
   using P = Pos<TQ, BC, BW>;
   constexpr int N = P::N;
   __shared__ CF s_f[1024];
@@ -602,7 +606,6 @@ __global__ __launch_bounds__(1024) void fit_kernel(const GroupDesc<CF>* descs, S
     B.g_stage[slot] = stage;
     B.g_done[slot] = done ? 1 : 0;
     if (done) atomicAdd(D.job_done, 1);
-    else atomicAdd(B.running, 1);
   }
 }
 
@@ -634,6 +637,7 @@ struct LaunchArgs {
   const void* descs;   // GroupDesc<CF>*
   const void* bufs;    // SlotBufs<CF>*
   int n_groups, group_size, n_filters;
+  int waves;           // fit kernel register budget: 4, 3 or 2 waves per SIMD
   // batch
   const PointRec* pts;
   const double* sum_w;
@@ -650,8 +654,18 @@ hipError_t launch_nf(const LaunchArgs& a) {
     hipLaunchKernelGGL((fit_init_kernel<CF, TQ, BC, BW>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
                        (const GroupDesc<CF>*)a.descs, *(const SlotBufs<CF>*)a.bufs);
   } else if (a.op == 1) {
-    hipLaunchKernelGGL((fit_kernel<CF, TQ, BC, BW, NF, MX>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
-                       (const GroupDesc<CF>*)a.descs, *(const SlotBufs<CF>*)a.bufs);
+    const GroupDesc<CF>* d = (const GroupDesc<CF>*)a.descs;
+    const SlotBufs<CF>& B = *(const SlotBufs<CF>*)a.bufs;
+    const int w = a.group_size > 256 ? 4 : a.waves;  // W < 4 kernels take groups of up to 256
+    if (w == 2)
+      hipLaunchKernelGGL((fit_kernel<CF, TQ, BC, BW, NF, MX, 2>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
+                         d, B);
+    else if (w == 3)
+      hipLaunchKernelGGL((fit_kernel<CF, TQ, BC, BW, NF, MX, 3>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
+                         d, B);
+    else
+      hipLaunchKernelGGL((fit_kernel<CF, TQ, BC, BW, NF, MX, 4>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
+                         d, B);
   } else {
     int blocks = (a.n + 255) / 256;
     hipLaunchKernelGGL((chi2_batch_kernel<CF, TQ, BC, BW, NF, MX>), dim3(blocks), dim3(256), 0, a.stream, a.pts,

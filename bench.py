@@ -123,6 +123,8 @@ def run_stage_one(prec, args):
     ctx = api.Context(v)
     ctx.set_model_space(space)
     ctx.upload_curve(curve)
+    tune = ctx.autotune([dict(space=space, curve=curve, n_fits=args.fits, n_steps=5000, eval_budget=args.budget,
+                              size_min=SIZE_MIN, seed=1)], cycles=4) if args.autotune else []
     ctx.rng_seed(1)
 
     def step():
@@ -158,11 +160,12 @@ def run_stage_one(prec, args):
     roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 2), "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4), "traffic": traffic_from_profile(name), "kernel": kernel,
             "flop_per_model_point": round(flop_per_eval / curve.n_data, 2), **split,
-            "avg_launch_ms": round(kern_ms / max(launches, 1), 3), "launches": launches}
+            "avg_launch_ms": round(kern_ms / max(launches, 1), 3), "launches": launches,
+            "waves_per_simd": tune[0]["waves"] if tune else 4}
     return {"value": tot_points / wall, "wall": wall, "ms_per_step": wall / args.steps * 1e3,
             "fits_per_s": args.fits * args.steps / wall, "evals_per_s": tot_evals / wall,
             "evals_per_fit": tot_evals / (args.fits * args.steps), "best_f": f, "best_params": params,
-            "roofline": roof}
+            "roofline": roof, "autotune": tune}
 
 
 def gate_stage_one(res_mx):
@@ -228,7 +231,7 @@ def main_stage_one(args):
                      "time_step_days": 1e-2, "precision": PREC_NAMES[head_prec]})
     out.update({"fits_per_s": head["fits_per_s"], "evals_per_s": head["evals_per_s"],
                 "evals_per_fit": round(head["evals_per_fit"], 1), "best_chi2": float(np.min(head["best_f"])),
-                "roofline": head["roofline"]})
+                "roofline": head["roofline"], "autotune": head["autotune"]})
     if MIXED in results:
         out["precision_gate"] = gate_stage_one(results[MIXED])
     for prec, r in results.items():
@@ -249,9 +252,10 @@ def main_stage_one(args):
 
 
 # ------------------------------------------------- other configurations
-def run_batch(jobs, prec, work_points=16000.0):
+def run_batch(jobs, prec, work_points=16000.0, autotune=True):
     from asteroid_amd import api, flops, variants
     with api.Context(variants.ls_torque(prec)) as ctx:
+        tune = ctx.autotune(jobs, work_points_per_launch=work_points, cycles=3) if autotune else []
         t0 = time.perf_counter()
         res = ctx.run_jobs(jobs, work_points_per_launch=work_points)
         wall = time.perf_counter() - t0
@@ -273,7 +277,9 @@ def run_batch(jobs, prec, work_points=16000.0):
     return res, {"value": mp / wall, "fits_per_s": fits / wall, "wall_s": wall, "kernel_ms": ms,
                  "launches": launches, "evals": int(sum(r["n_evals"] for r in res)),
                  "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 2),
-                              "unit": "TFLOP/s", "frac": round(achieved / peak, 4)}}
+                              "unit": "TFLOP/s", "frac": round(achieved / peak, 4)},
+                 "waves_per_simd": {f"tq{e['torque']}bc{e['bc']}bw{e['bw_ball']}nf{e['filter_capacity']}": e["waves"]
+                                    for e in tune}}
 
 
 def gate_jobs(jobs, res_mx):
@@ -335,9 +341,9 @@ def main_reopt(args):
     from asteroid_amd import workloads
     jobs_mx, truth = workloads.reopt_bc(MIXED)
     jobs64 = workloads.with_precision(jobs_mx, FP64)
-    res_mx, m_mx = run_batch(jobs_mx, MIXED)
+    res_mx, m_mx = run_batch(jobs_mx, MIXED, autotune=args.autotune)
     log(f"mixed: {m_mx}")
-    res_64, m_64 = run_batch(jobs64, FP64)
+    res_64, m_64 = run_batch(jobs64, FP64, autotune=args.autotune)
     log(f"fp64: {m_64}")
     b_mx = min(res_mx, key=lambda r: r["best_f"])
     b_64 = min(res_64, key=lambda r: r["best_f"])
@@ -359,9 +365,9 @@ def main_batched(args):
     jobs_mx, made = workloads.batched(MIXED, eval_budget=args.batch_budget)
     log(f"512 curves generated in {time.perf_counter() - t:.1f} s")
     jobs64 = workloads.with_precision(jobs_mx, FP64)
-    res_mx, m_mx = run_batch(jobs_mx, MIXED)
+    res_mx, m_mx = run_batch(jobs_mx, MIXED, autotune=args.autotune)
     log(f"mixed: {m_mx}")
-    res_64, m_64 = run_batch(jobs64, FP64)
+    res_64, m_64 = run_batch(jobs64, FP64, autotune=args.autotune)
     log(f"fp64: {m_64}")
     pts = sum(j["curve"].n_data for j in jobs_mx)
     out = base_line(m_mx["value"], 1, 0, m_mx["wall_s"] * 1e3, DTYPE[MIXED],
@@ -438,6 +444,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -309,6 +309,27 @@ typedef struct {
 
 int ast_run_jobs(ast_ctx* ctx, const ast_job* jobs, int n_jobs, const ast_batch_cfg* cfg, ast_result* results);
 
+/* ------------------------------------------------------------ autotuner
+ * No counterpart in the reference (its block size and registers are fixed
+ * at compile time, asteroid.h:183-199).  Every kernel class (variant,
+ * filter capacity, precision) has three builds of the fitting kernel with
+ * different register budgets: 4 waves per SIMD (128 VGPRs, groups of up to
+ * 1024 simplexes), 3 (168 VGPRs) and 2 (256 VGPRs), the last two for groups
+ * of up to 256.  They compute exactly the same results.  ast_autotune runs
+ * `cycles` launch cycles of the given jobs (one class at a time, from the
+ * same start for every budget, so the work is identical) at each budget,
+ * keeps the fastest for each class in the context, and reports the kernel
+ * times; ast_set_waves sets one class by hand.  Untuned classes use 4. */
+typedef struct {
+  int torque, bc, bw_ball, precision, filter_capacity;
+  int waves;             /* chosen budget */
+  double kernel_ms[3];   /* kernel time at 4, 3 and 2 waves per SIMD (0 = not run) */
+} ast_tune_entry;
+
+int ast_autotune(ast_ctx* ctx, const ast_job* jobs, int n_jobs, const ast_batch_cfg* cfg, int cycles,
+                 ast_tune_entry* out, int max_out, int* n_out);
+int ast_set_waves(ast_ctx* ctx, const ast_variant* variant, int n_filters, int waves);
+
 int ast_serve_start(ast_ctx* ctx, const ast_serve_cfg* cfg);
 /* Copies the job's inputs; returns its id. */
 int ast_submit(ast_ctx* ctx, const ast_job* job, uint64_t* id);
