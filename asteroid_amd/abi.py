@@ -53,7 +53,7 @@ class X2(C.Structure):
 class RunCfg(C.Structure):
     _fields_ = [("n_groups", C.c_int), ("group_size", C.c_int), ("nstages", C.c_int),
                 ("n_steps", C.c_int), ("eval_budget", C.c_int64), ("size_min", C.c_double),
-                ("ticks_per_launch", C.c_int)]
+                ("ticks_per_launch", C.c_int), ("use_graphs", C.c_int)]
 
 
 class ServeCfg(C.Structure):
@@ -79,7 +79,7 @@ class Result(C.Structure):
 
 
 class BatchCfg(C.Structure):
-    _fields_ = [("group_size", C.c_int), ("work_points_per_launch", C.c_double)]
+    _fields_ = [("group_size", C.c_int), ("work_points_per_launch", C.c_double), ("use_graphs", C.c_int)]
 
 
 class TuneEntry(C.Structure):

@@ -87,8 +87,9 @@ class Context:
         _chk(self.lib.ast_set_start(self.h, _dp(p)), "ast_set_start")
 
     def run(self, n_groups, group_size=256, nstages=1, n_steps=5000, eval_budget=0, size_min=0.0,
-            ticks_per_launch=0):
-        cfg = abi.RunCfg(n_groups, group_size, nstages, n_steps, eval_budget, size_min, ticks_per_launch)
+            ticks_per_launch=0, use_graphs=False):
+        cfg = abi.RunCfg(n_groups, group_size, nstages, n_steps, eval_budget, size_min, ticks_per_launch,
+                         int(use_graphs))
         _chk(self.lib.ast_run(self.h, C.byref(cfg)), "ast_run")
         self._last_groups = n_groups
         self._last_fits = n_groups * group_size
@@ -155,13 +156,13 @@ class Context:
                     n_evals=r.n_evals, model_points=r.model_points, latency_ms=r.submit_to_done_ms,
                     queue_ms=r.queue_ms, run_ms=r.run_ms)
 
-    def run_jobs(self, jobs, group_size=256, work_points_per_launch=0.0):
+    def run_jobs(self, jobs, group_size=256, work_points_per_launch=0.0, use_graphs=False):
         """Run a batch of jobs (dicts of Context.submit's arguments) to completion
         in shared launch cycles; returns their results in order."""
         built = [self._job(**j) for j in jobs]
         arr = (abi.Job * len(built))(*[b[0] for b in built])
         out = (abi.Result * len(built))()
-        cfg = abi.BatchCfg(int(group_size), float(work_points_per_launch))
+        cfg = abi.BatchCfg(int(group_size), float(work_points_per_launch), int(use_graphs))
         _chk(self.lib.ast_run_jobs(self.h, arr, len(built), C.byref(cfg), out), "ast_run_jobs")
         return [self._result(out[k]) for k in range(len(built))]
 
@@ -171,7 +172,7 @@ class Context:
         arr = (abi.Job * len(built))(*[b[0] for b in built])
         out = (abi.TuneEntry * 64)()
         n = C.c_int()
-        cfg = abi.BatchCfg(int(group_size), float(work_points_per_launch))
+        cfg = abi.BatchCfg(int(group_size), float(work_points_per_launch), 0)
         _chk(self.lib.ast_autotune(self.h, arr, len(built), C.byref(cfg), int(cycles), out, 64, C.byref(n)),
              "ast_autotune")
         return [dict(torque=e.torque, bc=e.bc, bw_ball=e.bw_ball, precision=e.precision,

@@ -566,6 +566,9 @@ struct ast_ctx {
   ClassBatch batch[MAX_CLASSES];
   std::atomic<uint64_t> next_uid{1};
   int waves[MAX_CLASSES];  // fit-kernel register budget per class (4, 3 or 2 waves per SIMD)
+  hipEvent_t ev_fork = nullptr, ev_join[NSTREAMS] = {nullptr};  // graph capture fork/join
+  hipGraphExec_t cg_exec = nullptr;  // the captured cycle
+  std::vector<uint64_t> cg_key;
   ast_ctx() { for (int k = 0; k < MAX_CLASSES; ++k) waves[k] = 4; }
 };
 
@@ -579,12 +582,12 @@ int set_device(ast_ctx* c) {
 // One cycle: one fit-kernel launch per kernel class over every running
 // group of every job of that class, every stream drained at the end; adds
 // the cycle's kernel time (the longest stream).
-int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, int* launches);
+int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, int* launches, bool graphs);
 
 // A failed cycle may leave kernels in flight: drain the device before the
 // caller frees anything they use.
-int run_cycle(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, int* launches) {
-  int rc = run_cycle_once(c, jobs, kernel_ms, launches);
+int run_cycle(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, int* launches, bool graphs = false) {
+  int rc = run_cycle_once(c, jobs, kernel_ms, launches, graphs);
   if (rc) (void)hipDeviceSynchronize();
   return rc;
 }
@@ -619,7 +622,67 @@ int class_batch_update(ast_ctx* c, int k, const std::vector<DevJob*>& members) {
   return 0;
 }
 
-int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, int* launches) {
+// One fit-kernel launch of class k over the class batch, on its stream.
+hipError_t launch_class(ast_ctx* c, int k, const DevJob& j0) {
+  LaunchArgs a{};
+  a.op = 1;
+  a.stream = c->streams[stream_of(k)];
+  a.descs = c->batch[k].d_descs;
+  SlotBufs<double> bd;
+  SlotBufs<float> bf;
+  if (j0.cf == 8) { bd = pool_bufs<double>(c->pool, k); a.bufs = &bd; }
+  else { bf = pool_bufs<float>(c->pool, k); a.bufs = &bf; }
+  a.n_groups = c->batch[k].n_groups;
+  a.group_size = c->pool.G;
+  a.n_filters = j0.n_filters;
+  a.waves = c->waves[k];
+  return dispatch(j0.space.v, a);
+}
+
+// The cycle as a HIP graph: every class launch, forked from stream 0 onto
+// the classes' streams and joined back.  Valid while the class batches, the
+// pool and the register budgets are unchanged (the key).
+int cycle_graph(ast_ctx* c, std::vector<DevJob*>* members, const bool* used_s) {
+  std::vector<uint64_t> key{(uint64_t)(uintptr_t)c->pool.X, (uint64_t)c->pool.nf, (uint64_t)c->pool.G};
+  for (int k = 0; k < MAX_CLASSES; ++k) {
+    if (members[k].empty()) continue;
+    key.push_back((uint64_t)k);
+    key.push_back((uint64_t)c->waves[k]);
+    key.push_back((uint64_t)(uintptr_t)c->batch[k].d_descs);
+    key.push_back((uint64_t)c->batch[k].n_groups);
+    for (uint64_t u : c->batch[k].members) key.push_back(u);
+  }
+  if (c->cg_exec && key == c->cg_key) return 0;
+  if (c->cg_exec) (void)hipGraphExecDestroy(c->cg_exec);
+  c->cg_exec = nullptr;
+  c->cg_key.clear();
+  const hipStream_t s0 = c->streams[0];
+  HIPCHK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipEventRecord(c->ev_fork, s0);
+  for (int k = 1; k < NSTREAMS && e == hipSuccess; ++k)
+    if (used_s[k]) e = hipStreamWaitEvent(c->streams[k], c->ev_fork, 0);
+  for (int k = 0; k < MAX_CLASSES && e == hipSuccess; ++k)
+    if (!members[k].empty()) e = launch_class(c, k, *members[k][0]);
+  for (int k = 1; k < NSTREAMS && e == hipSuccess; ++k)
+    if (used_s[k]) {
+      e = hipEventRecord(c->ev_join[k], c->streams[k]);
+      if (e == hipSuccess) e = hipStreamWaitEvent(s0, c->ev_join[k], 0);
+    }
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(s0, &g);  // always end the capture
+  if (e == hipSuccess) e = e2;
+  if (e == hipSuccess) e = hipGraphInstantiate(&c->cg_exec, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    c->cg_exec = nullptr;
+    snprintf(g_hip_err, sizeof g_hip_err, "%s (cycle graph)", hipGetErrorString(e));
+    return AST_E_HIP;
+  }
+  c->cg_key = key;
+  return 0;
+}
+
+int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, int* launches, bool graphs) {
   std::vector<DevJob*> members[MAX_CLASSES];
   bool used_s[NSTREAMS] = {false};
   for (DevJob* j : jobs) {
@@ -631,46 +694,52 @@ int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, in
     members[j->cls].push_back(j);
     used_s[stream_of(j->cls)] = true;
   }
+  int n_cls = 0;
   for (int k = 0; k < MAX_CLASSES; ++k)
-    if (!members[k].empty()) RETIF(class_batch_update(c, k, members[k]));
-  for (int k = 0; k < NSTREAMS; ++k)
-    if (used_s[k]) HIPCHK(hipEventRecord(c->ev0[k], c->streams[k]));
-  for (int k = 0; k < MAX_CLASSES; ++k) {
-    if (members[k].empty()) continue;
-    const DevJob& j0 = *members[k][0];
-    const hipStream_t s = c->streams[stream_of(k)];
-    LaunchArgs a{};
-    a.op = 1;
-    a.stream = s;
-    a.descs = c->batch[k].d_descs;
-    SlotBufs<double> bd;
-    SlotBufs<float> bf;
-    if (j0.cf == 8) { bd = pool_bufs<double>(c->pool, k); a.bufs = &bd; }
-    else { bf = pool_bufs<float>(c->pool, k); a.bufs = &bf; }
-    a.n_groups = c->batch[k].n_groups;
-    a.group_size = c->pool.G;
-    a.n_filters = j0.n_filters;
-    a.waves = c->waves[k];
-    HIPCHK(dispatch(j0.space.v, a));
-    (*launches)++;
-  }
-  for (int k = 0; k < NSTREAMS; ++k)
-    if (used_s[k]) HIPCHK(hipEventRecord(c->ev1[k], c->streams[k]));
-  // Per-job finished-group counters.
-  std::vector<int32_t> done(jobs.size());
-  for (size_t i = 0; i < jobs.size(); ++i)
-    HIPCHK(hipMemcpyAsync(&done[i], jobs[i]->done, sizeof(int32_t), hipMemcpyDeviceToHost,
-                          c->streams[stream_of(jobs[i]->cls)]));
+    if (!members[k].empty()) {
+      RETIF(class_batch_update(c, k, members[k]));
+      n_cls++;
+    }
   double ms_max = 0;
-  for (int k = 0; k < NSTREAMS; ++k) {
-    bool any = used_s[k];
-    for (size_t i = 0; i < jobs.size() && !any; ++i) any = stream_of(jobs[i]->cls) == k;
-    if (!any) continue;
-    HIPCHK(hipStreamSynchronize(c->streams[k]));
-    if (used_s[k]) {
-      float ms = 0;
-      HIPCHK(hipEventElapsedTime(&ms, c->ev0[k], c->ev1[k]));
-      if (ms > ms_max) ms_max = ms;
+  std::vector<int32_t> done(jobs.size());
+  if (graphs && n_cls > 0) {
+    // one graph launch on stream 0; every class stream joins back into it
+    RETIF(cycle_graph(c, members, used_s));
+    const hipStream_t s0 = c->streams[0];
+    HIPCHK(hipEventRecord(c->ev0[0], s0));
+    HIPCHK(hipGraphLaunch(c->cg_exec, s0));
+    HIPCHK(hipEventRecord(c->ev1[0], s0));
+    *launches += n_cls;
+    for (size_t i = 0; i < jobs.size(); ++i)
+      HIPCHK(hipMemcpyAsync(&done[i], jobs[i]->done, sizeof(int32_t), hipMemcpyDeviceToHost, s0));
+    HIPCHK(hipStreamSynchronize(s0));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0[0], c->ev1[0]));
+    ms_max = ms;
+  } else {
+    for (int k = 0; k < NSTREAMS; ++k)
+      if (used_s[k]) HIPCHK(hipEventRecord(c->ev0[k], c->streams[k]));
+    for (int k = 0; k < MAX_CLASSES; ++k) {
+      if (members[k].empty()) continue;
+      HIPCHK(launch_class(c, k, *members[k][0]));
+      (*launches)++;
+    }
+    for (int k = 0; k < NSTREAMS; ++k)
+      if (used_s[k]) HIPCHK(hipEventRecord(c->ev1[k], c->streams[k]));
+    // Per-job finished-group counters.
+    for (size_t i = 0; i < jobs.size(); ++i)
+      HIPCHK(hipMemcpyAsync(&done[i], jobs[i]->done, sizeof(int32_t), hipMemcpyDeviceToHost,
+                            c->streams[stream_of(jobs[i]->cls)]));
+    for (int k = 0; k < NSTREAMS; ++k) {
+      bool any = used_s[k];
+      for (size_t i = 0; i < jobs.size() && !any; ++i) any = stream_of(jobs[i]->cls) == k;
+      if (!any) continue;
+      HIPCHK(hipStreamSynchronize(c->streams[k]));
+      if (used_s[k]) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0[k], c->ev1[k]));
+        if (ms > ms_max) ms_max = ms;
+      }
     }
   }
   *kernel_ms += ms_max;
@@ -823,7 +892,7 @@ int make_job(ast_ctx* c, const ast_job* job, int G, std::unique_ptr<DevJob>* out
 // results (results may be null).  cycle_limit > 0 stops after that many
 // cycles (the autotuner's fixed work); 0 runs every job to completion.
 int batch_run(ast_ctx* c, const std::vector<const ast_job*>& jobs, int G, double work, long cycle_limit,
-              ast_result* results, double* kernel_ms, int* launches) {
+              ast_result* results, double* kernel_ms, int* launches, bool graphs) {
   const int n_jobs = (int)jobs.size();
   std::vector<std::unique_ptr<DevJob>> J(n_jobs);
   std::vector<CurveH> curves(n_jobs);
@@ -874,7 +943,7 @@ int batch_run(ast_ctx* c, const std::vector<const ast_job*>& jobs, int G, double
       rc = AST_E_STATE;
       break;
     }
-    rc = run_cycle(c, run, kernel_ms, launches);
+    rc = run_cycle(c, run, kernel_ms, launches, graphs);
   }
   if (rc) (void)hipDeviceSynchronize();
   int64_t ev_tot = 0, pts_tot = 0;
@@ -1093,7 +1162,9 @@ int ast_ctx_create(ast_ctx** out, const ast_variant* v, int device) {
     e = hipStreamCreateWithFlags(&c->streams[k], hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0[k]);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1[k]);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming);
   }
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
   if (e != hipSuccess) {
     set_err(hipGetErrorString(e));
     ast_ctx_destroy(c);
@@ -1116,11 +1187,14 @@ int ast_ctx_destroy(ast_ctx* c) {
   dev_free(c->best_f);
   dev_free(c->best_params);
   dev_free(c->best_dV);
+  if (c->cg_exec) (void)hipGraphExecDestroy(c->cg_exec);
   for (int k = 0; k < NSTREAMS; ++k) {
     if (c->streams[k]) (void)hipStreamDestroy(c->streams[k]);
     if (c->ev0[k]) (void)hipEventDestroy(c->ev0[k]);
     if (c->ev1[k]) (void)hipEventDestroy(c->ev1[k]);
+    if (c->ev_join[k]) (void)hipEventDestroy(c->ev_join[k]);
   }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   delete c;
   return AST_OK;
 }
@@ -1232,7 +1306,7 @@ int ast_run(ast_ctx* c, const ast_run_cfg* rc) {
       set_err("fit did not terminate");
       return AST_E_STATE;
     }
-    RETIF(run_cycle(c, jobs, &c->last_ms, &c->last_launches));
+    RETIF(run_cycle(c, jobs, &c->last_ms, &c->last_launches, rc->use_graphs != 0));
   }
   int64_t ev = 0;
   RETIF(job_evals(c, j, &ev));
@@ -1398,7 +1472,7 @@ int ast_run_jobs(ast_ctx* c, const ast_job* jobs, int n_jobs, const ast_batch_cf
   for (int i = 0; i < n_jobs; ++i) js.push_back(&jobs[i]);
   c->last_ms = 0;
   c->last_launches = 0;
-  return batch_run(c, js, G, work, 0, results, &c->last_ms, &c->last_launches);
+  return batch_run(c, js, G, work, 0, results, &c->last_ms, &c->last_launches, cfg && cfg->use_graphs);
 }
 
 int ast_set_waves(ast_ctx* c, const ast_variant* v, int n_filters, int waves) {
@@ -1438,7 +1512,7 @@ int ast_autotune(ast_ctx* c, const ast_job* jobs, int n_jobs, const ast_batch_cf
       int launches = 0;
       // the same jobs from the same start each time: identical work, so the
       // kernel times compare the register budgets alone
-      RETIF(batch_run(c, kv.second, G, work, cycles, nullptr, &ms[w], &launches));
+      RETIF(batch_run(c, kv.second, G, work, cycles, nullptr, &ms[w], &launches, false));
       if (best_ms == 0 || ms[w] < best_ms) { best_ms = ms[w]; best = cand[w]; }
     }
     c->waves[k] = best;

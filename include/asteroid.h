@@ -151,6 +151,7 @@ typedef struct {
   int64_t eval_budget; /* chi^2 evaluations per simplex and stage; 0 = none    */
   double size_min;     /* SIZE_MIN; 0 = the precision's default (asteroid.h:233-237) */
   int ticks_per_launch;/* evaluation rounds per kernel launch; 0 = automatic   */
+  int use_graphs;      /* 1: replay each launch cycle from a captured HIP graph (same results) */
 } ast_run_cfg;
 
 typedef struct ast_ctx ast_ctx;
@@ -305,6 +306,7 @@ typedef struct {
 typedef struct {
   int group_size;                /* simplexes per group; 0 = 256 */
   double work_points_per_launch; /* as in ast_serve_cfg; 0 = 16000 */
+  int use_graphs;                /* 1: replay unchanged launch cycles from a captured HIP graph */
 } ast_batch_cfg;
 
 int ast_run_jobs(ast_ctx* ctx, const ast_job* jobs, int n_jobs, const ast_batch_cfg* cfg, ast_result* results);
