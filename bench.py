@@ -129,7 +129,8 @@ def run_stage_one(prec, args):
 
     def step():
         ctx.reset_best()
-        ctx.run(n_groups, group, nstages=1, n_steps=5000, eval_budget=args.budget, size_min=SIZE_MIN)
+        ctx.run(n_groups, group, nstages=1, n_steps=5000, eval_budget=args.budget, size_min=SIZE_MIN,
+                use_graphs=args.graphs)
 
     name = PREC_NAMES[prec]
     for w in range(args.warmup):
@@ -228,7 +229,7 @@ def main_stage_one(args):
                     {"workload": "stage-one random search (LS ellipsoid + torque, P_PSI, INTERP)",
                      "n_obs": 400, "span_days": 3.0, "filters": 1, "simplexes": args.fits,
                      "evals_per_simplex": args.budget, "size_min": SIZE_MIN, "group_size": 256,
-                     "time_step_days": 1e-2, "precision": PREC_NAMES[head_prec]})
+                     "time_step_days": 1e-2, "precision": PREC_NAMES[head_prec], "hip_graphs": args.graphs})
     out.update({"fits_per_s": head["fits_per_s"], "evals_per_s": head["evals_per_s"],
                 "evals_per_fit": round(head["evals_per_fit"], 1), "best_chi2": float(np.min(head["best_f"])),
                 "roofline": head["roofline"], "autotune": head["autotune"]})
@@ -252,12 +253,12 @@ def main_stage_one(args):
 
 
 # ------------------------------------------------- other configurations
-def run_batch(jobs, prec, work_points=16000.0, autotune=True):
+def run_batch(jobs, prec, work_points=16000.0, autotune=True, graphs=True):
     from asteroid_amd import api, flops, variants
     with api.Context(variants.ls_torque(prec)) as ctx:
         tune = ctx.autotune(jobs, work_points_per_launch=work_points, cycles=3) if autotune else []
         t0 = time.perf_counter()
-        res = ctx.run_jobs(jobs, work_points_per_launch=work_points)
+        res = ctx.run_jobs(jobs, work_points_per_launch=work_points, use_graphs=graphs)
         wall = time.perf_counter() - t0
         ms, launches = ctx.last_timing()
     mp = sum(r["model_points"] for r in res)
@@ -341,9 +342,9 @@ def main_reopt(args):
     from asteroid_amd import workloads
     jobs_mx, truth = workloads.reopt_bc(MIXED)
     jobs64 = workloads.with_precision(jobs_mx, FP64)
-    res_mx, m_mx = run_batch(jobs_mx, MIXED, autotune=args.autotune)
+    res_mx, m_mx = run_batch(jobs_mx, MIXED, autotune=args.autotune, graphs=args.graphs)
     log(f"mixed: {m_mx}")
-    res_64, m_64 = run_batch(jobs64, FP64, autotune=args.autotune)
+    res_64, m_64 = run_batch(jobs64, FP64, autotune=args.autotune, graphs=args.graphs)
     log(f"fp64: {m_64}")
     b_mx = min(res_mx, key=lambda r: r["best_f"])
     b_64 = min(res_64, key=lambda r: r["best_f"])
@@ -365,9 +366,9 @@ def main_batched(args):
     jobs_mx, made = workloads.batched(MIXED, eval_budget=args.batch_budget)
     log(f"512 curves generated in {time.perf_counter() - t:.1f} s")
     jobs64 = workloads.with_precision(jobs_mx, FP64)
-    res_mx, m_mx = run_batch(jobs_mx, MIXED, autotune=args.autotune)
+    res_mx, m_mx = run_batch(jobs_mx, MIXED, autotune=args.autotune, graphs=args.graphs)
     log(f"mixed: {m_mx}")
-    res_64, m_64 = run_batch(jobs64, FP64, autotune=args.autotune)
+    res_64, m_64 = run_batch(jobs64, FP64, autotune=args.autotune, graphs=args.graphs)
     log(f"fp64: {m_64}")
     pts = sum(j["curve"].n_data for j in jobs_mx)
     out = base_line(m_mx["value"], 1, 0, m_mx["wall_s"] * 1e3, DTYPE[MIXED],
@@ -445,6 +446,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
+    ap.add_argument("--no-graphs", dest="graphs", action="store_false")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
