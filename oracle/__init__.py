@@ -38,7 +38,9 @@ def lib():
     global _lib
     if _lib is None:
         name = "liboracle_fma.so" if _has_fma() else "liboracle.so"
-        path = os.path.join(HERE, name)
+        # AST_ORACLE_LIB selects another build of the same source (the
+        # sanitizer test builds one with -fsanitize=address,undefined)
+        path = os.environ.get("AST_ORACLE_LIB") or os.path.join(HERE, name)
         if not os.path.exists(path):
             raise RuntimeError(f"{path} missing: run `make -C oracle`")
         L = C.CDLL(path)
