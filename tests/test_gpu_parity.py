@@ -87,3 +87,52 @@ def test_gpu_matches_golden_vectors(name):
         ctx.upload_curve(pb.curve)
         c, dv = ctx.chi2_batch(params)
         assert np.array_equal(c, fx(case["chi2_canon"])) and np.array_equal(dv.ravel(), fx(case["dV_canon"]))
+
+
+EDGE = {
+    # name: (variant, n_filters, n_obs, span_days)
+    "min_dof_ls": (lambda: variants.ls_torque(abi.PREC_FP64), 1, 13, 0.5),       # 11 params + 1 filter + 1
+    "nf8_class_bc": (lambda: variants.bc_free(abi.PREC_FP64), 6, 300, 2.0),     # the 8-filter kernels
+    "long_6000": (lambda: variants.ls_torque(abi.PREC_F32SIMPLEX), 2, 6000, 10.0),
+    "sparse_rk4": (lambda: variants.bw_ball_torque(abi.PREC_FP64), 1, 30, 8.0),  # ~27 RK4 steps a point
+}
+
+
+@pytest.mark.parametrize("name", list(EDGE))
+def test_chi2_edge_shapes_bitwise(name):
+    mk, nf, n_obs, span = EDGE[name]
+    v = mk()
+    curve, truth = synth.make_curve(17, v, n_obs, span, n_filters=nf)
+    space = variants.default_model_space(v)
+    pb = oracle.Problem(space, curve)
+    x = np.random.default_rng(4).uniform(0.02, 0.98, size=(200, space.n_params))
+    params, rc = pb.x2params(x, reopt=0)
+    params = np.vstack([truth, params[rc == 0]])
+    want_c, want_dv = pb.chi2(params, form=oracle.CANON)
+    with api.Context(v) as ctx:
+        ctx.set_model_space(space)
+        ctx.upload_curve(curve)
+        got_c, got_dv = ctx.chi2_batch(params)
+    ok = ~np.isnan(want_c)
+    assert np.array_equal(np.isnan(got_c), ~ok)
+    assert np.array_equal(got_c[ok], want_c[ok]) and np.array_equal(got_dv[ok], want_dv[ok])
+
+
+@pytest.mark.parametrize("name,G,gs", [("nf8_class_bc", 2, 64), ("min_dof_ls", 1, 1024), ("sparse_rk4", 3, 64)])
+def test_fit_edge_shapes_identical(name, G, gs):
+    mk, nf, n_obs, span = EDGE[name]
+    v = mk()
+    curve, truth = synth.make_curve(18, v, n_obs, span, n_filters=nf)
+    space = variants.default_model_space(v)
+    pb = oracle.Problem(space, curve)
+    want = pb.run(G, gs, nstages=2, n_steps=25, seed=77, n_threads=8)
+    with api.Context(v) as ctx:
+        ctx.set_model_space(space)
+        ctx.upload_curve(curve)
+        ctx.rng_seed(77)
+        ctx.run(G, gs, nstages=2, n_steps=25, ticks_per_launch=11)
+        f, p, dv = ctx.fetch()
+        lf, ln, ls = ctx.fetch_lanes()
+    assert np.array_equal(lf, want["lane_f"]) and np.array_equal(ln, want["lane_nev"])
+    assert np.array_equal(ls, want["lane_steps"])
+    assert np.array_equal(f, want["f"]) and np.array_equal(p, want["params"]) and np.array_equal(dv, want["dV"])
