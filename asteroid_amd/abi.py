@@ -79,7 +79,8 @@ class Result(C.Structure):
 
 
 class BatchCfg(C.Structure):
-    _fields_ = [("group_size", C.c_int), ("work_points_per_launch", C.c_double), ("use_graphs", C.c_int)]
+    _fields_ = [("group_size", C.c_int), ("work_points_per_launch", C.c_double), ("use_graphs", C.c_int),
+                ("schedule", C.c_int)]
 
 
 class TuneEntry(C.Structure):

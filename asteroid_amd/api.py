@@ -156,13 +156,13 @@ class Context:
                     n_evals=r.n_evals, model_points=r.model_points, latency_ms=r.submit_to_done_ms,
                     queue_ms=r.queue_ms, run_ms=r.run_ms)
 
-    def run_jobs(self, jobs, group_size=256, work_points_per_launch=0.0, use_graphs=False):
+    def run_jobs(self, jobs, group_size=256, work_points_per_launch=0.0, use_graphs=False, schedule=0):
         """Run a batch of jobs (dicts of Context.submit's arguments) to completion
         in shared launch cycles; returns their results in order."""
         built = [self._job(**j) for j in jobs]
         arr = (abi.Job * len(built))(*[b[0] for b in built])
         out = (abi.Result * len(built))()
-        cfg = abi.BatchCfg(int(group_size), float(work_points_per_launch), int(use_graphs))
+        cfg = abi.BatchCfg(int(group_size), float(work_points_per_launch), int(use_graphs), int(schedule))
         _chk(self.lib.ast_run_jobs(self.h, arr, len(built), C.byref(cfg), out), "ast_run_jobs")
         return [self._result(out[k]) for k in range(len(built))]
 
@@ -172,7 +172,7 @@ class Context:
         arr = (abi.Job * len(built))(*[b[0] for b in built])
         out = (abi.TuneEntry * 64)()
         n = C.c_int()
-        cfg = abi.BatchCfg(int(group_size), float(work_points_per_launch), 0)
+        cfg = abi.BatchCfg(int(group_size), float(work_points_per_launch), 0, 0)
         _chk(self.lib.ast_autotune(self.h, arr, len(built), C.byref(cfg), int(cycles), out, 64, C.byref(n)),
              "ast_autotune")
         return [dict(torque=e.torque, bc=e.bc, bw_ball=e.bw_ball, precision=e.precision,

@@ -252,6 +252,8 @@ struct Pool {
   double* PRM = nullptr;
   int32_t *ctl = nullptr, *lstep = nullptr, *nev = nullptr, *nev_tot = nullptr, *steps_tot = nullptr;
   int32_t *g_reopt = nullptr, *g_stage = nullptr, *g_done = nullptr;
+  int32_t* job_done = nullptr;     // [n_slots]: a job's finished-group counter sits at its first slot
+  std::vector<int32_t> job_done_h; // host copy, read back once per cycle
   std::vector<int> free_slots;
 };
 
@@ -259,7 +261,7 @@ void pool_free(Pool& p) {
   dev_free(p.X); dev_free(p.F); dev_free(p.X0); dev_free(p.XR); dev_free(p.XP); dev_free(p.DV);
   dev_free(p.fr); dev_free(p.sf); dev_free(p.IND); dev_free(p.PRM); dev_free(p.ctl); dev_free(p.lstep);
   dev_free(p.nev); dev_free(p.nev_tot); dev_free(p.steps_tot); dev_free(p.g_reopt); dev_free(p.g_stage);
-  dev_free(p.g_done);
+  dev_free(p.g_done); dev_free(p.job_done);
   p.nf = 0;
   p.n_slots = 0;
   p.free_slots.clear();
@@ -288,11 +290,13 @@ int pool_alloc(Pool& p, size_t nf, int G) {
   ALLOC(p.g_reopt, int32_t, nf / G);
   ALLOC(p.g_stage, int32_t, nf / G);
   ALLOC(p.g_done, int32_t, nf / G);
+  ALLOC(p.job_done, int32_t, nf / G);
 #undef ALLOC
   if (rc) { pool_free(p); return rc; }
   p.nf = nf;
   p.G = G;
   p.n_slots = (int)(nf / G);
+  p.job_done_h.assign((size_t)p.n_slots, 0);
   p.free_slots.clear();
   for (int s = p.n_slots - 1; s >= 0; --s) p.free_slots.push_back(s);
   return 0;
@@ -324,6 +328,7 @@ struct DevJob {
   int n_data = 0, n_filters = 0;
   int cls = 0;                 // kernel class index (for streams and counters)
   double cost = 1;             // one evaluation, in observation-equivalents (eval_cost)
+  int ticks = 1;               // evaluation rounds per launch
   int cf = 8;                  // simplex float bytes
   // device memory (owned unless borrowed)
   PointRec* pts = nullptr;
@@ -348,7 +353,8 @@ struct DevJob {
 };
 
 void job_free(DevJob& j) {
-  dev_free(j.pts); dev_free(j.sum_w); dev_free(j.sp); dev_free(j.params0); dev_free(j.done);
+  dev_free(j.pts); dev_free(j.sum_w); dev_free(j.sp); dev_free(j.params0);
+  j.done = nullptr;  // the pool's
   dev_free(j.descs);
   if (!j.borrowed) { dev_free(j.rng); dev_free(j.best_f); dev_free(j.best_params); dev_free(j.best_dV); }
 }
@@ -372,7 +378,8 @@ double eval_cost(const ast_variant& v, const CurveH& C) {
 // Device memory of one job (its own streams and bests unless borrowed).  The
 // clears run on `s`, the stream the job's kernels run on, so they are ordered
 // before its first launch.
-int job_upload(DevJob& j, const CurveH& C, const double* params0, hipStream_t s) {
+int job_upload(DevJob& j, const Pool& pool, const CurveH& C, const double* params0, hipStream_t s) {
+  if (j.slots.empty() || j.slots[0] < 0 || j.slots[0] >= pool.n_slots) return AST_E_STATE;
   const SpaceH& S = j.space;
   j.cf = S.v.precision == AST_PREC_FP64 ? 8 : 4;
   j.n_data = C.n_data;
@@ -383,7 +390,6 @@ int job_upload(DevJob& j, const CurveH& C, const double* params0, hipStream_t s)
   RETIF(dev_alloc(&j.sum_w, (size_t)AST_MAX_FILTERS));
   RETIF(dev_alloc((char**)&j.sp, sizeof(Space<double>)));
   RETIF(dev_alloc(&j.params0, (size_t)AST_MAX_PARAMS));
-  RETIF(dev_alloc(&j.done, (size_t)1));
   HIPCHK(hipMemcpy(j.pts, C.pts.data(), sizeof(PointRec) * C.pts.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(j.sum_w, C.sum_w, sizeof(double) * AST_MAX_FILTERS, hipMemcpyHostToDevice));
   if (j.cf == 8) {
@@ -397,6 +403,7 @@ int job_upload(DevJob& j, const CurveH& C, const double* params0, hipStream_t s)
   if (params0)
     for (int i = 0; i < N; ++i) p0[i] = params0[i];
   HIPCHK(hipMemcpy(j.params0, p0, sizeof p0, hipMemcpyHostToDevice));
+  j.done = pool.job_done + j.slots[0];
   HIPCHK(hipMemsetAsync(j.done, 0, sizeof(int32_t), s));
   if (!j.borrowed) {
     RETIF(dev_alloc(&j.rng, fits));
@@ -710,8 +717,6 @@ int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, in
     HIPCHK(hipGraphLaunch(c->cg_exec, s0));
     HIPCHK(hipEventRecord(c->ev1[0], s0));
     *launches += n_cls;
-    for (size_t i = 0; i < jobs.size(); ++i)
-      HIPCHK(hipMemcpyAsync(&done[i], jobs[i]->done, sizeof(int32_t), hipMemcpyDeviceToHost, s0));
     HIPCHK(hipStreamSynchronize(s0));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0[0], c->ev1[0]));
@@ -726,10 +731,6 @@ int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, in
     }
     for (int k = 0; k < NSTREAMS; ++k)
       if (used_s[k]) HIPCHK(hipEventRecord(c->ev1[k], c->streams[k]));
-    // Per-job finished-group counters.
-    for (size_t i = 0; i < jobs.size(); ++i)
-      HIPCHK(hipMemcpyAsync(&done[i], jobs[i]->done, sizeof(int32_t), hipMemcpyDeviceToHost,
-                            c->streams[stream_of(jobs[i]->cls)]));
     for (int k = 0; k < NSTREAMS; ++k) {
       bool any = used_s[k];
       for (size_t i = 0; i < jobs.size() && !any; ++i) any = stream_of(jobs[i]->cls) == k;
@@ -742,6 +743,12 @@ int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, in
       }
     }
   }
+  // Every job's finished-group counter in one read-back (they sit at the
+  // jobs' first slots; the streams are drained, so the counters are final).
+  Pool& P = c->pool;
+  if (!jobs.empty())
+    HIPCHK(hipMemcpy(P.job_done_h.data(), P.job_done, sizeof(int32_t) * P.n_slots, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < jobs.size(); ++i) done[i] = P.job_done_h[(size_t)jobs[i]->slots[0]];
   *kernel_ms += ms_max;
   const Clock::time_point now = Clock::now();
   for (size_t i = 0; i < jobs.size(); ++i) {
@@ -892,7 +899,7 @@ int make_job(ast_ctx* c, const ast_job* job, int G, std::unique_ptr<DevJob>* out
 // results (results may be null).  cycle_limit > 0 stops after that many
 // cycles (the autotuner's fixed work); 0 runs every job to completion.
 int batch_run(ast_ctx* c, const std::vector<const ast_job*>& jobs, int G, double work, long cycle_limit,
-              ast_result* results, double* kernel_ms, int* launches, bool graphs) {
+              ast_result* results, double* kernel_ms, int* launches, bool graphs, int schedule = 0) {
   const int n_jobs = (int)jobs.size();
   std::vector<std::unique_ptr<DevJob>> J(n_jobs);
   std::vector<CurveH> curves(n_jobs);
@@ -906,16 +913,43 @@ int batch_run(ast_ctx* c, const std::vector<const ast_job*>& jobs, int G, double
   RETIF(set_device(c));
   if ((size_t)c->pool.n_slots < groups || c->pool.G != G) RETIF(pool_alloc(c->pool, groups * G, G));
   c->last_groups = 0;  // the pool's rows no longer hold an ast_run's lanes
+  // Work of one simplex of each job: an upper bound of its evaluations (the
+  // budget, or the step limit) times the cost of one.
+  std::vector<double> evals(n_jobs), wj(n_jobs);
+  double w_max = 0;
+  for (int i = 0; i < n_jobs; ++i) {
+    const DevJob& j = *J[i];
+    double e = (double)(j.space.n_params + 1) + (double)j.rp.n_steps * (j.space.n_params + 2);
+    if (j.rp.eval_budget > 0 && (double)j.rp.eval_budget < e) e = (double)j.rp.eval_budget;
+    evals[i] = e * j.rp.nstages;
+    wj[i] = evals[i] * j.cost;
+    if (wj[i] > w_max) w_max = wj[i];
+  }
+  // Launch order (slots, descriptor order): the heaviest jobs first so they
+  // are always dispatched first (schedules 0 and 2), or the jobs' own order.
+  std::vector<int> order(n_jobs);
+  for (int i = 0; i < n_jobs; ++i) order[i] = i;
+  if (schedule != 1)
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return wj[a] > wj[b]; });
   std::vector<DevJob*> run;
   int slot = 0;
   const Clock::time_point t0 = Clock::now();
   int rc = 0;
-  for (int i = 0; i < n_jobs && !rc; ++i) {
+  for (int oi = 0; oi < n_jobs && !rc; ++oi) {
+    const int i = order[oi];
     DevJob& j = *J[i];
     for (int g = 0; g < j.rp.n_groups; ++g) j.slots.push_back(slot++);
     const hipStream_t s = c->streams[stream_of(j.cls)];
-    rc = job_upload(j, curves[i], p0[i].empty() ? nullptr : p0[i].data(), s);
-    if (!rc) rc = job_descs(j, ticks_for(j.cost, work));
+    // Ticks per launch: a share of the job's own total, so that every job
+    // ends near the cycle the heaviest one needs and the GPU stays full to
+    // the end (schedule 0), or equal work per launch for every job (1, 2).
+    int ticks = ticks_for(j.cost, work);
+    if (schedule == 0) {
+      const double t = std::ceil(evals[i] * work / w_max);
+      ticks = t < 1 ? 1 : (t > 4096 ? 4096 : (int)t);
+    }
+    rc = job_upload(j, c->pool, curves[i], p0[i].empty() ? nullptr : p0[i].data(), s);
+    if (!rc) rc = job_descs(j, ticks);
     if (!rc) {
       hipError_t e = job_launch(j, c->pool, 0, s);
       if (e != hipSuccess) {
@@ -924,6 +958,7 @@ int batch_run(ast_ctx* c, const std::vector<const ast_job*>& jobs, int G, double
       }
     }
     j.t_submit = j.t_admit = t0;
+    j.ticks = ticks;
     run.push_back(&j);
   }
   // Every tick completes one evaluation of each running simplex; a stage
@@ -931,7 +966,7 @@ int batch_run(ast_ctx* c, const std::vector<const ast_job*>& jobs, int G, double
   long max_cycles = 0;
   for (DevJob* j : run) {
     const double per_stage = (double)(j->space.n_params + 1) + (double)j->rp.n_steps * (j->space.n_params + 2);
-    const long mc = (long)((per_stage * j->rp.nstages + 16.0) / ticks_for(j->cost, work)) + 4;
+    const long mc = (long)((per_stage * j->rp.nstages + 16.0) / j->ticks) + 4;
     if (mc > max_cycles) max_cycles = mc;
   }
   for (long L = 0; !rc; ++L) {
@@ -952,7 +987,7 @@ int batch_run(ast_ctx* c, const std::vector<const ast_job*>& jobs, int G, double
     if (results) {
       ast_result& r = results[i];
       std::memset(&r, 0, sizeof r);
-      if (!rc && i < (int)run.size()) {
+      if (!rc && !j.slots.empty()) {
         int e = job_result(j, &r);
         int64_t ev = 0;
         if (!e) e = job_evals(c, j, &ev);
@@ -984,7 +1019,7 @@ int batch_run(ast_ctx* c, const std::vector<const ast_job*>& jobs, int G, double
 // under the scheduler lock when it was admitted.
 int serve_admit(ast_ctx* c, Serve& S, DevJob& j, const CurveH& curve, const std::vector<double>& p0) {
   const hipStream_t s = c->streams[stream_of(j.cls)];
-  RETIF(job_upload(j, curve, p0.empty() ? nullptr : p0.data(), s));
+  RETIF(job_upload(j, c->pool, curve, p0.empty() ? nullptr : p0.data(), s));
   RETIF(job_descs(j, ticks_for(j.cost, S.cfg.work_points_per_launch)));
   HIPCHK(job_launch(j, c->pool, 0, s));
   return 0;
@@ -1279,11 +1314,11 @@ int ast_run(ast_ctx* c, const ast_run_cfg* rc) {
   j.best_dV = c->best_dV;
   j.cls = class_of(c->v, c->curve.n_filters);
   const hipStream_t s = c->streams[stream_of(j.cls)];
-  RETIF(job_upload(j, c->curve, c->params0, s));
+  for (int g = 0; g < rc->n_groups; ++g) j.slots.push_back(g);
+  RETIF(job_upload(j, c->pool, c->curve, c->params0, s));
   // The context's streams and bests were cleared by synchronous-API calls on
   // the null stream, which the job's non-blocking stream does not wait for.
   HIPCHK(hipDeviceSynchronize());
-  for (int g = 0; g < rc->n_groups; ++g) j.slots.push_back(g);
   int ticks;
   if (rc->ticks_per_launch > 0) {
     ticks = rc->ticks_per_launch > 4096 ? 4096 : rc->ticks_per_launch;
@@ -1472,7 +1507,8 @@ int ast_run_jobs(ast_ctx* c, const ast_job* jobs, int n_jobs, const ast_batch_cf
   for (int i = 0; i < n_jobs; ++i) js.push_back(&jobs[i]);
   c->last_ms = 0;
   c->last_launches = 0;
-  return batch_run(c, js, G, work, 0, results, &c->last_ms, &c->last_launches, cfg && cfg->use_graphs);
+  return batch_run(c, js, G, work, 0, results, &c->last_ms, &c->last_launches, cfg && cfg->use_graphs,
+                   cfg ? cfg->schedule : 0);
 }
 
 int ast_set_waves(ast_ctx* c, const ast_variant* v, int n_filters, int waves) {

@@ -307,6 +307,10 @@ typedef struct {
   int group_size;                /* simplexes per group; 0 = 256 */
   double work_points_per_launch; /* as in ast_serve_cfg; 0 = 16000 */
   int use_graphs;                /* 1: replay unchanged launch cycles from a captured HIP graph */
+  int schedule;                  /* 0: heaviest jobs dispatched first, each job's launches sized so that
+                                    all jobs end together (the GPU stays full to the end); 1: equal
+                                    work per launch for every job; 2: as 1, heaviest first.  Results
+                                    never depend on it. */
 } ast_batch_cfg;
 
 int ast_run_jobs(ast_ctx* ctx, const ast_job* jobs, int n_jobs, const ast_batch_cfg* cfg, ast_result* results);

@@ -77,12 +77,24 @@ def test_batch_one_launch_per_class():
     for k in range(8):
         curve, _ = synth.make_curve(300 + k, v, 80, 1.5)
         jobs.append(dict(space=space, curve=curve, n_fits=256, n_steps=40, seed=k))
-    with api.Context(v) as ctx:
-        ctx.run_jobs(jobs, work_points_per_launch=2000)
+    with api.Context(v) as ctx:  # schedule 1: the same ticks per launch alone and in the batch
+        ctx.run_jobs(jobs, work_points_per_launch=2000, schedule=1)
         _, launches = ctx.last_timing()
         alone = []
         for j in jobs:
-            ctx.run_jobs([j], work_points_per_launch=2000)
+            ctx.run_jobs([j], work_points_per_launch=2000, schedule=1)
             alone.append(ctx.last_timing()[1])
     print("batch launches", launches, "alone", alone)
     assert launches == max(alone)
+
+
+def test_batch_schedules_agree():
+    """The three launch schedules (heaviest first with equal finish, equal
+    work per launch in job order, equal work heaviest first) change only how
+    the work is cut into launches: every result is the same bit for bit."""
+    jobs = _jobs()
+    with api.Context(variants.ls_torque()) as ctx:
+        runs = [ctx.run_jobs(jobs, work_points_per_launch=4000, schedule=s) for s in (0, 1, 2)]
+    for k in range(len(jobs)):
+        _same(runs[0][k], runs[1][k])
+        _same(runs[0][k], runs[2][k])
