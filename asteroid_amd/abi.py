@@ -153,6 +153,8 @@ def load():
         "ast_fetch": (I, [P, pD, pD, pD]),
         "ast_fetch_lanes": (I, [P, pD, pI64, pI64]),
         "ast_chi2_batch": (I, [P, pD, I, pD, pD]),
+        "ast_model_curve": (I, [P, pD, I, C.POINTER(Ephem3), I, pD, pD, pD, pD]),
+        "ast_periodogram": (I, [P, pD, pD, I, I, pD, pD, pI]),
         "ast_last_timing": (I, [P, pD, pI]),
         "ast_last_counts": (I, [P, pI64, pI64]),
         "ast_default_model_space": (I, [C.POINTER(Variant), pD, pI, pI, pI, pI, pD, C.POINTER(X2)]),
@@ -181,5 +183,5 @@ EXPORTED = (
     "ast_reset_best", "ast_set_start", "ast_run", "ast_fetch", "ast_fetch_lanes", "ast_chi2_batch",
     "ast_last_timing", "ast_last_counts", "ast_default_model_space", "ast_strerror",
     "ast_last_hip_error", "ast_serve_start", "ast_submit", "ast_poll", "ast_cancel", "ast_serve_stats",
-    "ast_serve_stop", "ast_run_jobs", "ast_autotune", "ast_set_waves",
+    "ast_serve_stop", "ast_run_jobs", "ast_autotune", "ast_set_waves", "ast_model_curve", "ast_periodogram",
 )

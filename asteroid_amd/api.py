@@ -119,6 +119,34 @@ class Context:
         _chk(self.lib.ast_chi2_batch(self.h, _dp(p), n, _dp(chi2), _dp(dv)), "ast_chi2_batch")
         return chi2, dv
 
+    def model_curve(self, params, n_plot=20000, curve=None):
+        """The -plot evaluation (chi2_plot, cuda.c:1865-1966) of one or more
+        models: (t, vmod, chi2, dV) with the grid times t (n_plot), vmod
+        (n_models x n_plot), chi2 (n_models) and dV (n_models x n_filters).
+        The grid's interior E,S come from the 3-point ephemeris of `curve`
+        (default: the uploaded curve)."""
+        p = np.ascontiguousarray(np.atleast_2d(params), np.float64)
+        n = p.shape[0]
+        eph = (curve or self.curve).ephem3()
+        t = np.zeros(n_plot)
+        vmod = np.zeros((n, n_plot))
+        chi2 = np.zeros(n)
+        dv = np.zeros((n, self.curve.n_filters))
+        _chk(self.lib.ast_model_curve(self.h, _dp(p), n, C.byref(eph), int(n_plot), _dp(t), _dp(vmod), _dp(chi2),
+                                      _dp(dv)), "ast_model_curve")
+        return t, vmod, chi2, dv
+
+    def periodogram(self, t, vm, n_clusters=5):
+        """minima() (misc.c:65-276) on one model curve: (cl_fr, cl_H, n_minima)."""
+        t = np.ascontiguousarray(t, np.float64)
+        vm = np.ascontiguousarray(vm, np.float64)
+        fr = np.zeros(max(n_clusters, 1))
+        h = np.zeros(max(n_clusters, 1))
+        nm = C.c_int()
+        _chk(self.lib.ast_periodogram(self.h, _dp(t), _dp(vm), len(t), int(n_clusters), _dp(fr), _dp(h),
+                                      C.byref(nm)), "ast_periodogram")
+        return fr[:n_clusters], h[:n_clusters], nm.value
+
     def last_timing(self):
         ms = C.c_double()
         n = C.c_int()

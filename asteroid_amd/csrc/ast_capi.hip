@@ -176,6 +176,7 @@ struct CurveH {  // a validated light curve as the kernels read it
   double sum_w[AST_MAX_FILTERS] = {0};
   int n_data = 0, n_filters = 0;
   int64_t rk4_steps = 0;  // RK4 steps of one evaluation
+  double t_first = 0, t_last = 0;  // times of the first and last observation
 };
 
 int make_curve(const ast_variant& v, const ast_obs* obs, int n_data, int n_filters, const ast_ephem3* eph,
@@ -215,7 +216,47 @@ int make_curve(const ast_variant& v, const ast_obs* obs, int n_data, int n_filte
     if (!seen[m]) return AST_E_DATA;
   C.n_data = n_data;
   C.n_filters = n_filters;
+  C.t_first = obs[0].mjd;
+  C.t_last = obs[n_data - 1].mjd;
   *out = std::move(C);
+  return AST_OK;
+}
+
+// The plotting grid (read_data.c:304-360): n_plot times evenly spaced from
+// the first to the last observation.  The two ends are the first and last
+// observations themselves, E and S included; the interior points take E,S
+// from the 3-point ephemeris (read_data.c:345, the INTERP quadratic, then
+// normalised).  RK4 intervals between grid points as between observations
+// (cuda.c:321-323).  V, w and the filter are unused on a grid.
+int make_grid(const ast_variant& v, const CurveH& C, const ast_ephem3& eph, int n_plot, std::vector<PointRec>* out,
+              std::vector<double>* t_out) {
+  if (n_plot < 2 || C.n_data < 1) return AST_E_ARG;
+  std::vector<PointRec> g((size_t)n_plot);
+  std::vector<double> t((size_t)n_plot);
+  const double h = (C.t_last - C.t_first) / (n_plot - 1);
+  for (int i = 0; i < n_plot; ++i) {
+    PointRec& p = g[i];
+    std::memset(&p, 0, sizeof p);
+    if (i == 0 || i == n_plot - 1) {
+      const PointRec& d = C.pts[i == 0 ? 0 : C.n_data - 1];
+      t[i] = i == 0 ? C.t_first : C.t_last;
+      for (int k = 0; k < 3; ++k) { p.E[k] = d.E[k]; p.S[k] = d.S[k]; }
+    } else {
+      t[i] = C.t_first + (double)i * h;
+      interp_canon(eph, t[i], p.E, p.S);
+    }
+    if (i > 0) {
+      const double t1 = t[i - 1], t2 = t[i];
+      const int nsteps = (int)((t2 - t1) / v.time_step + 1);
+      const double hs = (t2 - t1) / nsteps;
+      p.nsteps = nsteps;
+      p.h = hs;
+      p.hh = 0.5 * hs;
+      p.h6 = 1 / 6.0 * hs;
+    }
+  }
+  *out = std::move(g);
+  if (t_out) *t_out = std::move(t);
   return AST_OK;
 }
 
@@ -540,6 +581,140 @@ struct Serve {
 };
 
 }  // namespace
+
+// ------------------------------------------------------------- periodogram
+// The periodogram of the minima of a model light curve (minima(),
+// misc.c:65-276, the default branch: minima at grid times, step-function
+// histogram).  Constants of misc.c:117-121 and NMIN_MAX (misc.c:70).
+constexpr int PGRAM_BINS = 10000;        // NN
+constexpr double PGRAM_SGM = 0.1;        // sgm
+constexpr double PGRAM_DT0 = 1.0 / 24.0; // dt0, days
+constexpr double PGRAM_DT2 = 80.0 / 24.0;// dt2, days
+constexpr double PGRAM_S0 = 3;           // s0
+constexpr int PGRAM_MAX_MINIMA = 10000;  // NMIN_MAX
+
+// Histogram counts: for each frequency bin i (period p = 1/fr[i]), the
+// number of pairs of minima a < b whose interval is within a fraction sgm of
+// a whole number of periods (misc.c:128-146).  Each thread owns one bin;
+// blockIdx.y takes every n_chunks-th first minimum a, and the per-chunk
+// counts go to partial[chunk][bin] (summed on the host, exactly).  The
+// interval and the ratio are computed as the reference does (a subtraction
+// and an IEEE division), so the counts equal the reference's whatever the
+// order of the pairs.
+__global__ __launch_bounds__(256) void minima_hist_kernel(const double* __restrict__ tm, int n,
+                                                         const double* __restrict__ period, int n_bins,
+                                                         double sgm, int n_chunks, int32_t* __restrict__ partial) {
+  const int bin = blockIdx.x * blockDim.x + threadIdx.x;
+  const int chunk = blockIdx.y;
+  if (bin >= n_bins) return;
+  const double p = period[bin];
+  int32_t cnt = 0;
+  for (int a = chunk; a < n - 1; a += n_chunks) {
+    const double ta = tm[a];
+    for (int b = a + 1; b < n; ++b) {
+      const double q = (tm[b] - ta) / p;
+      const double dv = fabs(q - (double)(int)(q + 0.5));
+      cnt += dv < sgm ? 1 : 0;
+    }
+  }
+  partial[(size_t)chunk * n_bins + bin] = cnt;
+}
+
+// Frequency of bin i (misc.c:128).
+inline double pgram_fr(int i) {
+  const double fr0 = 1 / PGRAM_DT0, fr2 = 1 / PGRAM_DT2;
+  return fr2 + (double)i / (double)(PGRAM_BINS - 1) * (fr0 - fr2);
+}
+
+int pgram_counts(hipStream_t s, const std::vector<double>& tm, std::vector<int32_t>* counts, double* ms_out) {
+  counts->assign(PGRAM_BINS, 0);
+  *ms_out = 0;
+  const int n = (int)tm.size();
+  if (n < 2) return AST_OK;
+  std::vector<double> period(PGRAM_BINS);
+  for (int i = 0; i < PGRAM_BINS; ++i) period[i] = 1 / pgram_fr(i);
+  const int bx = (PGRAM_BINS + 255) / 256;
+  const int n_chunks = std::max(1, std::min(n - 1, (2048 + bx - 1) / bx));
+  double *td = nullptr, *pd = nullptr;
+  int32_t* part = nullptr;
+  int rc = dev_alloc(&td, (size_t)n);
+  if (!rc) rc = dev_alloc(&pd, (size_t)PGRAM_BINS);
+  if (!rc) rc = dev_alloc(&part, (size_t)n_chunks * PGRAM_BINS);
+  if (rc) { dev_free(td); dev_free(pd); dev_free(part); return rc; }
+  std::vector<int32_t> host((size_t)n_chunks * PGRAM_BINS);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t he = hipEventCreate(&e0);
+  if (he == hipSuccess) he = hipEventCreate(&e1);
+  if (he == hipSuccess) he = hipMemcpy(td, tm.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemcpy(pd, period.data(), sizeof(double) * PGRAM_BINS, hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipEventRecord(e0, s);
+  if (he == hipSuccess) {
+    hipLaunchKernelGGL(minima_hist_kernel, dim3(bx, n_chunks), dim3(256), 0, s, td, n, pd, PGRAM_BINS, PGRAM_SGM,
+                       n_chunks, part);
+    he = hipGetLastError();
+  }
+  if (he == hipSuccess) he = hipEventRecord(e1, s);
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  float ms = 0;
+  if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+  if (he == hipSuccess) he = hipMemcpy(host.data(), part, sizeof(int32_t) * host.size(), hipMemcpyDeviceToHost);
+  if (he != hipSuccess) {
+    (void)hipDeviceSynchronize();
+    snprintf(g_hip_err, sizeof g_hip_err, "%s (periodogram)", hipGetErrorString(he));
+    rc = AST_E_HIP;
+  } else {
+    for (int ch = 0; ch < n_chunks; ++ch)
+      for (int i = 0; i < PGRAM_BINS; ++i) (*counts)[i] += host[(size_t)ch * PGRAM_BINS + i];
+    *ms_out = ms;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  dev_free(td);
+  dev_free(pd);
+  dev_free(part);
+  return rc;
+}
+
+// Normalisation and clusters (misc.c:148-266), on the host in the
+// reference's order.  A cluster's left walk stops at bin 0 (the reference
+// reads H[-1] there, misc.c:240-252).  Clusters not found get frequency -1
+// and height 0.
+void pgram_clusters(const std::vector<int32_t>& counts, int ncl, double* cl_fr, double* cl_H) {
+  const int NN = PGRAM_BINS;
+  std::vector<double> H(NN), fr(NN);
+  std::vector<int> marked(NN, 0);
+  double m = 0.0;
+  for (int i = 0; i < NN; ++i) {
+    fr[i] = pgram_fr(i);
+    H[i] = (double)counts[i];
+    m = m + H[i];
+  }
+  m = m / NN;
+  double sd = 0;
+  for (int i = 0; i < NN; ++i) {
+    const double dev = H[i] - m;
+    sd = sd + dev * dev;
+  }
+  sd = std::sqrt(sd / (NN - 1));
+  for (int i = 0; i < NN; ++i) H[i] = (H[i] - m) / sd;
+  const double cl_min = PGRAM_S0;
+  for (int j = 0; j < ncl; ++j) { cl_fr[j] = -1.0; cl_H[j] = 0.0; }
+  int Ncl = 0;
+  for (int j = 0; j < ncl; ++j) {
+    double mx = -100;
+    int imax = -1;
+    cl_fr[j] = -1.0;
+    for (int i = 0; i < NN; ++i)
+      if (marked[i] == 0 && H[i] > mx) { mx = H[i]; imax = i; }
+    if (imax > -1 && H[imax] >= cl_min) {
+      cl_fr[Ncl] = fr[imax];
+      cl_H[Ncl] = H[imax];
+      Ncl++;
+      for (int i = imax; i < NN && H[i] > cl_min; ++i) marked[i] = 1;
+      for (int i = imax - 1; i >= 0 && H[i] > cl_min; --i) marked[i] = 1;
+    }
+  }
+}
 
 struct ast_ctx {
   int device = 0;
@@ -1446,6 +1621,90 @@ int ast_chi2_batch(ast_ctx* c, const double* params, int n, double* chi2, double
   dev_free(cd);
   dev_free(dd);
   return result;
+}
+
+int ast_model_curve(ast_ctx* c, const double* params, int n, const ast_ephem3* eph, int n_plot, double* t,
+                    double* vmod, double* chi2, double* dV) {
+  if (!c || !params || n <= 0 || !eph || n_plot < 2 || !vmod || !chi2 || !dV) return AST_E_ARG;
+  if (c->serve) return AST_E_BUSY;
+  if (!c->space_set || !c->curve_set) return AST_E_STATE;
+  if (c->curve.n_data - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
+  std::vector<PointRec> grid;
+  std::vector<double> tg;
+  RETIF(make_grid(c->v, c->curve, *eph, n_plot, &grid, &tg));
+  RETIF(set_device(c));
+  const int np = c->space.n_params, nf = c->curve.n_filters;
+  PointRec* gd = nullptr;
+  double *pd = nullptr, *cd = nullptr, *dd = nullptr, *vd = nullptr;
+  int rc = dev_alloc(&gd, (size_t)n_plot);
+  if (!rc) rc = dev_alloc(&pd, (size_t)n * np);
+  if (!rc) rc = dev_alloc(&cd, (size_t)n);
+  if (!rc) rc = dev_alloc(&dd, (size_t)n * nf);
+  if (!rc) rc = dev_alloc(&vd, (size_t)n * n_plot);
+  auto release = [&]() { dev_free(gd); dev_free(pd); dev_free(cd); dev_free(dd); dev_free(vd); };
+  if (rc) { release(); return rc; }
+  hipStream_t s = c->streams[0];
+  hipError_t he = hipMemcpy(gd, grid.data(), sizeof(PointRec) * n_plot, hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemcpy(pd, params, sizeof(double) * n * np, hipMemcpyHostToDevice);
+  LaunchArgs a{};
+  a.op = 3;
+  a.stream = s;
+  a.pts = c->pts;
+  a.sum_w = c->sum_w;
+  a.n_data = c->curve.n_data;
+  a.n_filters = nf;
+  a.dof = c->curve.n_data - np - nf;
+  a.params = pd;
+  a.n = n;
+  a.chi2 = cd;
+  a.dV = dd;
+  a.grid = gd;
+  a.n_plot = n_plot;
+  a.vmod = vd;
+  if (he == hipSuccess) he = hipEventRecord(c->ev0[0], s);
+  if (he == hipSuccess) he = dispatch(c->v, a);
+  if (he == hipSuccess) he = hipEventRecord(c->ev1[0], s);
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  float ms = 0;
+  if (he == hipSuccess) he = hipEventElapsedTime(&ms, c->ev0[0], c->ev1[0]);
+  if (he == hipSuccess) he = hipMemcpy(chi2, cd, sizeof(double) * n, hipMemcpyDeviceToHost);
+  if (he == hipSuccess) he = hipMemcpy(dV, dd, sizeof(double) * n * nf, hipMemcpyDeviceToHost);
+  if (he == hipSuccess) he = hipMemcpy(vmod, vd, sizeof(double) * n * (size_t)n_plot, hipMemcpyDeviceToHost);
+  int result = AST_OK;
+  if (he != hipSuccess) {
+    (void)hipDeviceSynchronize();
+    snprintf(g_hip_err, sizeof g_hip_err, "%s (model curve)", hipGetErrorString(he));
+    result = AST_E_HIP;
+  } else {
+    if (t) std::memcpy(t, tg.data(), sizeof(double) * n_plot);
+    c->last_ms = ms;
+    c->last_launches = 1;
+    c->last_evals = n;
+    c->last_points = (int64_t)n * (c->curve.n_data + n_plot);
+  }
+  release();
+  return result;
+}
+
+int ast_periodogram(ast_ctx* c, const double* t, const double* vm, int n, int n_clusters, double* cl_fr,
+                    double* cl_H, int* n_minima) {
+  if (!c || !t || !vm || n < 0 || n_clusters < 0 || (n_clusters > 0 && (!cl_fr || !cl_H))) return AST_E_ARG;
+  if (c->serve) return AST_E_BUSY;
+  // Brightness minima are maxima of the magnitude (misc.c:74-92).
+  std::vector<double> tm;
+  for (int i = 1; i <= n - 2; ++i)
+    if (vm[i] > vm[i - 1] && vm[i] >= vm[i + 1]) tm.push_back(t[i]);
+  if (n_minima) *n_minima = (int)tm.size();
+  if (tm.size() > (size_t)PGRAM_MAX_MINIMA) {
+    set_err("more than 10000 minima (misc.c:80-84)");
+    return AST_E_DATA;
+  }
+  std::vector<int32_t> counts;
+  RETIF(set_device(c));
+  RETIF(pgram_counts(c->streams[0], tm, &counts, &c->last_ms));
+  c->last_launches = tm.size() >= 2 ? 1 : 0;
+  pgram_clusters(counts, n_clusters, cl_fr, cl_H);
+  return AST_OK;
 }
 
 int ast_last_timing(ast_ctx* c, double* kernel_ms, int* launches) {

@@ -424,9 +424,13 @@ constexpr float MX_BACKLIT_COS = -0.8f;
 // (the mixed-precision form) the Euler angles and angular velocity are
 // integrated exactly as in the FP64 form, and each observation's geometry
 // and brightness run in FP32; the chi^2 sums stay FP64.
-template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX = false>
+// With PLOT (chi2one called with Nplot > 0, cuda.c:699-703, 884-885) the
+// points are a plotting grid: each point's Vmod + dv0 goes to
+// vplot[i * vstride], nothing is summed, and the result is 0.
+template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX = false, bool PLOT = false>
 __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_filters, int dof,
-                         const double* sum_w, CF* dV) {
+                         const double* sum_w, CF* dV, double* vplot = nullptr, size_t vstride = 0,
+                         double dv0 = 0.0) {
   using P = Pos<TQ, BC, BW>;
   double sy2[NF], sy[NF];
 #pragma unroll
@@ -546,6 +550,10 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
         Vmod = ls_tail_f64(u, K.bcp);
       }
     }
+    if constexpr (PLOT) {
+      vplot[(size_t)i * vstride] = Vmod + dv0;
+      continue;
+    }
     const double w = (double)pr.w;
     const double yv = (double)pr.V - Vmod;
     const int m = pr.filter;
@@ -561,6 +569,7 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
         }
     }
   }
+  if constexpr (PLOT) return CF(0.0);
   CF chi2a = 0.0;  // cuda.c:884-910
 #pragma unroll
   for (int m = 0; m < NF; ++m) {

@@ -628,9 +628,36 @@ __global__ __launch_bounds__(256) void chi2_batch_kernel(const PointRec* pts_g, 
   for (int m = 0; m < n_filters; ++m) dV[(size_t)k * n_filters + m] = dv[m];
 }
 
+// ------------------------------------------------------------ model curve
+// The -plot evaluation (chi2_plot, cuda.c:1865-1966, without PROFILES and
+// ANIMATE), one model per lane: chi^2 and delta_V of the model against the
+// observations (step one, cuda.c:1939-1942), then the model magnitude plus
+// delta_V[0] at each point of the plotting grid (step two, cuda.c:1949-1950),
+// integrated along the grid from its first time.  vmod is model-major: lane
+// k writes vmod[k * n_plot + i].
+template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX>
+__global__ __launch_bounds__(256) void model_curve_kernel(const PointRec* pts_g, const double* sum_w, int n_data,
+                                                         int n_filters, int dof, const PointRec* grid_g, int n_plot,
+                                                         const double* params, int n, double* chi2, double* dV,
+                                                         double* vmod) {
+  using P = Pos<TQ, BC, BW>;
+  constexpr int N = P::N;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double prm[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) prm[i] = params[(size_t)k * N + i];
+  CF dv[NF];
+  CF c = chi2_model<CF, TQ, BC, BW, NF, MX>(prm, (PointPtr)(uintptr_t)pts_g, n_data, n_filters, dof, sum_w, dv);
+  chi2[k] = c;
+  for (int m = 0; m < n_filters; ++m) dV[(size_t)k * n_filters + m] = dv[m];
+  chi2_model<CF, TQ, BC, BW, NF, MX, true>(prm, (PointPtr)(uintptr_t)grid_g, n_plot, n_filters, dof, sum_w, dv,
+                                          vmod + (size_t)k * n_plot, 1, (double)dv[0]);
+}
+
 // --------------------------------------------------------- launch interface
 struct LaunchArgs {
-  int op;  // 0 init groups, 1 fit ticks, 2 chi2 batch
+  int op;  // 0 init groups, 1 fit ticks, 2 chi2 batch, 3 model curve
   int precision;
   hipStream_t stream;
   // fit: descriptors of the groups of this launch (device), their count, G
@@ -646,6 +673,10 @@ struct LaunchArgs {
   int n;
   double* chi2;
   double* dV;
+  // model curve: the plotting grid and the magnitudes on it
+  const PointRec* grid;
+  int n_plot;
+  double* vmod;
 };
 
 template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX>
@@ -666,10 +697,15 @@ hipError_t launch_nf(const LaunchArgs& a) {
     else
       hipLaunchKernelGGL((fit_kernel<CF, TQ, BC, BW, NF, MX, 4>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
                          d, B);
-  } else {
+  } else if (a.op == 2) {
     int blocks = (a.n + 255) / 256;
     hipLaunchKernelGGL((chi2_batch_kernel<CF, TQ, BC, BW, NF, MX>), dim3(blocks), dim3(256), 0, a.stream, a.pts,
                        a.sum_w, a.n_data, a.n_filters, a.dof, a.params, a.n, a.chi2, a.dV);
+  } else {
+    int blocks = (a.n + 255) / 256;
+    hipLaunchKernelGGL((model_curve_kernel<CF, TQ, BC, BW, NF, MX>), dim3(blocks), dim3(256), 0, a.stream, a.pts,
+                       a.sum_w, a.n_data, a.n_filters, a.dof, a.grid, a.n_plot, a.params, a.n, a.chi2, a.dV,
+                       a.vmod);
   }
   return hipGetLastError();
 }

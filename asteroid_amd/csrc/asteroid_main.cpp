@@ -192,6 +192,83 @@ Ingested read_data(const char* data_file, bool interp) {
   return D;
 }
 
+// The -plot branch of main() (asteroid.c:833-1059 without PROFILES,
+// ANIMATE, MINIMA_TEST, PLOT_OMEGA, LAST, MY_L and NOPRINT): the model
+// curve on the NPLOT-point grid, delta_V, model_params.dat, the minima
+// periodogram line, the chi2_plot line, model.dat and data.dat.
+// chi_float: the reference's CHI_FLOAT is float (every form but -DACC), so
+// data.dat's converted magnitudes are computed in float as there.
+int plot_model(ast_ctx* ctx, const ast_variant& v, const Ingested& D, const std::vector<double>& params,
+               const int* property, int n_params, bool chi_float) {
+  const int NPLOT = 20000;  // asteroid.h:266
+  const int NCL_MAX = 5;    // asteroid.h:277
+  const int N_data = (int)D.obs.size(), N_filters = D.n_filters;
+  printf("\n*** Plotting ***\n\n");
+  std::vector<double> t(NPLOT), vm(NPLOT), dV(N_filters);
+  double chi2 = 0;
+  CHECK(ast_model_curve(ctx, params.data(), 1, &D.eph, NPLOT, t.data(), vm.data(), &chi2, dV.data()));
+  FILE* fV = fopen("delta_V", "w");
+  if (!fV) die("Cannot open delta_V!");
+  for (int m = 0; m < N_filters; ++m) fprintf(fV, "%8.4f\n", dV[m]);
+  fclose(fV);
+  // The initial model without the torque, photometric b,c and BW_BALL
+  // parameters (asteroid.c:907-933).
+  ast_type_ids T;
+  ast_variant_type_ids(&v, &T);
+  FILE* fmod = fopen("model_params.dat", "w");
+  if (!fmod) die("Cannot open model_params.dat!");
+  for (int k = 0; k < n_params; ++k) {
+    const int ty = property[k * AST_N_COLUMNS + AST_P_TYPE];
+    if (v.torque && (ty == T.Ti || ty == T.Ts || ty == T.Tl)) continue;
+    if (v.bc && (ty == T.c || ty == T.b)) continue;
+    if (v.bw_ball && (ty == T.theta_R || ty == T.phi_R || ty == T.kappa)) continue;
+    fprintf(fmod, "%15.11f ", params[k]);
+  }
+  fprintf(fmod, "\n");
+  fclose(fmod);
+  double cl_fr[NCL_MAX], cl_H[NCL_MAX];
+  CHECK(ast_periodogram(ctx, t.data(), vm.data(), NPLOT, NCL_MAX, cl_fr, cl_H, nullptr));
+  for (int j = 0; j < NCL_MAX; ++j) printf("%f ", cl_fr[j]);
+  for (int j = 0; j < NCL_MAX; ++j) printf("%f ", cl_H[j]);
+  printf("\n");
+  // asteroid.c:984-1023 averages h_dlsq2 over six time bins, but no kernel
+  // writes d_dlsq2 (chi2_plot takes it and leaves it alone) and it is never
+  // copied back, so h_dlsq2 keeps the zeros of its fresh allocation: the
+  // distance is 0 when every bin holds data and 0/0 otherwise.
+  double d2[6], w[6];
+  for (int i = 0; i < 6; ++i) { d2[i] = 0.0; w[i] = 0.0; }
+  for (int i = 0; i < N_data; ++i) {
+    const double tm = D.obs[i].mjd;
+    const int ind = tm < 0.5 ? 0 : tm < 1.5 ? 1 : tm < 2.7 ? 2 : tm < 3.7 ? 3 : tm < 4.5 ? 4 : 5;
+    d2[ind] = d2[ind] + 0.0;
+    w[ind] = w[ind] + 1;
+  }
+  double sum = 0.0, SW = 0.0;
+  for (int i = 0; i < 6; ++i) {
+    const double W = (i == 2 || i == 3) ? 2 : 1;  // extra weight on bins 2 and 3
+    sum = sum + W * d2[i] / w[i];
+    SW = SW + W;
+  }
+  const double dist = std::sqrt(sum / SW);
+  printf("chi2_plot = %13.6e, lsq = %13.6e\n", chi2, dist);
+  FILE* fp = fopen("model.dat", "w");
+  if (!fp) die("Cannot open model.dat!");
+  for (int i = 0; i < NPLOT; ++i) fprintf(fp, "%13.7f %13.6e\n", D.mjd0 + t[i], vm[i]);
+  fclose(fp);
+  fp = fopen("data.dat", "w");
+  if (!fp) die("Cannot open data.dat!");
+  for (const ast_obs& o : D.obs) {
+    // V converted to the first filter; the error bar 1/sqrt(w) in float
+    double Vc;
+    if (chi_float) Vc = (float)((float)(o.V - (float)dV[o.filter]) + (float)dV[0]);
+    else Vc = (double)o.V - dV[o.filter] + dV[0];
+    const float sg = 1 / std::sqrt(o.w);
+    fprintf(fp, "%13.7f %13.6e %13.6e w\n", D.mjd0 + o.mjd, Vc, sg);
+  }
+  fclose(fp);
+  return 0;
+}
+
 void usage() {
   printf("\n Command line arguments:\n\n");
   printf("-best : only keep the best result\n");
@@ -298,9 +375,8 @@ int main(int argc, char** argv) {
   if (!any_macro) { v.torque = 1; v.p_psi = 1; v.interp = 1; }
   v.precision = acc == 2 ? AST_PREC_MIXED : (acc ? AST_PREC_FP64 : AST_PREC_F32SIMPLEX);
   if (j_input == -1) die("-i parameter is missing!");
-  if (plot) die("-plot is not built in this version (fitting only)");
-  if (reopt && !model) die("-reopt and -plot switches require -m switch!");
-  if (!reopt && model) die("-m can only be used together with -reopt or -plot switches!");
+  if ((reopt || plot) && !model) die("-reopt and -plot switches require -m switch!");
+  if (!reopt && !plot && model) die("-m can only be used together with -reopt or -plot switches!");
 
   // Parameter tables of the variant (asteroid.c:86-169, 473-601).
   int n_params = 0, n_types = 0;
@@ -361,6 +437,11 @@ int main(int argc, char** argv) {
   CHECK(rc);
   CHECK(ast_set_model_space(ctx, n_params, property, n_types, types, limits, &x2));
   CHECK(ast_upload_curve(ctx, D.obs.data(), N_data, N_filters, &D.eph));
+  if (plot) {
+    int rc_plot = plot_model(ctx, v, D, params, property, n_params, acc != 1);
+    CHECK(ast_ctx_destroy(ctx));
+    return rc_plot;
+  }
 
   printf("\n*** Simplex optimization ***\n\n");
   printf(" N_threads = %d\n", n_blocks * bsize);

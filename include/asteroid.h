@@ -213,6 +213,33 @@ int ast_fetch_lanes(ast_ctx* ctx, double* f_last, int64_t* n_evals, int64_t* n_s
  * computed one model per GPU lane.  chi2 has n entries, dV n*n_filters. */
 int ast_chi2_batch(ast_ctx* ctx, const double* params, int n, double* chi2, double* dV);
 
+/* The -plot evaluation (chi2_plot, cuda.c:1865-1966, launched at
+ * asteroid.c:880; its grid is built at read_data.c:304-360).  For each of n
+ * models (physical parameters, n*n_params doubles): chi^2 and delta_V
+ * against the uploaded curve (cuda.c:1939-1942), then the model magnitude
+ * plus delta_V[0] (d_Vmod, cuda.c:699-703) at n_plot grid times, evenly
+ * spaced from the first to the last observation (NPLOT = 20000 in the
+ * reference, asteroid.h:266).  The grid's two ends are the first and last
+ * observations; its interior E,S come from `eph`, the 3-point ephemeris
+ * (read_data.c:345; required for every variant).  Outputs, caller-owned:
+ * t (n_plot grid times, may be NULL), vmod (n*n_plot, model k's curve at
+ * vmod[k*n_plot ...]), chi2 (n), dV (n*n_filters).  One model per GPU lane. */
+int ast_model_curve(ast_ctx* ctx, const double* params, int n, const ast_ephem3* eph, int n_plot,
+                    double* t, double* vmod, double* chi2, double* dV);
+
+/* The periodogram of a model light curve's brightness minima (minima(),
+ * misc.c:65-276, called at asteroid.c:972; the default branch: minima at
+ * grid times, step-function histogram of 10000 frequency bins from
+ * 0.3/day to 24/day).  t and vm are the n grid times and magnitudes
+ * (ast_model_curve's t and one model's vmod).  The top n_clusters clusters
+ * of the normalised histogram (cl_fr, cl_H of asteroid.c:974-979, NCL_MAX =
+ * 5 in the reference) come back in cl_fr (frequency, 1/day; -1 where no
+ * cluster reaches 3 sigma) and cl_H (height in sigmas; 0 where none);
+ * n_minima (may be NULL) receives the number of minima.  More than 10000
+ * minima is AST_E_DATA (misc.c:80-84). */
+int ast_periodogram(ast_ctx* ctx, const double* t, const double* vm, int n, int n_clusters,
+                    double* cl_fr, double* cl_H, int* n_minima);
+
 /* Timing of the last ast_run / ast_chi2_batch, from HIP events on the
  * context's stream: total kernel milliseconds and the number of launches. */
 int ast_last_timing(ast_ctx* ctx, double* kernel_ms, int* launches);

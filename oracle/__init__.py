@@ -52,6 +52,8 @@ def lib():
         L.orc_params2x.argtypes = [pP, C.c_int, pD, C.c_int, pD]
         L.orc_run.argtypes = [pP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_double,
                               C.c_uint64, C.c_int, pD, pU32, pD, pD, pD, pD, pI64, pI64, C.c_int]
+        L.orc_model_curve.argtypes = [pP, C.c_int, pD, C.c_int, C.c_int, pD, pD, pD, pD]
+        L.orc_periodogram.argtypes = [pD, pD, C.c_int, C.c_int, pD, pD, pI, pD]
         L.orc_math.argtypes = [C.c_int, pD, pD, pD, C.c_int]
         L.orc_philox.argtypes = [pU32, pU32, pU32]
         L.orc_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int, C.POINTER(C.c_float)]
@@ -105,6 +107,18 @@ class Problem:
             out.append(tr)
         return tuple(out)
 
+    def model_curve(self, params, n_plot=20000, form=CANON):
+        """chi2_plot on the reference's plotting grid: (t, vmod, chi2, dV)."""
+        params = np.ascontiguousarray(np.atleast_2d(params), np.float64)
+        n = params.shape[0]
+        t = np.zeros(n_plot)
+        vm = np.zeros((n, n_plot))
+        chi2 = np.zeros(n)
+        dV = np.zeros((n, self.curve.n_filters))
+        rc = lib().orc_model_curve(C.byref(self.pb), form, _p(params), n, n_plot, _p(t), _p(vm), _p(chi2), _p(dV))
+        assert rc == 0, rc
+        return t, vm, chi2, dV
+
     def x2params(self, x, reopt=0, form=CANON):
         x = np.ascontiguousarray(np.atleast_2d(x), np.float64)
         n = x.shape[0]
@@ -140,6 +154,21 @@ class Problem:
         assert rc == 0, rc
         return dict(rng=rng, f=d_f, params=d_params, dV=d_dV, lane_f=lane_f, lane_nev=lane_nev,
                     lane_steps=lane_steps)
+
+
+def periodogram(t, vm, n_clusters=5):
+    """minima() (misc.c:65-276): (cl_fr, cl_H, n_minima, H) or None when
+    there are more than 10000 minima."""
+    t = np.ascontiguousarray(t, np.float64)
+    vm = np.ascontiguousarray(vm, np.float64)
+    fr = np.zeros(max(n_clusters, 1))
+    h = np.zeros(max(n_clusters, 1))
+    H = np.zeros(10000)
+    nm = C.c_int()
+    rc = lib().orc_periodogram(_p(t), _p(vm), len(t), n_clusters, _p(fr), _p(h), C.byref(nm), _p(H))
+    if rc != 0:
+        return None
+    return fr[:n_clusters], h[:n_clusters], nm.value, H
 
 
 MATH = {"sin": 0, "cos": 1, "tan": 2, "atan2": 3, "log": 4, "log10": 5, "exp": 6, "asin": 7,

@@ -33,6 +33,7 @@
 // params2x starts with LAM = 0 (cuda.c:1022); the last delta_V of a thread
 // starts at 0 (cuda.c:1345).
 #include <cmath>
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <thread>
@@ -890,6 +891,125 @@ void run_group(const Problem& P, int form, const RunCfg& C, int reopt_in, const 
   }
 }
 
+// ================================================================ plotting
+// The plotting grid (read_data.c:304-360) as observations: n_plot times
+// (t_last - t_first)/(n_plot-1) apart from the first observation (the
+// reference's first observation is at time 0 and its grid times are
+// iplot*h), both ends copied from the first and last observation, E and S
+// included; the interior points' E,S interpolated from the 3-point
+// ephemeris and normalised (read_data.c:345-355; literal or canonical
+// arithmetic by form).  V = w = 0 and filter 0: only Vmod is read.
+inline std::vector<ast_obs> plot_grid(const Problem& P, int form, int n_plot) {
+  std::vector<ast_obs> g((size_t)n_plot);
+  const ast_obs& first = P.obs[0];
+  const ast_obs& last = P.obs[P.n_data - 1];
+  const double h = (last.mjd - first.mjd) / (n_plot - 1);
+  for (int i = 0; i < n_plot; ++i) {
+    ast_obs& o = g[i];
+    std::memset(&o, 0, sizeof o);
+    if (i == 0 || i == n_plot - 1) {
+      const ast_obs& d = i == 0 ? first : last;
+      o.mjd = d.mjd;
+      for (int c = 0; c < 3; ++c) { o.E[c] = d.E[c]; o.S[c] = d.S[c]; }
+    } else {
+      o.mjd = first.mjd + i * h;
+      if (form == 0) interp_literal(P.eph, o.mjd, o.E, o.S);
+      else interp_canon(P.eph, o.mjd, o.E, o.S);
+    }
+  }
+  return g;
+}
+
+// chi2_plot (cuda.c:1865-1966, no PROFILES/ANIMATE): chi^2 and delta_V on
+// the observations, then Vmod + delta_V[0] (cuda.c:699-703) along the grid,
+// integrated from the grid's first time.
+template <class CF>
+inline CF model_curve(const Problem& P, int form, const double* params, const std::vector<ast_obs>& grid, CF* dV,
+                      double* vmod) {
+  const CF chi2 = chi2_eval<CF>(P, form, params, dV);
+  Problem G = P;
+  G.obs = grid.data();
+  G.n_data = (int)grid.size();
+  G.n_filters = 1;
+  double sy2[AST_MAX_FILTERS], sy[AST_MAX_FILTERS], sw[AST_MAX_FILTERS];
+  ModelOut out{vmod, nullptr};
+  if (form == 0) model_literal(G, params, sy2, sy, sw, &out);
+  else model_canon<double>(G, params, sy2, sy, sw, &out);
+  for (int i = 0; i < G.n_data; ++i) vmod[i] = vmod[i] + dV[0];
+  return chi2;
+}
+
+// minima() (misc.c:65-276) without PARABOLIC_MAX, MINIMA_SPLINE and
+// MINIMA_PRINT: brightness minima (magnitude maxima) of the curve, every
+// pairwise interval, a 10000-bin histogram over frequency of the intervals
+// within 10% of a whole number of periods, normalised by its mean and
+// standard deviation, and the top ncl clusters above 3 sigma.  The left walk
+// of a cluster stops at bin 0 (the reference reads H[-1] there,
+// misc.c:240-252).  Clusters not found: frequency -1, height 0.  H (10000
+// values, may be null) receives the normalised histogram.
+inline int periodogram(const double* t, const double* Vm, int Nplot, int ncl, double* cl_fr, double* cl_H,
+                       int* n_min, double* H_out) {
+  const int NMIN_MAX = 10000;
+  std::vector<double> tm;
+  for (int i = 1; i <= Nplot - 2; ++i)
+    if (Vm[i] > Vm[i - 1] && Vm[i] >= Vm[i + 1]) tm.push_back(t[i]);
+  const int N = (int)tm.size();
+  if (n_min) *n_min = N;
+  if (N > NMIN_MAX) return AST_E_DATA;
+  std::vector<double> dt;
+  for (int i = 0; i <= N - 2; ++i)
+    for (int j = i + 1; j <= N - 1; ++j) dt.push_back(tm[j] - tm[i]);
+  std::sort(dt.begin(), dt.end());
+  const int NN = 10000;
+  const double sgm = 0.1, dt0 = 1.0 / 24.0, dt2 = 80.0 / 24.0, s0 = 3;
+  const double fr0 = 1 / dt0, fr2 = 1 / dt2;
+  std::vector<double> fr(NN), H(NN);
+  std::vector<int> marked(NN, 0);
+  double m = 0.0;
+  for (int i = 0; i < NN; ++i) {
+    fr[i] = fr2 + (double)i / (double)(NN - 1) * (fr0 - fr2);
+    H[i] = 0.0;
+    const double dt1 = 1 / fr[i];
+    for (size_t j = 0; j < dt.size(); ++j) {
+      const double dv = std::fabs(dt[j] / dt1 - (int)(dt[j] / dt1 + 0.5));
+      if (dv < sgm) H[i] = H[i] + 1.0;
+    }
+    m = m + H[i];
+  }
+  m = m / NN;
+  double s = 0;
+  for (int i = 0; i < NN; ++i) {
+    const double dev = H[i] - m;
+    s = s + dev * dev;
+  }
+  s = std::sqrt(s / (NN - 1));
+  for (int i = 0; i < NN; ++i) H[i] = (H[i] - m) / s;
+  if (H_out) for (int i = 0; i < NN; ++i) H_out[i] = H[i];
+  const double cl_min = s0;
+  int Ncl = 0;
+  for (int j = 0; j < ncl; ++j) cl_H[j] = 0.0;
+  for (int j = 0; j < ncl; ++j) {
+    double max = -100;
+    int imax = -1;
+    cl_fr[j] = -1.0;
+    for (int i = 0; i < NN; ++i)
+      if (marked[i] == 0 && H[i] > max) {
+        max = H[i];
+        imax = i;
+      }
+    if (imax > -1 && H[imax] >= cl_min) {
+      cl_fr[Ncl] = fr[imax];
+      cl_H[Ncl] = H[imax];
+      Ncl++;
+      int i = imax;
+      while (i < NN && H[i] > cl_min) marked[i++] = 1;
+      i = imax - 1;
+      while (i >= 0 && H[i] > cl_min) marked[i--] = 1;
+    }
+  }
+  return 0;
+}
+
 // ============================================================ op counting
 // A scalar that counts the floating-point operations performed on it.  The
 // canonical evaluator and the elementary functions are templates over the
@@ -985,6 +1105,38 @@ int orc_chi2(const orc_problem* pb, int form, const double* params, int n, doubl
     }
   }
   return 0;
+}
+
+// The -plot evaluation of n models on an n_plot-point grid: t (n_plot),
+// vmod (n*n_plot, model-major), chi2 (n), dV (n*n_filters).
+int orc_model_curve(const orc_problem* pb, int form, const double* params, int n, int n_plot, double* t,
+                    double* vmod, double* chi2, double* dV) {
+  orc::Problem P;
+  int rc = load_problem(pb, &P);
+  if (rc) return rc;
+  if (n_plot < 2) return AST_E_ARG;
+  const std::vector<ast_obs> grid = orc::plot_grid(P, form, n_plot);
+  if (t) for (int i = 0; i < n_plot; ++i) t[i] = grid[i].mjd;
+  for (int k = 0; k < n; ++k) {
+    const double* p = params + (size_t)k * P.n_params;
+    double* vm = vmod + (size_t)k * n_plot;
+    if (simplex_float(P.v)) {
+      float d[AST_MAX_FILTERS];
+      chi2[k] = orc::model_curve<float>(P, form, p, grid, d, vm);
+      for (int m = 0; m < P.n_filters; ++m) dV[(size_t)k * P.n_filters + m] = d[m];
+    } else {
+      double d[AST_MAX_FILTERS];
+      chi2[k] = orc::model_curve<double>(P, form, p, grid, d, vm);
+      for (int m = 0; m < P.n_filters; ++m) dV[(size_t)k * P.n_filters + m] = d[m];
+    }
+  }
+  return 0;
+}
+
+int orc_periodogram(const double* t, const double* vm, int n, int ncl, double* cl_fr, double* cl_H, int* n_min,
+                    double* H) {
+  if (!t || !vm || n < 0 || ncl < 0) return AST_E_ARG;
+  return orc::periodogram(t, vm, n, ncl, cl_fr, cl_H, n_min, H);
 }
 
 // x2params on n points (x in double, converted to the simplex float type).

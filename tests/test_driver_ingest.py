@@ -122,3 +122,7 @@ def test_ingest_rejects_bad_input(tmp_path):
     assert r.returncode == 0 and "-Nstages" in r.stdout
     r = subprocess.run([EXE, "-i", "lc.txt", "-reopt"], cwd=tmp_path, stdout=subprocess.PIPE, text=True)
     assert r.returncode == 1 and "require -m" in r.stdout
+    r = subprocess.run([EXE, "-i", "lc.txt", "-plot"], cwd=tmp_path, stdout=subprocess.PIPE, text=True)
+    assert r.returncode == 1 and "require -m" in r.stdout
+    r = subprocess.run([EXE, "-i", "lc.txt", "-m"] + ["0.5"] * 11, cwd=tmp_path, stdout=subprocess.PIPE, text=True)
+    assert r.returncode == 1 and "-m can only be used together with -reopt or -plot" in r.stdout

@@ -80,3 +80,47 @@ def test_driver_matches_oracle(tmp_path, flag, prec):
     np.testing.assert_allclose(file_p, want["params"], rtol=0, atol=1e-11)
     dv = np.array([[float(x) for x in l[1:1 + curve.n_filters]] for l in lines])
     np.testing.assert_allclose(dv, want["dV"], rtol=1e-6, atol=1e-12)
+
+
+def test_driver_plot_matches_oracle(tmp_path):
+    """`asteroid -plot -m ...` (asteroid.c:833-1059): delta_V, model_params.dat,
+    the periodogram line, the chi2_plot line, model.dat and data.dat, against
+    the oracle's chi2_plot and minima() on the driver's own ingested data."""
+    _scene(tmp_path, n_obs=80, filters="RG")
+    curve = _ingested(tmp_path)
+    mjd0 = float(open(tmp_path / "dump.txt").readline().split()[2])
+    v = variants.ls_torque(abi.PREC_F32SIMPLEX)
+    space = variants.default_model_space(v)
+    pb = oracle.Problem(space, curve)
+    p, rc = pb.x2params(np.full((1, space.n_params), 0.37))
+    assert rc[0] == 0
+    params = p[0]
+    cmd = [EXE, "-i", "lc.txt", "-plot", "-m"] + ["%.17g" % x for x in params]
+    r = subprocess.run(cmd, cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout
+    t, vm, chi2, dv = pb.model_curve(params, n_plot=20000)
+    fr, H, nmin, _ = oracle.periodogram(t, vm[0], 5)
+    out = r.stdout.splitlines()
+    assert "*** Plotting ***" in out
+    nums = [n for n in map(_numbers, out) if n is not None and len(n) == 10]
+    assert len(nums) == 1
+    assert nums[0] == [float("%f" % x) for x in list(fr) + list(H)]
+    line = [l for l in out if l.startswith("chi2_plot = ")]
+    # lsq averages a buffer no kernel writes (zeros) over six time bins: 0, or 0/0 with an empty bin
+    bins = np.digitize(curve.mjd, [0.5, 1.5, 2.7, 3.7, 4.5])
+    assert len(line) == 1 and line[0].startswith("chi2_plot = %13.6e, lsq = " % chi2[0])
+    lsq = line[0].split("lsq =")[1].strip()
+    assert lsq == ("0.000000e+00" if len(set(bins)) == 6 else lsq) and (len(set(bins)) == 6 or "nan" in lsq)
+    assert [l.rstrip("\n") for l in open(tmp_path / "delta_V")] == ["%8.4f" % x for x in dv[0]]
+    keep = [k for k in range(space.n_params) if k not in (3, 4, 5)]  # the torque parameters are left out
+    assert open(tmp_path / "model_params.dat").read() == "".join("%15.11f " % params[k] for k in keep) + "\n"
+    model = [l.split() for l in open(tmp_path / "model.dat")]
+    assert len(model) == 20000
+    assert [m[0] for m in model[::997]] == [("%13.7f" % (mjd0 + x)).strip() for x in t[::997]]
+    assert [m[1] for m in model] == ["%.6e" % x for x in vm[0]]
+    data = [l.split() for l in open(tmp_path / "data.dat")]
+    assert len(data) == curve.n_data and all(d[3] == "w" for d in data)
+    f32 = np.float32
+    want_v = [f32(f32(curve.V[i] - f32(dv[0, curve.filter[i]])) + f32(dv[0, 0])) for i in range(curve.n_data)]
+    assert [d[1] for d in data] == ["%.6e" % x for x in want_v]
+    assert [d[2] for d in data] == ["%.6e" % f32(1 / np.sqrt(curve.w[i])) for i in range(curve.n_data)]
