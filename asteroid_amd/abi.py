@@ -83,6 +83,11 @@ class BatchCfg(C.Structure):
                 ("schedule", C.c_int)]
 
 
+class RmsdCfg(C.Structure):
+    _fields_ = [("n_groups", C.c_int), ("group_size", C.c_int), ("n_samples", C.c_int), ("dx", C.c_float),
+                ("spherical_k", C.c_int)]
+
+
 class TuneEntry(C.Structure):
     _fields_ = [("torque", C.c_int), ("bc", C.c_int), ("bw_ball", C.c_int), ("precision", C.c_int),
                 ("filter_capacity", C.c_int), ("waves", C.c_int), ("kernel_ms", C.c_double * 3)]
@@ -155,6 +160,7 @@ def load():
         "ast_chi2_batch": (I, [P, pD, I, pD, pD]),
         "ast_model_curve": (I, [P, pD, I, C.POINTER(Ephem3), I, pD, pD, pD, pD]),
         "ast_periodogram": (I, [P, pD, pD, I, I, pD, pD, pI]),
+        "ast_rmsd": (I, [P, C.POINTER(RmsdCfg), C.POINTER(C.c_float), C.POINTER(C.c_float), pI64, pI64, pD, pD]),
         "ast_last_timing": (I, [P, pD, pI]),
         "ast_last_counts": (I, [P, pI64, pI64]),
         "ast_default_model_space": (I, [C.POINTER(Variant), pD, pI, pI, pI, pI, pD, C.POINTER(X2)]),
@@ -184,4 +190,5 @@ EXPORTED = (
     "ast_last_timing", "ast_last_counts", "ast_default_model_space", "ast_strerror",
     "ast_last_hip_error", "ast_serve_start", "ast_submit", "ast_poll", "ast_cancel", "ast_serve_stats",
     "ast_serve_stop", "ast_run_jobs", "ast_autotune", "ast_set_waves", "ast_model_curve", "ast_periodogram",
+    "ast_rmsd",
 )

@@ -147,6 +147,21 @@ class Context:
                                       C.byref(nm)), "ast_periodogram")
         return fr[:n_clusters], h[:n_clusters], nm.value
 
+    def rmsd(self, n_groups, group_size, n_samples, dx, spherical_k=False):
+        """chi2_gpu_rms (cuda.c:2148-2389) around the start model (set_start):
+        per-group float ranges par_min, par_max (n_groups x n_params), this
+        call's n_tot and n_bad, and the start model's f0 and threshold f1."""
+        N = self.space.n_params
+        pmin = np.zeros((n_groups, N), np.float32)
+        pmax = np.zeros((n_groups, N), np.float32)
+        nt, nb = C.c_int64(), C.c_int64()
+        f0, f1 = C.c_double(), C.c_double()
+        cfg = abi.RmsdCfg(int(n_groups), int(group_size), int(n_samples), float(dx), int(spherical_k))
+        pf = C.POINTER(C.c_float)
+        _chk(self.lib.ast_rmsd(self.h, C.byref(cfg), pmin.ctypes.data_as(pf), pmax.ctypes.data_as(pf), C.byref(nt),
+                               C.byref(nb), C.byref(f0), C.byref(f1)), "ast_rmsd")
+        return dict(par_min=pmin, par_max=pmax, n_tot=nt.value, n_bad=nb.value, f0=f0.value, f1=f1.value)
+
     def last_timing(self):
         ms = C.c_double()
         n = C.c_int()

@@ -582,6 +582,16 @@ struct Serve {
 
 }  // namespace
 
+// ------------------------------------------------------------------- RMSD
+struct RmsdOut {
+  float* par_min;
+  float* par_max;
+  int64_t n_tot, n_bad;
+  double f0, f1;
+  float ms;
+};
+
+
 // ------------------------------------------------------------- periodogram
 // The periodogram of the minima of a model light curve (minima(),
 // misc.c:65-276, the default branch: minima at grid times, step-function
@@ -1336,6 +1346,86 @@ void serve_loop(ast_ctx* c) {
 
 }  // namespace
 
+// One RMSD launch pair: the input model, then the shifted models.
+template <class CF>
+int rmsd_run(ast_ctx* c, const ast_rmsd_cfg& cfg, RmsdOut* out) {
+  const int N = c->space.n_params, G = cfg.group_size, ng = cfg.n_groups;
+  const size_t lanes = (size_t)ng * G;
+  RETIF(ensure_streams(c, lanes, (size_t)ng));
+  const Space<CF> sp_h = space_table<CF>(c->space);
+  Space<CF>* sp = nullptr;
+  RmsdRef<CF>* ref = nullptr;
+  double* p0 = nullptr;
+  float *pmin = nullptr, *pmax = nullptr;
+  int32_t* cnt = nullptr;
+  int rc = dev_alloc(&sp, 1);
+  if (!rc) rc = dev_alloc(&ref, 1);
+  if (!rc) rc = dev_alloc(&p0, (size_t)N);
+  if (!rc) rc = dev_alloc(&pmin, (size_t)ng * N);
+  if (!rc) rc = dev_alloc(&pmax, (size_t)ng * N);
+  if (!rc) rc = dev_alloc(&cnt, (size_t)ng * 2);
+  auto release = [&]() { dev_free(sp); dev_free(ref); dev_free(p0); dev_free(pmin); dev_free(pmax); dev_free(cnt); };
+  if (rc) { release(); return rc; }
+  RmsdArgs<CF> A{};
+  A.pts = c->pts;
+  A.sum_w = c->sum_w;
+  A.n_data = c->curve.n_data;
+  A.n_filters = c->curve.n_filters;
+  A.dof = c->curve.n_data - N - c->curve.n_filters;
+  A.sp = sp;
+  A.reopt = c->space.x2.reopt;
+  A.params0 = p0;
+  A.ref = ref;
+  A.rng = c->rng;
+  A.seed = c->seed;
+  A.n_samples = cfg.n_samples;
+  A.dx = cfg.dx;
+  A.spherical_k = cfg.spherical_k;
+  A.par_min = pmin;
+  A.par_max = pmax;
+  A.counts = cnt;
+  hipStream_t s = c->streams[0];
+  hipError_t he = hipMemcpy(sp, &sp_h, sizeof sp_h, hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemcpy(p0, c->params0, sizeof(double) * N, hipMemcpyHostToDevice);
+  LaunchArgs a{};
+  a.stream = s;
+  a.n_filters = c->curve.n_filters;
+  a.n_groups = ng;
+  a.group_size = G;
+  a.rmsd = &A;
+  a.op = 4;
+  if (he == hipSuccess) he = dispatch(c->v, a);
+  if (he == hipSuccess) he = hipEventRecord(c->ev0[0], s);
+  a.op = 5;
+  if (he == hipSuccess) he = dispatch(c->v, a);
+  if (he == hipSuccess) he = hipEventRecord(c->ev1[0], s);
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  float ms = 0;
+  if (he == hipSuccess) he = hipEventElapsedTime(&ms, c->ev0[0], c->ev1[0]);
+  std::vector<int32_t> ch((size_t)ng * 2);
+  RmsdRef<CF> rh;
+  if (he == hipSuccess) he = hipMemcpy(out->par_min, pmin, sizeof(float) * ng * N, hipMemcpyDeviceToHost);
+  if (he == hipSuccess) he = hipMemcpy(out->par_max, pmax, sizeof(float) * ng * N, hipMemcpyDeviceToHost);
+  if (he == hipSuccess) he = hipMemcpy(ch.data(), cnt, sizeof(int32_t) * ch.size(), hipMemcpyDeviceToHost);
+  if (he == hipSuccess) he = hipMemcpy(&rh, ref, sizeof rh, hipMemcpyDeviceToHost);
+  if (he != hipSuccess) {
+    (void)hipDeviceSynchronize();
+    snprintf(g_hip_err, sizeof g_hip_err, "%s (rmsd)", hipGetErrorString(he));
+    release();
+    return AST_E_HIP;
+  }
+  out->n_tot = out->n_bad = 0;
+  for (int g = 0; g < ng; ++g) {
+    out->n_tot += ch[(size_t)g * 2];
+    out->n_bad += ch[(size_t)g * 2 + 1];
+  }
+  out->f0 = rh.f0;
+  out->f1 = rh.f1;
+  out->ms = ms;
+  release();
+  return AST_OK;
+}
+
 extern "C" {
 
 const char* ast_strerror(int code) {
@@ -1621,6 +1711,30 @@ int ast_chi2_batch(ast_ctx* c, const double* params, int n, double* chi2, double
   dev_free(cd);
   dev_free(dd);
   return result;
+}
+
+int ast_rmsd(ast_ctx* c, const ast_rmsd_cfg* cfg, float* par_min, float* par_max, int64_t* n_tot, int64_t* n_bad,
+             double* f0, double* f1) {
+  if (!c || !cfg || !par_min || !par_max) return AST_E_ARG;
+  if (cfg->n_groups <= 0 || cfg->group_size <= 0 || cfg->group_size > 1024 || cfg->n_samples < 0 ||
+      !(cfg->dx >= 0))
+    return AST_E_ARG;
+  if (c->serve) return AST_E_BUSY;
+  if (!c->space_set || !c->curve_set) return AST_E_STATE;
+  if (c->curve.n_data - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
+  RETIF(set_device(c));
+  RmsdOut o{par_min, par_max, 0, 0, 0, 0, 0};
+  const int rc = c->v.precision == AST_PREC_FP64 ? rmsd_run<double>(c, *cfg, &o) : rmsd_run<float>(c, *cfg, &o);
+  if (rc) return rc;
+  if (n_tot) *n_tot = o.n_tot;
+  if (n_bad) *n_bad = o.n_bad;
+  if (f0) *f0 = o.f0;
+  if (f1) *f1 = o.f1;
+  c->last_ms = o.ms;
+  c->last_launches = 1;
+  c->last_evals = o.n_tot;
+  c->last_points = o.n_tot * c->curve.n_data;
+  return AST_OK;
 }
 
 int ast_model_curve(ast_ctx* c, const double* params, int n, const ast_ephem3* eph, int n_plot, double* t,

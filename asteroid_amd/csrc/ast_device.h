@@ -424,10 +424,14 @@ constexpr float MX_BACKLIT_COS = -0.8f;
 // (the mixed-precision form) the Euler angles and angular velocity are
 // integrated exactly as in the FP64 form, and each observation's geometry
 // and brightness run in FP32; the chi^2 sums stay FP64.
-// With PLOT (chi2one called with Nplot > 0, cuda.c:699-703, 884-885) the
-// points are a plotting grid: each point's Vmod + dv0 goes to
-// vplot[i * vstride], nothing is summed, and the result is 0.
-template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX = false, bool PLOT = false>
+// FORM selects what the model is scored by: FORM_CHI2, chi^2 per degree of
+// freedom (cuda.c:907-908); FORM_PLOT (chi2one called with Nplot > 0,
+// cuda.c:699-703, 884-885), where the points are a plotting grid, each
+// point's Vmod + dv0 goes to vplot[i * vstride], nothing is summed and the
+// result is 0; FORM_RMSD (the RMSD build, cuda.c:890-906), the root of the
+// summed filter chi^2 over the summed weights.
+enum { FORM_CHI2 = 0, FORM_PLOT = 1, FORM_RMSD = 2 };
+template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX = false, int FORM = FORM_CHI2>
 __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_filters, int dof,
                          const double* sum_w, CF* dV, double* vplot = nullptr, size_t vstride = 0,
                          double dv0 = 0.0) {
@@ -550,7 +554,7 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
         Vmod = ls_tail_f64(u, K.bcp);
       }
     }
-    if constexpr (PLOT) {
+    if constexpr (FORM == FORM_PLOT) {
       vplot[(size_t)i * vstride] = Vmod + dv0;
       continue;
     }
@@ -569,17 +573,25 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
         }
     }
   }
-  if constexpr (PLOT) return CF(0.0);
+  if constexpr (FORM == FORM_PLOT) return CF(0.0);
   CF chi2a = 0.0;  // cuda.c:884-910
+  CF sum_all = 0.0;
 #pragma unroll
   for (int m = 0; m < NF; ++m) {
     if (m < n_filters) {
       CF chi2m = sy2[m] - sy[m] * sy[m] / sum_w[m];
       chi2a = chi2a + chi2m;
       dV[m] = sy[m] / sum_w[m];
+      if constexpr (FORM == FORM_RMSD) sum_all = sum_all + sum_w[m];
     }
   }
-  chi2a = chi2a / dof;
+  if constexpr (FORM == FORM_RMSD) {
+    // the root is taken in double: rounded to float it is the correctly
+    // rounded float root
+    chi2a = (CF)__builtin_sqrt((double)(chi2a / sum_all));
+  } else {
+    chi2a = chi2a / dof;
+  }
   return chi2a;
 }
 

@@ -651,8 +651,161 @@ __global__ __launch_bounds__(256) void model_curve_kernel(const PointRec* pts_g,
   CF c = chi2_model<CF, TQ, BC, BW, NF, MX>(prm, (PointPtr)(uintptr_t)pts_g, n_data, n_filters, dof, sum_w, dv);
   chi2[k] = c;
   for (int m = 0; m < n_filters; ++m) dV[(size_t)k * n_filters + m] = dv[m];
-  chi2_model<CF, TQ, BC, BW, NF, MX, true>(prm, (PointPtr)(uintptr_t)grid_g, n_plot, n_filters, dof, sum_w, dv,
-                                          vmod + (size_t)k * n_plot, 1, (double)dv[0]);
+  chi2_model<CF, TQ, BC, BW, NF, MX, FORM_PLOT>(prm, (PointPtr)(uintptr_t)grid_g, n_plot, n_filters, dof, sum_w,
+                                               dv, vmod + (size_t)k * n_plot, 1, (double)dv[0]);
+}
+
+// ------------------------------------------------------------------- RMSD
+// Confidence intervals by random shifts around an input model
+// (chi2_gpu_rms, cuda.c:2148-2389, the RMSD build).  RmsdRef holds what
+// every thread of the reference computes first (cuda.c:2217-2247): the
+// input model in scale-free coordinates, its RMSD f0 and the acceptance
+// threshold f1 = f0 + f0/sqrt(N_data - N_PARAMS - N_filters).
+template <class CF>
+struct RmsdRef {
+  CF x0[MAXP];
+  CF f0, f1;
+  double phi_M0, phi_00, phi_K0;  // the input model's phi_M, phi_0 and torque azimuth
+};
+
+template <class CF>
+struct RmsdArgs {
+  const PointRec* pts;
+  const double* sum_w;
+  int n_data, n_filters, dof;
+  const Space<CF>* sp;
+  int reopt;
+  const double* params0;  // d_params0
+  RmsdRef<CF>* ref;
+  uint32_t* rng;          // per-lane stream counters (the curandState array)
+  uint64_t seed;
+  int n_samples;          // Nstages: random points per lane
+  float dx;               // dx_rand
+  int spherical_k;        // SPHERICAL_K
+  float* par_min;         // [group][N_PARAMS] (dpar_min)
+  float* par_max;
+  int32_t* counts;        // [group][2]: Ntot, Nbad
+};
+
+template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX>
+__global__ __launch_bounds__(64) void rmsd_ref_kernel(RmsdArgs<CF> A) {
+  using P = Pos<TQ, BC, BW>;
+  constexpr int N = P::N;
+  if (threadIdx.x != 0) return;
+  double prm[N];
+  for (int i = 0; i < N; ++i) prm[i] = A.params0[i];
+  CF x[N];
+  params2x<CF, TQ, BC, BW>(x, prm, *A.sp);
+  CF dv[NF];
+  const CF f0 = chi2_model<CF, TQ, BC, BW, NF, MX, FORM_RMSD>(prm, (PointPtr)(uintptr_t)A.pts, A.n_data,
+                                                              A.n_filters, A.dof, A.sum_w, dv);
+  RmsdRef<CF>& R = *A.ref;
+  for (int i = 0; i < N; ++i) R.x0[i] = x[i];
+  R.f0 = f0;
+  R.f1 = f0 + f0 / (CF)__builtin_sqrt((double)(CF)A.dof);
+  R.phi_M0 = prm[P::phi_M];
+  R.phi_00 = prm[P::phi_0];
+  R.phi_K0 = 0.0;
+  if (TQ) {
+    const double bt = prm[P::b_tumb], ct = prm[P::c_tumb];
+    const double Is = (1.0 + bt * bt) / (bt * bt + ct * ct);
+    const double Ii = (1.0 + ct * ct) / (bt * bt + ct * ct);
+    R.phi_K0 = astm::atan2(Is * prm[P::Ts], Ii * prm[P::Ti]);
+  }
+}
+
+// One lane per thread of the reference: n_samples shifted models each,
+// the range of the accepted ones per parameter, reduced per group in lane
+// order by the group's first thread (cuda.c:2356-2381).
+template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX, int W>
+__global__ __launch_bounds__(W == 4 ? 1024 : 256) __attribute__((amdgpu_waves_per_eu(W, W))) void rmsd_kernel(
+    RmsdArgs<CF> A) {
+  using P = Pos<TQ, BC, BW>;
+  constexpr int N = P::N;
+  constexpr double PI = 3.141592653589793238;
+  constexpr double SMALL = 1e-8;  // cuda.c:2258
+  __shared__ float s_min[1024], s_max[1024];
+  __shared__ int32_t s_cnt[2][1024];
+  const int t = threadIdx.x;
+  const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + t;
+  const Space<CF>& sp = *A.sp;
+  const RmsdRef<CF>& R = *A.ref;
+  CF pmin[N], pmax[N];
+  for (int i = 0; i < N; ++i) { pmin[i] = (CF)1e30; pmax[i] = (CF)-1e30; }
+  int32_t ntot = 0, nbad = 0;
+  uint32_t ctr = A.rng[id];
+  double prm[N];
+  for (int s = 0; s < A.n_samples; ++s) {
+    int LAM = 0;
+    CF x1[N];
+    for (int i = 0; i < N; ++i) {  // cuda.c:2276-2287
+      const float z = astrng::normal01(A.seed, id, ctr);
+      ctr += 2;
+      CF xx = R.x0[i] + (CF)(A.dx * z);
+      const int per = sp.periodic[i];
+      if (xx < SMALL && hard_left(per, LAM)) xx = (CF)SMALL;
+      else if (xx > 1.0 - SMALL && hard_right(per, LAM)) xx = (CF)(1.0 - SMALL);
+      x1[i] = xx;
+      if (i == P::Es) LAM = x1[i] >= (CF)0.5;
+    }
+    if (x2params<CF, TQ, BC, BW>(x1, prm, sp, A.reopt)) continue;
+    CF dv[NF];
+    const CF f = chi2_model<CF, TQ, BC, BW, NF, MX, FORM_RMSD>(prm, (PointPtr)(uintptr_t)A.pts, A.n_data,
+                                                               A.n_filters, A.dof, A.sum_w, dv);
+    ntot++;
+    if (f < R.f1) {
+      // cuda.c:2304-2311, as written: an angle beyond +-pi of the input's is
+      // set to the input's value -2 pi, which the second test then moves to
+      // the input's value +2 pi
+      if (prm[P::phi_M] > R.phi_M0 + PI) prm[P::phi_M] = R.phi_M0 - 2 * PI;
+      if (prm[P::phi_M] < R.phi_M0 - PI) prm[P::phi_M] = R.phi_M0 + 2 * PI;
+      if (prm[P::phi_0] > R.phi_00 + PI) prm[P::phi_0] = R.phi_00 - 2 * PI;
+      if (prm[P::phi_0] < R.phi_00 - PI) prm[P::phi_0] = R.phi_00 + 2 * PI;
+      if (TQ && A.spherical_k) {  // cuda.c:2312-2333
+        const double bt = prm[P::b_tumb], ct = prm[P::c_tumb];
+        const double Is = (1.0 + bt * bt) / (bt * bt + ct * ct);
+        const double Ii = (1.0 + ct * ct) / (bt * bt + ct * ct);
+        const double Ki = Ii * prm[P::Ti], Ks = Is * prm[P::Ts], Kl = prm[P::Tl];
+        const double K = __builtin_sqrt(Ki * Ki + Ks * Ks + Kl * Kl);
+        prm[P::Ti] = K;
+        prm[P::Ts] = astm::acos(Kl / K);
+        double phi = astm::atan2(Ks, Ki);
+        if (phi > R.phi_K0 + PI) phi = phi - 2 * PI;
+        if (phi < R.phi_K0 - PI) phi = phi + 2 * PI;
+        prm[P::Tl] = phi;
+      }
+      for (int i = 0; i < N; ++i) {
+        if (prm[i] < pmin[i]) pmin[i] = (CF)prm[i];
+        if (prm[i] > pmax[i]) pmax[i] = (CF)prm[i];
+      }
+    } else {
+      nbad++;
+    }
+  }
+  A.rng[id] = ctr;
+  s_cnt[0][t] = ntot;
+  s_cnt[1][t] = nbad;
+  for (int i = 0; i < N; ++i) {
+    s_min[t] = (float)pmin[i];
+    s_max[t] = (float)pmax[i];
+    __syncthreads();
+    if (t == 0) {
+      float fmin = 1e30f, fmax = -1e30f;
+      for (int j = 0; j < (int)blockDim.x; ++j) {
+        if (s_min[j] < fmin) fmin = s_min[j];
+        if (s_max[j] > fmax) fmax = s_max[j];
+      }
+      A.par_min[(size_t)blockIdx.x * N + i] = fmin;
+      A.par_max[(size_t)blockIdx.x * N + i] = fmax;
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    int32_t a = 0, b = 0;
+    for (int j = 0; j < (int)blockDim.x; ++j) { a += s_cnt[0][j]; b += s_cnt[1][j]; }
+    A.counts[(size_t)blockIdx.x * 2] = a;
+    A.counts[(size_t)blockIdx.x * 2 + 1] = b;
+  }
 }
 
 // --------------------------------------------------------- launch interface
@@ -677,6 +830,8 @@ struct LaunchArgs {
   const PointRec* grid;
   int n_plot;
   double* vmod;
+  // RMSD: RmsdArgs<CF> (op 4: the input model, op 5: the shifted models)
+  const void* rmsd;
 };
 
 template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX>
@@ -701,11 +856,20 @@ hipError_t launch_nf(const LaunchArgs& a) {
     int blocks = (a.n + 255) / 256;
     hipLaunchKernelGGL((chi2_batch_kernel<CF, TQ, BC, BW, NF, MX>), dim3(blocks), dim3(256), 0, a.stream, a.pts,
                        a.sum_w, a.n_data, a.n_filters, a.dof, a.params, a.n, a.chi2, a.dV);
-  } else {
+  } else if (a.op == 3) {
     int blocks = (a.n + 255) / 256;
     hipLaunchKernelGGL((model_curve_kernel<CF, TQ, BC, BW, NF, MX>), dim3(blocks), dim3(256), 0, a.stream, a.pts,
                        a.sum_w, a.n_data, a.n_filters, a.dof, a.grid, a.n_plot, a.params, a.n, a.chi2, a.dV,
                        a.vmod);
+  } else if (a.op == 4) {
+    hipLaunchKernelGGL((rmsd_ref_kernel<CF, TQ, BC, BW, NF, MX>), dim3(1), dim3(64), 0, a.stream,
+                       *(const RmsdArgs<CF>*)a.rmsd);
+  } else if (a.group_size > 256) {  // groups of up to 1024: 128 registers
+    hipLaunchKernelGGL((rmsd_kernel<CF, TQ, BC, BW, NF, MX, 4>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
+                       *(const RmsdArgs<CF>*)a.rmsd);
+  } else {
+    hipLaunchKernelGGL((rmsd_kernel<CF, TQ, BC, BW, NF, MX, 2>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
+                       *(const RmsdArgs<CF>*)a.rmsd);
   }
   return hipGetLastError();
 }

@@ -269,6 +269,47 @@ int plot_model(ast_ctx* ctx, const ast_variant& v, const Ingested& D, const std:
   return 0;
 }
 
+// The RMSD build's main loop (asteroid.c:643-725): one confidence-interval
+// launch per cycle; the cumulative rejected fraction and the cumulative
+// range of every parameter go to stdout, the ranges to the results file.
+int rmsd_loop(ast_ctx* ctx, int n_blocks, int bsize, int Nstages, float dx_rand, int spherical_k, int n_params,
+              int Ncases, const char* results) {
+  std::vector<float> PAR_min(n_params, 1e30f), PAR_max(n_params, -1e30f);
+  std::vector<float> hmin((size_t)n_blocks * n_params), hmax((size_t)n_blocks * n_params);
+  printf("\n");
+  FILE* fp = fopen(results, "w");
+  if (!fp) die("Cannot open the output file!");
+  ast_rmsd_cfg cfg{n_blocks, bsize, Nstages, dx_rand, spherical_k};
+  int64_t Ntot = 0, Nbad = 0;  // d_Ntot, d_Nbad: never reset
+  int loop_counter = 0;
+  while (true) {
+    loop_counter++;
+    int64_t nt = 0, nb = 0;
+    double f0 = 0, f1 = 0;
+    CHECK(ast_rmsd(ctx, &cfg, hmin.data(), hmax.data(), &nt, &nb, &f0, &f1));
+    Ntot += nt;
+    Nbad += nb;
+    if (loop_counter == 1) printf("f0=%f, f1=%f\n\n", (float)f0, (float)f1);
+    printf("%e ", Nbad / (double)Ntot);
+    for (int j = 0; j < n_params; ++j) {
+      for (int i = 0; i < n_blocks; ++i) {
+        const size_t k = (size_t)i * n_params + j;
+        if (hmin[k] < PAR_min[j]) PAR_min[j] = hmin[k];
+        if (hmax[k] > PAR_max[j]) PAR_max[j] = hmax[k];
+      }
+      printf("%13.6e %13.6e ; ", PAR_min[j], PAR_max[j]);
+      fprintf(fp, "%13.6e %13.6e ; ", PAR_min[j], PAR_max[j]);
+    }
+    printf("\n");
+    fflush(stdout);
+    fprintf(fp, "\n");
+    fflush(fp);
+    if (loop_counter == Ncases) break;
+  }
+  fclose(fp);
+  return 0;
+}
+
 void usage() {
   printf("\n Command line arguments:\n\n");
   printf("-best : only keep the best result\n");
@@ -285,7 +326,9 @@ void usage() {
   printf("-reopt : reoptimize the model provided with -m switch\n");
   printf("-seed SEED : use the SEED number to initialize the random number generator\n");
   printf("-t : travelling reoptimization\n");
-  printf("-D<MACRO> : model variant (TORQUE, P_PSI, BC, BW_BALL, INTERP, ACC, MIXED); default -DP_PSI -DTORQUE -DINTERP\n");
+  printf("-dx value : maximum shift for parameters in scale-free units (0...1), with -DRMSD\n");
+  printf("-D<MACRO> : model variant (TORQUE, P_PSI, BC, BW_BALL, INTERP, ACC, MIXED) and mode (RMSD, SPHERICAL_K);\n"
+         "            default -DP_PSI -DTORQUE -DINTERP\n");
   printf("-blocks n, -bsize n : reseed groups per launch and simplexes per group (N_BLOCKS, BSIZE)\n");
   printf("-steps n : simplex steps per stage (N_STEPS); -budget n : chi^2 evaluations per stage\n");
   printf("-time_step d : ODE time step in days (TIME_STEP); -device n : HIP device\n");
@@ -305,7 +348,8 @@ int main(int argc, char** argv) {
   int acc = 0;
   v.time_step = 1e-2;
   int j_input = -1, j_results = -1;
-  int travel = 0, keep = 0, best = 0, reopt = 0, model = 0, plot = 0;
+  int travel = 0, keep = 0, best = 0, reopt = 0, model = 0, plot = 0, rmsd = 0, spherical_k = 0;
+  float dx_rand = 0.0f;
   unsigned long seed = 0;
   int Ncases = -1, Nstages = 1;
   int n_blocks = 1024, bsize = 256, n_steps = 5000, device = 0;
@@ -331,6 +375,8 @@ int main(int argc, char** argv) {
       else if (mac == "INTERP") v.interp = 1;
       else if (mac == "ACC") acc = 1;
       else if (mac == "MIXED") acc = 2;
+      else if (mac == "RMSD") rmsd = 1;
+      else if (mac == "SPHERICAL_K") spherical_k = 1;
       else {
         printf("Unsupported macro -D%s\n", mac.c_str());
         return 1;
@@ -370,9 +416,15 @@ int main(int argc, char** argv) {
     else if (!strcmp(a, "-time_step")) { need(1); v.time_step = atof(argv[j + 1]); j += 2; }
     else if (!strcmp(a, "-device")) { need(1); device = atoi(argv[j + 1]); j += 2; }
     else if (!strcmp(a, "-dump_ingest")) { need(1); dump = argv[j + 1]; j += 2; }
+    else if (!strcmp(a, "-dx")) { need(1); dx_rand = (float)atof(argv[j + 1]); j += 2; }
     else die("Bad arguments!");
   }
-  if (!any_macro) { v.torque = 1; v.p_psi = 1; v.interp = 1; }
+  // The default variant; -DRMSD and -DSPHERICAL_K alone keep it.
+  if (!any_macro || !(v.torque || v.p_psi || v.bc || v.bw_ball || v.interp || acc)) {
+    v.torque = 1;
+    v.p_psi = 1;
+    v.interp = 1;
+  }
   v.precision = acc == 2 ? AST_PREC_MIXED : (acc ? AST_PREC_FP64 : AST_PREC_F32SIMPLEX);
   if (j_input == -1) die("-i parameter is missing!");
   if ((reopt || plot) && !model) die("-reopt and -plot switches require -m switch!");
@@ -448,6 +500,12 @@ int main(int argc, char** argv) {
   if (j_results == -1) die("Bad output file name!");
   CHECK(ast_rng_seed(ctx, seed == 0 ? (uint64_t)time(nullptr) : (uint64_t)seed));
   if (reopt) CHECK(ast_set_start(ctx, params.data()));
+  if (rmsd) {
+    int rc_rmsd = rmsd_loop(ctx, n_blocks, bsize, Nstages, dx_rand, spherical_k, n_params, Ncases,
+                            argv[j_results]);
+    CHECK(ast_ctx_destroy(ctx));
+    return rc_rmsd;
+  }
   ast_run_cfg cfg{n_blocks, bsize, Nstages, n_steps, budget, 0.0, 0};
   std::vector<double> h_f(n_blocks), h_params((size_t)n_blocks * n_params), h_dV((size_t)n_blocks * N_filters);
   int loop_counter = 0;

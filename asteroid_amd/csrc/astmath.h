@@ -197,6 +197,11 @@ template <class R> AST_HD R asin(R x) {
   return atan2(x, p_sqrt((R(1.0) - x) * (R(1.0) + x)));
 }
 
+// acos(x) = atan2(sqrt((1-x)(1+x)), x); NaN for |x| > 1.
+template <class R> AST_HD R acos(R x) {
+  return atan2(p_sqrt((R(1.0) - x) * (R(1.0) + x)), x);
+}
+
 // Natural logarithm: -inf at 0, NaN below 0 or for NaN, +inf at +inf.
 template <class R> AST_HD R log(R x) {
   double xv = p_val(x);

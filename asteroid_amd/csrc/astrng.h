@@ -15,6 +15,7 @@
 // ((u >> 8) + 1) * 2^-24, exact in single precision.
 #pragma once
 #include <stdint.h>
+#include "astmath.h"
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
@@ -52,6 +53,18 @@ AST_RNG_HD uint32_t draw_u32(uint64_t seed, uint64_t fit, uint32_t k) {
 
 AST_RNG_HD float uniform01(uint64_t seed, uint64_t fit, uint32_t k) {
   return (float)((draw_u32(seed, fit, k) >> 8) + 1u) * (1.0f / 16777216.0f);
+}
+
+// A standard normal deviate (the role of curand_normal, cuda.c:2276) from
+// draws k and k+1 by the Box-Muller transform (Box & Muller, Ann. Math.
+// Statist. 29, 1958): sqrt(-2 ln u1) cos(2 pi u2), evaluated in double with
+// the project's own log and cos and rounded to float.
+AST_RNG_HD float normal01(uint64_t seed, uint64_t fit, uint32_t k) {
+  const double u1 = (double)uniform01(seed, fit, k);
+  const double u2 = (double)uniform01(seed, fit, k + 1);
+  const double two_pi = 6.283185307179586232;
+  const double r = astm::p_sqrt(-2.0 * astm::log(u1));
+  return (float)(r * astm::cos(two_pi * u2));
 }
 
 }  // namespace astrng

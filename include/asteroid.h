@@ -240,6 +240,31 @@ int ast_model_curve(ast_ctx* ctx, const double* params, int n, const ast_ephem3*
 int ast_periodogram(ast_ctx* ctx, const double* t, const double* vm, int n, int n_clusters,
                     double* cl_fr, double* cl_H, int* n_minima);
 
+/* One launch of the RMSD confidence-interval kernel (chi2_gpu_rms,
+ * cuda.c:2148-2389, the -DRMSD build; launched at asteroid.c:673). */
+typedef struct {
+  int n_groups;      /* N_BLOCKS                                                  */
+  int group_size;    /* BSIZE, at most 1024                                       */
+  int n_samples;     /* -Nstages: random models per simplex lane (cuda.c:2255)    */
+  float dx;          /* -dx: standard deviation of the shift of every scale-free
+                        coordinate (dx_rand, asteroid.c:271-277)                 */
+  int spherical_k;   /* -DSPHERICAL_K: torque ranges as K, theta_K, phi_K        */
+} ast_rmsd_cfg;
+
+/* Around the start model (ast_set_start, the d_params0 role) every lane
+ * draws n_samples models: each scale-free coordinate shifted by dx times a
+ * standard normal deviate (from the lane's random stream, which persists
+ * across calls as the curandState array does), clamped at hard limits, and
+ * mapped with the model space's reopt.  A model whose RMSD (the RMSD build's
+ * merit, cuda.c:890-906) is below f1 = f0 + f0/sqrt(N_data - N_PARAMS -
+ * N_filters), f0 the start model's RMSD, widens its lane's per-parameter
+ * range; each group's ranges come back as float par_min, par_max
+ * (n_groups*n_params: dpar_min, dpar_max, asteroid.c:692-693).  n_tot and
+ * n_bad (may be NULL) receive this call's number of models evaluated and
+ * rejected (d_Ntot, d_Nbad), f0 and f1 the two RMSD values (d_f0, d_f1). */
+int ast_rmsd(ast_ctx* ctx, const ast_rmsd_cfg* cfg, float* par_min, float* par_max, int64_t* n_tot,
+             int64_t* n_bad, double* f0, double* f1);
+
 /* Timing of the last ast_run / ast_chi2_batch, from HIP events on the
  * context's stream: total kernel milliseconds and the number of launches. */
 int ast_last_timing(ast_ctx* ctx, double* kernel_ms, int* launches);

@@ -54,9 +54,12 @@ def lib():
                               C.c_uint64, C.c_int, pD, pU32, pD, pD, pD, pD, pI64, pI64, C.c_int]
         L.orc_model_curve.argtypes = [pP, C.c_int, pD, C.c_int, C.c_int, pD, pD, pD, pD]
         L.orc_periodogram.argtypes = [pD, pD, C.c_int, C.c_int, pD, pD, pI, pD]
+        L.orc_rmsd.argtypes = [pP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_uint64, C.c_int, pD,
+                               pU32, C.POINTER(C.c_float), C.POINTER(C.c_float), pI64, pD, C.c_int]
         L.orc_math.argtypes = [C.c_int, pD, pD, pD, C.c_int]
         L.orc_philox.argtypes = [pU32, pU32, pU32]
         L.orc_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int, C.POINTER(C.c_float)]
+        L.orc_normal.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int, C.POINTER(C.c_float)]
         _lib = L
     return _lib
 
@@ -119,6 +122,23 @@ class Problem:
         assert rc == 0, rc
         return t, vm, chi2, dV
 
+    def rmsd(self, n_groups, group_size, n_samples, dx, params0, seed=1, reopt=1, spherical_k=False, rng=None,
+             form=CANON, n_threads=1):
+        """chi2_gpu_rms: per-group float (par_min, par_max), counts (Ntot, Nbad per
+        group), f0, f1; rng (per-lane stream counters) is advanced in place."""
+        N = self.space.n_params
+        rng = np.zeros(n_groups * group_size, np.uint32) if rng is None else rng
+        pmin = np.zeros((n_groups, N), np.float32)
+        pmax = np.zeros((n_groups, N), np.float32)
+        counts = np.zeros((n_groups, 2), np.int64)
+        f = np.zeros(2)
+        p0 = np.ascontiguousarray(params0, np.float64)
+        rc = lib().orc_rmsd(C.byref(self.pb), form, n_groups, group_size, n_samples, float(dx), int(spherical_k),
+                            seed, reopt, _p(p0), _p(rng, C.c_uint32), _p(pmin, C.c_float), _p(pmax, C.c_float),
+                            _p(counts, C.c_int64), _p(f), n_threads)
+        assert rc == 0, rc
+        return dict(par_min=pmin, par_max=pmax, counts=counts, f0=f[0], f1=f[1], rng=rng)
+
     def x2params(self, x, reopt=0, form=CANON):
         x = np.ascontiguousarray(np.atleast_2d(x), np.float64)
         n = x.shape[0]
@@ -172,7 +192,7 @@ def periodogram(t, vm, n_clusters=5):
 
 
 MATH = {"sin": 0, "cos": 1, "tan": 2, "atan2": 3, "log": 4, "log10": 5, "exp": 6, "asin": 7,
-        "atan": 8, "expf": 9, "sin_mx": 10, "cos_mx": 11}
+        "atan": 8, "expf": 9, "sin_mx": 10, "cos_mx": 11, "acos": 12}
 
 
 def math(fn, x, y=None):
@@ -194,4 +214,10 @@ def philox(ctr, key):
 def uniform(seed, fit, k0, n):
     out = np.zeros(n, np.float32)
     lib().orc_uniform(seed, fit, k0, n, out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+def normal(seed, fit, k0, n):
+    out = np.zeros(n, np.float32)
+    lib().orc_normal(seed, fit, k0, n, out.ctypes.data_as(C.POINTER(C.c_float)))
     return out

@@ -71,6 +71,7 @@ struct LibmMath {
   static double asin(double x) { return std::asin(x); }
   static double atan(double x) { return std::atan(x); }
   static double atan2(double y, double x) { return std::atan2(y, x); }
+  static double acos(double x) { return std::acos(x); }
   static double log(double x) { return std::log(x); }
   static double log10(double x) { return std::log10(x); }
   static double exp(double x) { return std::exp(x); }
@@ -84,6 +85,7 @@ struct AstMath {
   static double asin(double x) { return astm::asin(x); }
   static double atan(double x) { return astm::atan(x); }
   static double atan2(double y, double x) { return astm::atan2(y, x); }
+  static double acos(double x) { return astm::acos(x); }
   static double log(double x) { return astm::log(x); }
   static double log10(double x) { return astm::log10(x); }
   static double exp(double x) { return astm::exp(x); }
@@ -891,6 +893,138 @@ void run_group(const Problem& P, int form, const RunCfg& C, int reopt_in, const 
   }
 }
 
+// ==================================================================== RMSD
+// The RMSD build's merit function (cuda.c:890-906): the root of the summed
+// filter chi^2 over the summed weights, in CHI_FLOAT.
+template <class CF>
+inline CF rmsd_from_sums(const Problem& P, const double* sy2, const double* sy, const double* sw, CF* dV) {
+  CF chi2a = 0.0;
+  CF SUM_w = 0.0;
+  for (int m = 0; m < P.n_filters; ++m) {
+    CF chi2m = sy2[m] - sy[m] * sy[m] / sw[m];
+    chi2a = chi2a + chi2m;
+    dV[m] = sy[m] / sw[m];
+    SUM_w = SUM_w + sw[m];
+  }
+  return std::sqrt(chi2a / SUM_w);
+}
+
+template <class CF>
+inline CF rmsd_eval(const Problem& P, int form, const double* params, CF* dV) {
+  double sy2[AST_MAX_FILTERS], sy[AST_MAX_FILTERS], sw[AST_MAX_FILTERS];
+  if (form == 0) model_literal(P, params, sy2, sy, sw, nullptr);
+  else model_canon<double>(P, params, sy2, sy, sw, nullptr);
+  return rmsd_from_sums<CF>(P, sy2, sy, sw, dV);
+}
+
+struct RmsdCfg {
+  int group_size, n_samples;
+  float dx;
+  int spherical_k, reopt;
+  uint64_t seed;
+};
+
+// One block of chi2_gpu_rms (cuda.c:2148-2389): each thread draws
+// n_samples models around params0 (a normal shift of dx per scale-free
+// coordinate, clamped at hard limits), keeps the range of the parameters of
+// those within f1 of the input's RMSD, and the block reduces the ranges
+// serially in float.  counts[2g], counts[2g+1]: Ntot, Nbad of the block.
+template <class CF, class M>
+void rmsd_group(const Problem& P, int form, const RmsdCfg& C, const double* params0, int g, uint32_t* rng,
+                float* par_min, float* par_max, int64_t* counts, double* f0f1) {
+  const ast_type_ids& T = P.T;
+  const int N = P.n_params, G = C.group_size;
+  const double SMALL = 1e-8;  // cuda.c:2258
+  double prm[AST_MAX_PARAMS];
+  for (int i = 0; i < N; ++i) prm[i] = params0[i];
+  CF x0[AST_MAX_PARAMS];
+  params2x<CF, M>(P, x0, prm);
+  CF dV[AST_MAX_FILTERS];
+  const CF f0 = rmsd_eval<CF>(P, form, prm, dV);
+  const CF f1 = f0 + f0 / std::sqrt((CF)(P.n_data - N - P.n_filters));
+  if (f0f1) { f0f1[0] = f0; f0f1[1] = f1; }
+  const int iphiM = P.types[T.phi_M], iphi0 = P.types[T.phi_0];
+  const double phi_M0 = prm[iphiM], phi_00 = prm[iphi0];
+  double phi0 = 0.0;
+  const bool sph = C.spherical_k && P.v.torque;
+  auto moments = [&](const double* p, double* Is, double* Ii) {
+    const double bt = p[P.types[T.b_tumb]], ct = p[P.types[T.c_tumb]];
+    *Is = (1.0 + bt * bt) / (bt * bt + ct * ct);
+    *Ii = (1.0 + ct * ct) / (bt * bt + ct * ct);
+  };
+  if (P.v.torque) {
+    double Is, Ii;
+    moments(prm, &Is, &Ii);
+    phi0 = M::atan2(Is * prm[P.types[T.Ts]], Ii * prm[P.types[T.Ti]]);
+  }
+  std::vector<CF> pmin((size_t)G * N), pmax((size_t)G * N);
+  int64_t Ntot_b = 0, Nbad_b = 0;
+  for (int t = 0; t < G; ++t) {
+    const uint64_t id = (uint64_t)g * G + t;
+    uint32_t ctr = rng[id];
+    CF* mn = &pmin[(size_t)t * N];
+    CF* mx = &pmax[(size_t)t * N];
+    for (int i = 0; i < N; ++i) { mn[i] = 1e30; mx[i] = -1e30; }
+    int Ntot = 0, Nbad = 0;
+    for (int istage = 0; istage < C.n_samples; ++istage) {
+      int LAM = 0;
+      CF x1[AST_MAX_PARAMS];
+      for (int i = 0; i < N; ++i) {
+        CF xx = x0[i] + C.dx * astrng::normal01(C.seed, id, ctr);
+        ctr += 2;
+        const int per = P.prop[i][AST_P_PERIODIC];
+        if (xx < SMALL && hard_left(per, LAM)) xx = SMALL;
+        else if (xx > 1.0 - SMALL && hard_right(per, LAM)) xx = 1.0 - SMALL;
+        x1[i] = xx;
+        if (P.prop[i][AST_P_TYPE] == T.Es) LAM = x1[i] >= 0.5;
+      }
+      if (x2params<CF, M>(P, x1, prm, C.reopt)) continue;
+      const CF f = rmsd_eval<CF>(P, form, prm, dV);
+      Ntot++;
+      if (f < f1) {
+        if (prm[iphiM] > phi_M0 + PI) prm[iphiM] = phi_M0 - 2 * PI;
+        if (prm[iphiM] < phi_M0 - PI) prm[iphiM] = phi_M0 + 2 * PI;
+        if (prm[iphi0] > phi_00 + PI) prm[iphi0] = phi_00 - 2 * PI;
+        if (prm[iphi0] < phi_00 - PI) prm[iphi0] = phi_00 + 2 * PI;
+        if (sph) {
+          double Is, Ii;
+          moments(prm, &Is, &Ii);
+          const double Ki = Ii * prm[P.types[T.Ti]];
+          const double Ks = Is * prm[P.types[T.Ts]];
+          const double Kl = prm[P.types[T.Tl]];
+          prm[P.types[T.Ti]] = std::sqrt(Ki * Ki + Ks * Ks + Kl * Kl);
+          prm[P.types[T.Ts]] = M::acos(Kl / prm[P.types[T.Ti]]);
+          double phi = M::atan2(Ks, Ki);
+          if (phi > phi0 + PI) phi = phi - 2 * PI;
+          if (phi < phi0 - PI) phi = phi + 2 * PI;
+          prm[P.types[T.Tl]] = phi;
+        }
+        for (int i = 0; i < N; ++i) {
+          if (prm[i] < mn[i]) mn[i] = prm[i];
+          if (prm[i] > mx[i]) mx[i] = prm[i];
+        }
+      } else {
+        Nbad++;
+      }
+    }
+    rng[id] = ctr;
+    Ntot_b += Ntot;
+    Nbad_b += Nbad;
+  }
+  for (int i = 0; i < N; ++i) {
+    float fmin = 1e30, fmax = -1e30;
+    for (int j = 0; j < G; ++j) {
+      const float a = pmin[(size_t)j * N + i], b = pmax[(size_t)j * N + i];
+      if (a < fmin) fmin = a;
+      if (b > fmax) fmax = b;
+    }
+    par_min[(size_t)g * N + i] = fmin;
+    par_max[(size_t)g * N + i] = fmax;
+  }
+  counts[2 * (size_t)g] = Ntot_b;
+  counts[2 * (size_t)g + 1] = Nbad_b;
+}
+
 // ================================================================ plotting
 // The plotting grid (read_data.c:304-360) as observations: n_plot times
 // (t_last - t_first)/(n_plot-1) apart from the first observation (the
@@ -1133,6 +1267,37 @@ int orc_model_curve(const orc_problem* pb, int form, const double* params, int n
   return 0;
 }
 
+// chi2_gpu_rms over n_groups blocks (threaded over blocks); rng holds the
+// per-lane stream counters and is advanced in place.
+int orc_rmsd(const orc_problem* pb, int form, int n_groups, int group_size, int n_samples, float dx, int spherical_k,
+             uint64_t seed, int reopt, const double* params0, uint32_t* rng, float* par_min, float* par_max,
+             int64_t* counts, double* f0f1, int n_threads) {
+  orc::Problem P;
+  int rc = load_problem(pb, &P);
+  if (rc) return rc;
+  if (n_groups <= 0 || group_size <= 0 || n_samples < 0) return AST_E_ARG;
+  orc::RmsdCfg C{group_size, n_samples, dx, spherical_k, reopt, seed};
+  auto work = [&](int g) {
+    double* ff = g == 0 ? f0f1 : nullptr;
+    if (simplex_float(P.v)) {
+      if (form) orc::rmsd_group<float, orc::AstMath>(P, form, C, params0, g, rng, par_min, par_max, counts, ff);
+      else orc::rmsd_group<float, orc::LibmMath>(P, form, C, params0, g, rng, par_min, par_max, counts, ff);
+    } else {
+      if (form) orc::rmsd_group<double, orc::AstMath>(P, form, C, params0, g, rng, par_min, par_max, counts, ff);
+      else orc::rmsd_group<double, orc::LibmMath>(P, form, C, params0, g, rng, par_min, par_max, counts, ff);
+    }
+  };
+  if (n_threads <= 1) {
+    for (int g = 0; g < n_groups; ++g) work(g);
+  } else {
+    std::vector<std::thread> th;
+    for (int w = 0; w < n_threads; ++w)
+      th.emplace_back([&, w]() { for (int g = w; g < n_groups; g += n_threads) work(g); });
+    for (auto& t : th) t.join();
+  }
+  return 0;
+}
+
 int orc_periodogram(const double* t, const double* vm, int n, int ncl, double* cl_fr, double* cl_H, int* n_min,
                     double* H) {
   if (!t || !vm || n < 0 || ncl < 0) return AST_E_ARG;
@@ -1227,6 +1392,7 @@ int orc_math(int fn, const double* x, const double* y, double* out, int n) {
       case 9: out[k] = astm::expf((float)a); break;
       case 10: { double sn, cs; astm::sincos_mx(a, &sn, &cs); out[k] = sn; break; }
       case 11: { double sn, cs; astm::sincos_mx(a, &sn, &cs); out[k] = cs; break; }
+      case 12: out[k] = astm::acos(a); break;
       default: return AST_E_ARG;
     }
   }
@@ -1256,6 +1422,12 @@ int orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
 
 int orc_uniform(uint64_t seed, uint64_t fit, uint32_t k0, int n, float* out) {
   for (int k = 0; k < n; ++k) out[k] = astrng::uniform01(seed, fit, k0 + (uint32_t)k);
+  return 0;
+}
+
+// n normal deviates of one stream from draw k0 on (two draws each).
+int orc_normal(uint64_t seed, uint64_t fit, uint32_t k0, int n, float* out) {
+  for (int k = 0; k < n; ++k) out[k] = astrng::normal01(seed, fit, k0 + 2u * (uint32_t)k);
   return 0;
 }
 

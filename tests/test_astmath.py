@@ -28,6 +28,7 @@ CASES = [
     ("log10", lambda r: np.exp(r.uniform(-700, 700, 4000)), mp.log10, 1.5),
     ("exp", lambda r: r.uniform(-700, 700, 4000), mp.exp, 1.0),
     ("asin", lambda r: r.uniform(-1, 1, 4000), mp.asin, 3.0),
+    ("acos", lambda r: r.uniform(-1, 1, 4000), mp.acos, 3.0),
     ("atan", lambda r: r.normal(scale=10, size=4000), mp.atan, 2.0),
 ]
 
@@ -84,6 +85,20 @@ def test_uniform_stream_layout():
         oracle.philox([1, 0, fit, 0], [seed & 0xFFFFFFFF, seed >> 32])
     want = np.array([((w >> 8) + 1) * 2.0 ** -24 for w in words], np.float32)
     assert np.array_equal(u, want)
+
+
+def test_normal_deviates():
+    """normal01: Box-Muller on draws k, k+1 of the stream: the transform of
+    the stream's own uniforms, and standard normal in distribution."""
+    from scipy import stats
+    seed, fit = 99, 5
+    z = oracle.normal(seed, fit, 0, 4)
+    u = oracle.uniform(seed, fit, 0, 8).astype(np.float64)
+    want = np.sqrt(-2 * np.log(u[0::2])) * np.cos(2 * np.pi * u[1::2])
+    np.testing.assert_allclose(z, want.astype(np.float32), rtol=2e-7, atol=1e-7)
+    big = np.concatenate([oracle.normal(seed, f, 10, 2000) for f in range(50)]).astype(np.float64)
+    assert abs(big.mean()) < 0.02 and abs(big.std() - 1) < 0.02
+    assert stats.kstest(big, "norm").pvalue > 1e-3
 
 
 def test_coefficients_regenerate():

@@ -124,3 +124,39 @@ def test_driver_plot_matches_oracle(tmp_path):
     want_v = [f32(f32(curve.V[i] - f32(dv[0, curve.filter[i]])) + f32(dv[0, 0])) for i in range(curve.n_data)]
     assert [d[1] for d in data] == ["%.6e" % x for x in want_v]
     assert [d[2] for d in data] == ["%.6e" % f32(1 / np.sqrt(curve.w[i])) for i in range(curve.n_data)]
+
+
+def test_driver_rmsd_matches_oracle(tmp_path):
+    """`asteroid -DRMSD -dx ... -Nstages n -reopt -m ...` (asteroid.c:643-725):
+    the f0/f1 line, then per cycle the cumulative rejected fraction and the
+    cumulative range of every parameter, against the oracle's chi2_gpu_rms
+    run twice on the driver's ingested data with the same seed."""
+    _scene(tmp_path, n_obs=80, filters="RG")
+    curve = _ingested(tmp_path)
+    v = variants.ls_torque(abi.PREC_F32SIMPLEX)
+    space = variants.default_model_space(v, ppsi_hours=(2.0, 4800.0), reopt=1)
+    pb = oracle.Problem(space, curve)
+    p, rc = pb.x2params(np.full((1, space.n_params), 0.41), reopt=1)
+    params = p[0]
+    cmd = [EXE, "-DP_PSI", "-DTORQUE", "-DINTERP", "-DRMSD", "-dx", "3e-4", "-Nstages", "3", "-reopt", "-seed", "9",
+           "-blocks", "2", "-bsize", "64", "-N", "2", "-i", "lc.txt", "-o", "ci.txt", "-m"] + ["%.17g" % x for x in params]
+    r = subprocess.run(cmd, cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout
+    rng = np.zeros(128, np.uint32)
+    runs = [pb.rmsd(2, 64, 3, 3e-4, params, seed=9, rng=rng, n_threads=4) for _ in range(2)]
+    out = r.stdout.splitlines()
+    assert "f0=%f, f1=%f" % (np.float32(runs[0]["f0"]), np.float32(runs[0]["f1"])) in out
+    lines = [l for l in out if l.count(";") == space.n_params]
+    assert len(lines) == 2
+    ntot = nbad = 0
+    pmin = np.full(space.n_params, np.float32(1e30))
+    pmax = np.full(space.n_params, np.float32(-1e30))
+    file_lines = open(tmp_path / "ci.txt").read().splitlines()
+    for k, w in enumerate(runs):
+        ntot += w["counts"][:, 0].sum()
+        nbad += w["counts"][:, 1].sum()
+        pmin = np.minimum(pmin, w["par_min"].min(0))
+        pmax = np.maximum(pmax, w["par_max"].max(0))
+        ranges = "".join("%13.6e %13.6e ; " % (a, b) for a, b in zip(pmin, pmax))
+        assert lines[k] == "%e " % (nbad / ntot) + ranges
+        assert file_lines[k] == ranges

@@ -1,0 +1,89 @@
+"""RMSD confidence intervals on the GPU (ast_rmsd) against the oracle's
+chi2_gpu_rms: per-group ranges, counts, f0 and f1 bit for bit in the FP64 and
+float-simplex forms (with and without spherical torque components, groups of
+up to 256 and of 1024), the random streams continuing across calls; the
+mixed form against FP64 within its tolerance."""
+import numpy as np
+import pytest
+
+import oracle
+from asteroid_amd import abi, api, synth, variants
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = {
+    "ls_torque": lambda p: variants.ls_torque(p),
+    "bc": lambda p: variants.bc_free(p),
+    "bw_torque": lambda p: variants.bw_ball_torque(p),
+    "ls_plain": lambda p: variants.make_variant(False, False, False, True, True, p, 1e-2),
+}
+
+
+def _case(vname, prec, nf=2, n=120, seed=41):
+    v = VARIANTS[vname](prec)
+    curve, truth = synth.make_curve(seed, v, n, 2.0, n_filters=nf)
+    space = variants.default_model_space(v, reopt=1)
+    return v, curve, truth, space
+
+
+def _ctx(v, space, curve, truth, seed):
+    ctx = api.Context(v)
+    ctx.set_model_space(space)
+    ctx.upload_curve(curve)
+    ctx.rng_seed(seed)
+    ctx.set_start(truth)
+    return ctx
+
+
+def _same(g, w, groups):
+    assert np.array_equal(g["par_min"], w["par_min"])
+    assert np.array_equal(g["par_max"], w["par_max"])
+    assert g["n_tot"] == w["counts"][:, 0].sum() and g["n_bad"] == w["counts"][:, 1].sum()
+    assert g["f0"] == w["f0"] and g["f1"] == w["f1"]
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_F32SIMPLEX], ids=["fp64", "f32simplex"])
+@pytest.mark.parametrize("vname", list(VARIANTS))
+def test_rmsd_bitwise(vname, prec):
+    v, curve, truth, space = _case(vname, prec)
+    G, ng, S, dx, seed = 64, 3, 6, 3e-4, 17
+    with _ctx(v, space, curve, truth, seed) as ctx:
+        a = ctx.rmsd(ng, G, S, dx)
+        b = ctx.rmsd(ng, G, S, dx)  # the streams continue
+    pb = oracle.Problem(space, curve)
+    rng = np.zeros(ng * G, np.uint32)
+    wa = pb.rmsd(ng, G, S, dx, truth, seed=seed, rng=rng, n_threads=4)
+    wb = pb.rmsd(ng, G, S, dx, truth, seed=seed, rng=rng, n_threads=4)
+    print(vname, "accepted", a["n_tot"] - a["n_bad"], "of", a["n_tot"])
+    _same(a, wa, ng)
+    _same(b, wb, ng)
+    assert 0 < a["n_bad"] < a["n_tot"]  # the shifts reach beyond the interval
+
+
+@pytest.mark.parametrize("G", [256, 1024])
+def test_rmsd_spherical_and_group_sizes(G):
+    v, curve, truth, space = _case("ls_torque", abi.PREC_F32SIMPLEX, nf=1)
+    with _ctx(v, space, curve, truth, 3) as ctx:
+        got = ctx.rmsd(2, G, 2, 3e-4, spherical_k=True)
+        ms, _ = ctx.last_timing()
+    print(f"G={G}: {ms:.2f} ms")
+    want = oracle.Problem(space, curve).rmsd(2, G, 2, 3e-4, truth, seed=3, spherical_k=True, n_threads=8)
+    _same(got, want, 2)
+
+
+def test_rmsd_mixed_within_tolerance():
+    v64, curve, truth, space64 = _case("ls_torque", abi.PREC_FP64, nf=2)
+    vmx = VARIANTS["ls_torque"](abi.PREC_MIXED)
+    spacemx = variants.default_model_space(vmx, reopt=1)
+    with _ctx(vmx, spacemx, curve, truth, 5) as ctx:
+        mx = ctx.rmsd(4, 128, 4, 3e-4)
+    # the same draws in float coordinates: the float-simplex form of FP64
+    v32 = VARIANTS["ls_torque"](abi.PREC_F32SIMPLEX)
+    with _ctx(v32, variants.default_model_space(v32, reopt=1), curve, truth, 5) as ctx:
+        f32 = ctx.rmsd(4, 128, 4, 3e-4)
+    assert mx["n_tot"] == f32["n_tot"]
+    assert abs(mx["n_bad"] - f32["n_bad"]) <= max(2, f32["n_tot"] // 1000)
+    assert abs(mx["f0"] - f32["f0"]) <= 1e-6 * f32["f0"]
+    span = np.maximum(f32["par_max"].max(0) - f32["par_min"].min(0), 1e-12)
+    assert np.all(np.abs(mx["par_min"].min(0) - f32["par_min"].min(0)) <= 0.05 * span)
+    assert np.all(np.abs(mx["par_max"].max(0) - f32["par_max"].max(0)) <= 0.05 * span)
