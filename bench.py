@@ -24,8 +24,9 @@ fitting kernel (algorithmic FLOPs from asteroid_amd/flops.py over its
 HIP-event-timed launch durations) and the CPU oracle timed on a bounded
 sample of the same workload on this host's cores.
 
---workload small|reopt|batched|serving measures the other configurations,
-one JSON line each (see DESIGN.md for what each one is).
+--workload small|reopt|batched|serving|plot|rmsd measures the other
+configurations and paths, one JSON line each (see DESIGN.md for what each
+one is).
 """
 import argparse
 import json
@@ -380,6 +381,139 @@ def main_batched(args):
     return out
 
 
+def _models_near(truth, n, seed, scale=1e-3):
+    """truth plus n-1 models with every parameter shifted by a relative
+    normal deviate of `scale` (candidate models around a fit)."""
+    r = np.random.default_rng(seed).standard_normal((n - 1, len(truth)))
+    return np.vstack([truth[None], truth[None] * (1 + scale * r)])
+
+
+def main_plot(args):
+    """The -plot path: model light curves on the reference's 20000-point grid
+    (chi2_plot) for a batch of models on the Stage One curve, the reference's
+    single-model case, and the minima periodogram of one curve."""
+    import oracle
+    from asteroid_amd import api, flops, workloads
+    prec = F32SIMPLEX if args.precision in ("both", "f32simplex") else {"fp64": FP64, "mixed": MIXED}[args.precision]
+    v, curve, truth, space = workloads.stage_one(prec)
+    n_models, n_plot = args.plot_models, 20000
+    params = _models_near(truth, n_models, 5)
+    with api.Context(v) as ctx:
+        ctx.set_model_space(space)
+        ctx.upload_curve(curve)
+        ctx.model_curve(params[:64], n_plot=n_plot)  # warm-up
+        t0 = time.perf_counter()
+        t, vm, chi2, dv = ctx.model_curve(params, n_plot=n_plot)
+        wall = time.perf_counter() - t0
+        ms, _ = ctx.last_timing()
+        t0 = time.perf_counter()
+        t1, vm1, _, _ = ctx.model_curve(truth, n_plot=n_plot)
+        one_wall = time.perf_counter() - t0
+        one_ms, _ = ctx.last_timing()
+        t0 = time.perf_counter()
+        fr, H, nmin = ctx.periodogram(t1, vm1[0], 5)
+        pg_wall = time.perf_counter() - t0
+        pg_ms, _ = ctx.last_timing()
+    pts = n_models * (curve.n_data + n_plot)
+    f_model = flops.eval_flops(v, curve.mjd) + flops.eval_flops(v, t)
+    achieved = f_model * n_models / (ms / 1e3) / 1e12
+    log(f"{n_models} models x {n_plot}: kernel {ms:.1f} ms, call {wall * 1e3:.1f} ms; one model {one_ms:.2f} ms")
+    out = base_line(pts / wall, 1, 1, wall * 1e3, DTYPE[prec],
+                    {"workload": f"plot: {n_models} models (the truth and models within 0.1% of it) x "
+                                 f"{n_plot}-point grid on the Stage One 400-point curve, chi2_plot per model",
+                     "models": n_models, "n_plot": n_plot, "precision": PREC_NAMES[prec]})
+    out.update({"kernel_ms": ms, "model_points_per_s_kernel": pts / (ms / 1e3),
+                "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                             "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                             "kernel": "model_curve_kernel", "flop_per_model": round(f_model, 1),
+                             "lanes": n_models},
+                "single_model": {"kernel_ms": one_ms, "call_ms": one_wall * 1e3,
+                                 "periodogram_ms": pg_wall * 1e3, "periodogram_kernel_ms": pg_ms,
+                                 "minima": nmin, "clusters_per_day": [float(x) for x in fr]},
+                "cpu_baseline": None})
+    if args.no_cpu:
+        return out
+    pb = oracle.Problem(space, curve)
+    n = cores()
+    # calibrate on one model per core, then size the sample to ~cpu_seconds
+    t0 = time.perf_counter()
+    pb.model_curve(params[:n], n_plot=n_plot, n_threads=n)
+    t1 = time.perf_counter() - t0
+    sample = params[: min(n_models, n * max(1, int(round(args.cpu_seconds / max(t1, 1e-3)))))]
+    t0 = time.perf_counter()
+    want = pb.model_curve(sample, n_plot=n_plot, n_threads=n)
+    cpu_dt = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pb.model_curve(truth, n_plot=n_plot)
+    cpu_one = time.perf_counter() - t0
+    same = bool(np.array_equal(want[1], vm[: len(sample)]) and np.array_equal(want[2], chi2[: len(sample)]))
+    out["single_model"]["cpu_oracle_ms"] = cpu_one * 1e3
+    out.update({"identical_to_oracle": same,
+                "cpu_baseline": {"value": len(sample) * (curve.n_data + n_plot) / cpu_dt, "unit": "model-point evals/s",
+                                 "cores": n, "kind": "port",
+                                 "sample": f"{len(sample)} of the models ({cpu_dt:.2f} s wall, oracle canonical form, "
+                                           f"{n} threads)"}})
+    return out
+
+
+def main_rmsd(args):
+    """RMSD confidence intervals (chi2_gpu_rms) around the Stage One truth
+    model: 2^18 lanes (1024 groups of 256), args.rmsd_samples shifted models
+    each, dx = args.dx; float-simplex form (the reference's default) with the
+    mixed form beside it."""
+    import oracle
+    from asteroid_amd import api, flops, variants, workloads
+    res = {}
+    for prec in (F32SIMPLEX, MIXED):
+        v, curve, truth, _ = workloads.stage_one(prec)
+        space = variants.default_model_space(v, ppsi_hours=(2.0, 4800.0), reopt=1)
+        with api.Context(v) as ctx:
+            ctx.set_model_space(space)
+            ctx.upload_curve(curve)
+            ctx.rng_seed(21)
+            ctx.set_start(truth)
+            ctx.rmsd(1024, 256, 4, args.dx)  # warm-up
+            t0 = time.perf_counter()
+            r = ctx.rmsd(1024, 256, args.rmsd_samples, args.dx)
+            wall = time.perf_counter() - t0
+            ms, _ = ctx.last_timing()
+        f_eval = flops.eval_flops(v, curve.mjd) if prec != MIXED else sum(flops.eval_flops_mixed(v, curve.mjd))
+        peak = peak_of(prec, v, curve.mjd)[0]
+        achieved = f_eval * r["n_tot"] / (ms / 1e3) / 1e12
+        res[prec] = dict(value=r["n_tot"] * curve.n_data / wall, wall_s=wall, kernel_ms=ms, n_tot=r["n_tot"],
+                         n_bad=r["n_bad"], f0=r["f0"], f1=r["f1"],
+                         ranges=[[float(a), float(b)] for a, b in zip(r["par_min"].min(0), r["par_max"].max(0))],
+                         roofline={"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 2),
+                                   "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "kernel": "rmsd_kernel"})
+        log(f"{PREC_NAMES[prec]}: {res[prec]['value']:.3e} model-pts/s, kernel {ms:.0f} ms, "
+            f"accepted {r['n_tot'] - r['n_bad']} of {r['n_tot']}")
+    a = res[F32SIMPLEX]
+    out = base_line(a["value"], 1, 1, a["wall_s"] * 1e3, DTYPE[F32SIMPLEX],
+                    {"workload": f"RMSD confidence intervals around the Stage One truth model: 1024 groups x 256 "
+                                 f"lanes x {args.rmsd_samples} models, dx {args.dx}, 400-point curve",
+                     "lanes": 262144, "samples_per_lane": args.rmsd_samples, "dx": args.dx})
+    out.update({"roofline": a["roofline"], "f32simplex": a, "mixed": res[MIXED], "cpu_baseline": None})
+    if args.no_cpu:
+        return out
+    v, curve, truth, _ = workloads.stage_one(F32SIMPLEX)
+    space = variants.default_model_space(v, ppsi_hours=(2.0, 4800.0), reopt=1)
+    pb = oracle.Problem(space, curve)
+    n = cores()
+    # calibrate on one group of 16 lanes x 8 models per core, then size the sample to ~cpu_seconds
+    t0 = time.perf_counter()
+    pb.rmsd(n, 16, 8, args.dx, truth, seed=21, n_threads=n)
+    t1 = (time.perf_counter() - t0) / 8
+    S = max(1, int(round(args.cpu_seconds / max(t1, 1e-4))))
+    t0 = time.perf_counter()
+    w = pb.rmsd(n, 16, S, args.dx, truth, seed=21, n_threads=n)
+    dt = time.perf_counter() - t0
+    out.update({"cpu_baseline": {"value": float(w["counts"][:, 0].sum()) * curve.n_data / dt,
+                                 "unit": "model-point evals/s", "cores": n, "kind": "port",
+                                 "sample": f"{n} groups x 16 lanes x {S} models ({dt:.2f} s wall, oracle "
+                                           f"canonical form, {n} threads)"}})
+    return out
+
+
 def fifo_latency(arrivals, service):
     """Latencies (ms) of a single FIFO server with these arrival and service times (s)."""
     free, lat = 0.0, []
@@ -432,7 +566,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["stage1", "small", "reopt", "batched", "serving"], default="stage1")
+    ap.add_argument("--workload", choices=["stage1", "small", "reopt", "batched", "serving", "plot", "rmsd"],
+                    default="stage1")
+    ap.add_argument("--plot-models", type=int, default=4096)
+    ap.add_argument("--rmsd-samples", type=int, default=64)
+    ap.add_argument("--dx", type=float, default=0.003)
     ap.add_argument("--precision", choices=["both", "mixed", "fp64", "f32simplex"], default="both")
     ap.add_argument("--fits", type=int, default=1 << 18)
     ap.add_argument("--budget", type=int, default=4800)
@@ -454,7 +592,7 @@ def main():
               "without communication (see DESIGN.md); run it with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     fn = {"stage1": main_stage_one, "small": main_small, "reopt": main_reopt, "batched": main_batched,
-          "serving": main_serving}[args.workload]
+          "serving": main_serving, "plot": main_plot, "rmsd": main_rmsd}[args.workload]
     print(json.dumps(fn(args)), flush=True)
 
 

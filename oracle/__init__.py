@@ -52,7 +52,7 @@ def lib():
         L.orc_params2x.argtypes = [pP, C.c_int, pD, C.c_int, pD]
         L.orc_run.argtypes = [pP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_double,
                               C.c_uint64, C.c_int, pD, pU32, pD, pD, pD, pD, pI64, pI64, C.c_int]
-        L.orc_model_curve.argtypes = [pP, C.c_int, pD, C.c_int, C.c_int, pD, pD, pD, pD]
+        L.orc_model_curve.argtypes = [pP, C.c_int, pD, C.c_int, C.c_int, pD, pD, pD, pD, C.c_int]
         L.orc_periodogram.argtypes = [pD, pD, C.c_int, C.c_int, pD, pD, pI, pD]
         L.orc_rmsd.argtypes = [pP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_uint64, C.c_int, pD,
                                pU32, C.POINTER(C.c_float), C.POINTER(C.c_float), pI64, pD, C.c_int]
@@ -110,7 +110,7 @@ class Problem:
             out.append(tr)
         return tuple(out)
 
-    def model_curve(self, params, n_plot=20000, form=CANON):
+    def model_curve(self, params, n_plot=20000, form=CANON, n_threads=1):
         """chi2_plot on the reference's plotting grid: (t, vmod, chi2, dV)."""
         params = np.ascontiguousarray(np.atleast_2d(params), np.float64)
         n = params.shape[0]
@@ -118,7 +118,8 @@ class Problem:
         vm = np.zeros((n, n_plot))
         chi2 = np.zeros(n)
         dV = np.zeros((n, self.curve.n_filters))
-        rc = lib().orc_model_curve(C.byref(self.pb), form, _p(params), n, n_plot, _p(t), _p(vm), _p(chi2), _p(dV))
+        rc = lib().orc_model_curve(C.byref(self.pb), form, _p(params), n, n_plot, _p(t), _p(vm), _p(chi2), _p(dV),
+                                   n_threads)
         assert rc == 0, rc
         return t, vm, chi2, dV
 

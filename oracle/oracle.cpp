@@ -1244,14 +1244,14 @@ int orc_chi2(const orc_problem* pb, int form, const double* params, int n, doubl
 // The -plot evaluation of n models on an n_plot-point grid: t (n_plot),
 // vmod (n*n_plot, model-major), chi2 (n), dV (n*n_filters).
 int orc_model_curve(const orc_problem* pb, int form, const double* params, int n, int n_plot, double* t,
-                    double* vmod, double* chi2, double* dV) {
+                    double* vmod, double* chi2, double* dV, int n_threads) {
   orc::Problem P;
   int rc = load_problem(pb, &P);
   if (rc) return rc;
   if (n_plot < 2) return AST_E_ARG;
   const std::vector<ast_obs> grid = orc::plot_grid(P, form, n_plot);
   if (t) for (int i = 0; i < n_plot; ++i) t[i] = grid[i].mjd;
-  for (int k = 0; k < n; ++k) {
+  auto work = [&](int k) {
     const double* p = params + (size_t)k * P.n_params;
     double* vm = vmod + (size_t)k * n_plot;
     if (simplex_float(P.v)) {
@@ -1263,6 +1263,14 @@ int orc_model_curve(const orc_problem* pb, int form, const double* params, int n
       chi2[k] = orc::model_curve<double>(P, form, p, grid, d, vm);
       for (int m = 0; m < P.n_filters; ++m) dV[(size_t)k * P.n_filters + m] = d[m];
     }
+  };
+  if (n_threads <= 1) {
+    for (int k = 0; k < n; ++k) work(k);
+  } else {
+    std::vector<std::thread> th;
+    for (int w = 0; w < n_threads; ++w)
+      th.emplace_back([&, w]() { for (int k = w; k < n; k += n_threads) work(k); });
+    for (auto& x : th) x.join();
   }
   return 0;
 }
