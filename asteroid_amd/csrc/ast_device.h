@@ -384,40 +384,65 @@ __device__ inline double log2_f32x(float x) {
 constexpr float LN2_F = 0.693147180559945309f;
 constexpr double M2P5_LOG10_2 = -0.75257498915995295;  // -2.5 log10(2): magnitude from log2
 
-// Lommel-Seeliger magnitude from the scaled directions in FP32.  sin(alpha')
-// comes from |u_S x u_E| and the bisector normaliser from |u_S + u_E|, so
-// nothing cancels at small phase angles.  ca_out = cos(alpha').
-__device__ inline double ls_tail_f32(const ScaledDirs& ud, float bcp, float* ca_out) {
-  const float uSb = (float)ud.uSb, uSc = (float)ud.uSc, uSa = (float)ud.uSa;
-  const float uEb = (float)ud.uEb, uEc = (float)ud.uEc, uEa = (float)ud.uEa;
-  const float sS2 = dot3(uSb, uSc, uSa, uSb, uSc, uSa);
-  const float sE2 = dot3(uEb, uEc, uEa, uEb, uEc, uEa);
-  const float rS = __builtin_amdgcn_rsqf(sS2), rE = __builtin_amdgcn_rsqf(sE2);
-  const float sS = sS2 * rS, sE = sE2 * rE;
-  const float k = rS * rE;  // 1 / (sS sE)
-  const float ca = dot3(uSb, uSc, uSa, uEb, uEc, uEa) * k;
-  *ca_out = ca;
-  const float x0 = fmx(-uSa, uEc, uSc * uEa);
-  const float x1 = fmx(-uSb, uEa, uSa * uEb);
-  const float x2 = fmx(-uSc, uEb, uSb * uEc);
-  const float cr2 = dot3(x0, x1, x2, x0, x1, x2);
-  const float sa = (cr2 * __builtin_amdgcn_rsqf(cr2)) * k;
-  const float t0 = uSb + uEb, t1 = uSc + uEc, t2 = uSa + uEa;
-  const float isc = __builtin_amdgcn_rsqf(dot3(t0, t1, t2, t0, t1, t2));
-  const float cl = fmx(sE, ca, sS) * isc;
-  const float sl = (sE * sa) * isc;
-  const float cd = fmx(cl, ca, sl * sa);
-  const float sd = fmx(-cl, sa, sl * ca);
-  const float lg = LN2_F * log2_f32(((1.0f + cl) * (1.0f + cd)) * __builtin_amdgcn_rcpf(sl * (-sd)));
-  const float inner = fmx(sl * sd, lg, cd + cl);
-  return M2P5_LOG10_2 * log2_f32x((((bcp * sS) * sE) * isc) * inner);
+// 1/sqrt(x) in FP64 from the hardware estimate and one Newton step.
+__device__ inline double rsq_nr(double x) {
+  const double r = __builtin_amdgcn_rsq(x);
+  const double e = __builtin_fma(-(x * r), r, 1.0);
+  return __builtin_fma(0.5 * r, e, r);
 }
 
-// The mixed form redoes the Lommel-Seeliger tail in FP64 when the scaled
-// Sun and Earth directions are more than about 143 degrees apart
-// (cos < -0.8).  There the sum cancels to a small fraction of its terms (flat
-// shapes seen from the unlit side) and FP32 would lose several digits.
-constexpr float MX_BACKLIT_COS = -0.8f;
+// Lommel-Seeliger magnitude of the mixed form, without cancellation at any
+// phase.  With s^, e^ the unit scaled Sun and Earth directions, t = u_S + u_E
+// and t^ its unit vector, h = s^ + e^: the phase angle alpha' is 2 s' with
+// cos s' = |h|/2, and lambda' (the angle from s^ to t^) gives
+// d' = lambda' - alpha'/2 with cos d' = t^.h/|h|.  With x = cos s'/cos d',
+//   cos(alpha'-lambda') + cos lambda' + sin lambda' sin(lambda'-alpha')
+//     ln(cot(lambda'/2) cot((alpha'-lambda')/2)) = 2 cos^2 d' g(x),
+//   g(x) = x - (1 - x^2) artanh x,
+// which vanishes like x^3 towards backscatter (x -> 0).  Below X0 g comes from
+// its series sum_k 2 x^(2k+1)/((2k-1)(2k+1)); above, from the closed form with
+// y = 1 - x = |s^ - t^| |e^ - t^| / (2 cos d') (that is 2 sin(lambda'/2)
+// sin((alpha'-lambda')/2) / cos d'), so nothing cancels as x -> 1 either.  The
+// unit vectors, their sum and differences are formed in FP64, the rest in
+// FP32; the magnitude error stays below ~1.2e-6 for every geometry.
+constexpr float MX_SERIES_X0 = 0.6f;
+constexpr int MX_SERIES_TERMS = 14;  // 0.36^14 < 1e-6 of the leading term at X0
+
+__device__ inline double ls_tail_mx(const ScaledDirs& ud, float bcp) {
+  const double sS2 = dot3(ud.uSb, ud.uSc, ud.uSa, ud.uSb, ud.uSc, ud.uSa);
+  const double sE2 = dot3(ud.uEb, ud.uEc, ud.uEa, ud.uEb, ud.uEc, ud.uEa);
+  const double tb = ud.uSb + ud.uEb, tc = ud.uSc + ud.uEc, ta = ud.uSa + ud.uEa;
+  const double tt2 = dot3(tb, tc, ta, tb, tc, ta);
+  const double rS = rsq_nr(sS2), rE = rsq_nr(sE2), rT = rsq_nr(tt2);
+  const double sb = ud.uSb * rS, sc = ud.uSc * rS, sa = ud.uSa * rS;  // s^
+  const double eb = ud.uEb * rE, ec = ud.uEc * rE, ea = ud.uEa * rE;  // e^
+  const double qb = tb * rT, qc = tc * rT, qa = ta * rT;              // t^
+  const float hb = (float)(sb + eb), hc = (float)(sc + ec), ha = (float)(sa + ea);
+  const float pb = (float)(sb - qb), pc = (float)(sc - qc), pa = (float)(sa - qa);
+  const float wb = (float)(eb - qb), wc = (float)(ec - qc), wa = (float)(ea - qa);
+  const float sS = (float)(sS2 * rS), sE = (float)(sE2 * rE), isc = (float)rT;
+  const float hh = dot3(hb, hc, ha, hb, hc, ha);
+  const float rh = __builtin_amdgcn_rsqf(hh);
+  const float cdp = dot3((float)qb, (float)qc, (float)qa, hb, hc, ha) * rh;  // cos d'
+  const float icd = __builtin_amdgcn_rcpf(cdp);
+  const float x = (0.5f * (hh * rh)) * icd;
+  const float pp = dot3(pb, pc, pa, pb, pc, pa), ww = dot3(wb, wc, wa, wb, wc, wa);
+  const float y = (0.5f * (__builtin_amdgcn_sqrtf(pp) * __builtin_amdgcn_sqrtf(ww))) * icd;  // 1 - x
+  // series
+  const float xs = __builtin_fminf(x, MX_SERIES_X0);
+  const float x2 = xs * xs;
+  float acc = 2.0f / ((2.0f * MX_SERIES_TERMS - 1.0f) * (2.0f * MX_SERIES_TERMS + 1.0f));
+#pragma unroll
+  for (int k = MX_SERIES_TERMS - 1; k >= 1; --k) acc = fmx(acc, x2, 2.0f / ((2.0f * k - 1.0f) * (2.0f * k + 1.0f)));
+  const float gs = (acc * x2) * xs;
+  // closed form (y = 0 only at zero phase, where the limit is g = x)
+  const float yc = __builtin_fmaxf(y, 1e-30f);
+  const float art = (0.5f * LN2_F) * log2_f32((2.0f - y) * __builtin_amdgcn_rcpf(yc));  // artanh x
+  const float gc = x - (y * (2.0f - y)) * art;
+  const float g = x < MX_SERIES_X0 ? gs : gc;
+  const float inner = ((2.0f * cdp) * cdp) * g;
+  return M2P5_LOG10_2 * log2_f32x((((bcp * sS) * sE) * isc) * inner);
+}
 
 // chi^2 of one model (chi2one, cuda.c:81-1011, in-scope branches), summed
 // per filter; NF is the compile-time filter capacity (>= n_filters).  With MX
@@ -547,9 +572,7 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
       const double S[3] = {pr.S[0], pr.S[1], pr.S[2]};
       const ScaledDirs u = scaled_dirs(a, b, c, E, S, K);
       if (MX) {
-        float ca;
-        Vmod = ls_tail_f32(u, bcpf, &ca);
-        if (!(ca >= MX_BACKLIT_COS)) Vmod = ls_tail_f64(u, K.bcp);  // ill-conditioned (or NaN): FP64
+        Vmod = ls_tail_mx(u, bcpf);
       } else {
         Vmod = ls_tail_f64(u, K.bcp);
       }

@@ -54,21 +54,21 @@ def eval_flops(variant, mjd):
 # The mixed-precision evaluator runs the per-model set-up, the RK4
 # integration, the body axes, the scaled Sun/Earth directions and the chi^2
 # sums exactly as the FP64 form (the rows above), and replaces the FP64
-# brightness tail by an FP32 one.  Counted from ast_device.h:
+# brightness tail by its own.  Counted from ast_device.h:
 #   FP64 Lommel-Seeliger tail (ls_tail_f64, from |u_S|^2 to the magnitude,
 #     with the log and log10 of astmath.h): 16 add, 32 mul, 27 fma, 5 div,
 #     4 sqrt; the Lambert + ball tail (log10 and the -2.5 factor): 6 add,
 #     8 mul, 8 fma, 1 div.
-#   FP32 Lommel-Seeliger tail (ls_tail_f32): 7 add, 28 mul, 17 fma and 7
-#     "special" instructions (hardware rsq, rcp or log2, 1 FLOP each); the
-#     Lambert + ball tail: 1 log2.  Both add 1 FP64 add and 1 FP64 mul (the
-#     magnitude from the exponent and the log2 of the mantissa).
-# Observations the mixed form redoes in FP64 (backlit scaled geometry) are
-# extra work and not counted.
+#   Mixed Lommel-Seeliger tail (ls_tail_mx): in FP64 the norms, the unit
+#     vectors and their sum and differences, 12 add, 20 mul, 12 fma and 3
+#     hardware rsq (1 FLOP each, in the sqrt column); in FP32, 3 add, 25 mul,
+#     21 fma and 7 hardware rsq/rcp/sqrt/log2 ("special", 1 FLOP each).  The
+#     Lambert + ball tail: 1 FP32 log2.  Both add 1 FP64 add and 1 FP64 mul
+#     (the magnitude from the exponent and the log2 of the mantissa).
 MIXED = 2
 _TAIL64 = {"ls": np.array([16, 32, 27, 5, 4, 0], float), "bw": np.array([6, 8, 8, 1, 0, 0], float)}
-_TAIL32 = {"ls": np.array([7, 28, 17, 7], float), "bw": np.array([0, 0, 0, 1], float)}  # add mul fma special
-_MX_F64_EXTRA = np.array([1, 1, 0, 0, 0, 0], float)
+_TAIL32 = {"ls": np.array([3, 25, 21, 7], float), "bw": np.array([0, 0, 0, 1], float)}  # add mul fma special
+_MX_F64_EXTRA = {"ls": np.array([13, 21, 12, 0, 3, 0], float), "bw": np.array([1, 1, 0, 0, 0, 0], float)}
 _W4 = np.array([1, 1, 2, 1], float)
 
 
@@ -77,7 +77,7 @@ def eval_flops_mixed(variant, mjd):
     kind = "bw" if variant.bw_ball else "ls"
     t = TABLE[(bool(variant.torque), kind)]
     n = len(mjd)
-    f64_pt = np.array(t["per_point"], float) - _TAIL64[kind] + _MX_F64_EXTRA
+    f64_pt = np.array(t["per_point"], float) - _TAIL64[kind] + _MX_F64_EXTRA[kind]
     f64 = (np.array(t["per_model"], float) @ _W + rk4_steps(mjd, variant.time_step) * (np.array(t["per_rk4"], float) @ _W)
            + n * (f64_pt @ _W))
     return float(f64), float(n * (_TAIL32[kind] @ _W4))

@@ -37,7 +37,7 @@ def test_mixed_table_consistent():
     """The FP64 brightness tail the mixed form replaces holds every division
     and square root of an observation (the rest of the point is fma/mul/add
     work), and the mixed counts are the FP64 counts minus that tail plus the
-    FP32 tail."""
+    mixed tail (its FP64 and FP32 parts)."""
     from asteroid_amd import abi, synth, variants
     for torque, kind in ((True, "ls"), (False, "ls"), (True, "bw")):
         per_point = np.array(flops.TABLE[(torque, kind)]["per_point"], float)
@@ -47,5 +47,5 @@ def test_mixed_table_consistent():
     curve, _ = synth.make_curve(3, v64, 50, 1.0)
     f64, f32 = flops.eval_flops_mixed(vmx, curve.mjd)
     w = np.array([1, 1, 2, 1, 1, 1], float)
-    expect64 = flops.eval_flops(v64, curve.mjd) - 50 * (flops._TAIL64["ls"] - flops._MX_F64_EXTRA) @ w
-    assert f64 == expect64 and f32 == 50 * 76 and flops.eval_flops(vmx, curve.mjd) == f64 + f32
+    expect64 = flops.eval_flops(v64, curve.mjd) - 50 * (flops._TAIL64["ls"] - flops._MX_F64_EXTRA["ls"]) @ w
+    assert f64 == expect64 and f32 == 50 * 77 and flops.eval_flops(vmx, curve.mjd) == f64 + f32
