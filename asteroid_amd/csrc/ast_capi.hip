@@ -35,13 +35,13 @@
 #include "ast_kernels.h"
 
 namespace ast {
-template <bool TQ, bool BC, bool BW> hipError_t launch_variant(const LaunchArgs& a);
-extern template hipError_t launch_variant<false, false, false>(const LaunchArgs&);
-extern template hipError_t launch_variant<false, true, false>(const LaunchArgs&);
-extern template hipError_t launch_variant<false, false, true>(const LaunchArgs&);
-extern template hipError_t launch_variant<true, false, false>(const LaunchArgs&);
-extern template hipError_t launch_variant<true, true, false>(const LaunchArgs&);
-extern template hipError_t launch_variant<true, false, true>(const LaunchArgs&);
+template <int VC> hipError_t launch_variant(const LaunchArgs& a);
+extern template hipError_t launch_variant<0>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_BC>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_BW>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_TQ>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_TQ | VF_BC>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_TQ | VF_BW>(const LaunchArgs&);
 }  // namespace ast
 
 using namespace ast;
@@ -265,12 +265,12 @@ int nf_class(int n_filters) { return n_filters <= 1 ? 1 : (n_filters <= 4 ? 4 : 
 hipError_t dispatch(const ast_variant& v, LaunchArgs& a) {
   a.precision = v.precision;
   const bool tq = v.torque, bc = v.bc, bw = v.bw_ball;
-  if (!tq && !bc && !bw) return launch_variant<false, false, false>(a);
-  if (!tq && bc) return launch_variant<false, true, false>(a);
-  if (!tq && bw) return launch_variant<false, false, true>(a);
-  if (tq && !bc && !bw) return launch_variant<true, false, false>(a);
-  if (tq && bc) return launch_variant<true, true, false>(a);
-  return launch_variant<true, false, true>(a);
+  if (!tq && !bc && !bw) return launch_variant<0>(a);
+  if (!tq && bc) return launch_variant<VF_BC>(a);
+  if (!tq && bw) return launch_variant<VF_BW>(a);
+  if (tq && !bc && !bw) return launch_variant<VF_TQ>(a);
+  if (tq && bc) return launch_variant<VF_TQ | VF_BC>(a);
+  return launch_variant<VF_TQ | VF_BW>(a);
 }
 
 // ------------------------------------------------------------- slot pool

@@ -38,17 +38,41 @@ struct PointRec {
 };
 static_assert(sizeof(PointRec) == 96, "PointRec layout");
 
+// Model variant as one compile-time code: the reference's macros
+// (makefile:3-36) as bit flags.  TORQUE2 implies TORQUE and RECT implies BC
+// (asteroid.h:32-38).
+enum : int { VF_TQ = 1, VF_BC = 2, VF_BW = 4, VF_TREND = 8, VF_TQ2 = 16, VF_ROT = 32, VF_RECT = 64 };
+template <int VC>
+struct VarF {
+  static constexpr bool TQ = (VC & (VF_TQ | VF_TQ2)) != 0;  // constant torque
+  static constexpr bool T2 = (VC & VF_TQ2) != 0;            // a second torque after a split time
+  static constexpr bool BC = (VC & (VF_BC | VF_RECT)) != 0; // photometric b, c
+  static constexpr bool BW = (VC & VF_BW) != 0;             // black-and-white ball
+  static constexpr bool TR = (VC & VF_TREND) != 0;          // de-trending by the phase angle
+  static constexpr bool RO = (VC & VF_ROT) != 0;            // brightness frame rotated
+  static constexpr bool RC = (VC & VF_RECT) != 0;           // rectangular prism brightness
+};
+#define AST_VFLAGS(VC)                                                                           \
+  constexpr bool TQ = VarF<VC>::TQ, BC = VarF<VC>::BC, BW = VarF<VC>::BW, T2 = VarF<VC>::T2,   \
+                 TR = VarF<VC>::TR, RO = VarF<VC>::RO, RC = VarF<VC>::RC
+
 // Parameter positions of a variant (the Property order, asteroid.c:86-125).
-template <bool TQ, bool BC, bool BW>
+template <int VC>
 struct Pos {
-  static constexpr int o = TQ ? 3 : 0;
+  using F = VarF<VC>;
+  static constexpr int t = F::TR ? 1 : 0;
+  static constexpr int o = t + (F::TQ ? 3 : 0) + (F::T2 ? 4 : 0);
   static constexpr int theta_M = 0, phi_M = 1, phi_0 = 2;
-  static constexpr int Ti = TQ ? 3 : -1, Ts = TQ ? 4 : -1, Tl = TQ ? 5 : -1;
+  static constexpr int A = F::TR ? 3 : -1;
+  static constexpr int Ti = F::TQ ? 3 + t : -1, Ts = F::TQ ? 4 + t : -1, Tl = F::TQ ? 5 + t : -1;
+  static constexpr int T2i = F::T2 ? 6 + t : -1, T2s = F::T2 ? 7 + t : -1, T2l = F::T2 ? 8 + t : -1,
+                       Tt = F::T2 ? 9 + t : -1;
   static constexpr int c_tumb = 3 + o, b_tumb = 4 + o, Es = 5 + o, L = 6 + o, psi_0 = 7 + o;
-  static constexpr int c = BC ? 8 + o : -1, b = BC ? 9 + o : -1;
-  static constexpr int ob = o + (BC ? 2 : 0);
-  static constexpr int theta_R = BW ? 8 + ob : -1, phi_R = BW ? 9 + ob : -1, kappa = BW ? 10 + ob : -1;
-  static constexpr int N = 8 + o + (BC ? 2 : 0) + (BW ? 3 : 0);
+  static constexpr int c = F::BC ? 8 + o : -1, b = F::BC ? 9 + o : -1;
+  static constexpr int ob = o + (F::BC ? 2 : 0);
+  static constexpr int theta_R = (F::BW || F::RO) ? 8 + ob : -1, phi_R = (F::BW || F::RO) ? 9 + ob : -1;
+  static constexpr int psi_R = F::RO ? 10 + ob : -1, kappa = F::BW ? 10 + ob : -1;
+  static constexpr int N = 8 + o + (F::BC ? 2 : 0) + ((F::BW || F::RO) ? 3 : 0);
 };
 
 // Model space by parameter position (the reference indexes limits by type;
@@ -79,9 +103,10 @@ __device__ inline float cf_floor(float x) { return __builtin_floorf(x); }
 // ------------------------------------------------------------------ x2params
 // cuda.c:1120-1319 (P_PSI branch of the L mapping).  Same operation order
 // and types as the oracle.  Returns nonzero on a hard-limit crossing.
-template <class CF, bool TQ, bool BC, bool BW>
+template <class CF, int VC>
 __device__ int x2params(const CF* x, double* params, const Space<CF>& sp, int reopt) {
-  using P = Pos<TQ, BC, BW>;
+  using P = Pos<VC>;
+  AST_VFLAGS(VC);
   constexpr int N = P::N;
   int LAM = 0;
 #pragma unroll
@@ -155,9 +180,10 @@ __device__ int x2params(const CF* x, double* params, const Space<CF>& sp, int re
 
 // ------------------------------------------------------------------ params2x
 // cuda.c:1016-1115.
-template <class CF, bool TQ, bool BC, bool BW>
+template <class CF, int VC>
 __device__ void params2x(CF* x, const double* params, const Space<CF>& sp) {
-  using P = Pos<TQ, BC, BW>;
+  using P = Pos<VC>;
+  AST_VFLAGS(VC);
   constexpr int N = P::N;
   int LAM = 0, i_Es = 0;
   double b_tumb = params[P::b_tumb];
@@ -456,11 +482,12 @@ __device__ inline double ls_tail_mx(const ScaledDirs& ud, float bcp) {
 // result is 0; FORM_RMSD (the RMSD build, cuda.c:890-906), the root of the
 // summed filter chi^2 over the summed weights.
 enum { FORM_CHI2 = 0, FORM_PLOT = 1, FORM_RMSD = 2 };
-template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX = false, int FORM = FORM_CHI2>
+template <class CF, int VC, int NF, bool MX = false, int FORM = FORM_CHI2>
 __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_filters, int dof,
                          const double* sum_w, CF* dV, double* vplot = nullptr, size_t vstride = 0,
                          double dv0 = 0.0) {
-  using P = Pos<TQ, BC, BW>;
+  using P = Pos<VC>;
+  AST_VFLAGS(VC);
   double sy2[NF], sy[NF];
 #pragma unroll
   for (int m = 0; m < NF; ++m) { sy2[m] = 0.0; sy[m] = 0.0; }

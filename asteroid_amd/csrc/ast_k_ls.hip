@@ -2,5 +2,5 @@
 // one translation unit per variant so the variants compile in parallel.
 #include "ast_kernels.h"
 namespace ast {
-template hipError_t launch_variant<false, false, false>(const LaunchArgs&);
+template hipError_t launch_variant<0>(const LaunchArgs&);
 }

@@ -113,10 +113,11 @@ struct StageCfg {
   int n_steps, eval_budget;
 };
 
-template <class CF, bool TQ, bool BC, bool BW>
+template <class CF, int VC>
 struct Machine {
-  using P = Pos<TQ, BC, BW>;
+  using P = Pos<VC>;
   static constexpr int N = P::N;
+  static constexpr bool BC = VarF<VC>::BC;
   const Space<CF>& sp;
   const SlotBufs<CF>& B;
   const StageCfg<CF>& C;
@@ -138,7 +139,7 @@ struct Machine {
     double prm[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) prm[i] = PRM(i);
-    int rc = x2params<CF, TQ, BC, BW>(xv, prm, sp, reopt);
+    int rc = x2params<CF, VC>(xv, prm, sp, reopt);
 #pragma unroll
     for (int i = 0; i < N; ++i) PRM(i) = prm[i];
     return rc == 0;
@@ -407,9 +408,10 @@ struct Machine {
 // A group's start (one workgroup of G threads per new group): identity sort
 // order, stage-0 start point (params2x of the reoptimisation model,
 // cuda.c:1409-1416), counters, group flags.
-template <class CF, bool TQ, bool BC, bool BW>
+template <class CF, int VC>
 __global__ void fit_init_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
-  using P = Pos<TQ, BC, BW>;
+  using P = Pos<VC>;
+  AST_VFLAGS(VC);
   constexpr int N = P::N;
   constexpr int KC = 8 / sizeof(CF);
   const GroupDesc<CF>& D = descs[blockIdx.x];
@@ -421,7 +423,7 @@ __global__ void fit_init_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
   for (int i = 0; i < N; ++i) prm[i] = D.reopt ? D.params0[i] : 0.0;
   for (int i = 0; i < N; ++i) B.PRM[i * nf + fit] = prm[i];
   CF x[N];
-  if (D.reopt) params2x<CF, TQ, BC, BW>(x, prm, *D.sp);
+  if (D.reopt) params2x<CF, VC>(x, prm, *D.sp);
   else
     for (int i = 0; i < N; ++i) x[i] = 0;
   for (int i = 0; i < N; ++i) B.X[(size_t)i * rs + fo] = x[i];
@@ -444,11 +446,11 @@ __global__ void fit_init_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
 // lane's params[] buffer, delta_V to its last-dV slot.  Kept out of line so
 // the evaluator's register allocation does not compete with the simplex
 // bookkeeping live across it.
-template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX>
+template <class CF, int VC, int NF, bool MX>
 __device__ __attribute__((noinline)) CF eval_lane(const double* PRM, CF* DV, size_t nf, size_t fit, size_t rs,
                                                    size_t fo, const PointRec* pts_g, int n_data, int n_filters, int dof,
                                                    const double* sum_w) {
-  constexpr int N = Pos<TQ, BC, BW>::N;
+  constexpr int N = Pos<VC>::N;
   // Arguments of a device function arrive in vector registers; the table
   // pointer and the sizes are wave-uniform, so move them to scalar registers
   // to keep the observation loop on scalar loads and scalar branches.
@@ -463,7 +465,7 @@ __device__ __attribute__((noinline)) CF eval_lane(const double* PRM, CF* DV, siz
 #pragma unroll
   for (int i = 0; i < N; ++i) prm[i] = PRM[(size_t)i * nf + fit];
   CF dv[NF];
-  CF fv = chi2_model<CF, TQ, BC, BW, NF, MX>(prm, pts, n_data, n_filters, dof, sum_w, dv);
+  CF fv = chi2_model<CF, VC, NF, MX>(prm, pts, n_data, n_filters, dof, sum_w, dv);
 #pragma unroll
   for (int m = 0; m < NF; ++m)
     if (m < n_filters) DV[(size_t)m * rs + fo] = dv[m];
@@ -477,10 +479,11 @@ __device__ __attribute__((noinline)) CF eval_lane(const double* PRM, CF* DV, siz
 // 1024 simplexes), 3 (168 VGPRs) or 2 (256 VGPRs), both for groups of up to
 // 256.  The arithmetic is the same for every W; only speed differs, and the
 // autotuner picks W per kernel class by measurement.
-template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX, int W>
+template <class CF, int VC, int NF, bool MX, int W>
 __global__ __launch_bounds__(W == 4 ? 1024 : 256) __attribute__((amdgpu_waves_per_eu(W, W)))
 void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
-  using P = Pos<TQ, BC, BW>;
+  using P = Pos<VC>;
+  AST_VFLAGS(VC);
   constexpr int N = P::N;
   __shared__ CF s_f[1024];
   __shared__ CF s_x0[MAXP];
@@ -516,13 +519,13 @@ void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
   R.sf = B.sf[fo];
   bool done = false;
   {
-    Machine<CF, TQ, BC, BW> M{sp, B, SC, fit, nf, fo, rs, jfit, reopt};
+    Machine<CF, VC> M{sp, B, SC, fit, nf, fo, rs, jfit, reopt};
     M.advance(R, (CF)0, false);  // a lane at a stage start draws its simplex first
   }
   for (int tick = 0; tick < ticks && !done; ++tick) {
-    Machine<CF, TQ, BC, BW> M{sp, B, SC, fit, nf, fo, rs, jfit, reopt};
+    Machine<CF, VC> M{sp, B, SC, fit, nf, fo, rs, jfit, reopt};
     if (phase_pending(R.phase)) {
-      CF fv = eval_lane<CF, TQ, BC, BW, NF, MX>(B.PRM, B.DV, nf, fit, rs, fo, D.pts, D.n_data, D.n_filters, D.dof,
+      CF fv = eval_lane<CF, VC, NF, MX>(B.PRM, B.DV, nf, fit, rs, fo, D.pts, D.n_data, D.n_filters, D.dof,
                                             D.sum_w);
       R.nev++;
       R.nev_tot++;
@@ -554,8 +557,8 @@ void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
           double prm[N];
 #pragma unroll
           for (int i = 0; i < N; ++i) prm[i] = M.PRM(i);
-          x2params<CF, TQ, BC, BW>(xb, prm, sp, reopt);
-          params2x<CF, TQ, BC, BW>(xb, prm, sp);
+          x2params<CF, VC>(xb, prm, sp, reopt);
+          params2x<CF, VC>(xb, prm, sp);
 #pragma unroll
           for (int i = 0; i < N; ++i) { M.PRM(i) = prm[i]; M.X(b, i) = xb[i]; }
         }
@@ -568,7 +571,7 @@ void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
 #pragma unroll
       for (int i = 0; i < N; ++i) M.X(0, i) = s_x0[i];
       R.phase = PH_START;
-      Machine<CF, TQ, BC, BW> M2{sp, B, SC, fit, nf, fo, rs, jfit, reopt};
+      Machine<CF, VC> M2{sp, B, SC, fit, nf, fo, rs, jfit, reopt};
       M2.advance(R, (CF)0, false);
     } else {
       const int g = D.group_in_job;
@@ -579,7 +582,7 @@ void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
         const int b = M.IND(0);
 #pragma unroll
         for (int i = 0; i < N; ++i) { xb[i] = M.X(b, i); prm[i] = M.PRM(i); }
-        x2params<CF, TQ, BC, BW>(xb, prm, sp, reopt);
+        x2params<CF, VC>(xb, prm, sp, reopt);
 #pragma unroll
         for (int i = 0; i < N; ++i) {
           M.PRM(i) = prm[i];
@@ -610,11 +613,12 @@ void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
 }
 
 // ------------------------------------------------------------- chi2 batch
-template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX>
+template <class CF, int VC, int NF, bool MX>
 __global__ __launch_bounds__(256) void chi2_batch_kernel(const PointRec* pts_g, const double* sum_w, int n_data,
                                                         int n_filters, int dof, const double* params, int n,
                                                         double* chi2, double* dV) {
-  using P = Pos<TQ, BC, BW>;
+  using P = Pos<VC>;
+  AST_VFLAGS(VC);
   constexpr int N = P::N;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
@@ -623,7 +627,7 @@ __global__ __launch_bounds__(256) void chi2_batch_kernel(const PointRec* pts_g, 
 #pragma unroll
   for (int i = 0; i < N; ++i) prm[i] = params[(size_t)k * N + i];
   CF dv[NF];
-  CF c = chi2_model<CF, TQ, BC, BW, NF, MX>(prm, pts, n_data, n_filters, dof, sum_w, dv);
+  CF c = chi2_model<CF, VC, NF, MX>(prm, pts, n_data, n_filters, dof, sum_w, dv);
   chi2[k] = c;
   for (int m = 0; m < n_filters; ++m) dV[(size_t)k * n_filters + m] = dv[m];
 }
@@ -635,12 +639,13 @@ __global__ __launch_bounds__(256) void chi2_batch_kernel(const PointRec* pts_g, 
 // delta_V[0] at each point of the plotting grid (step two, cuda.c:1949-1950),
 // integrated along the grid from its first time.  vmod is model-major: lane
 // k writes vmod[k * n_plot + i].
-template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX>
+template <class CF, int VC, int NF, bool MX>
 __global__ __launch_bounds__(256) void model_curve_kernel(const PointRec* pts_g, const double* sum_w, int n_data,
                                                          int n_filters, int dof, const PointRec* grid_g, int n_plot,
                                                          const double* params, int n, double* chi2, double* dV,
                                                          double* vmod) {
-  using P = Pos<TQ, BC, BW>;
+  using P = Pos<VC>;
+  AST_VFLAGS(VC);
   constexpr int N = P::N;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
@@ -648,10 +653,10 @@ __global__ __launch_bounds__(256) void model_curve_kernel(const PointRec* pts_g,
 #pragma unroll
   for (int i = 0; i < N; ++i) prm[i] = params[(size_t)k * N + i];
   CF dv[NF];
-  CF c = chi2_model<CF, TQ, BC, BW, NF, MX>(prm, (PointPtr)(uintptr_t)pts_g, n_data, n_filters, dof, sum_w, dv);
+  CF c = chi2_model<CF, VC, NF, MX>(prm, (PointPtr)(uintptr_t)pts_g, n_data, n_filters, dof, sum_w, dv);
   chi2[k] = c;
   for (int m = 0; m < n_filters; ++m) dV[(size_t)k * n_filters + m] = dv[m];
-  chi2_model<CF, TQ, BC, BW, NF, MX, FORM_PLOT>(prm, (PointPtr)(uintptr_t)grid_g, n_plot, n_filters, dof, sum_w,
+  chi2_model<CF, VC, NF, MX, FORM_PLOT>(prm, (PointPtr)(uintptr_t)grid_g, n_plot, n_filters, dof, sum_w,
                                                dv, vmod + (size_t)k * n_plot, 1, (double)dv[0]);
 }
 
@@ -687,17 +692,18 @@ struct RmsdArgs {
   int32_t* counts;        // [group][2]: Ntot, Nbad
 };
 
-template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX>
+template <class CF, int VC, int NF, bool MX>
 __global__ __launch_bounds__(64) void rmsd_ref_kernel(RmsdArgs<CF> A) {
-  using P = Pos<TQ, BC, BW>;
+  using P = Pos<VC>;
+  AST_VFLAGS(VC);
   constexpr int N = P::N;
   if (threadIdx.x != 0) return;
   double prm[N];
   for (int i = 0; i < N; ++i) prm[i] = A.params0[i];
   CF x[N];
-  params2x<CF, TQ, BC, BW>(x, prm, *A.sp);
+  params2x<CF, VC>(x, prm, *A.sp);
   CF dv[NF];
-  const CF f0 = chi2_model<CF, TQ, BC, BW, NF, MX, FORM_RMSD>(prm, (PointPtr)(uintptr_t)A.pts, A.n_data,
+  const CF f0 = chi2_model<CF, VC, NF, MX, FORM_RMSD>(prm, (PointPtr)(uintptr_t)A.pts, A.n_data,
                                                               A.n_filters, A.dof, A.sum_w, dv);
   RmsdRef<CF>& R = *A.ref;
   for (int i = 0; i < N; ++i) R.x0[i] = x[i];
@@ -717,10 +723,11 @@ __global__ __launch_bounds__(64) void rmsd_ref_kernel(RmsdArgs<CF> A) {
 // One lane per thread of the reference: n_samples shifted models each,
 // the range of the accepted ones per parameter, reduced per group in lane
 // order by the group's first thread (cuda.c:2356-2381).
-template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX, int W>
+template <class CF, int VC, int NF, bool MX, int W>
 __global__ __launch_bounds__(W == 4 ? 1024 : 256) __attribute__((amdgpu_waves_per_eu(W, W))) void rmsd_kernel(
     RmsdArgs<CF> A) {
-  using P = Pos<TQ, BC, BW>;
+  using P = Pos<VC>;
+  AST_VFLAGS(VC);
   constexpr int N = P::N;
   constexpr double PI = 3.141592653589793238;
   constexpr double SMALL = 1e-8;  // cuda.c:2258
@@ -748,9 +755,9 @@ __global__ __launch_bounds__(W == 4 ? 1024 : 256) __attribute__((amdgpu_waves_pe
       x1[i] = xx;
       if (i == P::Es) LAM = x1[i] >= (CF)0.5;
     }
-    if (x2params<CF, TQ, BC, BW>(x1, prm, sp, A.reopt)) continue;
+    if (x2params<CF, VC>(x1, prm, sp, A.reopt)) continue;
     CF dv[NF];
-    const CF f = chi2_model<CF, TQ, BC, BW, NF, MX, FORM_RMSD>(prm, (PointPtr)(uintptr_t)A.pts, A.n_data,
+    const CF f = chi2_model<CF, VC, NF, MX, FORM_RMSD>(prm, (PointPtr)(uintptr_t)A.pts, A.n_data,
                                                                A.n_filters, A.dof, A.sum_w, dv);
     ntot++;
     if (f < R.f1) {
@@ -834,61 +841,61 @@ struct LaunchArgs {
   const void* rmsd;
 };
 
-template <class CF, bool TQ, bool BC, bool BW, int NF, bool MX>
+template <class CF, int VC, int NF, bool MX>
 hipError_t launch_nf(const LaunchArgs& a) {
   if (a.op == 0) {
-    hipLaunchKernelGGL((fit_init_kernel<CF, TQ, BC, BW>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
+    hipLaunchKernelGGL((fit_init_kernel<CF, VC>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
                        (const GroupDesc<CF>*)a.descs, *(const SlotBufs<CF>*)a.bufs);
   } else if (a.op == 1) {
     const GroupDesc<CF>* d = (const GroupDesc<CF>*)a.descs;
     const SlotBufs<CF>& B = *(const SlotBufs<CF>*)a.bufs;
     const int w = a.group_size > 256 ? 4 : a.waves;  // W < 4 kernels take groups of up to 256
     if (w == 2)
-      hipLaunchKernelGGL((fit_kernel<CF, TQ, BC, BW, NF, MX, 2>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
+      hipLaunchKernelGGL((fit_kernel<CF, VC, NF, MX, 2>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
                          d, B);
     else if (w == 3)
-      hipLaunchKernelGGL((fit_kernel<CF, TQ, BC, BW, NF, MX, 3>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
+      hipLaunchKernelGGL((fit_kernel<CF, VC, NF, MX, 3>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
                          d, B);
     else
-      hipLaunchKernelGGL((fit_kernel<CF, TQ, BC, BW, NF, MX, 4>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
+      hipLaunchKernelGGL((fit_kernel<CF, VC, NF, MX, 4>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
                          d, B);
   } else if (a.op == 2) {
     int blocks = (a.n + 255) / 256;
-    hipLaunchKernelGGL((chi2_batch_kernel<CF, TQ, BC, BW, NF, MX>), dim3(blocks), dim3(256), 0, a.stream, a.pts,
+    hipLaunchKernelGGL((chi2_batch_kernel<CF, VC, NF, MX>), dim3(blocks), dim3(256), 0, a.stream, a.pts,
                        a.sum_w, a.n_data, a.n_filters, a.dof, a.params, a.n, a.chi2, a.dV);
   } else if (a.op == 3) {
     int blocks = (a.n + 255) / 256;
-    hipLaunchKernelGGL((model_curve_kernel<CF, TQ, BC, BW, NF, MX>), dim3(blocks), dim3(256), 0, a.stream, a.pts,
+    hipLaunchKernelGGL((model_curve_kernel<CF, VC, NF, MX>), dim3(blocks), dim3(256), 0, a.stream, a.pts,
                        a.sum_w, a.n_data, a.n_filters, a.dof, a.grid, a.n_plot, a.params, a.n, a.chi2, a.dV,
                        a.vmod);
   } else if (a.op == 4) {
-    hipLaunchKernelGGL((rmsd_ref_kernel<CF, TQ, BC, BW, NF, MX>), dim3(1), dim3(64), 0, a.stream,
+    hipLaunchKernelGGL((rmsd_ref_kernel<CF, VC, NF, MX>), dim3(1), dim3(64), 0, a.stream,
                        *(const RmsdArgs<CF>*)a.rmsd);
   } else if (a.group_size > 256) {  // groups of up to 1024: 128 registers
-    hipLaunchKernelGGL((rmsd_kernel<CF, TQ, BC, BW, NF, MX, 4>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
+    hipLaunchKernelGGL((rmsd_kernel<CF, VC, NF, MX, 4>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
                        *(const RmsdArgs<CF>*)a.rmsd);
   } else {
-    hipLaunchKernelGGL((rmsd_kernel<CF, TQ, BC, BW, NF, MX, 2>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
+    hipLaunchKernelGGL((rmsd_kernel<CF, VC, NF, MX, 2>), dim3(a.n_groups), dim3(a.group_size), 0, a.stream,
                        *(const RmsdArgs<CF>*)a.rmsd);
   }
   return hipGetLastError();
 }
 
-template <class CF, bool TQ, bool BC, bool BW, bool MX>
+template <class CF, int VC, bool MX>
 hipError_t launch_cf(const LaunchArgs& a) {
-  if (a.n_filters <= 1) return launch_nf<CF, TQ, BC, BW, 1, MX>(a);
-  if (a.n_filters <= 4) return launch_nf<CF, TQ, BC, BW, 4, MX>(a);
-  return launch_nf<CF, TQ, BC, BW, MAXF, MX>(a);
+  if (a.n_filters <= 1) return launch_nf<CF, VC, 1, MX>(a);
+  if (a.n_filters <= 4) return launch_nf<CF, VC, 4, MX>(a);
+  return launch_nf<CF, VC, MAXF, MX>(a);
 }
 
 // Simplex float and evaluator form by precision: FP64 (double simplex, FP64
 // evaluator), F32SIMPLEX (float simplex, FP64 evaluator: the reference's
 // default CHI_FLOAT), MIXED (float simplex, mixed-precision evaluator).
-template <bool TQ, bool BC, bool BW>
+template <int VC>
 hipError_t launch_variant(const LaunchArgs& a) {
-  if (a.precision == AST_PREC_FP64) return launch_cf<double, TQ, BC, BW, false>(a);
-  if (a.precision == AST_PREC_MIXED) return launch_cf<float, TQ, BC, BW, true>(a);
-  return launch_cf<float, TQ, BC, BW, false>(a);
+  if (a.precision == AST_PREC_FP64) return launch_cf<double, VC, false>(a);
+  if (a.precision == AST_PREC_MIXED) return launch_cf<float, VC, true>(a);
+  return launch_cf<float, VC, false>(a);
 }
 
 }  // namespace ast
