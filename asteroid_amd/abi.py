@@ -28,13 +28,14 @@ E_CANCELLED = -8
 
 class Variant(C.Structure):
     _fields_ = [("torque", C.c_int), ("bc", C.c_int), ("bw_ball", C.c_int), ("p_psi", C.c_int),
-                ("interp", C.c_int), ("precision", C.c_int), ("time_step", C.c_double)]
+                ("interp", C.c_int), ("precision", C.c_int), ("time_step", C.c_double), ("trend", C.c_int),
+                ("torque2", C.c_int), ("rotate", C.c_int), ("rect", C.c_int)]
 
 
 class TypeIds(C.Structure):
     _fields_ = [(n, C.c_int) for n in (
         "theta_M", "phi_M", "phi_0", "L", "Ti", "Ts", "Tl", "c_tumb", "b_tumb", "Es", "psi_0",
-        "c", "b", "theta_R", "phi_R", "kappa", "n_types", "n_params")]
+        "c", "b", "theta_R", "phi_R", "kappa", "n_types", "n_params", "A", "T2i", "T2s", "T2l", "Tt", "psi_R")]
 
 
 class Obs(C.Structure):
@@ -104,30 +105,47 @@ assert C.sizeof(Obs) == 72
 
 def type_ids(variant):
     """ast_variant_type_ids of include/asteroid.h (asteroid.h:137-178 order)."""
+    torque = bool(variant.torque or variant.torque2)
+    bc = bool(variant.bc or variant.rect)
     ids = {}
     k = 0
     for name in ("theta_M", "phi_M", "phi_0", "L"):
         ids[name] = k
         k += 1
-    for name in ("Ti", "Ts", "Tl"):
+    ids["A"] = -1
+    if variant.trend:
+        ids["A"] = k
+        k += 1
+    for name in ("Ti", "Ts", "Tl", "T2i", "T2s", "T2l", "Tt"):
         ids[name] = -1
-    if variant.torque:
+    if torque:
         for name in ("Ti", "Ts", "Tl"):
+            ids[name] = k
+            k += 1
+    if variant.torque2:
+        for name in ("T2i", "T2s", "T2l", "Tt"):
             ids[name] = k
             k += 1
     for name in ("c_tumb", "b_tumb", "Es", "psi_0"):
         ids[name] = k
         k += 1
     ids["c"] = ids["b"] = -1
-    if variant.bc:
+    if bc:
         ids["c"], ids["b"] = k, k + 1
         k += 2
-    ids["theta_R"] = ids["phi_R"] = ids["kappa"] = -1
+    ids["theta_R"] = ids["phi_R"] = ids["psi_R"] = ids["kappa"] = -1
+    if variant.rotate or variant.bw_ball:
+        ids["theta_R"], ids["phi_R"] = k, k + 1
+        k += 2
+    if variant.rotate:
+        ids["psi_R"] = k
+        k += 1
     if variant.bw_ball:
-        ids["theta_R"], ids["phi_R"], ids["kappa"] = k, k + 1, k + 2
-        k += 3
+        ids["kappa"] = k
+        k += 1
     ids["n_types"] = k
-    ids["n_params"] = 8 + 2 * bool(variant.bc) + 3 * bool(variant.torque) + 3 * bool(variant.bw_ball)
+    ids["n_params"] = (8 + 2 * bc + 3 * bool(variant.rotate) + bool(variant.trend)
+                       + (7 if variant.torque2 else 3 * torque) + 3 * bool(variant.bw_ball))
     return ids
 
 

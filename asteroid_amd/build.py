@@ -1,6 +1,6 @@
 """Build libasteroid_hip.so (gfx950) and the host driver in-tree.
 
-hipcc compiles the six variant translation units, the C-ABI and the table
+hipcc compiles the variant translation units, the C-ABI and the table
 helper in parallel; everything links into asteroid_amd/lib/libasteroid_hip.so.
 All HIP code is built with -ffp-contract=off (the canonical evaluation order
 spells every fma out, so host and device results are bitwise reproducible)
@@ -23,7 +23,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-math-errno", 
           "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unknown-pragmas"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-VARIANT_TUS = ["ast_k_ls", "ast_k_bc", "ast_k_bw", "ast_k_tq_ls", "ast_k_tq_bc", "ast_k_tq_bw"]
+VARIANT_TUS = ["ast_k_ls", "ast_k_bc", "ast_k_bw", "ast_k_tq_ls", "ast_k_tq_bc", "ast_k_tq_bw", "ast_k_tq_trend",
+               "ast_k_tq2", "ast_k_bc_rot", "ast_k_tq_bc_rot", "ast_k_rect", "ast_k_tq_rect"]
 
 
 def _run(cmd):

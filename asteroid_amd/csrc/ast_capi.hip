@@ -42,6 +42,12 @@ extern template hipError_t launch_variant<VF_BW>(const LaunchArgs&);
 extern template hipError_t launch_variant<VF_TQ>(const LaunchArgs&);
 extern template hipError_t launch_variant<VF_TQ | VF_BC>(const LaunchArgs&);
 extern template hipError_t launch_variant<VF_TQ | VF_BW>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_TQ | VF_TREND>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_TQ2>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_BC | VF_ROT>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_TQ | VF_BC | VF_ROT>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_RECT>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_TQ | VF_RECT>(const LaunchArgs&);
 }  // namespace ast
 
 using namespace ast;
@@ -86,8 +92,29 @@ void dev_free(T*& p) {
 }
 
 // ------------------------------------------------------------ validation
+// The variant codes built (one kernel family each), indexed by variant_index.
+constexpr int VARIANT_CODES[] = {0, VF_TQ, VF_BC, VF_TQ | VF_BC, VF_BW, VF_TQ | VF_BW,
+                                 VF_TQ | VF_TREND, VF_TQ2, VF_BC | VF_ROT, VF_TQ | VF_BC | VF_ROT,
+                                 VF_RECT, VF_TQ | VF_RECT};
+constexpr int N_VARIANTS = (int)(sizeof VARIANT_CODES / sizeof VARIANT_CODES[0]);
+
+// Variant code of the macros (TORQUE2 implies TORQUE, RECT implies BC).
+int variant_code(const ast_variant& v) {
+  const bool t2 = v.torque2, tq = v.torque || t2, rc = v.rect, bc = v.bc || rc;
+  return (tq && !t2 ? VF_TQ : 0) | (t2 ? VF_TQ2 : 0) | (bc && !rc ? VF_BC : 0) | (v.bw_ball ? VF_BW : 0) |
+         (v.trend ? VF_TREND : 0) | (v.rotate ? VF_ROT : 0) | (rc ? VF_RECT : 0);
+}
+
+// Index of the variant's kernel family, or -1 if it is not built.
+int variant_index(const ast_variant& v) {
+  const int code = variant_code(v);
+  for (int k = 0; k < N_VARIANTS; ++k)
+    if (VARIANT_CODES[k] == code) return k;
+  return -1;
+}
+
 bool variant_supported(const ast_variant& v) {
-  if (v.bc && v.bw_ball) return false;
+  if (variant_index(v) < 0) return false;
   if (v.precision != AST_PREC_FP64 && v.precision != AST_PREC_F32SIMPLEX && v.precision != AST_PREC_MIXED)
     return false;
   if (!(v.time_step > 0)) return false;
@@ -113,10 +140,11 @@ int make_space(const ast_variant& v, int n_params, const int* property, int n_ty
   if (n_params != S.T.n_params || n_types != S.T.n_types) return AST_E_ARG;
   // The kernels fix each parameter's role by its position: the Property
   // order of the reference (asteroid.c:86-125).
-  const int expect[] = {S.T.theta_M, S.T.phi_M, S.T.phi_0, S.T.Ti, S.T.Ts, S.T.Tl, S.T.c_tumb, S.T.b_tumb,
-                        S.T.Es, S.T.L, S.T.psi_0, S.T.c, S.T.b, S.T.theta_R, S.T.phi_R, S.T.kappa};
+  const int expect[] = {S.T.theta_M, S.T.phi_M, S.T.phi_0, S.T.A, S.T.Ti, S.T.Ts, S.T.Tl, S.T.T2i,
+                        S.T.T2s, S.T.T2l, S.T.Tt, S.T.c_tumb, S.T.b_tumb, S.T.Es, S.T.L, S.T.psi_0,
+                        S.T.c, S.T.b, S.T.theta_R, S.T.phi_R, S.T.psi_R, S.T.kappa};
   int pos = 0;
-  for (int k = 0; k < 16; ++k) {
+  for (int k = 0; k < (int)(sizeof expect / sizeof expect[0]); ++k) {
     if (expect[k] < 0) continue;
     if (property[pos * AST_N_COLUMNS + AST_P_TYPE] != expect[k]) return AST_E_UNSUPPORTED;
     if (types[expect[k]] != pos) return AST_E_UNSUPPORTED;
@@ -171,6 +199,12 @@ void interp_canon(const ast_ephem3& e, double t, double E[3], double S[3]) {
   for (int k = 0; k < 3; ++k) { E[k] = E[k] / En; S[k] = S[k] / Sn; }
 }
 
+// Phase angle of unit Sun and Earth directions, acos(S.E) in the canonical
+// form (TREND, cuda.c:694).
+double phase_angle(const double E[3], const double S[3]) {
+  return astm::acos(std::fma(S[2], E[2], std::fma(S[1], E[1], S[0] * E[0])));
+}
+
 struct CurveH {  // a validated light curve as the kernels read it
   std::vector<PointRec> pts;
   double sum_w[AST_MAX_FILTERS] = {0};
@@ -209,6 +243,9 @@ int make_curve(const ast_variant& v, const ast_obs* obs, int n_data, int n_filte
     p.V = o.V;
     p.w = o.w;
     p.filter = o.filter;
+    p.mjd = o.mjd;
+    p.alpha = phase_angle(p.E, p.S);
+    p.ts = v.time_step;
     C.sum_w[o.filter] = C.sum_w[o.filter] + (double)o.w;
     seen[o.filter] = 1;
   }
@@ -245,6 +282,9 @@ int make_grid(const ast_variant& v, const CurveH& C, const ast_ephem3& eph, int 
       t[i] = C.t_first + (double)i * h;
       interp_canon(eph, t[i], p.E, p.S);
     }
+    p.mjd = t[i];
+    p.alpha = phase_angle(p.E, p.S);
+    p.ts = v.time_step;
     if (i > 0) {
       const double t1 = t[i - 1], t2 = t[i];
       const int nsteps = (int)((t2 - t1) / v.time_step + 1);
@@ -264,20 +304,28 @@ int nf_class(int n_filters) { return n_filters <= 1 ? 1 : (n_filters <= 4 ? 4 : 
 
 hipError_t dispatch(const ast_variant& v, LaunchArgs& a) {
   a.precision = v.precision;
-  const bool tq = v.torque, bc = v.bc, bw = v.bw_ball;
-  if (!tq && !bc && !bw) return launch_variant<0>(a);
-  if (!tq && bc) return launch_variant<VF_BC>(a);
-  if (!tq && bw) return launch_variant<VF_BW>(a);
-  if (tq && !bc && !bw) return launch_variant<VF_TQ>(a);
-  if (tq && bc) return launch_variant<VF_TQ | VF_BC>(a);
-  return launch_variant<VF_TQ | VF_BW>(a);
+  switch (variant_code(v)) {
+    case 0: return launch_variant<0>(a);
+    case VF_BC: return launch_variant<VF_BC>(a);
+    case VF_BW: return launch_variant<VF_BW>(a);
+    case VF_TQ: return launch_variant<VF_TQ>(a);
+    case VF_TQ | VF_BC: return launch_variant<VF_TQ | VF_BC>(a);
+    case VF_TQ | VF_BW: return launch_variant<VF_TQ | VF_BW>(a);
+    case VF_TQ | VF_TREND: return launch_variant<VF_TQ | VF_TREND>(a);
+    case VF_TQ2: return launch_variant<VF_TQ2>(a);
+    case VF_BC | VF_ROT: return launch_variant<VF_BC | VF_ROT>(a);
+    case VF_TQ | VF_BC | VF_ROT: return launch_variant<VF_TQ | VF_BC | VF_ROT>(a);
+    case VF_RECT: return launch_variant<VF_RECT>(a);
+    case VF_TQ | VF_RECT: return launch_variant<VF_TQ | VF_RECT>(a);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 // ------------------------------------------------------------- slot pool
 // Device state of up to `nf` resident simplexes.  Element storage is 8 bytes
 // wide so that float and double simplexes share the pool; arrays have room
 // for AST_MAX_PARAMS parameters.
-constexpr int MAX_CLASSES = 54;  // 6 variants x 3 filter capacities x 3 precision forms
+constexpr int MAX_CLASSES = N_VARIANTS * 3 * 3;  // variants x 3 filter capacities x 3 precision forms
 constexpr int NSTREAMS = 4;      // classes share streams round-robin
 constexpr size_t SLOT_BYTES = 8 * ((AST_MAX_PARAMS + 1) * AST_MAX_PARAMS + (AST_MAX_PARAMS + 1) +
                                    4 * AST_MAX_PARAMS + AST_MAX_FILTERS + 2) +
@@ -540,9 +588,9 @@ hipError_t job_launch(DevJob& j, const Pool& pool, int op, hipStream_t s) {
 }
 
 // Kernel class of a job: one fit-kernel instantiation (variant, filter
-// capacity, precision form).  BC and BW_BALL are exclusive, so k < 6.
+// capacity, precision form).
 int class_of(const ast_variant& v, int n_filters) {
-  int k = (v.torque ? 1 : 0) + (v.bc ? 2 : 0) + (v.bw_ball ? 4 : 0);
+  int k = variant_index(v);
   int f = n_filters <= 1 ? 0 : (n_filters <= 4 ? 1 : 2);
   int p = v.precision == AST_PREC_FP64 ? 0 : (v.precision == AST_PREC_F32SIMPLEX ? 1 : 2);
   return (k * 3 + f) * 3 + p;

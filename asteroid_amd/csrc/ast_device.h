@@ -34,9 +34,11 @@ struct PointRec {
   double h, hh, h6;
   float V, w;
   int filter, nsteps;
-  int pad[2];
+  double mjd;    // time of the observation (for an interval split inside a model, TORQUE2)
+  double alpha;  // phase angle acos(S.E) (TREND)
+  double ts;     // the variant's TIME_STEP (for the split interval's steps)
 };
-static_assert(sizeof(PointRec) == 96, "PointRec layout");
+static_assert(sizeof(PointRec) == 112, "PointRec layout");
 
 // Model variant as one compile-time code: the reference's macros
 // (makefile:3-36) as bit flags.  TORQUE2 implies TORQUE and RECT implies BC
@@ -315,6 +317,7 @@ typedef __attribute__((address_space(4))) const PointRec* PointPtr;
 // Per-model constants of the brightness.
 struct ModelConsts {
   double ib, ic, bcp, kappa, sR, cR, stR, ctR;
+  double sPR, cPR;  // ROTATE: sin, cos of psi_R
 };
 
 #define AST_AI __device__ __attribute__((always_inline)) inline
@@ -330,6 +333,63 @@ AST_AI void axes_at(const SpinFrame<double>& Fd, double phi, double psi, const d
   if (TQ) { sp = sc[2]; cp = sc[3]; }
   else sincos_e<MX>(psi, &sp, &cp);
   body_axes<double>(Fd, sf, cf, st, ctt, sp, cp, a, b, c);
+}
+
+// ROTATE (cuda.c:475-516): the brightness ellipsoid's axes turned by phi_R,
+// theta_R, psi_R about the kinematic ones, by the same construction as the
+// body axes with (b, c, a) in place of the spin frame; the first two steps
+// are also those of the BW_BALL axis (bw_arg_f64).
+AST_AI void rotate_axes(double* a, double* b, double* c, const ModelConsts& K) {
+  const double N_x = __builtin_fma(c[0], K.sR, b[0] * K.cR);
+  const double N_y = __builtin_fma(c[1], K.sR, b[1] * K.cR);
+  const double N_z = __builtin_fma(c[2], K.sR, b[2] * K.cR);
+  const double p_x = __builtin_fma(-N_z, a[1], N_y * a[2]);
+  const double p_y = __builtin_fma(-N_x, a[2], N_z * a[0]);
+  const double p_z = __builtin_fma(-N_y, a[0], N_x * a[1]);
+  a[0] = __builtin_fma(p_x, K.stR, a[0] * K.ctR);
+  a[1] = __builtin_fma(p_y, K.stR, a[1] * K.ctR);
+  a[2] = __builtin_fma(p_z, K.stR, a[2] * K.ctR);
+  const double w_x = __builtin_fma(-a[2], N_y, a[1] * N_z);
+  const double w_y = __builtin_fma(-a[0], N_z, a[2] * N_x);
+  const double w_z = __builtin_fma(-a[1], N_x, a[0] * N_y);
+  b[0] = __builtin_fma(w_x, K.sPR, N_x * K.cPR);
+  b[1] = __builtin_fma(w_y, K.sPR, N_y * K.cPR);
+  b[2] = __builtin_fma(w_z, K.sPR, N_z * K.cPR);
+  c[0] = __builtin_fma(-a[2], b[1], a[1] * b[2]);
+  c[1] = __builtin_fma(-a[0], b[2], a[2] * b[0]);
+  c[2] = __builtin_fma(-a[1], b[0], a[0] * b[1]);
+}
+
+// RECT (cuda.c:609-661): magnitude of a rectangular prism of half-lengths
+// 1, bL, cL along a, b, c at phase 0, as the reference writes it: each axis
+// (b and c scaled by their lengths) is crossed with the vector whose x, y, z
+// components are E's projections on b, c, a, and the three face terms are
+// the norms of the pairwise cross products of those.
+AST_AI double rect_mag(const double* a, const double* b, const double* c, const double* E, double bL, double cL) {
+  const double Eb = dot3(b[0], b[1], b[2], E[0], E[1], E[2]);
+  const double Ec = dot3(c[0], c[1], c[2], E[0], E[1], E[2]);
+  const double Ea = dot3(a[0], a[1], a[2], E[0], E[1], E[2]);
+  const double bx = bL * b[0], by = bL * b[1], bz = bL * b[2];
+  const double cx = cL * c[0], cy = cL * c[1], cz = cL * c[2];
+  const double ap_x = __builtin_fma(-a[2], Ec, a[1] * Ea), ap_y = __builtin_fma(-a[0], Ea, a[2] * Eb),
+               ap_z = __builtin_fma(-a[1], Eb, a[0] * Ec);
+  const double bp_x = __builtin_fma(-bz, Ec, by * Ea), bp_y = __builtin_fma(-bx, Ea, bz * Eb),
+               bp_z = __builtin_fma(-by, Eb, bx * Ec);
+  const double cp_x = __builtin_fma(-cz, Ec, cy * Ea), cp_y = __builtin_fma(-cx, Ea, cz * Eb),
+               cp_z = __builtin_fma(-cy, Eb, cx * Ec);
+  const double ab_x = __builtin_fma(-ap_z, bp_y, ap_y * bp_z), ab_y = __builtin_fma(-ap_x, bp_z, ap_z * bp_x),
+               ab_z = __builtin_fma(-ap_y, bp_x, ap_x * bp_y);
+  const double ac_x = __builtin_fma(-ap_z, cp_y, ap_y * cp_z), ac_y = __builtin_fma(-ap_x, cp_z, ap_z * cp_x),
+               ac_z = __builtin_fma(-ap_y, cp_x, ap_x * cp_y);
+  const double bc_x = __builtin_fma(-bp_z, cp_y, bp_y * cp_z), bc_y = __builtin_fma(-bp_x, cp_z, bp_z * cp_x),
+               bc_z = __builtin_fma(-bp_y, cp_x, bp_x * cp_y);
+  double sab = dot3(ab_x, ab_y, ab_z, ab_x, ab_y, ab_z);
+  double sac = dot3(ac_x, ac_y, ac_z, ac_x, ac_y, ac_z);
+  double sbc = dot3(bc_x, bc_y, bc_z, bc_x, bc_y, bc_z);
+  if (sab < 0.0) sab = 0.0;
+  if (sbc < 0.0) sbc = 0.0;
+  if (sac < 0.0) sac = 0.0;
+  return -2.5 * astm::log10(4.0 * ((__builtin_sqrt(sab) + __builtin_sqrt(sbc)) + __builtin_sqrt(sac)));
 }
 
 // Sun and Earth directions in the body frame scaled by the ellipsoid's axes
@@ -548,7 +608,12 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
     y[1] = theta;
     y[2] = psi;
   }
-  ModelConsts K{0, 0, 0, 0, 0, 0, 0, 0};
+  ModelConsts K{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if constexpr (RO) {
+    astm::sincos(params[P::phi_R], &K.sR, &K.cR);
+    astm::sincos(params[P::theta_R], &K.stR, &K.ctR);
+    astm::sincos(params[P::psi_R], &K.sPR, &K.cPR);
+  }
   if (BW) {
     astm::sincos(params[P::phi_R], &K.sR, &K.cR);
     astm::sincos(params[P::theta_R], &K.stR, &K.ctR);
@@ -561,10 +626,43 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
     K.bcp = b * c;
   }
   const float bcpf = (float)K.bcp;  // mixed form
+  double trendA = 0.0, t_split = 0.0;
+  if constexpr (TR) trendA = params[P::A];
+  bool switched = false;
+  if constexpr (T2)  // cuda.c:291: the torque changes at this fraction of the span
+    t_split = __builtin_fma(params[P::Tt], pts[n_data - 1].mjd - pts[0].mjd, pts[0].mjd);
   for (int i = 0; i < n_data; ++i) {
     const auto& pr = pts[i];
-    const int nsteps = pr.nsteps;  // 0 for the first observation
-    const double h = pr.h, hh = pr.hh, h6 = pr.h6;
+    int nsteps = pr.nsteps;  // 0 for the first observation
+    double h = pr.h, hh = pr.hh, h6 = pr.h6;
+    int nsub = 1;
+    double t2_end = 0.0;
+    if constexpr (T2) {
+      // The interval holding t_split runs as two sub-intervals, the second
+      // under the second torque (cuda.c:288-317); the torque then stays.
+      if (i > 0 && !switched && t_split >= pts[i - 1].mjd && t_split < pr.mjd) {
+        const double t1 = pts[i - 1].mjd;
+        nsub = 2;
+        t2_end = pr.mjd;
+        nsteps = (int)((t_split - t1) / pr.ts + 1);
+        h = (t_split - t1) / nsteps;
+        hh = 0.5 * h;
+        h6 = 1 / 6.0 * h;
+      }
+    }
+    for (int isub = 0; isub < nsub; ++isub) {
+    if constexpr (T2) {
+      if (isub == 1) {
+        nsteps = (int)((t2_end - t_split) / pr.ts + 1);
+        h = (t2_end - t_split) / nsteps;
+        hh = 0.5 * h;
+        h6 = 1 / 6.0 * h;
+        mu[3] = params[P::T2i];
+        mu[4] = params[P::T2s];
+        mu[5] = params[P::T2l];
+        switched = true;
+      }
+    }
     for (int l = 0; l < nsteps; ++l) {  // RK4, cuda.c:348-372
       double f[6], K4[6], s[6];
       ode<TQ, true>(y, K4, mu, mu02, sc);
@@ -586,13 +684,17 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
         astm::sincos(y[1], &sc[0], &sc[1]);
       }
     }
+    }  // sub-intervals
     if (TQ) { phi = y[3]; theta = y[4]; psi = y[5]; }
     else { phi = y[0]; theta = y[1]; psi = y[2]; }
     const double E[3] = {pr.E[0], pr.E[1], pr.E[2]};
     double a[3], b[3], c[3];
     axes_at<TQ, MX>(Fd, phi, psi, sc, a, b, c);
+    if constexpr (RO) rotate_axes(a, b, c, K);
     double Vmod;
-    if (BW) {
+    if constexpr (RC) {
+      Vmod = rect_mag(a, b, c, E, params[P::b], params[P::c]);
+    } else if (BW) {
       const double arg = bw_arg_f64(a, b, c, E, K);
       Vmod = MX ? M2P5_LOG10_2 * log2_f32x((float)arg) : -2.5 * astm::log10(arg);
     } else {
@@ -604,6 +706,7 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
         Vmod = ls_tail_f64(u, K.bcp);
       }
     }
+    if constexpr (TR) Vmod = __builtin_fma(-trendA, pr.alpha, Vmod);  // cuda.c:692-697
     if constexpr (FORM == FORM_PLOT) {
       vplot[(size_t)i * vstride] = Vmod + dv0;
       continue;

@@ -1,6 +1,7 @@
 // ast_tables.cpp -- ast_default_model_space(): the parameter tables the
 // reference's main() builds for a variant (asteroid.c:86-169 Property and
 // Types, asteroid.c:473-564 hard-coded limits, asteroid.c:599-600 P_psi).
+// The row budget is AST_MAX_PARAMS; variants beyond it are not built.
 #include <cmath>
 #include <cstring>
 
@@ -11,6 +12,7 @@ extern "C" int ast_default_model_space(const ast_variant* v, const double* ppsi_
   if (!v || !n_params || !property || !n_types || !types || !limits || !x2) return AST_E_ARG;
   ast_type_ids T;
   ast_variant_type_ids(v, &T);
+  if (T.n_params > AST_MAX_PARAMS || T.n_types > AST_MAX_TYPES) return AST_E_UNSUPPORTED;
   const double PI = 3.141592653589793238, RAD = 180.0 / PI, Tmax = 10.0;
   int rows[AST_MAX_PARAMS][AST_N_COLUMNS];
   int n = 0;
@@ -26,49 +28,69 @@ extern "C" int ast_default_model_space(const ast_variant* v, const double* ppsi_
   add(T.theta_M, 1, 0, AST_HARD_BOTH);
   add(T.phi_M, 1, 0, AST_PERIODIC);
   add(T.phi_0, 1, 0, AST_PERIODIC);
-  if (v->torque) {
+  if (v->trend) add(T.A, 1, 1, AST_HARD_BOTH);
+  if (v->torque || v->torque2) {
     add(T.Ti, 1, 0, AST_SOFT_BOTH);
     add(T.Ts, 1, 0, AST_SOFT_BOTH);
     add(T.Tl, 1, 0, AST_SOFT_BOTH);
+  }
+  if (v->torque2) {
+    add(T.T2i, 1, 0, AST_SOFT_BOTH);
+    add(T.T2s, 1, 0, AST_SOFT_BOTH);
+    add(T.T2l, 1, 0, AST_SOFT_BOTH);
+    add(T.Tt, 1, 0, AST_HARD_BOTH);
   }
   add(T.c_tumb, 1, 1, AST_HARD_RIGHT);
   add(T.b_tumb, 0, 1, AST_HARD_BOTH);
   add(T.Es, 0, 0, AST_HARD_BOTH);
   add(T.L, 1, 0, AST_HARD_LEFT);
   add(T.psi_0, 0, 0, AST_PERIODIC_LAM);
-  if (v->bc) {
+  if (v->bc || v->rect) {
     add(T.c, 1, 1, AST_HARD_RIGHT);
     add(T.b, 0, 1, AST_HARD_BOTH);
   }
-  if (v->bw_ball) {
+  if (v->rotate || v->bw_ball) {
     add(T.theta_R, 1, 1, AST_HARD_BOTH);
     add(T.phi_R, 1, 1, AST_PERIODIC);
-    add(T.kappa, 1, 1, AST_SOFT_BOTH);
   }
+  if (v->rotate) add(T.psi_R, 1, 1, AST_PERIODIC);
+  if (v->bw_ball) add(T.kappa, 1, 1, AST_SOFT_BOTH);
   const int nt = T.n_types;
   double lo[AST_MAX_TYPES] = {0}, hi[AST_MAX_TYPES] = {0};
   lo[T.theta_M] = 0.001 / RAD;
   hi[T.theta_M] = 179.999 / RAD;
   lo[T.L] = 48.0 * PI / 10;
   hi[T.L] = 48.0 * PI / 0.1;
-  if (v->torque) {
+  if (v->trend) {  // asteroid.c:498-503
+    lo[T.A] = -5;
+    hi[T.A] = 5;
+  }
+  if (v->torque || v->torque2) {
     lo[T.Ti] = -Tmax; hi[T.Ti] = Tmax;
     lo[T.Ts] = -Tmax; hi[T.Ts] = Tmax;
     lo[T.Tl] = -Tmax; hi[T.Tl] = Tmax;
+  }
+  if (v->torque2) {  // asteroid.c:515-525
+    lo[T.T2i] = -Tmax; hi[T.T2i] = Tmax;
+    lo[T.T2s] = -Tmax; hi[T.T2s] = Tmax;
+    lo[T.T2l] = -Tmax; hi[T.T2l] = Tmax;
+    lo[T.Tt] = 0.001; hi[T.Tt] = 0.999;
   }
   lo[T.c_tumb] = std::log(0.01);
   hi[T.c_tumb] = std::log(1.0);
   lo[T.b_tumb] = 0;
   hi[T.b_tumb] = 1;
-  if (v->bc) {
+  if (v->bc || v->rect) {
     lo[T.c] = lo[T.c_tumb];
     hi[T.c] = hi[T.c_tumb];
     lo[T.b] = 0;
     hi[T.b] = 1;
   }
-  if (v->bw_ball) {
+  if (v->rotate || v->bw_ball) {
     lo[T.theta_R] = 0.001 / RAD;
     hi[T.theta_R] = 179.999 / RAD;
+  }
+  if (v->bw_ball) {
     lo[T.kappa] = std::log(0.01);
     hi[T.kappa] = std::log(0.1);
   }

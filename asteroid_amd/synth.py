@@ -10,10 +10,12 @@ external files, README.md:11-41), so every workload here is generated:
 * magnitudes of a ground-truth model plus Gaussian noise.
 
 The ground-truth magnitudes come from an independent numpy model of the
-same physics (Euler's equations with constant torque integrated by
-scipy's DOP853 at tight tolerance, body axes from a z-x-z Euler rotation
-matrix, Lommel-Seeliger ellipsoid of Muinonen & Lumme 2015 or the
-black-and-white ball), not from the fitting code itself.
+same physics (Euler's equations with constant or piecewise-constant torque
+integrated by scipy's DOP853 at tight tolerance, body axes from a z-x-z
+Euler rotation matrix, the brightness frame turned by a second one, the
+Lommel-Seeliger ellipsoid of Muinonen & Lumme 2015, the black-and-white
+ball, or the projected area of a rectangular prism, de-trended by the phase
+angle), not from the fitting code itself.
 """
 import numpy as np
 
@@ -147,10 +149,20 @@ def truth_state(variant, params, ids, t, rtol=1e-12, atol=1e-12):
     theta0 = np.arcsin(np.sqrt((Es - 1) / (np.sin(psi0) ** 2 * (1 / Ii - 1 / Is) + 1 / Is - 1)))
     y0 = [L / Ii * np.sin(theta0) * np.sin(psi0), L / Is * np.sin(theta0) * np.cos(psi0),
           L * np.cos(theta0), p["phi_0"], theta0, psi0]
-    T = [p["Ti"], p["Ts"], p["Tl"]] if variant.torque else [0.0, 0.0, 0.0]
-    sol = solve_ivp(_euler_rhs(Ii, Is, T), (t[0], t[-1]), y0, method="DOP853", t_eval=t,
-                    rtol=rtol, atol=atol)
-    return sol.y  # (6, n)
+    T = [p["Ti"], p["Ts"], p["Tl"]] if (variant.torque or variant.torque2) else [0.0, 0.0, 0.0]
+    if not variant.torque2:
+        sol = solve_ivp(_euler_rhs(Ii, Is, T), (t[0], t[-1]), y0, method="DOP853", t_eval=t,
+                        rtol=rtol, atol=atol)
+        return sol.y  # (6, n)
+    # TORQUE2: the second torque from t_split = t0 + Tt (t_last - t0) on
+    t_split = t[0] + p["Tt"] * (t[-1] - t[0])
+    before = t[t < t_split]
+    sol1 = solve_ivp(_euler_rhs(Ii, Is, T), (t[0], t_split), y0, method="DOP853",
+                     t_eval=np.append(before, t_split), rtol=rtol, atol=atol)
+    T2 = [p["T2i"], p["T2s"], p["T2l"]]
+    sol2 = solve_ivp(_euler_rhs(Ii, Is, T2), (t_split, t[-1]), sol1.y[:, -1], method="DOP853",
+                     t_eval=t[t >= t_split], rtol=rtol, atol=atol)
+    return np.hstack([sol1.y[:, :-1], sol2.y])
 
 
 def _param_index(ids, variant):
@@ -172,16 +184,33 @@ def truth_vmod(variant, params, t, E, S):
     YM = np.cross(M, XM)
     frame = np.stack([XM, YM, M], axis=1)         # M-frame basis as columns
     B = frame[None] @ _zxz(phi, theta, psi)       # columns: b, c, a in inertial axes
+    if variant.rotate:                            # the brightness frame, turned in the body frame
+        B = B @ _zxz(np.array(p["phi_R"]), np.array(p["theta_R"]), np.array(p["psi_R"]))
     Eb = np.einsum("nij,ni->nj", B, E)            # (Ep_b, Ep_c, Ep_a)
     Sb = np.einsum("nij,ni->nj", B, S)
+    trend = 0.0
+    if variant.trend:                             # de-trending by the phase angle
+        trend = -p["A"] * np.arccos(np.clip(np.sum(E * S, axis=1), -1, 1))
     if variant.bw_ball:
         stR, ctR = np.sin(p["theta_R"]), np.cos(p["theta_R"])
         sfR, cfR = np.sin(p["phi_R"]), np.cos(p["phi_R"])
         a_rot = np.array([stR * sfR, -stR * cfR, ctR])
         cz = Eb @ a_rot
-        return -2.5 * np.log10(0.5 * (p["kappa"] * (1 + cz) + 0.5 * (1 - cz)))
-    b = p["b"] if variant.bc else p["b_tumb"]
-    c = p["c"] if variant.bc else p["c_tumb"]
+        return -2.5 * np.log10(0.5 * (p["kappa"] * (1 + cz) + 0.5 * (1 - cz))) + trend
+    bc = variant.bc or variant.rect
+    b = p["b"] if bc else p["b_tumb"]
+    c = p["c"] if bc else p["c_tumb"]
+    if variant.rect:
+        # The reference crosses the inertial axes with Et = (E.b, E.c, E.a) read
+        # as an inertial vector; for unit a, b, c = a x b and any Et,
+        # |(u x Et) x (v x Et)| = |Et| |Et.(u x v)|, so the three face terms are
+        # b |Et.c|, c |Et.b| and b c |Et.a| (|Et| = 1): the projected area of
+        # the prism when the frames coincide.
+        Et = Eb
+        fa = np.abs(np.einsum("ni,ni->n", Et, B[:, :, 2]))
+        fb = np.abs(np.einsum("ni,ni->n", Et, B[:, :, 0]))
+        fc = np.abs(np.einsum("ni,ni->n", Et, B[:, :, 1]))
+        return -2.5 * np.log10(4 * (b * fc + c * fb + b * c * fa)) + trend
     Cm = np.array([1 / b ** 2, 1 / c ** 2, 1.0])
     sS = np.sqrt((Sb * Sb * Cm).sum(1))
     sE = np.sqrt((Eb * Eb * Cm).sum(1))
@@ -191,7 +220,7 @@ def truth_vmod(variant, params, t, E, S):
     lam = np.arctan2(sE * np.sin(alpha), sS + sE * ca)
     f = np.cos(lam - alpha) + np.cos(lam) + np.sin(lam) * np.sin(lam - alpha) * np.log(
         1 / np.tan(lam / 2) / np.tan((alpha - lam) / 2))
-    return -2.5 * np.log10(b * c * sS * sE / sc * f)
+    return -2.5 * np.log10(b * c * sS * sE / sc * f) + trend
 
 
 def random_truth(rng, variant, ppsi_hours=(4.0, 10.0), torque_scale=0.2):
@@ -223,15 +252,24 @@ def random_truth(rng, variant, ppsi_hours=(4.0, 10.0), torque_scale=0.2):
     p[idx["Es"]] = Es
     p[idx["L"]] = 48 * np.pi / period
     p[idx["psi_0"]] = psi0
-    if variant.torque:
+    if variant.torque or variant.torque2:
         for k in ("Ti", "Ts", "Tl"):
             p[idx[k]] = rng.normal(scale=torque_scale)
-    if variant.bc:
+    if variant.torque2:
+        for k in ("T2i", "T2s", "T2l"):
+            p[idx[k]] = rng.normal(scale=torque_scale)
+        p[idx["Tt"]] = rng.uniform(0.2, 0.8)
+    if variant.trend:
+        p[idx["A"]] = rng.uniform(-1.0, 1.0)
+    if variant.bc or variant.rect:
         p[idx["c"]] = c_t * np.exp(rng.uniform(-0.2, 0.2))
         p[idx["b"]] = min(1.0, b_t * np.exp(rng.uniform(-0.2, 0.2)))
-    if variant.bw_ball:
+    if variant.bw_ball or variant.rotate:
         p[idx["theta_R"]] = rng.uniform(0.3, 2.8)
         p[idx["phi_R"]] = rng.uniform(0, 2 * np.pi)
+    if variant.rotate:
+        p[idx["psi_R"]] = rng.uniform(0, 2 * np.pi)
+    if variant.bw_ball:
         p[idx["kappa"]] = np.exp(rng.uniform(np.log(0.01), np.log(0.1)))
     return p
 

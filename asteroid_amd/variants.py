@@ -18,22 +18,31 @@ TMAX = 10.0                        # asteroid.c:507
 
 
 def make_variant(torque=True, bc=False, bw_ball=False, p_psi=True, interp=True,
-                 precision=abi.PREC_FP64, time_step=1e-2):
-    return abi.Variant(int(torque), int(bc), int(bw_ball), int(p_psi), int(interp), int(precision),
-                       float(time_step))
+                 precision=abi.PREC_FP64, time_step=1e-2, trend=False, torque2=False, rotate=False, rect=False):
+    """The reference's macros; TORQUE2 implies TORQUE and RECT implies BC (asteroid.h:32-38)."""
+    return abi.Variant(int(torque or torque2), int(bc or rect), int(bw_ball), int(p_psi), int(interp),
+                       int(precision), float(time_step), int(trend), int(torque2), int(rotate), int(rect))
 
 
 def param_positions(variant):
     """Position of each named parameter in the parameter vector (the Property
     order, asteroid.c:86-125); type ids (abi.type_ids) are a different order."""
     order = ["theta_M", "phi_M", "phi_0"]
-    if variant.torque:
+    if variant.trend:
+        order += ["A"]
+    if variant.torque or variant.torque2:
         order += ["Ti", "Ts", "Tl"]
+    if variant.torque2:
+        order += ["T2i", "T2s", "T2l", "Tt"]
     order += ["c_tumb", "b_tumb", "Es", "L", "psi_0"]
-    if variant.bc:
+    if variant.bc or variant.rect:
         order += ["c", "b"]
+    if variant.rotate or variant.bw_ball:
+        order += ["theta_R", "phi_R"]
+    if variant.rotate:
+        order += ["psi_R"]
     if variant.bw_ball:
-        order += ["theta_R", "phi_R", "kappa"]
+        order += ["kappa"]
     return {n: i for i, n in enumerate(order)}
 
 
@@ -102,9 +111,14 @@ def default_model_space(variant, ppsi_hours=(2.0, 4800.0), reopt=0, frozen=None,
         (T["phi_M"], 1, 0, 0, 0, abi.PERIODIC),
         (T["phi_0"], 1, 0, 0, 0, abi.PERIODIC),
     ]
-    if variant.torque:
+    if variant.trend:
+        rows += [(T["A"], 1, 0, 0, 1, abi.HARD_BOTH)]
+    if variant.torque or variant.torque2:
         rows += [(T["Ti"], 1, 0, 0, 0, abi.SOFT_BOTH), (T["Ts"], 1, 0, 0, 0, abi.SOFT_BOTH),
                  (T["Tl"], 1, 0, 0, 0, abi.SOFT_BOTH)]
+    if variant.torque2:
+        rows += [(T["T2i"], 1, 0, 0, 0, abi.SOFT_BOTH), (T["T2s"], 1, 0, 0, 0, abi.SOFT_BOTH),
+                 (T["T2l"], 1, 0, 0, 0, abi.SOFT_BOTH), (T["Tt"], 1, 0, 0, 0, abi.HARD_BOTH)]
     rows += [
         (T["c_tumb"], 1, 0, 0, 1, abi.HARD_RIGHT),
         (T["b_tumb"], 0, 0, 0, 1, abi.HARD_BOTH),
@@ -112,11 +126,14 @@ def default_model_space(variant, ppsi_hours=(2.0, 4800.0), reopt=0, frozen=None,
         (T["L"], 1, 0, 0, 0, abi.HARD_LEFT),
         (T["psi_0"], 0, 0, 0, 0, abi.PERIODIC_LAM),
     ]
-    if variant.bc:
+    if variant.bc or variant.rect:
         rows += [(T["c"], 1, 0, 0, 1, abi.HARD_RIGHT), (T["b"], 0, 0, 0, 1, abi.HARD_BOTH)]
+    if variant.rotate or variant.bw_ball:
+        rows += [(T["theta_R"], 1, 0, 0, 1, abi.HARD_BOTH), (T["phi_R"], 1, 0, 0, 1, abi.PERIODIC)]
+    if variant.rotate:
+        rows += [(T["psi_R"], 1, 0, 0, 1, abi.PERIODIC)]
     if variant.bw_ball:
-        rows += [(T["theta_R"], 1, 0, 0, 1, abi.HARD_BOTH), (T["phi_R"], 1, 0, 0, 1, abi.PERIODIC),
-                 (T["kappa"], 1, 0, 0, 1, abi.SOFT_BOTH)]
+        rows += [(T["kappa"], 1, 0, 0, 1, abi.SOFT_BOTH)]
     rows = [list(r) for r in rows]
     nt = T["n_types"]
     types = [-1] * nt
@@ -126,16 +143,23 @@ def default_model_space(variant, ppsi_hours=(2.0, 4800.0), reopt=0, frozen=None,
     hi = [0.0] * nt
     lo[T["theta_M"]], hi[T["theta_M"]] = 0.001 / RAD, 179.999 / RAD
     lo[T["L"]], hi[T["L"]] = 48.0 * PI / 10, 48.0 * PI / 0.1
-    if variant.torque:
+    if variant.trend:
+        lo[T["A"]], hi[T["A"]] = -5.0, 5.0                   # asteroid.c:498-503
+    if variant.torque or variant.torque2:
         for n in ("Ti", "Ts", "Tl"):
             lo[T[n]], hi[T[n]] = -TMAX, TMAX
+    if variant.torque2:                                      # asteroid.c:515-525
+        for n in ("T2i", "T2s", "T2l"):
+            lo[T[n]], hi[T[n]] = -TMAX, TMAX
+        lo[T["Tt"]], hi[T["Tt"]] = 0.001, 0.999
     lo[T["c_tumb"]], hi[T["c_tumb"]] = math.log(0.01), math.log(1.0)
     lo[T["b_tumb"]], hi[T["b_tumb"]] = 0.0, 1.0
-    if variant.bc:
+    if variant.bc or variant.rect:
         lo[T["c"]], hi[T["c"]] = lo[T["c_tumb"]], hi[T["c_tumb"]]
         lo[T["b"]], hi[T["b"]] = 0.0, 1.0
-    if variant.bw_ball:
+    if variant.rotate or variant.bw_ball:
         lo[T["theta_R"]], hi[T["theta_R"]] = 0.001 / RAD, 179.999 / RAD
+    if variant.bw_ball:
         lo[T["kappa"]], hi[T["kappa"]] = math.log(0.01), math.log(0.1)
     for itype, value in (frozen or {}).items():
         for r in rows:
@@ -149,3 +173,23 @@ def default_model_space(variant, ppsi_hours=(2.0, 4800.0), reopt=0, frozen=None,
     ppsi1 = np.float32(24.0 / ppsi_hours[1])   # asteroid.c:599-600
     ppsi2 = np.float32(24.0 / ppsi_hours[0])
     return ModelSpace(variant, rows, types, lo, hi, reopt, ppsi1, ppsi2)
+
+
+def ls_torque_trend(precision=abi.PREC_FP64, time_step=1e-2, interp=True):
+    """LS ellipsoid with torque, de-trended by the phase angle (-DTREND), 12 parameters."""
+    return make_variant(True, False, False, True, interp, precision, time_step, trend=True)
+
+
+def ls_torque2(precision=abi.PREC_FP64, time_step=1e-2, interp=True):
+    """LS ellipsoid with a torque that changes at the split time (-DTORQUE2), 15 parameters."""
+    return make_variant(True, False, False, True, interp, precision, time_step, torque2=True)
+
+
+def bc_rotate(precision=abi.PREC_FP64, time_step=1e-2, interp=True, torque=False):
+    """BC with the brightness ellipsoid rotated (-DBC -DROTATE), 13 (16 with torque) parameters."""
+    return make_variant(torque, True, False, True, interp, precision, time_step, rotate=True)
+
+
+def rect(precision=abi.PREC_FP64, time_step=1e-2, interp=True, torque=False):
+    """Rectangular prism brightness (-DRECT), 10 (13 with torque) parameters."""
+    return make_variant(torque, True, False, True, interp, precision, time_step, rect=True)

@@ -64,7 +64,11 @@ enum {
 };
 
 /* Model variant: the compile-time macros of the reference (makefile:3-36),
- * chosen at run time here.  TREND, ROTATE, RECT, TORQUE2, SEGMENT, P_PHI,
+ * chosen at run time here.  TORQUE2 implies TORQUE and RECT implies BC, as in
+ * the reference (asteroid.h:32-38).  The kernels are built for LS, BC and
+ * BW_BALL with and without torque, and for TREND with torque, TORQUE2,
+ * ROTATE (with BC, with and without torque) and RECT (with and without
+ * torque); other combinations return AST_E_UNSUPPORTED.  SEGMENT, P_PHI,
  * P_BOTH and NUDGE are not built. */
 typedef struct {
   int torque;     /* -DTORQUE : 6-ODE constant-torque rotation                 */
@@ -74,6 +78,10 @@ typedef struct {
   int interp;     /* -DINTERP : E,S from a 3-point ephemeris                   */
   int precision;  /* AST_PREC_*                                                */
   double time_step; /* TIME_STEP, days (asteroid.h:202)                        */
+  int trend;      /* -DTREND  : magnitude de-trended by A times the phase angle (cuda.c:692-697) */
+  int torque2;    /* -DTORQUE2: a second torque after the split time Tt (cuda.c:288-317)       */
+  int rotate;     /* -DROTATE : brightness ellipsoid rotated by theta_R, phi_R, psi_R (cuda.c:475-516) */
+  int rect;       /* -DRECT   : rectangular prism brightness, phase 0 (cuda.c:609-661)          */
 } ast_variant;
 
 /* Parameter type ids of a variant: the reference's __COUNTER__ order
@@ -83,28 +91,38 @@ typedef struct {
   int theta_M, phi_M, phi_0, L, Ti, Ts, Tl, c_tumb, b_tumb, Es, psi_0;
   int c, b, theta_R, phi_R, kappa;
   int n_types;   /* N_TYPES  */
-  int n_params;  /* N_PARAMS (asteroid.h:110) */
+  int n_params;  /* N_PARAMS (asteroid.h:106-110) */
+  int A;         /* TREND                */
+  int T2i, T2s, T2l, Tt;  /* TORQUE2     */
+  int psi_R;     /* ROTATE               */
 } ast_type_ids;
 
 static inline int ast_variant_type_ids(const ast_variant* v, ast_type_ids* t) {
   int id = 0;
   if (!v || !t) return AST_E_ARG;
+  const int torque = v->torque || v->torque2, bc = v->bc || v->rect;
   t->theta_M = id++;
   t->phi_M = id++;
   t->phi_0 = id++;
   t->L = id++;
+  t->A = v->trend ? id++ : -1;
   t->Ti = t->Ts = t->Tl = -1;
-  if (v->torque) { t->Ti = id++; t->Ts = id++; t->Tl = id++; }
+  if (torque) { t->Ti = id++; t->Ts = id++; t->Tl = id++; }
+  t->T2i = t->T2s = t->T2l = t->Tt = -1;
+  if (v->torque2) { t->T2i = id++; t->T2s = id++; t->T2l = id++; t->Tt = id++; }
   t->c_tumb = id++;
   t->b_tumb = id++;
   t->Es = id++;
   t->psi_0 = id++;
   t->c = t->b = -1;
-  if (v->bc) { t->c = id++; t->b = id++; }
-  t->theta_R = t->phi_R = t->kappa = -1;
-  if (v->bw_ball) { t->theta_R = id++; t->phi_R = id++; t->kappa = id++; }
+  if (bc) { t->c = id++; t->b = id++; }
+  t->theta_R = t->phi_R = t->kappa = t->psi_R = -1;
+  if (v->rotate || v->bw_ball) { t->theta_R = id++; t->phi_R = id++; }
+  if (v->rotate) t->psi_R = id++;
+  if (v->bw_ball) t->kappa = id++;
   t->n_types = id;
-  t->n_params = 8 + (v->bc ? 2 : 0) + (v->torque ? 3 : 0) + (v->bw_ball ? 3 : 0);
+  t->n_params = 8 + (bc ? 2 : 0) + (v->rotate ? 3 : 0) + (v->trend ? 1 : 0) + (v->torque2 ? 7 : (torque ? 3 : 0)) +
+                (v->bw_ball ? 3 : 0);
   return AST_OK;
 }
 

@@ -61,6 +61,9 @@ struct Problem {
   int n_data, n_filters;
   const ast_obs* obs;
   ast_ephem3 eph;
+  // TORQUE2 implies TORQUE and RECT implies BC (asteroid.h:32-38)
+  bool tq() const { return v.torque || v.torque2; }
+  bool bc() const { return v.bc || v.rect; }
 };
 
 // ------------------------------------------------------ elementary functions
@@ -160,7 +163,7 @@ inline void ode_literal(bool torque, const double* y, double* f, const double* m
 inline void model_literal(const Problem& P, const double* params, double* sy2, double* sy,
                           double* sw, ModelOut* out) {
   const ast_type_ids& T = P.T;
-  const bool torque = P.v.torque;
+  const bool torque = P.tq();
   for (int m = 0; m < P.n_filters; ++m) sy2[m] = sy[m] = sw[m] = 0.0;
   double tM = params[P.types[T.theta_M]], pM = params[P.types[T.phi_M]];
   double M_x = std::sin(tM) * std::cos(pM);
@@ -201,11 +204,29 @@ inline void model_literal(const Problem& P, const double* params, double* sy2, d
     mu[2] = Im;
   }
   const int n_ode = torque ? 6 : 3;
+  // TORQUE2: the torque changes at this fraction of the span (cuda.c:288-289)
+  const double t_split = P.v.torque2 ? params[P.types[T.Tt]] * (P.obs[P.n_data - 1].mjd - P.obs[0].mjd) + P.obs[0].mjd
+                                     : 0.0;
   for (int i = 0; i < P.n_data; ++i) {
     if (i > 0) {
+     double t1 = P.obs[i - 1].mjd, t2 = P.obs[i].mjd;
+     // the interval holding t_split runs as two sub-intervals (cuda.c:290-316)
+     const int n_split = (P.v.torque2 && t_split >= t1 && t_split < t2) ? 2 : 1;
+     for (int isplit = 0; isplit < n_split; ++isplit) {
+      if (n_split == 2) {
+        if (isplit == 0) {
+          t2 = t_split;
+        } else {
+          t1 = t_split;
+          t2 = P.obs[i].mjd;
+          mu[3] = params[P.types[T.T2i]];
+          mu[4] = params[P.types[T.T2s]];
+          mu[5] = params[P.types[T.T2l]];
+        }
+      }
       int N_steps;
       double h;
-      rk4_interval(P.obs[i - 1].mjd, P.obs[i].mjd, P.v.time_step, &N_steps, &h);
+      rk4_interval(t1, t2, P.v.time_step, &N_steps, &h);
       double y[6];
       if (torque) { y[0] = Om_i; y[1] = Om_s; y[2] = Om_l; y[3] = phi; y[4] = theta; y[5] = psi; }
       else { y[0] = phi; y[1] = theta; y[2] = psi; }
@@ -222,6 +243,7 @@ inline void model_literal(const Problem& P, const double* params, double* sy2, d
       }
       if (torque) { Om_i = y[0]; Om_s = y[1]; Om_l = y[2]; phi = y[3]; theta = y[4]; psi = y[5]; }
       else { phi = y[0]; theta = y[1]; psi = y[2]; }
+     }
     }
     if (out && out->traj) {
       double* t6 = out->traj + 6 * i;
@@ -249,7 +271,7 @@ inline void model_literal(const Problem& P, const double* params, double* sy2, d
     double c_x = a_y * b_z - a_z * b_y;
     double c_y = a_z * b_x - a_x * b_z;
     double c_z = a_x * b_y - a_y * b_x;
-    if (P.v.bw_ball) {  // cuda.c:475-497
+    if (P.v.bw_ball || P.v.rotate) {  // cuda.c:475-497
       double cp = std::cos(params[P.types[T.phi_R]]), sp = std::sin(params[P.types[T.phi_R]]);
       N_x = b_x * cp + c_x * sp;
       N_y = b_y * cp + c_y * sp;
@@ -262,6 +284,18 @@ inline void model_literal(const Problem& P, const double* params, double* sy2, d
       a_y = a_y * ctR + p_y * stR;
       a_z = a_z * ctR + p_z * stR;
     }
+    if (P.v.rotate) {  // cuda.c:499-516
+      w_x = a_y * N_z - a_z * N_y;
+      w_y = a_z * N_x - a_x * N_z;
+      w_z = a_x * N_y - a_y * N_x;
+      double sR = std::sin(params[P.types[T.psi_R]]), cR = std::cos(params[P.types[T.psi_R]]);
+      b_x = N_x * cR + w_x * sR;
+      b_y = N_y * cR + w_y * sR;
+      b_z = N_z * cR + w_z * sR;
+      c_x = a_y * b_z - a_z * b_y;
+      c_y = a_z * b_x - a_x * b_z;
+      c_z = a_x * b_y - a_y * b_x;
+    }
     double E[3], S[3];
     if (P.v.interp) interp_literal(P.eph, P.obs[i].mjd, E, S);
     else for (int c = 0; c < 3; ++c) { E[c] = P.obs[i].E[c]; S[c] = P.obs[i].S[c]; }
@@ -271,13 +305,29 @@ inline void model_literal(const Problem& P, const double* params, double* sy2, d
     double Sp_b = b_x * S[0] + b_y * S[1] + b_z * S[2];
     double Sp_c = c_x * S[0] + c_y * S[1] + c_z * S[2];
     double Sp_a = a_x * S[0] + a_y * S[1] + a_z * S[2];
-    double b = P.v.bc ? params[P.types[T.b]] : bt;
-    double c = P.v.bc ? params[P.types[T.c]] : ct;
+    double b = P.bc() ? params[P.types[T.b]] : bt;
+    double c = P.bc() ? params[P.types[T.c]] : ct;
     double Vmod;
     if (P.v.bw_ball) {  // cuda.c:603-607
       double cos_zeta = Ep_a;
       double kappa = params[P.types[T.kappa]];
       Vmod = -2.5 * std::log10(0.5 * (kappa * (1 + cos_zeta) + 0.5 * (1 - cos_zeta)));
+    } else if (P.v.rect) {  // cuda.c:609-661
+      double bx = b * b_x, by = b * b_y, bz = b * b_z;
+      double cx = c * c_x, cy = c * c_y, cz = c * c_z;
+      double ap_x = a_y * Ep_a - a_z * Ep_c, ap_y = a_z * Ep_b - a_x * Ep_a, ap_z = a_x * Ep_c - a_y * Ep_b;
+      double bp_x = by * Ep_a - bz * Ep_c, bp_y = bz * Ep_b - bx * Ep_a, bp_z = bx * Ep_c - by * Ep_b;
+      double cp_x = cy * Ep_a - cz * Ep_c, cp_y = cz * Ep_b - cx * Ep_a, cp_z = cx * Ep_c - cy * Ep_b;
+      double ab_x = ap_y * bp_z - ap_z * bp_y, ab_y = ap_z * bp_x - ap_x * bp_z, ab_z = ap_x * bp_y - ap_y * bp_x;
+      double ac_x = ap_y * cp_z - ap_z * cp_y, ac_y = ap_z * cp_x - ap_x * cp_z, ac_z = ap_x * cp_y - ap_y * cp_x;
+      double bc_x = bp_y * cp_z - bp_z * cp_y, bc_y = bp_z * cp_x - bp_x * cp_z, bc_z = bp_x * cp_y - bp_y * cp_x;
+      double ab = ab_x * ab_x + ab_y * ab_y + ab_z * ab_z;
+      double ac = ac_x * ac_x + ac_y * ac_y + ac_z * ac_z;
+      double bcs = bc_x * bc_x + bc_y * bc_y + bc_z * bc_z;
+      if (ab < 0.0) ab = 0.0;
+      if (bcs < 0.0) bcs = 0.0;
+      if (ac < 0.0) ac = 0.0;
+      Vmod = -2.5 * std::log10(4 * (std::sqrt(ab) + std::sqrt(bcs) + std::sqrt(ac)));
     } else {            // cuda.c:666-689
       double scalar_Sun = std::sqrt(Sp_b * Sp_b / (b * b) + Sp_c * Sp_c / (c * c) + Sp_a * Sp_a);
       double scalar_Earth = std::sqrt(Ep_b * Ep_b / (b * b) + Ep_c * Ep_c / (c * c) + Ep_a * Ep_a);
@@ -292,6 +342,10 @@ inline void model_literal(const Problem& P, const double* params, double* sy2, d
                                (std::cos(lambda_p - alpha_p) + cos_lambda_p +
                                 sin_lambda_p * std::sin(lambda_p - alpha_p) *
                                     std::log(1.0 / std::tan(0.5 * lambda_p) / std::tan(0.5 * (alpha_p - lambda_p)))));
+    }
+    if (P.v.trend) {  // cuda.c:692-697: de-trending by the solar phase angle
+      double alpha = std::acos(Sp_b * Ep_b + Sp_c * Ep_c + Sp_a * Ep_a);
+      Vmod = Vmod - params[P.types[T.A]] * alpha;
     }
     if (out && out->vmod) out->vmod[i] = Vmod;
     int m = P.obs[i].filter;  // cuda.c:708-714
@@ -342,7 +396,7 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
   using astm::p_sqrt;
   using astm::p_val;
   const ast_type_ids& T = P.T;
-  const bool torque = P.v.torque;
+  const bool torque = P.tq();
   for (int m = 0; m < P.n_filters; ++m) sy2[m] = sy[m] = sw[m] = R(0.0);
   R sTM, cTM, sPM, cPM;
   astm::sincos(R(params[P.types[T.theta_M]]), &sTM, &cTM);
@@ -394,23 +448,48 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
   // Photometric shape (LS) or ball constants (BW_BALL).
   R ib = R(0.0), ic = R(0.0), bcp = R(0.0), kappa = R(0.0);
   R sR = R(0.0), cR = R(0.0), stR = R(0.0), ctR = R(0.0);
+  R sPR = R(0.0), cPR = R(0.0);
+  if (P.v.rotate) {
+    astm::sincos(R(params[P.types[T.phi_R]]), &sR, &cR);
+    astm::sincos(R(params[P.types[T.theta_R]]), &stR, &ctR);
+    astm::sincos(R(params[P.types[T.psi_R]]), &sPR, &cPR);
+  }
   if (P.v.bw_ball) {
     astm::sincos(R(params[P.types[T.phi_R]]), &sR, &cR);
     astm::sincos(R(params[P.types[T.theta_R]]), &stR, &ctR);
     kappa = R(params[P.types[T.kappa]]);
   } else {
-    R b = P.v.bc ? R(params[P.types[T.b]]) : bt;
-    R c = P.v.bc ? R(params[P.types[T.c]]) : ct;
+    R b = P.bc() ? R(params[P.types[T.b]]) : bt;
+    R c = P.bc() ? R(params[P.types[T.c]]) : ct;
     ib = R(1.0) / b;
     ic = R(1.0) / c;
     bcp = b * c;
   }
   const int n_ode = torque ? 6 : 3;
+  // TORQUE2: split time (cuda.c:288-289); the interval holding it runs as two
+  // sub-intervals, the second and every later one under the second torque.
+  const double t_split = P.v.torque2 ? std::fma(params[P.types[T.Tt]], P.obs[P.n_data - 1].mjd - P.obs[0].mjd,
+                                                P.obs[0].mjd)
+                                     : 0.0;
   for (int i = 0; i < P.n_data; ++i) {
     if (i > 0) {
+     double t1 = P.obs[i - 1].mjd, t2 = P.obs[i].mjd;
+     const int n_split = (P.v.torque2 && t_split >= t1 && t_split < t2) ? 2 : 1;
+     for (int isplit = 0; isplit < n_split; ++isplit) {
+      if (n_split == 2) {
+        if (isplit == 0) {
+          t2 = t_split;
+        } else {
+          t1 = t_split;
+          t2 = P.obs[i].mjd;
+          mu[3] = R(params[P.types[T.T2i]]);
+          mu[4] = R(params[P.types[T.T2s]]);
+          mu[5] = R(params[P.types[T.T2l]]);
+        }
+      }
       int N_steps;
       double hd;
-      rk4_interval(P.obs[i - 1].mjd, P.obs[i].mjd, P.v.time_step, &N_steps, &hd);
+      rk4_interval(t1, t2, P.v.time_step, &N_steps, &hd);
       R h = R(hd), hh = R(0.5 * hd), h6 = R(1 / 6.0 * hd);
       R y[6];
       if (torque) { y[0] = Om_i; y[1] = Om_s; y[2] = Om_l; y[3] = phi; y[4] = theta; y[5] = psi; }
@@ -434,6 +513,7 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
       }
       if (torque) { Om_i = y[0]; Om_s = y[1]; Om_l = y[2]; phi = y[3]; theta = y[4]; psi = y[5]; }
       else { phi = y[0]; theta = y[1]; psi = y[2]; }
+     }
     }
     if (out && out->traj) {
       double* t6 = out->traj + 6 * i;
@@ -463,6 +543,26 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
     R c_x = p_fma(-a_z, b_y, a_y * b_z);
     R c_y = p_fma(-a_x, b_z, a_z * b_x);
     R c_z = p_fma(-a_y, b_x, a_x * b_y);
+    if (P.v.rotate) {  // cuda.c:475-516, the body-axes construction on (b, c, a)
+      R n_x = p_fma(c_x, sR, b_x * cR);
+      R n_y = p_fma(c_y, sR, b_y * cR);
+      R n_z = p_fma(c_z, sR, b_z * cR);
+      R q_x = p_fma(-n_z, a_y, n_y * a_z);
+      R q_y = p_fma(-n_x, a_z, n_z * a_x);
+      R q_z = p_fma(-n_y, a_x, n_x * a_y);
+      a_x = p_fma(q_x, stR, a_x * ctR);
+      a_y = p_fma(q_y, stR, a_y * ctR);
+      a_z = p_fma(q_z, stR, a_z * ctR);
+      R v_x = p_fma(-a_z, n_y, a_y * n_z);
+      R v_y = p_fma(-a_x, n_z, a_z * n_x);
+      R v_z = p_fma(-a_y, n_x, a_x * n_y);
+      b_x = p_fma(v_x, sPR, n_x * cPR);
+      b_y = p_fma(v_y, sPR, n_y * cPR);
+      b_z = p_fma(v_z, sPR, n_z * cPR);
+      c_x = p_fma(-a_z, b_y, a_y * b_z);
+      c_y = p_fma(-a_x, b_z, a_z * b_x);
+      c_z = p_fma(-a_y, b_x, a_x * b_y);
+    }
     double Ed[3], Sd[3];
     if (P.v.interp) interp_canon(P.eph, P.obs[i].mjd, Ed, Sd);
     else for (int c = 0; c < 3; ++c) { Ed[c] = P.obs[i].E[c]; Sd[c] = P.obs[i].S[c]; }
@@ -480,6 +580,29 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
       R r_z = p_fma(q_z, stR, a_z * ctR);
       R cz = dot3(r_x, r_y, r_z, E0, E1, E2);
       Vmod = R(-2.5) * astm::log10(R(0.5) * p_fma(kappa, R(1.0) + cz, R(0.5) * (R(1.0) - cz)));
+    } else if (P.v.rect) {  // cuda.c:609-661
+      R bL = R(params[P.types[T.b]]), cL = R(params[P.types[T.c]]);
+      R Eb = dot3(b_x, b_y, b_z, E0, E1, E2);
+      R Ec = dot3(c_x, c_y, c_z, E0, E1, E2);
+      R Ea = dot3(a_x, a_y, a_z, E0, E1, E2);
+      R bx = bL * b_x, by = bL * b_y, bz = bL * b_z;
+      R cx = cL * c_x, cy = cL * c_y, cz = cL * c_z;
+      R ap_x = p_fma(-a_z, Ec, a_y * Ea), ap_y = p_fma(-a_x, Ea, a_z * Eb), ap_z = p_fma(-a_y, Eb, a_x * Ec);
+      R bp_x = p_fma(-bz, Ec, by * Ea), bp_y = p_fma(-bx, Ea, bz * Eb), bp_z = p_fma(-by, Eb, bx * Ec);
+      R cp_x = p_fma(-cz, Ec, cy * Ea), cp_y = p_fma(-cx, Ea, cz * Eb), cp_z = p_fma(-cy, Eb, cx * Ec);
+      R ab_x = p_fma(-ap_z, bp_y, ap_y * bp_z), ab_y = p_fma(-ap_x, bp_z, ap_z * bp_x),
+        ab_z = p_fma(-ap_y, bp_x, ap_x * bp_y);
+      R ac_x = p_fma(-ap_z, cp_y, ap_y * cp_z), ac_y = p_fma(-ap_x, cp_z, ap_z * cp_x),
+        ac_z = p_fma(-ap_y, cp_x, ap_x * cp_y);
+      R bc_x = p_fma(-bp_z, cp_y, bp_y * cp_z), bc_y = p_fma(-bp_x, cp_z, bp_z * cp_x),
+        bc_z = p_fma(-bp_y, cp_x, bp_x * cp_y);
+      R sab = dot3(ab_x, ab_y, ab_z, ab_x, ab_y, ab_z);
+      R sac = dot3(ac_x, ac_y, ac_z, ac_x, ac_y, ac_z);
+      R sbc = dot3(bc_x, bc_y, bc_z, bc_x, bc_y, bc_z);
+      if (p_val(sab) < 0.0) sab = R(0.0);
+      if (p_val(sbc) < 0.0) sbc = R(0.0);
+      if (p_val(sac) < 0.0) sac = R(0.0);
+      Vmod = R(-2.5) * astm::log10(R(4.0) * ((p_sqrt(sab) + p_sqrt(sbc)) + p_sqrt(sac)));
     } else {
       R S0 = R(Sd[0]), S1 = R(Sd[1]), S2 = R(Sd[2]);
       R bs_x = b_x * ib, bs_y = b_y * ib, bs_z = b_z * ib;
@@ -505,6 +628,10 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
       R lg = astm::log(((R(1.0) + cl) * (R(1.0) + cd)) / (sl * (-sd)));
       R inner = p_fma(sl * sd, lg, cd + cl);
       Vmod = R(-2.5) * astm::log10((((bcp * sS) * sE) * isc) * inner);
+    }
+    if (P.v.trend) {  // cuda.c:692-697: acos(S.E) in the inertial frame (a property of the point)
+      const double alpha = astm::acos(std::fma(Sd[2], Ed[2], std::fma(Sd[1], Ed[1], Sd[0] * Ed[0])));
+      Vmod = p_fma(-R(params[P.types[T.A]]), R(alpha), Vmod);
     }
     if (out && out->vmod) out->vmod[i] = p_val(Vmod);
     int m = P.obs[i].filter;
@@ -574,7 +701,7 @@ int x2params(const Problem& P, const CF* x, double* params, int reopt) {
       }
     } else if (t == T.psi_0) {
       params[i] = x[i] * (psi_max - psi_min) + psi_min;
-    } else if (P.v.bc && t == T.b) {
+    } else if (P.bc() && t == T.b) {
       double log_b = log_c * (x[i] * (L1 - L0) + L0);
       if (std::fabs(log_b - log_b_tumb) > BC_DEV_MAX) return 1;
       params[i] = M::exp(log_b);
@@ -585,7 +712,7 @@ int x2params(const Problem& P, const CF* x, double* params, int reopt) {
       if (t == T.c_tumb) {
         log_c_tumb = params[i];
         params[i] = M::exp(log_c_tumb);
-      } else if (P.v.bc && t == T.c) {
+      } else if (P.bc() && t == T.c) {
         log_c = params[i];
         if (std::fabs(log_c - log_c_tumb) > BC_DEV_MAX) return 1;
         params[i] = M::exp(log_c);
@@ -634,7 +761,7 @@ void params2x(const Problem& P, CF* x, const double* params) {
         x[i] = par / (2 * PI);
         x[i] = x[i] - cf_floor(x[i]);
       } else {
-        if (P.v.bc && t == T.c) {
+        if (P.bc() && t == T.c) {
           CF C0 = (CF)P.lim[0][T.c_tumb], C1 = (CF)P.lim[1][T.c_tumb];
           x[i] = (M::log(params[i]) - C0) / (C1 - C0);
           continue;
@@ -660,7 +787,7 @@ void params2x(const Problem& P, CF* x, const double* params) {
           psi_min = -psi_max;
         }
         x[i] = (params[i] - psi_min) / (psi_max - psi_min);
-      } else if (P.v.bc && t == T.b) {
+      } else if (P.bc() && t == T.b) {
         double par = M::log(params[i]) / M::log(params[P.types[T.c]]);
         x[i] = (par - L0) / (L1 - L0);
       }
@@ -715,7 +842,7 @@ CF run_stage(const Problem& P, int form, const RunCfg& C, int reopt, uint64_t fi
   for (int i = 0; i < N; ++i) {  // cuda.c:1459-1508
     float r = rnd();
     int t = P.prop[i][AST_P_TYPE];
-    if (P.v.bc && !reopt) {
+    if (P.bc() && !reopt) {
       if (t == T.c) { S.x[0][i] = S.x[0][P.types[T.c_tumb]]; continue; }
       if (t == T.b) { S.x[0][i] = S.x[0][P.types[T.b_tumb]]; continue; }
     }
@@ -946,13 +1073,13 @@ void rmsd_group(const Problem& P, int form, const RmsdCfg& C, const double* para
   const int iphiM = P.types[T.phi_M], iphi0 = P.types[T.phi_0];
   const double phi_M0 = prm[iphiM], phi_00 = prm[iphi0];
   double phi0 = 0.0;
-  const bool sph = C.spherical_k && P.v.torque;
+  const bool sph = C.spherical_k && P.tq();
   auto moments = [&](const double* p, double* Is, double* Ii) {
     const double bt = p[P.types[T.b_tumb]], ct = p[P.types[T.c_tumb]];
     *Is = (1.0 + bt * bt) / (bt * bt + ct * ct);
     *Ii = (1.0 + ct * ct) / (bt * bt + ct * ct);
   };
-  if (P.v.torque) {
+  if (P.tq()) {
     double Is, Ii;
     moments(prm, &Is, &Ii);
     phi0 = M::atan2(Is * prm[P.types[T.Ts]], Ii * prm[P.types[T.Ti]]);

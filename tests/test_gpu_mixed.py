@@ -27,6 +27,10 @@ CASES = {
     "bw_torque": (lambda p: variants.bw_ball_torque(p), 2, 200, 2.0),
     "ls_plain": (lambda p: variants.make_variant(False, False, False, True, True, p, 1e-2), 1, 200, 2.0),
     "ls_noninterp": (lambda p: variants.ls_torque(p, interp=False), 2, 150, 1.5),
+    "trend": (lambda p: variants.ls_torque_trend(p), 2, 200, 2.0),
+    "torque2": (lambda p: variants.ls_torque2(p), 1, 200, 2.0),
+    "bc_rotate_torque": (lambda p: variants.bc_rotate(p, torque=True), 2, 200, 2.0),
+    "rect_torque": (lambda p: variants.rect(p, torque=True), 1, 200, 2.0),
 }
 
 

@@ -16,6 +16,12 @@ VARIANTS = {
     "bw_torque": (lambda prec: variants.bw_ball_torque(prec), 2, (2.0, 4800.0)),
     "ls_plain": (lambda prec: variants.make_variant(False, False, False, True, True, prec, 1e-2), 1, (2.0, 4800.0)),
     "ls_noninterp": (lambda prec: variants.ls_torque(prec, interp=False), 2, (2.0, 4800.0)),
+    "trend": (lambda prec: variants.ls_torque_trend(prec), 2, (2.0, 4800.0)),
+    "torque2": (lambda prec: variants.ls_torque2(prec), 1, (2.0, 4800.0)),
+    "bc_rotate": (lambda prec: variants.bc_rotate(prec), 2, (2.0, 4800.0)),
+    "bc_rotate_torque": (lambda prec: variants.bc_rotate(prec, torque=True), 1, (2.0, 4800.0)),
+    "rect": (lambda prec: variants.rect(prec), 2, (2.0, 4800.0)),
+    "rect_torque": (lambda prec: variants.rect(prec, torque=True), 1, (2.0, 4800.0)),
 }
 PRECS = {"fp64": abi.PREC_FP64, "f32simplex": abi.PREC_F32SIMPLEX}
 
@@ -52,7 +58,8 @@ def test_chi2_batch_bitwise(name, prec):
 
 @pytest.mark.parametrize("prec", list(PRECS))
 @pytest.mark.parametrize("name,nstages,reopt", [("ls_torque", 2, 0), ("bc", 1, 1), ("bw_torque", 1, 0),
-                                                ("ls_plain", 2, 0)])
+                                                ("ls_plain", 2, 0), ("trend", 1, 0), ("torque2", 1, 0),
+                                                ("bc_rotate_torque", 1, 1), ("rect", 1, 0)])
 def test_fit_trajectories_identical(name, prec, nstages, reopt):
     v, curve, truth, space = _setup(name, prec, n_obs=60, span=1.0, reopt=reopt)
     pb = oracle.Problem(space, curve)

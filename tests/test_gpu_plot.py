@@ -17,6 +17,10 @@ VARIANTS = {
     "bw_torque": lambda p: variants.bw_ball_torque(p),
     "ls_plain": lambda p: variants.make_variant(False, False, False, True, True, p, 1e-2),
     "ls_noninterp": lambda p: variants.ls_torque(p, interp=False),
+    "trend": lambda p: variants.ls_torque_trend(p),
+    "torque2": lambda p: variants.ls_torque2(p),
+    "bc_rotate": lambda p: variants.bc_rotate(p),
+    "rect_torque": lambda p: variants.rect(p, torque=True),
 }
 
 

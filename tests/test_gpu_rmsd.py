@@ -16,6 +16,9 @@ VARIANTS = {
     "bc": lambda p: variants.bc_free(p),
     "bw_torque": lambda p: variants.bw_ball_torque(p),
     "ls_plain": lambda p: variants.make_variant(False, False, False, True, True, p, 1e-2),
+    "torque2": lambda p: variants.ls_torque2(p),
+    "bc_rotate_torque": lambda p: variants.bc_rotate(p, torque=True),
+    "rect": lambda p: variants.rect(p),
 }
 
 

@@ -18,7 +18,15 @@ index, a transposed operand) fails it:
 * the P_psi mapping of x2params produces an L whose integrated psi motion
   has exactly the requested period;
 * delta_V and chi^2 equal a brute-force weighted least-squares minimum;
-* the canonical (GPU) form equals the literal restatement to rounding.
+* the canonical (GPU) form equals the literal restatement to rounding;
+* TREND shifts every magnitude by exactly -A times the phase angle;
+* ROTATE by zero angles is BC, bit for bit;
+* TORQUE2 with the second torque equal to the first is TORQUE up to the RK4
+  error of the extra step boundary, and a split time moved across an
+  observation interval changes only the models after it;
+* the independent model (piecewise-constant torque in scipy, a second z-x-z
+  rotation of the brightness frame, the prism's projected area written from
+  the face normals) pins TORQUE2, ROTATE and RECT.
 """
 import numpy as np
 import pytest
@@ -132,15 +140,95 @@ def test_rk4_is_fourth_order():
 
 @pytest.mark.parametrize("name,v", [("ls_torque", variants.ls_torque(time_step=1e-3)),
                                     ("bc", variants.bc_free(time_step=1e-3)),
-                                    ("bw_torque", variants.bw_ball_torque(time_step=1e-3))])
+                                    ("bw_torque", variants.bw_ball_torque(time_step=1e-3)),
+                                    ("trend", variants.ls_torque_trend(time_step=1e-3)),
+                                    ("torque2", variants.ls_torque2(time_step=1e-3)),
+                                    ("bc_rotate", variants.bc_rotate(time_step=1e-3)),
+                                    ("bc_rotate_torque", variants.bc_rotate(time_step=1e-3, torque=True)),
+                                    ("rect", variants.rect(time_step=1e-3)),
+                                    ("rect_torque", variants.rect(time_step=1e-3, torque=True))])
 @pytest.mark.parametrize("seed", [5, 6])
-def test_independent_numpy_model(name, v, seed):
+@pytest.mark.parametrize("form", [oracle.LITERAL, oracle.CANON])
+def test_independent_numpy_model(name, v, seed, form):
     # A fine TIME_STEP makes the RK4 truncation error negligible, so the
     # comparison tests the geometry and brightness formulas themselves.
     pb, curve, truth = _problem(v, n=200, span=3.0, nf=2, seed=seed)
-    _, _, vm = pb.chi2(truth, form=oracle.LITERAL, want_vmod=True)
+    _, _, vm = pb.chi2(truth, form=form, want_vmod=True)
     want = synth.truth_vmod(v, truth, curve.mjd, curve.E, curve.S)
     assert np.max(np.abs(vm[0] - want)) < 1e-7
+
+
+@pytest.mark.parametrize("form", [oracle.LITERAL, oracle.CANON])
+def test_trend_is_minus_a_times_phase_angle(form):
+    v = variants.ls_torque_trend()
+    pb, curve, truth = _problem(v, n=150, nf=2)
+    ids = variants.param_positions(v)
+    flat = truth.copy()
+    flat[ids["A"]] = 0.0
+    _, _, vm = pb.chi2(np.vstack([truth, flat]), form=form, want_vmod=True)
+    alpha = _phase_angle(curve)
+    assert np.max(np.abs((vm[0] - vm[1]) + truth[ids["A"]] * alpha)) < 1e-12
+    # and the untrended model is the plain torque model
+    v0 = variants.ls_torque()
+    pb0 = oracle.Problem(variants.default_model_space(v0), curve)
+    p0 = np.delete(flat, ids["A"])
+    _, _, vm0 = pb0.chi2(p0, form=form, want_vmod=True)
+    assert np.array_equal(vm0[0], vm[1])
+
+
+@pytest.mark.parametrize("torque", [False, True])
+@pytest.mark.parametrize("form", [oracle.LITERAL, oracle.CANON])
+def test_rotate_by_zero_angles_is_bc(torque, form):
+    v = variants.bc_rotate(torque=torque)
+    pb, curve, truth = _problem(v, n=150, nf=2)
+    ids = variants.param_positions(v)
+    for k in ("theta_R", "phi_R", "psi_R"):
+        truth[ids[k]] = 0.0
+    vb = variants.make_variant(torque, True, False, True, True)
+    pbb = oracle.Problem(variants.default_model_space(vb), curve)
+    pbc = np.delete(truth, [ids["theta_R"], ids["phi_R"], ids["psi_R"]])
+    cr, dr, vmr = pb.chi2(truth, form=form, want_vmod=True)
+    cb, db, vmb = pbb.chi2(pbc, form=form, want_vmod=True)
+    assert np.array_equal(vmr, vmb)
+    # a turned brightness frame is a different light curve
+    truth[ids["theta_R"]] = 0.4
+    _, _, vm2 = pb.chi2(truth, form=form, want_vmod=True)
+    assert np.max(np.abs(vm2 - vmb)) > 1e-3
+
+
+@pytest.mark.parametrize("form", [oracle.LITERAL, oracle.CANON])
+def test_torque2_with_equal_torques_is_torque(form):
+    v = variants.ls_torque2(time_step=1e-3)
+    pb, curve, truth = _problem(v, n=150, nf=1)
+    ids = variants.param_positions(v)
+    for a, b in (("T2i", "Ti"), ("T2s", "Ts"), ("T2l", "Tl")):
+        truth[ids[a]] = truth[ids[b]]
+    v1 = variants.ls_torque(time_step=1e-3)
+    pb1 = oracle.Problem(variants.default_model_space(v1), curve)
+    p1 = np.delete(truth, [ids[k] for k in ("T2i", "T2s", "T2l", "Tt")])
+    _, _, vm2 = pb.chi2(truth, form=form, want_vmod=True)
+    _, _, vm1 = pb1.chi2(p1, form=form, want_vmod=True)
+    err = np.max(np.abs(vm2 - vm1))
+    assert 0 < err < 1e-9  # only the extra RK4 step boundary differs
+
+
+def test_torque2_split_changes_only_later_points():
+    v = variants.ls_torque2()
+    pb, curve, truth = _problem(v, n=150, nf=1)
+    ids = variants.param_positions(v)
+    t = curve.mjd
+    span = t[-1] - t[0]
+    # two split times inside the same observation interval: identical up to
+    # that interval, different after it
+    k = 70
+    lo = (t[k] - t[0]) / span
+    hi = (t[k + 1] - t[0]) / span
+    p = np.vstack([truth, truth])
+    p[0, ids["Tt"]] = lo + 0.25 * (hi - lo)
+    p[1, ids["Tt"]] = lo + 0.75 * (hi - lo)
+    _, _, vm = pb.chi2(p, form=oracle.CANON, want_vmod=True)
+    assert np.array_equal(vm[0, :k + 1], vm[1, :k + 1])
+    assert np.all(vm[0, k + 1:] != vm[1, k + 1:])
 
 
 def _psi_period(Ii, Is, L, Es, psi0, P_guess):
@@ -218,7 +306,11 @@ def test_delta_v_and_chi2_are_weighted_least_squares(form):
 @pytest.mark.parametrize("name,v,nf", [("ls_torque", variants.ls_torque(), 1), ("bc", variants.bc_free(), 3),
                                        ("bw_torque", variants.bw_ball_torque(), 2),
                                        ("ls_plain", variants.make_variant(False, False, False, True, True), 1),
-                                       ("ls_torque_f32", variants.ls_torque(abi.PREC_F32SIMPLEX), 1)])
+                                       ("ls_torque_f32", variants.ls_torque(abi.PREC_F32SIMPLEX), 1),
+                                       ("trend", variants.ls_torque_trend(), 2), ("torque2", variants.ls_torque2(), 1),
+                                       ("bc_rotate", variants.bc_rotate(), 2),
+                                       ("bc_rotate_torque", variants.bc_rotate(torque=True), 1),
+                                       ("rect", variants.rect(), 2), ("rect_torque", variants.rect(torque=True), 1)])
 def test_canonical_form_equals_literal(name, v, nf):
     pb, curve, truth = _problem(v, n=150, span=2.0, nf=nf)
     rng = np.random.default_rng(4)
