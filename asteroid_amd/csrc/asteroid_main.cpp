@@ -5,8 +5,9 @@
 // quadratic_interpolation() (misc.c:20-35), with the same flags and the same
 // stdout and result-file lines, on top of the C ABI of libasteroid_hip.so.
 // The reference chooses its model variant at compile time (makefile:3-36);
-// here the same macros are run-time flags (-DTORQUE, -DP_PSI, -DBC,
-// -DBW_BALL, -DINTERP, -DACC), defaulting to -DP_PSI -DTORQUE -DINTERP;
+// here the same macros are run-time flags (-DTORQUE, -DTORQUE2, -DP_PSI,
+// -DBC, -DRECT, -DROTATE, -DTREND, -DBW_BALL, -DINTERP, -DACC), defaulting to
+// -DP_PSI -DTORQUE -DINTERP;
 // -DMIXED (no counterpart in the reference) selects the mixed-precision form.
 // A failing library call prints the error and exits with status 1 (the
 // reference's ERR macro, cuda_errors.h:4-14).
@@ -211,8 +212,9 @@ int plot_model(ast_ctx* ctx, const ast_variant& v, const Ingested& D, const std:
   if (!fV) die("Cannot open delta_V!");
   for (int m = 0; m < N_filters; ++m) fprintf(fV, "%8.4f\n", dV[m]);
   fclose(fV);
-  // The initial model without the torque, photometric b,c and BW_BALL
-  // parameters (asteroid.c:907-933).
+  // The initial model without the torque, photometric b,c, TREND's A and
+  // BW_BALL parameters (asteroid.c:904-932); TORQUE2's second torque and
+  // split time and ROTATE's angles are written.
   ast_type_ids T;
   ast_variant_type_ids(&v, &T);
   FILE* fmod = fopen("model_params.dat", "w");
@@ -221,6 +223,7 @@ int plot_model(ast_ctx* ctx, const ast_variant& v, const Ingested& D, const std:
     const int ty = property[k * AST_N_COLUMNS + AST_P_TYPE];
     if (v.torque && (ty == T.Ti || ty == T.Ts || ty == T.Tl)) continue;
     if (v.bc && (ty == T.c || ty == T.b)) continue;
+    if (v.trend && ty == T.A) continue;
     if (v.bw_ball && (ty == T.theta_R || ty == T.phi_R || ty == T.kappa)) continue;
     fprintf(fmod, "%15.11f ", params[k]);
   }
@@ -327,7 +330,8 @@ void usage() {
   printf("-seed SEED : use the SEED number to initialize the random number generator\n");
   printf("-t : travelling reoptimization\n");
   printf("-dx value : maximum shift for parameters in scale-free units (0...1), with -DRMSD\n");
-  printf("-D<MACRO> : model variant (TORQUE, P_PSI, BC, BW_BALL, INTERP, ACC, MIXED) and mode (RMSD, SPHERICAL_K);\n"
+  printf("-D<MACRO> : model variant (TORQUE, TORQUE2, P_PSI, BC, RECT, ROTATE, TREND, BW_BALL, INTERP, ACC, MIXED)\n"
+         "            and mode (RMSD, SPHERICAL_K);\n"
          "            default -DP_PSI -DTORQUE -DINTERP\n");
   printf("-blocks n, -bsize n : reseed groups per launch and simplexes per group (N_BLOCKS, BSIZE)\n");
   printf("-steps n : simplex steps per stage (N_STEPS); -budget n : chi^2 evaluations per stage\n");
@@ -372,6 +376,10 @@ int main(int argc, char** argv) {
       else if (mac == "P_PSI") v.p_psi = 1;
       else if (mac == "BC") v.bc = 1;
       else if (mac == "BW_BALL") v.bw_ball = 1;
+      else if (mac == "TREND") v.trend = 1;
+      else if (mac == "TORQUE2") v.torque2 = v.torque = 1;  // asteroid.h:32-34
+      else if (mac == "ROTATE") v.rotate = 1;
+      else if (mac == "RECT") v.rect = v.bc = 1;            // asteroid.h:36-38
       else if (mac == "INTERP") v.interp = 1;
       else if (mac == "ACC") acc = 1;
       else if (mac == "MIXED") acc = 2;
@@ -420,7 +428,7 @@ int main(int argc, char** argv) {
     else die("Bad arguments!");
   }
   // The default variant; -DRMSD and -DSPHERICAL_K alone keep it.
-  if (!any_macro || !(v.torque || v.p_psi || v.bc || v.bw_ball || v.interp || acc)) {
+  if (!any_macro || !(v.torque || v.p_psi || v.bc || v.bw_ball || v.interp || v.trend || v.rotate || acc)) {
     v.torque = 1;
     v.p_psi = 1;
     v.interp = 1;

@@ -17,7 +17,10 @@ import numpy as np
 COLUMNS = ("add", "mul", "fma", "div", "sqrt", "rint")
 _W = np.array([1, 1, 2, 1, 1, 1], float)
 
-# key: (torque, brightness) with brightness "ls" (LS ellipsoid, BC or not) or "bw".
+# key: (torque, brightness) with brightness "ls" (LS ellipsoid, BC or not),
+# "bw", "rect", and the LS forms with "+trend" (TREND) or "+rot" (ROTATE).
+# TORQUE2 is the torque row: its split interval runs as two RK4 stretches,
+# counted by rk4_steps when the split fraction is given.
 TABLE = {
     (True, "ls"): dict(per_model=(28, 41, 77, 11, 3, 4), per_point=(22, 68, 73, 5, 4, 1),
                        per_rk4=(38, 60, 180, 4, 0, 8)),
@@ -27,34 +30,69 @@ TABLE = {
                        per_rk4=(38, 60, 180, 4, 0, 8)),
     (False, "bw"): dict(per_model=(35, 44, 107, 9, 3, 6), per_point=(18, 49, 69, 1, 0, 2),
                         per_rk4=(39, 60, 142, 0, 0, 8)),
+    (True, "ls+trend"): dict(per_model=(28, 41, 77, 11, 3, 4), per_point=(22, 68, 74, 5, 4, 1),
+                             per_rk4=(38, 60, 180, 4, 0, 8)),
+    (True, "ls+rot"): dict(per_model=(40, 56, 122, 11, 3, 7), per_point=(22, 86, 91, 5, 4, 1),
+                           per_rk4=(38, 60, 180, 4, 0, 8)),
+    (False, "ls+rot"): dict(per_model=(39, 50, 122, 11, 3, 7), per_point=(26, 91, 106, 5, 4, 2),
+                            per_rk4=(39, 60, 142, 0, 0, 8)),
+    (True, "rect"): dict(per_model=(28, 41, 77, 11, 3, 4), per_point=(14, 63, 72, 1, 3, 1),
+                         per_rk4=(38, 60, 180, 4, 0, 8)),
+    (False, "rect"): dict(per_model=(27, 35, 77, 11, 3, 4), per_point=(18, 68, 87, 1, 3, 2),
+                          per_rk4=(39, 60, 142, 0, 0, 8)),
 }
 
 
-def rk4_steps(mjd, time_step):
-    d = np.diff(np.asarray(mjd, float))
-    return int(np.sum((d / time_step + 1).astype(np.int64)))
+def table_key(variant):
+    torque = bool(variant.torque or variant.torque2)
+    if variant.bw_ball:
+        return torque, "bw"
+    if variant.rect:
+        return torque, "rect"
+    return torque, "ls+trend" if variant.trend else "ls+rot" if variant.rotate else "ls"
 
 
-def eval_ops(variant, mjd):
+def rk4_steps(mjd, time_step, split=None):
+    """RK4 steps of one evaluation (cuda.c:321); split = TORQUE2's Tt, whose
+    interval runs as two stretches (cuda.c:288-323)."""
+    mjd = np.asarray(mjd, float)
+    d = np.diff(mjd)
+    n = int(np.sum((d / time_step + 1).astype(np.int64)))
+    if split is not None:
+        ts = split * (mjd[-1] - mjd[0]) + mjd[0]
+        i = int(np.searchsorted(mjd, ts, side="right"))
+        if 0 < i < len(mjd):
+            t1, t2 = mjd[i - 1], mjd[i]
+            n += int((ts - t1) / time_step + 1) + int((t2 - ts) / time_step + 1) - int((t2 - t1) / time_step + 1)
+    return n
+
+
+def eval_ops(variant, mjd, split=None):
     """Operation counts (6 columns) of one evaluation on a curve with times mjd."""
-    t = TABLE[(bool(variant.torque), "bw" if variant.bw_ball else "ls")]
+    t = TABLE[table_key(variant)]
+    split = split if variant.torque2 else None
     return (np.array(t["per_model"], float) + len(mjd) * np.array(t["per_point"], float)
-            + rk4_steps(mjd, variant.time_step) * np.array(t["per_rk4"], float))
+            + rk4_steps(mjd, variant.time_step, split) * np.array(t["per_rk4"], float))
 
 
-def eval_flops(variant, mjd):
-    """FLOPs of one evaluation; for the mixed form, its FP64 plus FP32 FLOPs."""
+def eval_flops(variant, mjd, split=None):
+    """FLOPs of one evaluation; for the mixed form, its FP64 plus FP32 FLOPs.
+    TORQUE2 without `split` counts its split interval as one stretch (at most
+    one RK4 step short)."""
     if variant.precision == MIXED:
-        f64, f32 = eval_flops_mixed(variant, mjd)
+        f64, f32 = eval_flops_mixed(variant, mjd, split)
         return f64 + f32
-    return float(eval_ops(variant, mjd) @ _W)
+    return float(eval_ops(variant, mjd, split) @ _W)
 
 
 # ------------------------------------------------------------ mixed form
 # The mixed-precision evaluator runs the per-model set-up, the RK4
-# integration, the body axes, the scaled Sun/Earth directions and the chi^2
-# sums exactly as the FP64 form (the rows above), and replaces the FP64
-# brightness tail by its own.  Counted from ast_device.h:
+# integration, the scaled Sun/Earth directions and the chi^2 sums exactly as
+# the FP64 form (the rows above), takes the body axes' per-point sin/cos
+# (of phi, and of psi without torque) from the shorter sincos_mx (3 fma
+# fewer than sincos each: two-part reduction, degree-4 kernels), and
+# replaces the FP64 brightness tail by its own (RECT keeps its FP64 tail).
+# Counted from ast_device.h and astmath.h:
 #   FP64 Lommel-Seeliger tail (ls_tail_f64, from |u_S|^2 to the magnitude,
 #     with the log and log10 of astmath.h): 16 add, 32 mul, 27 fma, 5 div,
 #     4 sqrt; the Lambert + ball tail (log10 and the -2.5 factor): 6 add,
@@ -69,15 +107,23 @@ MIXED = 2
 _TAIL64 = {"ls": np.array([16, 32, 27, 5, 4, 0], float), "bw": np.array([6, 8, 8, 1, 0, 0], float)}
 _TAIL32 = {"ls": np.array([3, 25, 21, 7], float), "bw": np.array([0, 0, 0, 1], float)}  # add mul fma special
 _MX_F64_EXTRA = {"ls": np.array([13, 21, 12, 0, 3, 0], float), "bw": np.array([1, 1, 0, 0, 0, 0], float)}
+_MX_SINCOS_SAVED = np.array([0, 0, 3, 0, 0, 0], float)  # per sincos_mx call
 _W4 = np.array([1, 1, 2, 1], float)
 
 
-def eval_flops_mixed(variant, mjd):
+def eval_flops_mixed(variant, mjd, split=None):
     """(FP64 FLOPs, FP32 FLOPs) of one mixed-form evaluation."""
-    kind = "bw" if variant.bw_ball else "ls"
-    t = TABLE[(bool(variant.torque), kind)]
+    key = table_key(variant)
+    torque, kind = key
+    t = TABLE[key]
     n = len(mjd)
-    f64_pt = np.array(t["per_point"], float) - _TAIL64[kind] + _MX_F64_EXTRA[kind]
-    f64 = (np.array(t["per_model"], float) @ _W + rk4_steps(mjd, variant.time_step) * (np.array(t["per_rk4"], float) @ _W)
-           + n * (f64_pt @ _W))
-    return float(f64), float(n * (_TAIL32[kind] @ _W4))
+    tail = "rect" if kind == "rect" else "bw" if kind == "bw" else "ls"
+    f64_pt = np.array(t["per_point"], float) - (1 if torque else 2) * _MX_SINCOS_SAVED
+    f32_pt = 0.0
+    if tail != "rect":
+        f64_pt = f64_pt - _TAIL64[tail] + _MX_F64_EXTRA[tail]
+        f32_pt = _TAIL32[tail] @ _W4
+    split = split if variant.torque2 else None
+    f64 = (np.array(t["per_model"], float) @ _W
+           + rk4_steps(mjd, variant.time_step, split) * (np.array(t["per_rk4"], float) @ _W) + n * (f64_pt @ _W))
+    return float(f64), float(n * f32_pt)

@@ -24,9 +24,11 @@ fitting kernel (algorithmic FLOPs from asteroid_amd/flops.py over its
 HIP-event-timed launch durations) and the CPU oracle timed on a bounded
 sample of the same workload on this host's cores.
 
---workload small|reopt|batched|serving|plot|rmsd measures the other
-configurations and paths, one JSON line each (see DESIGN.md for what each
-one is).
+--workload small|reopt|batched|serving|plot|rmsd|variants measures the other
+configurations and paths, one JSON line each: the plain FP64 small search
+with its full CPU run, the Stage Two BC reoptimisation, batched fits of many
+curves, the serving sweep, the -plot model curves, the RMSD intervals, and
+the Stage One search for every built model variant.
 """
 import argparse
 import json
@@ -561,12 +563,78 @@ def main_serving(args):
     return out
 
 
+VARIANT_RUNS = {
+    # name: (factory(precision, time_step), n_filters)
+    "ls_torque": (lambda p, ts: variants_mod().ls_torque(p, time_step=ts), 1),
+    "ls": (lambda p, ts: variants_mod().make_variant(False, False, False, True, True, p, ts), 1),
+    "bc": (lambda p, ts: variants_mod().make_variant(False, True, False, True, True, p, ts), 1),
+    "bc_torque": (lambda p, ts: variants_mod().make_variant(True, True, False, True, True, p, ts), 1),
+    "bw": (lambda p, ts: variants_mod().make_variant(False, False, True, True, True, p, ts), 1),
+    "bw_torque": (lambda p, ts: variants_mod().bw_ball_torque(p, time_step=ts), 1),
+    "trend_torque": (lambda p, ts: variants_mod().ls_torque_trend(p, time_step=ts), 1),
+    "torque2": (lambda p, ts: variants_mod().ls_torque2(p, time_step=ts), 1),
+    "bc_rotate": (lambda p, ts: variants_mod().bc_rotate(p, time_step=ts), 1),
+    "bc_rotate_torque": (lambda p, ts: variants_mod().bc_rotate(p, time_step=ts, torque=True), 1),
+    "rect": (lambda p, ts: variants_mod().rect(p, time_step=ts), 1),
+    "rect_torque": (lambda p, ts: variants_mod().rect(p, time_step=ts, torque=True), 1),
+}
+
+
+def variants_mod():
+    from asteroid_amd import variants
+    return variants
+
+
+def main_variants(args):
+    """The Stage One random search (400 points over 3 days, TIME_STEP 0.01,
+    P_psi 2-4800 h, args.budget evaluations per simplex) for every built
+    variant, args.fits simplexes each, in one precision form: model-point
+    evals/s and the fit kernel's fraction of the op-mix peak per variant."""
+    from asteroid_amd import api, flops, synth, variants
+    prec = {"both": MIXED, "mixed": MIXED, "fp64": FP64, "f32simplex": F32SIMPLEX}[args.precision]
+    group = 256
+    rows = {}
+    for name, (mk, nf) in VARIANT_RUNS.items():
+        v = mk(prec, 1e-2)
+        curve, truth = synth.make_curve(2024, v, 400, 3.0, n_filters=nf, sigma_range=(0.05, 0.3))
+        space = variants.default_model_space(v, ppsi_hours=(2.0, 4800.0), reopt=0)
+        with api.Context(v) as ctx:
+            ctx.set_model_space(space)
+            ctx.upload_curve(curve)
+            ctx.rng_seed(5)
+            run = dict(nstages=1, n_steps=5000, eval_budget=args.budget, size_min=SIZE_MIN, use_graphs=args.graphs)
+            ctx.run(args.fits // group // 4, group, **run)  # warm-up
+            ctx.reset_best()
+            t0 = time.perf_counter()
+            ctx.run(args.fits // group, group, **run)
+            wall = time.perf_counter() - t0
+            e, pts = ctx.last_counts()
+            ms, nl = ctx.last_timing()
+        split = truth[variants.param_positions(v)["Tt"]] if v.torque2 else None
+        fl = flops.eval_flops(v, curve.mjd, split)
+        peak, mix = peak_of(prec, v, curve.mjd)
+        achieved = fl * e / (ms / 1e3) / 1e12
+        rows[name] = {"n_params": space.n_params, "value": pts / wall, "fits_per_s": args.fits / wall,
+                      "evals_per_fit": round(e / args.fits, 1), "kernel_ms": round(ms, 2), "launches": nl,
+                      "flop_per_model_point": round(fl / curve.n_data, 2), "achieved_tflops": round(achieved, 3),
+                      "peak_tflops": round(peak, 2), "frac": round(achieved / peak, 4)}
+        log(f"{name}: {pts / wall:.3e} model-points/s, {achieved:.2f} TFLOP/s ({achieved / peak:.1%} of {peak:.1f})")
+    head = rows["ls_torque"]
+    out = base_line(head["value"], 1, 1, None, DTYPE[prec],
+                    {"workload": "stage-one random search per variant (400 points, 3 days, TIME_STEP 0.01)",
+                     "simplexes": args.fits, "evals_per_simplex": args.budget, "group_size": group,
+                     "precision": PREC_NAMES[prec]})
+    out["variants"] = rows
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["stage1", "small", "reopt", "batched", "serving", "plot", "rmsd"],
+    ap.add_argument("--workload", choices=["stage1", "small", "reopt", "batched", "serving", "plot", "rmsd",
+                                           "variants"],
                     default="stage1")
     ap.add_argument("--plot-models", type=int, default=4096)
     ap.add_argument("--rmsd-samples", type=int, default=64)
@@ -589,10 +657,10 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the fits are independent and shard across GPUs "
-              "without communication (see DESIGN.md); run it with --gpus 1.", file=sys.stderr)
+              "without communication; run it with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     fn = {"stage1": main_stage_one, "small": main_small, "reopt": main_reopt, "batched": main_batched,
-          "serving": main_serving, "plot": main_plot, "rmsd": main_rmsd}[args.workload]
+          "serving": main_serving, "plot": main_plot, "rmsd": main_rmsd, "variants": main_variants}[args.workload]
     print(json.dumps(fn(args)), flush=True)
 
 

@@ -21,11 +21,14 @@ def _count(v, curve, params):
 
 @pytest.mark.parametrize("v", [variants.ls_torque(), variants.bc_free(), variants.bw_ball_torque(),
                                variants.make_variant(False, False, False, True, True),
-                               variants.make_variant(False, False, True, True, True)])
+                               variants.make_variant(False, False, True, True, True), variants.ls_torque_trend(),
+                               variants.ls_torque2(), variants.bc_rotate(), variants.bc_rotate(torque=True),
+                               variants.rect(), variants.rect(torque=True)])
 @pytest.mark.parametrize("n,span", [(37, 0.7), (400, 3.0), (250, 8.0)])
 def test_table_matches_counting_oracle(v, n, span):
     curve, truth = synth.make_curve(17, v, n, span)
-    got, table = _count(v, curve, truth), flops.eval_ops(v, curve.mjd)
+    split = truth[variants.param_positions(v)["Tt"]] if v.torque2 else None
+    got, table = _count(v, curve, truth), flops.eval_ops(v, curve.mjd, split)
     # The per-point and per-step work is branch-free; the once-per-model
     # asin(...) of theta_0 goes through atan2, whose octant reduction costs
     # up to two adds, one division and two adds more on some arguments.
@@ -48,4 +51,26 @@ def test_mixed_table_consistent():
     f64, f32 = flops.eval_flops_mixed(vmx, curve.mjd)
     w = np.array([1, 1, 2, 1, 1, 1], float)
     expect64 = flops.eval_flops(v64, curve.mjd) - 50 * (flops._TAIL64["ls"] - flops._MX_F64_EXTRA["ls"]) @ w
+    expect64 -= 50 * 6  # sincos_mx of phi: 3 fma fewer
     assert f64 == expect64 and f32 == 50 * 77 and flops.eval_flops(vmx, curve.mjd) == f64 + f32
+    # RECT keeps its FP64 tail: only the body axes' sin/cos change
+    r64, rmx = variants.rect(abi.PREC_FP64), variants.rect(abi.PREC_MIXED)
+    f64, f32 = flops.eval_flops_mixed(rmx, curve.mjd)
+    assert f32 == 0 and f64 == flops.eval_flops(r64, curve.mjd) - 50 * 12  # psi and phi without torque
+
+
+def test_sincos_mx_saves_three_fma():
+    """The mixed body axes' sin/cos: the FP64 sincos has a three-part pi/2
+    reduction and degree-5 kernels, sincos_mx a two-part one and degree-4
+    kernels; their fma counts differ by 3, read off astmath.h's source."""
+    import os
+    import re
+    src = open(os.path.join(os.path.dirname(flops.__file__), "csrc", "astmath.h")).read()
+
+    def body(name):
+        m = re.search(r"template <class R> AST_HD (?:R|void) %s\(R [a-z]\w*(?:, R\* s, R\* c)?\) \{(.*?)\n\}" % name,
+                      src, re.S)
+        return m.group(1)
+    n_full = body("sincos").count("p_fma") + body("k_sin").count("p_fma") + body("k_cos").count("p_fma")
+    n_mx = body("sincos_mx").count("p_fma")
+    assert n_full - n_mx == flops._MX_SINCOS_SAVED[2] == 3

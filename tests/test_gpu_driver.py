@@ -126,6 +126,41 @@ def test_driver_plot_matches_oracle(tmp_path):
     assert [d[2] for d in data] == ["%.6e" % f32(1 / np.sqrt(curve.w[i])) for i in range(curve.n_data)]
 
 
+@pytest.mark.parametrize("flags,mk,skip", [
+    (["-DTREND", "-DTORQUE"], lambda: variants.ls_torque_trend(abi.PREC_F32SIMPLEX), ("A", "Ti", "Ts", "Tl")),
+    (["-DTORQUE2"], lambda: variants.ls_torque2(abi.PREC_F32SIMPLEX), ("Ti", "Ts", "Tl")),
+    (["-DBC", "-DROTATE", "-DACC"], lambda: variants.bc_rotate(abi.PREC_FP64), ("c", "b")),
+    (["-DRECT", "-DTORQUE", "-DMIXED"], lambda: variants.rect(abi.PREC_MIXED, torque=True), ("Ti", "Ts", "Tl", "c", "b")),
+], ids=["trend", "torque2", "bc_rotate_acc", "rect_torque_mixed"])
+def test_driver_plot_new_variants(tmp_path, flags, mk, skip):
+    """-DTREND, -DTORQUE2, -DROTATE, -DRECT select the variants; chi2_plot
+    and delta_V match the oracle, and model_params.dat leaves out TREND's A
+    with the torque and b, c parameters (asteroid.c:904-932) but keeps
+    TORQUE2's second torque and split time and ROTATE's angles."""
+    _scene(tmp_path, n_obs=80, filters="RG")
+    curve = _ingested(tmp_path)
+    v = mk()
+    space = variants.default_model_space(v)
+    pb = oracle.Problem(space, curve)
+    p, rc = pb.x2params(np.full((1, space.n_params), 0.41))
+    assert rc[0] == 0
+    params = p[0]
+    cmd = [EXE, "-DP_PSI", "-DINTERP"] + flags + ["-i", "lc.txt", "-plot", "-m"] + ["%.17g" % x for x in params]
+    r = subprocess.run(cmd, cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout
+    line = [l for l in r.stdout.splitlines() if l.startswith("chi2_plot = ")]
+    t, vm, chi2, dv = pb.model_curve(params, n_plot=20000)
+    if v.precision == abi.PREC_MIXED:
+        got = float(line[0].split(",")[0].split("=")[1])
+        assert abs(got - chi2[0]) <= 1e-5 * chi2[0]
+    else:
+        assert line[0].startswith("chi2_plot = %13.6e, lsq = " % chi2[0])
+        assert [l.rstrip("\n") for l in open(tmp_path / "delta_V")] == ["%8.4f" % x for x in dv[0]]
+    pos = variants.param_positions(v)
+    keep = [k for k in range(space.n_params) if k not in [pos[n] for n in skip]]
+    assert open(tmp_path / "model_params.dat").read() == "".join("%15.11f " % params[k] for k in keep) + "\n"
+
+
 def test_driver_rmsd_matches_oracle(tmp_path):
     """`asteroid -DRMSD -dx ... -Nstages n -reopt -m ...` (asteroid.c:643-725):
     the f0/f1 line, then per cycle the cumulative rejected fraction and the
