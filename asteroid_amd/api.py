@@ -76,6 +76,13 @@ class Context:
              "ast_upload_curve")
         self.curve = curve
 
+    def set_minima(self, t_obs, V_obs):
+        """NUDGE's observed minima (times on the curve's clock, converted magnitudes)."""
+        t = np.ascontiguousarray(t_obs, np.float32)
+        v = np.ascontiguousarray(V_obs, np.float32)
+        _chk(self.lib.ast_set_minima(self.h, t.ctypes.data_as(C.POINTER(C.c_float)),
+                                     v.ctypes.data_as(C.POINTER(C.c_float)), len(t)), "ast_set_minima")
+
     def rng_seed(self, seed):
         _chk(self.lib.ast_rng_seed(self.h, int(seed)), "ast_rng_seed")
 

@@ -24,7 +24,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-math-errno", 
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 VARIANT_TUS = ["ast_k_ls", "ast_k_bc", "ast_k_bw", "ast_k_tq_ls", "ast_k_tq_bc", "ast_k_tq_bw", "ast_k_tq_trend",
-               "ast_k_tq2", "ast_k_bc_rot", "ast_k_tq_bc_rot", "ast_k_rect", "ast_k_tq_rect"]
+               "ast_k_tq2", "ast_k_bc_rot", "ast_k_tq_bc_rot", "ast_k_rect", "ast_k_tq_rect", "ast_k_tq_nudge",
+               "ast_k_tq_bc_nudge"]
 
 
 def _run(cmd):

@@ -48,6 +48,8 @@ extern template hipError_t launch_variant<VF_BC | VF_ROT>(const LaunchArgs&);
 extern template hipError_t launch_variant<VF_TQ | VF_BC | VF_ROT>(const LaunchArgs&);
 extern template hipError_t launch_variant<VF_RECT>(const LaunchArgs&);
 extern template hipError_t launch_variant<VF_TQ | VF_RECT>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_TQ | VF_NUDGE>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_TQ | VF_BC | VF_NUDGE>(const LaunchArgs&);
 }  // namespace ast
 
 using namespace ast;
@@ -95,14 +97,14 @@ void dev_free(T*& p) {
 // The variant codes built (one kernel family each), indexed by variant_index.
 constexpr int VARIANT_CODES[] = {0, VF_TQ, VF_BC, VF_TQ | VF_BC, VF_BW, VF_TQ | VF_BW,
                                  VF_TQ | VF_TREND, VF_TQ2, VF_BC | VF_ROT, VF_TQ | VF_BC | VF_ROT,
-                                 VF_RECT, VF_TQ | VF_RECT};
+                                 VF_RECT, VF_TQ | VF_RECT, VF_TQ | VF_NUDGE, VF_TQ | VF_BC | VF_NUDGE};
 constexpr int N_VARIANTS = (int)(sizeof VARIANT_CODES / sizeof VARIANT_CODES[0]);
 
 // Variant code of the macros (TORQUE2 implies TORQUE, RECT implies BC).
 int variant_code(const ast_variant& v) {
   const bool t2 = v.torque2, tq = v.torque || t2, rc = v.rect, bc = v.bc || rc;
   return (tq && !t2 ? VF_TQ : 0) | (t2 ? VF_TQ2 : 0) | (bc && !rc ? VF_BC : 0) | (v.bw_ball ? VF_BW : 0) |
-         (v.trend ? VF_TREND : 0) | (v.rotate ? VF_ROT : 0) | (rc ? VF_RECT : 0);
+         (v.trend ? VF_TREND : 0) | (v.rotate ? VF_ROT : 0) | (rc ? VF_RECT : 0) | (v.nudge ? VF_NUDGE : 0);
 }
 
 // Index of the variant's kernel family, or -1 if it is not built.
@@ -115,6 +117,10 @@ int variant_index(const ast_variant& v) {
 
 bool variant_supported(const ast_variant& v) {
   if (variant_index(v) < 0) return false;
+  // NUDGE's merit jumps where a model minimum appears or vanishes, so the
+  // mixed form's 1e-6 chi^2 tolerance cannot hold for it; the reference's
+  // Stage Three runs -DACC (README.md:89-93).
+  if (v.nudge && v.precision == AST_PREC_MIXED) return false;
   if (v.precision != AST_PREC_FP64 && v.precision != AST_PREC_F32SIMPLEX && v.precision != AST_PREC_MIXED)
     return false;
   if (!(v.time_step > 0)) return false;
@@ -206,7 +212,7 @@ double phase_angle(const double E[3], const double S[3]) {
 }
 
 struct CurveH {  // a validated light curve as the kernels read it
-  std::vector<PointRec> pts;
+  std::vector<PointRec> pts;  // n_data observations, then the minima record (zero unless NUDGE)
   double sum_w[AST_MAX_FILTERS] = {0};
   int n_data = 0, n_filters = 0;
   int64_t rk4_steps = 0;  // RK4 steps of one evaluation
@@ -219,7 +225,7 @@ int make_curve(const ast_variant& v, const ast_obs* obs, int n_data, int n_filte
   if (n_filters > AST_MAX_FILTERS) return AST_E_DATA;
   if (v.interp && !eph) return AST_E_ARG;
   CurveH C;
-  C.pts.resize(n_data);
+  C.pts.assign((size_t)n_data + 1, PointRec{});
   std::vector<int> seen(n_filters, 0);
   for (int i = 0; i < n_data; ++i) {
     const ast_obs& o = obs[i];
@@ -268,7 +274,8 @@ int make_curve(const ast_variant& v, const ast_obs* obs, int n_data, int n_filte
 int make_grid(const ast_variant& v, const CurveH& C, const ast_ephem3& eph, int n_plot, std::vector<PointRec>* out,
               std::vector<double>* t_out) {
   if (n_plot < 2 || C.n_data < 1) return AST_E_ARG;
-  std::vector<PointRec> g((size_t)n_plot);
+  std::vector<PointRec> g((size_t)n_plot + 1);
+  g[n_plot] = C.pts[C.n_data];  // the minima record
   std::vector<double> t((size_t)n_plot);
   const double h = (C.t_last - C.t_first) / (n_plot - 1);
   for (int i = 0; i < n_plot; ++i) {
@@ -317,6 +324,8 @@ hipError_t dispatch(const ast_variant& v, LaunchArgs& a) {
     case VF_TQ | VF_BC | VF_ROT: return launch_variant<VF_TQ | VF_BC | VF_ROT>(a);
     case VF_RECT: return launch_variant<VF_RECT>(a);
     case VF_TQ | VF_RECT: return launch_variant<VF_TQ | VF_RECT>(a);
+    case VF_TQ | VF_NUDGE: return launch_variant<VF_TQ | VF_NUDGE>(a);
+    case VF_TQ | VF_BC | VF_NUDGE: return launch_variant<VF_TQ | VF_BC | VF_NUDGE>(a);
     default: return hipErrorInvalidValue;
   }
 }
@@ -786,6 +795,8 @@ struct ast_ctx {
   CurveH curve;
   PointRec* pts = nullptr;   // for ast_chi2_batch
   double* sum_w = nullptr;
+  MinimaRec minima{};         // NUDGE's observed minima (the curve table's last record)
+  bool minima_set = false;
   double params0[AST_MAX_PARAMS]{};
   uint64_t seed = 0;
   uint32_t* rng = nullptr;
@@ -1101,6 +1112,7 @@ int make_job(ast_ctx* c, const ast_job* job, int G, std::unique_ptr<DevJob>* out
              std::vector<double>* p0) {
   if (job->n_fits <= 0 || job->n_fits % G || job->nstages <= 0 || job->n_steps < 0 || job->eval_budget < 0)
     return AST_E_ARG;
+  if (job->variant.nudge) return AST_E_UNSUPPORTED;  // a job carries no observed minima
   std::unique_ptr<DevJob> j(new DevJob());
   j->uid = c->next_uid++;
   RETIF(make_space(job->variant, job->n_params, job->property, job->n_types, job->types, job->limits, &job->x2,
@@ -1561,15 +1573,39 @@ int ast_upload_curve(ast_ctx* c, const ast_obs* obs, int n_data, int n_filters, 
   if (!c) return AST_E_ARG;
   CurveH C;
   RETIF(make_curve(c->v, obs, n_data, n_filters, eph, &C));
+  std::memcpy(&C.pts[n_data], &c->minima, sizeof(MinimaRec));
   RETIF(set_device(c));
   dev_free(c->pts);
   dev_free(c->sum_w);
-  RETIF(dev_alloc(&c->pts, (size_t)n_data));
+  RETIF(dev_alloc(&c->pts, C.pts.size()));
   RETIF(dev_alloc(&c->sum_w, (size_t)AST_MAX_FILTERS));
-  HIPCHK(hipMemcpy(c->pts, C.pts.data(), sizeof(PointRec) * n_data, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->pts, C.pts.data(), sizeof(PointRec) * C.pts.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->sum_w, C.sum_w, sizeof(double) * AST_MAX_FILTERS, hipMemcpyHostToDevice));
   c->curve = std::move(C);
   c->curve_set = true;
+  return AST_OK;
+}
+
+int ast_set_minima(ast_ctx* c, const float* t_obs, const float* V_obs, int n_obs) {
+  if (!c) return AST_E_ARG;
+  if (!c->v.nudge) return AST_E_STATE;
+  if (!t_obs || !V_obs || n_obs < 1 || n_obs > AST_MAX_MINIMA) return AST_E_ARG;
+  MinimaRec r{};
+  r.n_obs = n_obs;
+  r.v1s = c->v.v1s ? 1 : 0;
+  for (int k = 0; k < n_obs; ++k) {
+    if (!std::isfinite(t_obs[k]) || !std::isfinite(V_obs[k])) return AST_E_ARG;
+    r.t_obs[k] = t_obs[k];
+    r.V_obs[k] = V_obs[k];
+  }
+  c->minima = r;
+  c->minima_set = true;
+  if (c->curve_set) {
+    const int n = c->curve.n_data;
+    std::memcpy(&c->curve.pts[n], &r, sizeof r);
+    RETIF(set_device(c));
+    HIPCHK(hipMemcpy(c->pts + n, &c->curve.pts[n], sizeof(PointRec), hipMemcpyHostToDevice));
+  }
   return AST_OK;
 }
 
@@ -1597,7 +1633,7 @@ int ast_set_start(ast_ctx* c, const double* params0) {
 int ast_run(ast_ctx* c, const ast_run_cfg* rc) {
   if (!c || !rc) return AST_E_ARG;
   if (c->serve) return AST_E_BUSY;
-  if (!c->space_set || !c->curve_set) return AST_E_STATE;
+  if (!c->space_set || !c->curve_set || (c->v.nudge && !c->minima_set)) return AST_E_STATE;
   const int G = rc->group_size;
   if (rc->n_groups <= 0 || G < 64 || G > 1024 || (G & 63) || rc->nstages <= 0 || rc->n_steps < 0 ||
       rc->eval_budget < 0)
@@ -1715,7 +1751,7 @@ int ast_fetch_lanes(ast_ctx* c, double* f_last, int64_t* n_evals, int64_t* n_ste
 int ast_chi2_batch(ast_ctx* c, const double* params, int n, double* chi2, double* dV) {
   if (!c || !params || n <= 0 || !chi2 || !dV) return AST_E_ARG;
   if (c->serve) return AST_E_BUSY;  // its streams and events belong to the scheduler
-  if (!c->space_set || !c->curve_set) return AST_E_STATE;
+  if (!c->space_set || !c->curve_set || (c->v.nudge && !c->minima_set)) return AST_E_STATE;
   if (c->curve.n_data - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
   RETIF(set_device(c));
   double *pd = nullptr, *cd = nullptr, *dd = nullptr;
@@ -1768,7 +1804,7 @@ int ast_rmsd(ast_ctx* c, const ast_rmsd_cfg* cfg, float* par_min, float* par_max
       !(cfg->dx >= 0))
     return AST_E_ARG;
   if (c->serve) return AST_E_BUSY;
-  if (!c->space_set || !c->curve_set) return AST_E_STATE;
+  if (!c->space_set || !c->curve_set || (c->v.nudge && !c->minima_set)) return AST_E_STATE;
   if (c->curve.n_data - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
   RETIF(set_device(c));
   RmsdOut o{par_min, par_max, 0, 0, 0, 0, 0};
@@ -1789,7 +1825,7 @@ int ast_model_curve(ast_ctx* c, const double* params, int n, const ast_ephem3* e
                     double* vmod, double* chi2, double* dV) {
   if (!c || !params || n <= 0 || !eph || n_plot < 2 || !vmod || !chi2 || !dV) return AST_E_ARG;
   if (c->serve) return AST_E_BUSY;
-  if (!c->space_set || !c->curve_set) return AST_E_STATE;
+  if (!c->space_set || !c->curve_set || (c->v.nudge && !c->minima_set)) return AST_E_STATE;
   if (c->curve.n_data - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
   std::vector<PointRec> grid;
   std::vector<double> tg;
@@ -1798,7 +1834,7 @@ int ast_model_curve(ast_ctx* c, const double* params, int n, const ast_ephem3* e
   const int np = c->space.n_params, nf = c->curve.n_filters;
   PointRec* gd = nullptr;
   double *pd = nullptr, *cd = nullptr, *dd = nullptr, *vd = nullptr;
-  int rc = dev_alloc(&gd, (size_t)n_plot);
+  int rc = dev_alloc(&gd, grid.size());
   if (!rc) rc = dev_alloc(&pd, (size_t)n * np);
   if (!rc) rc = dev_alloc(&cd, (size_t)n);
   if (!rc) rc = dev_alloc(&dd, (size_t)n * nf);
@@ -1806,7 +1842,7 @@ int ast_model_curve(ast_ctx* c, const double* params, int n, const ast_ephem3* e
   auto release = [&]() { dev_free(gd); dev_free(pd); dev_free(cd); dev_free(dd); dev_free(vd); };
   if (rc) { release(); return rc; }
   hipStream_t s = c->streams[0];
-  hipError_t he = hipMemcpy(gd, grid.data(), sizeof(PointRec) * n_plot, hipMemcpyHostToDevice);
+  hipError_t he = hipMemcpy(gd, grid.data(), sizeof(PointRec) * grid.size(), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(pd, params, sizeof(double) * n * np, hipMemcpyHostToDevice);
   LaunchArgs a{};
   a.op = 3;

@@ -27,7 +27,9 @@ constexpr int MAXF = AST_MAX_FILTERS;
 
 // One observation as the kernels read it: unit E and S at the observation,
 // the RK4 interval from the previous observation (cuda.c:321-323; h, h/2,
-// h/6 and the step count), V, w and the filter.  96 bytes.
+// h/6 and the step count), V, w, the filter, the time, the phase angle and
+// TIME_STEP.  112 bytes.  A table of n_data records carries one more record
+// after them: the observed minima (MinimaRec) for NUDGE, zero otherwise.
 struct PointRec {
   double E[3];
   double S[3];
@@ -40,10 +42,24 @@ struct PointRec {
 };
 static_assert(sizeof(PointRec) == 112, "PointRec layout");
 
+// NUDGE (asteroid.h:216-230, 319-337; misc.c:279-323): the observed
+// brightness minima, times relative to the curve's first observation and
+// magnitudes in the converted system, as floats; v1s is the V1S macro
+// (one-sided magnitude distance, cuda.c:940-945).
+constexpr int NOBS_MAX = 10;  // asteroid.h:219
+struct MinimaRec {
+  int n_obs;
+  int v1s;
+  float t_obs[NOBS_MAX];
+  float V_obs[NOBS_MAX];
+  int pad[6];
+};
+static_assert(sizeof(MinimaRec) == sizeof(PointRec), "the minima record takes one PointRec slot");
+
 // Model variant as one compile-time code: the reference's macros
 // (makefile:3-36) as bit flags.  TORQUE2 implies TORQUE and RECT implies BC
 // (asteroid.h:32-38).
-enum : int { VF_TQ = 1, VF_BC = 2, VF_BW = 4, VF_TREND = 8, VF_TQ2 = 16, VF_ROT = 32, VF_RECT = 64 };
+enum : int { VF_TQ = 1, VF_BC = 2, VF_BW = 4, VF_TREND = 8, VF_TQ2 = 16, VF_ROT = 32, VF_RECT = 64, VF_NUDGE = 128 };
 template <int VC>
 struct VarF {
   static constexpr bool TQ = (VC & (VF_TQ | VF_TQ2)) != 0;  // constant torque
@@ -53,10 +69,11 @@ struct VarF {
   static constexpr bool TR = (VC & VF_TREND) != 0;          // de-trending by the phase angle
   static constexpr bool RO = (VC & VF_ROT) != 0;            // brightness frame rotated
   static constexpr bool RC = (VC & VF_RECT) != 0;           // rectangular prism brightness
+  static constexpr bool NU = (VC & VF_NUDGE) != 0;          // model minima nudged to observed ones
 };
 #define AST_VFLAGS(VC)                                                                           \
   constexpr bool TQ = VarF<VC>::TQ, BC = VarF<VC>::BC, BW = VarF<VC>::BW, T2 = VarF<VC>::T2,   \
-                 TR = VarF<VC>::TR, RO = VarF<VC>::RO, RC = VarF<VC>::RC
+                 TR = VarF<VC>::TR, RO = VarF<VC>::RO, RC = VarF<VC>::RC, NU = VarF<VC>::NU
 
 // Parameter positions of a variant (the Property order, asteroid.c:86-125).
 template <int VC>
@@ -313,6 +330,121 @@ __device__ inline void ode(const double* y, double* f, const double* mu, double 
 }
 
 typedef __attribute__((address_space(4))) const PointRec* PointPtr;
+typedef __attribute__((address_space(4))) const MinimaRec* MinimaPtr;
+
+// ------------------------------------------------------------------ NUDGE
+// Constants of asteroid.h:216-230, evaluated with the reference's types
+// (float constants, derived ones through double where it writes double).
+constexpr int NU_M_MAX = 30;
+constexpr float NU_M_MAX2 = 20.0f;
+constexpr float NU_DT_MAX = 0.12f;
+constexpr float NU_DV_MAX = 24.0f;
+constexpr float NU_DT_MAX2 = (float)(1.5 * (double)NU_DT_MAX);
+constexpr float NU_P_MIN = 0.01f;
+constexpr float NU_CHI2_0 = 10.0f;
+constexpr float NU_CHI2_1 = 30.0f;
+constexpr float NU_L_RC = 0.05f;
+constexpr float NU_L_RC2 = NU_L_RC * NU_L_RC;
+constexpr float NU_L_A = (float)(1.0 / (1.0 - (double)NU_L_RC2 / (1.0 + (double)NU_L_RC2)));
+constexpr float NU_P_MIN2 = 1 / NU_P_MIN - 1;
+
+// Model minima of one evaluation (cuda.c:716-758): a point whose magnitude
+// is above both neighbours' (a brightness minimum) and lies within DT_MAX2
+// of an observed minimum is located by the parabola through the three
+// points and kept; the 31st such minimum fails the model (1e30).  Times and
+// magnitudes are kept as floats, the parabola in double, as there.  The
+// located minima live in arrays of the caller's (t_mod, V_mod): apart from
+// the struct, so its scalars stay in registers.
+struct NudgeState {
+  float t0, t1, V0, V1;  // the last two points (t_old[], V_old[] of the reference)
+  float* t_mod;
+  float* V_mod;
+  int M = 0;
+  bool failed = false;
+
+  __device__ void point(int i, double mjd, double Vmod, MinimaPtr mr) {
+    if (failed) return;
+    if (i == 0) {
+      t0 = (float)mjd;
+      V0 = (float)Vmod;
+      return;
+    }
+    if (i == 1) {
+      t1 = (float)mjd;
+      V1 = (float)Vmod;
+      return;
+    }
+    if (V1 > V0 && (double)V1 >= Vmod) {
+      bool local = false;
+      for (int ii = 0; ii < mr->n_obs; ++ii)
+        if (__builtin_fabsf(t1 - mr->t_obs[ii]) < NU_DT_MAX2) local = true;
+      if (local) {
+        ++M;
+        if (M > NU_M_MAX) {
+          failed = true;
+          return;
+        }
+        const float q = (V1 - V0) / (t1 - t0);
+        const double a = ((Vmod - (double)V1) / (mjd - (double)t1) - (double)q) / (mjd - (double)t0);
+        const double b = (double)q - a * (double)(t1 + t0);
+        const double c = (double)V1 - a * (double)t1 * (double)t1 - b * (double)t1;
+        const float tm = (float)(-b / 2.0 / a);
+        t_mod[M - 1] = tm;
+        V_mod[M - 1] = (float)(a * (double)tm * (double)tm + b * (double)tm + c);
+      }
+    }
+    t0 = t1;
+    V0 = V1;
+    t1 = (float)mjd;
+    V1 = (float)Vmod;
+  }
+
+  // The merit (cuda.c:912-992): chi2 times the reward for model minima
+  // close to observed ones in (t, V) and the penalty for too many of them.
+  template <class CF>
+  __device__ CF merit(CF chi2a, CF dv0, MinimaPtr mr) const {
+    float S_M = 0.0f, P_tot = 1.0f;
+    for (int imod = 0; imod < M; ++imod) {
+      for (int iobs = 0; iobs < mr->n_obs; ++iobs) {
+        const float dt = __builtin_fabsf(t_mod[imod] - mr->t_obs[iobs]);
+        if (dt < NU_DT_MAX2) {
+          if (dt > NU_DT_MAX) {
+            const float x = (NU_DT_MAX2 - dt) / (NU_DT_MAX2 - NU_DT_MAX);
+            S_M = (float)((double)S_M + (double)(x * x) * (-2.0 * (double)x + 3.0));
+          } else {
+            S_M = (float)((double)S_M + 1.0);
+            float dV = (float)(((CF)V_mod[imod] + dv0) - (CF)mr->V_obs[iobs]);
+            if (mr->v1s && dV > 0.0f) dV = 0.0f;
+            const float x = __builtin_sqrtf(dt * dt / NU_DT_MAX / NU_DT_MAX + dV * dV / NU_DV_MAX / NU_DV_MAX);
+            if (x < 1.0f) {
+              const float P_i = NU_L_A * x * x / (x * x + NU_L_RC2);
+              P_tot = (float)((double)P_tot * ((double)NU_P_MIN * (1.0 + (double)(NU_P_MIN2 * P_i))));
+            }
+          }
+        }
+      }
+    }
+    P_tot = astm::powf(P_tot, (float)(1.0 / mr->n_obs));
+    if (P_tot < NU_P_MIN) P_tot = NU_P_MIN;
+    if (chi2a > (CF)NU_CHI2_1) {
+      P_tot = 1.0f;
+    } else if (chi2a > (CF)NU_CHI2_0) {
+      const float x = (float)(((CF)NU_CHI2_1 - chi2a) / (CF)(NU_CHI2_1 - NU_CHI2_0));
+      const float beta = x * x * (-2 * x + 3);
+      P_tot = astm::powf(P_tot, beta);
+    }
+    float P_M;
+    if (S_M < NU_M_MAX2) {
+      P_M = 1.0f;
+    } else if (S_M < NU_M_MAX) {
+      const float x = (S_M - NU_M_MAX2) / (NU_M_MAX - NU_M_MAX2);
+      P_M = (float)(1.0 + (double)(3 * x * x * (-2 * x + 3)));
+    } else {
+      P_M = 4.0f;
+    }
+    return chi2a * (CF)P_tot * (CF)P_M;
+  }
+};
 
 // Per-model constants of the brightness.
 struct ModelConsts {
@@ -626,6 +758,11 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
     K.bcp = b * c;
   }
   const float bcpf = (float)K.bcp;  // mixed form
+  float nu_t[NU ? NU_M_MAX : 1], nu_V[NU ? NU_M_MAX : 1];
+  NudgeState nu;
+  nu.t_mod = nu_t;
+  nu.V_mod = nu_V;
+  MinimaPtr mr = (MinimaPtr)(pts + n_data);
   double trendA = 0.0, t_split = 0.0;
   if constexpr (TR) trendA = params[P::A];
   bool switched = false;
@@ -708,7 +845,10 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
     }
     if constexpr (TR) Vmod = __builtin_fma(-trendA, pr.alpha, Vmod);  // cuda.c:692-697
     if constexpr (FORM == FORM_PLOT) {
-      vplot[(size_t)i * vstride] = Vmod + dv0;
+      // Under NUDGE the reference stops the curve at a failed model
+      // (cuda.c:735-738); the rest of the curve is NaN here.
+      vplot[(size_t)i * vstride] = (NU && nu.failed) ? __builtin_nan("") : Vmod + dv0;
+      if constexpr (NU) nu.point(i, pr.mjd, Vmod, mr);
       continue;
     }
     const double w = (double)pr.w;
@@ -725,8 +865,12 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
           sy[k] = __builtin_fma(yv, w, sy[k]);
         }
     }
+    if constexpr (NU) nu.point(i, pr.mjd, Vmod, mr);
   }
   if constexpr (FORM == FORM_PLOT) return CF(0.0);
+  // A failed NUDGE model returns before delta_V is written (cuda.c:737).
+  if constexpr (NU)
+    if (nu.failed) return CF(1e30);
   CF chi2a = 0.0;  // cuda.c:884-910
   CF sum_all = 0.0;
 #pragma unroll
@@ -745,6 +889,7 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
   } else {
     chi2a = chi2a / dof;
   }
+  if constexpr (NU) chi2a = nu.merit<CF>(chi2a, dV[0], mr);
   return chi2a;
 }
 

@@ -465,6 +465,11 @@ __device__ __attribute__((noinline)) CF eval_lane(const double* PRM, CF* DV, siz
 #pragma unroll
   for (int i = 0; i < N; ++i) prm[i] = PRM[(size_t)i * nf + fit];
   CF dv[NF];
+  if constexpr (VarF<VC>::NU) {
+    // a failed NUDGE model leaves delta_V as the lane's last evaluation left it
+#pragma unroll
+    for (int m = 0; m < NF; ++m) dv[m] = m < n_filters ? DV[(size_t)m * rs + fo] : CF(0);
+  }
   CF fv = chi2_model<CF, VC, NF, MX>(prm, pts, n_data, n_filters, dof, sum_w, dv);
 #pragma unroll
   for (int m = 0; m < NF; ++m)
@@ -627,6 +632,8 @@ __global__ __launch_bounds__(256) void chi2_batch_kernel(const PointRec* pts_g, 
 #pragma unroll
   for (int i = 0; i < N; ++i) prm[i] = params[(size_t)k * N + i];
   CF dv[NF];
+#pragma unroll
+  for (int m = 0; m < NF; ++m) dv[m] = NU ? CF(__builtin_nan("")) : CF(0);  // NaN: a failed NUDGE model
   CF c = chi2_model<CF, VC, NF, MX>(prm, pts, n_data, n_filters, dof, sum_w, dv);
   chi2[k] = c;
   for (int m = 0; m < n_filters; ++m) dV[(size_t)k * n_filters + m] = dv[m];
@@ -653,6 +660,8 @@ __global__ __launch_bounds__(256) void model_curve_kernel(const PointRec* pts_g,
 #pragma unroll
   for (int i = 0; i < N; ++i) prm[i] = params[(size_t)k * N + i];
   CF dv[NF];
+#pragma unroll
+  for (int m = 0; m < NF; ++m) dv[m] = NU ? CF(__builtin_nan("")) : CF(0);  // NaN: a failed NUDGE model
   CF c = chi2_model<CF, VC, NF, MX>(prm, (PointPtr)(uintptr_t)pts_g, n_data, n_filters, dof, sum_w, dv);
   chi2[k] = c;
   for (int m = 0; m < n_filters; ++m) dV[(size_t)k * n_filters + m] = dv[m];

@@ -6,8 +6,9 @@
 // stdout and result-file lines, on top of the C ABI of libasteroid_hip.so.
 // The reference chooses its model variant at compile time (makefile:3-36);
 // here the same macros are run-time flags (-DTORQUE, -DTORQUE2, -DP_PSI,
-// -DBC, -DRECT, -DROTATE, -DTREND, -DBW_BALL, -DINTERP, -DACC), defaulting to
-// -DP_PSI -DTORQUE -DINTERP;
+// -DBC, -DRECT, -DROTATE, -DTREND, -DBW_BALL, -DNUDGE, -DV1S, -DINTERP, -DACC),
+// defaulting to -DP_PSI -DTORQUE -DINTERP; with -DNUDGE the observed minima
+// come from observed.min in the working directory;
 // -DMIXED (no counterpart in the reference) selects the mixed-precision form.
 // A failing library call prints the error and exits with status 1 (the
 // reference's ERR macro, cuda_errors.h:4-14).
@@ -330,7 +331,8 @@ void usage() {
   printf("-seed SEED : use the SEED number to initialize the random number generator\n");
   printf("-t : travelling reoptimization\n");
   printf("-dx value : maximum shift for parameters in scale-free units (0...1), with -DRMSD\n");
-  printf("-D<MACRO> : model variant (TORQUE, TORQUE2, P_PSI, BC, RECT, ROTATE, TREND, BW_BALL, INTERP, ACC, MIXED)\n"
+  printf("-D<MACRO> : model variant (TORQUE, TORQUE2, P_PSI, BC, RECT, ROTATE, TREND, BW_BALL, NUDGE, V1S, INTERP,\n"
+         "            ACC, MIXED)\n"
          "            and mode (RMSD, SPHERICAL_K);\n"
          "            default -DP_PSI -DTORQUE -DINTERP\n");
   printf("-blocks n, -bsize n : reseed groups per launch and simplexes per group (N_BLOCKS, BSIZE)\n");
@@ -341,6 +343,38 @@ void usage() {
 }
 
 }  // namespace
+
+// NUDGE: the observed minima of observed.min (prepare_chi2_params,
+// misc.c:279-323): "t V" per line, read as floats; only minima within 0.05
+// days of the data range are kept, at most NOBS_MAX of them, with times on
+// the curve's clock.
+void load_minima(ast_ctx* ctx, const Ingested& D) {
+  FILE* f1 = fopen("observed.min", "r");
+  if (!f1) die("Cannot open observed.min!");
+  char line[1024];
+  std::vector<float> t_min, V_min;
+  const double last = D.obs.back().mjd;
+  while (fgets(line, sizeof line, f1)) {
+    float t_obs = 0, V_obs = 0;
+    if (sscanf(line, "%f %f", &t_obs, &V_obs) != 2) continue;
+    const double small = 0.05;
+    if (t_obs >= D.mjd0 - small && t_obs <= last + D.mjd0 + small) {
+      if ((int)t_min.size() >= AST_MAX_MINIMA) {
+        printf("Too many lines in observed.min file! (>NOBS_MAX)!\n");
+        exit(1);
+      }
+      t_min.push_back((float)(t_obs - D.mjd0));
+      V_min.push_back(V_obs);
+    }
+  }
+  fclose(f1);
+  if (t_min.empty()) {
+    printf("No local minima in the data range!\n");
+    exit(1);
+  }
+  printf("%d minima in observed.min\n", (int)t_min.size());
+  CHECK(ast_set_minima(ctx, t_min.data(), V_min.data(), (int)t_min.size()));
+}
 
 int main(int argc, char** argv) {
   if (argc == 1) {
@@ -380,6 +414,8 @@ int main(int argc, char** argv) {
       else if (mac == "TORQUE2") v.torque2 = v.torque = 1;  // asteroid.h:32-34
       else if (mac == "ROTATE") v.rotate = 1;
       else if (mac == "RECT") v.rect = v.bc = 1;            // asteroid.h:36-38
+      else if (mac == "NUDGE") v.nudge = 1;
+      else if (mac == "V1S") v.v1s = 1;
       else if (mac == "INTERP") v.interp = 1;
       else if (mac == "ACC") acc = 1;
       else if (mac == "MIXED") acc = 2;
@@ -497,6 +533,7 @@ int main(int argc, char** argv) {
   CHECK(rc);
   CHECK(ast_set_model_space(ctx, n_params, property, n_types, types, limits, &x2));
   CHECK(ast_upload_curve(ctx, D.obs.data(), N_data, N_filters, &D.eph));
+  if (v.nudge) load_minima(ctx, D);
   if (plot) {
     int rc_plot = plot_model(ctx, v, D, params, property, n_params, acc != 1);
     CHECK(ast_ctx_destroy(ctx));

@@ -250,4 +250,8 @@ template <class R> AST_HD R exp(R x) {
 // the double result rounded to float.
 AST_HD float expf(float x) { return (float)exp<double>((double)x); }
 
+// Single-precision power for x >= 0 (the NUDGE merit's powf): exp(y log x)
+// in double, rounded to float; 0 for x = 0 and y > 0, 1 for x = 1.
+AST_HD float powf(float x, float y) { return (float)exp<double>((double)y * log<double>((double)x)); }
+
 }  // namespace astm

@@ -18,10 +18,12 @@ TMAX = 10.0                        # asteroid.c:507
 
 
 def make_variant(torque=True, bc=False, bw_ball=False, p_psi=True, interp=True,
-                 precision=abi.PREC_FP64, time_step=1e-2, trend=False, torque2=False, rotate=False, rect=False):
+                 precision=abi.PREC_FP64, time_step=1e-2, trend=False, torque2=False, rotate=False, rect=False,
+                 nudge=False, v1s=False):
     """The reference's macros; TORQUE2 implies TORQUE and RECT implies BC (asteroid.h:32-38)."""
     return abi.Variant(int(torque or torque2), int(bc or rect), int(bw_ball), int(p_psi), int(interp),
-                       int(precision), float(time_step), int(trend), int(torque2), int(rotate), int(rect))
+                       int(precision), float(time_step), int(trend), int(torque2), int(rotate), int(rect),
+                       int(nudge), int(v1s))
 
 
 def param_positions(variant):
@@ -193,3 +195,9 @@ def bc_rotate(precision=abi.PREC_FP64, time_step=1e-2, interp=True, torque=False
 def rect(precision=abi.PREC_FP64, time_step=1e-2, interp=True, torque=False):
     """Rectangular prism brightness (-DRECT), 10 (13 with torque) parameters."""
     return make_variant(torque, True, False, True, interp, precision, time_step, rect=True)
+
+
+def ls_torque_nudge(precision=abi.PREC_FP64, time_step=1e-2, interp=True, bc=False, v1s=False):
+    """Stage Three (README.md:87-101): LS (or BC) with torque, chi^2 nudged by
+    the observed minima (-DNUDGE), 11 (13 with BC) parameters."""
+    return make_variant(True, bc, False, True, interp, precision, time_step, nudge=True, v1s=v1s)

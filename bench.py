@@ -577,7 +577,18 @@ VARIANT_RUNS = {
     "bc_rotate_torque": (lambda p, ts: variants_mod().bc_rotate(p, time_step=ts, torque=True), 1),
     "rect": (lambda p, ts: variants_mod().rect(p, time_step=ts), 1),
     "rect_torque": (lambda p, ts: variants_mod().rect(p, time_step=ts, torque=True), 1),
+    "nudge_torque": (lambda p, ts: variants_mod().ls_torque_nudge(p, time_step=ts), 1),
 }
+
+
+def minima_of(v, curve, truth, k=4):
+    """k observed minima for a NUDGE run: local maxima of the ground truth's
+    magnitudes (the synthetic model, not the fitted one) at the observations."""
+    from asteroid_amd import synth
+    vm = synth.truth_vmod(v, truth, curve.mjd, curve.E, curve.S)
+    idx = [i - 1 for i in range(2, len(vm)) if vm[i - 1] > vm[i - 2] and vm[i - 1] >= vm[i]]
+    pick = idx[::max(1, len(idx) // k)][:k]
+    return (np.array([curve.mjd[i] for i in pick], np.float32), np.array([vm[i] for i in pick], np.float32))
 
 
 def variants_mod():
@@ -594,13 +605,23 @@ def main_variants(args):
     prec = {"both": MIXED, "mixed": MIXED, "fp64": FP64, "f32simplex": F32SIMPLEX}[args.precision]
     group = 256
     rows = {}
-    for name, (mk, nf) in VARIANT_RUNS.items():
+    names = args.variant_list.split(",") if args.variant_list else list(VARIANT_RUNS)
+    for name in names:
+        mk, nf = VARIANT_RUNS[name]
         v = mk(prec, 1e-2)
+        if v.nudge and prec == MIXED:
+            continue  # NUDGE runs in the FP64 and float-simplex forms only
         curve, truth = synth.make_curve(2024, v, 400, 3.0, n_filters=nf, sigma_range=(0.05, 0.3))
-        space = variants.default_model_space(v, ppsi_hours=(2.0, 4800.0), reopt=0)
+        # NUDGE is Stage Three (README.md:87-101): a reoptimisation from a model
+        # (here the truth perturbed by 2%), not a random search
+        stage3 = bool(v.nudge)
+        space = variants.default_model_space(v, ppsi_hours=(2.0, 4800.0), reopt=1 if stage3 else 0)
         with api.Context(v) as ctx:
             ctx.set_model_space(space)
             ctx.upload_curve(curve)
+            if stage3:
+                ctx.set_minima(*minima_of(v, curve, truth))
+                ctx.set_start(truth * (1.0 + np.random.default_rng(3).uniform(-0.02, 0.02, truth.shape)))
             ctx.rng_seed(5)
             run = dict(nstages=1, n_steps=5000, eval_budget=args.budget, size_min=SIZE_MIN, use_graphs=args.graphs)
             ctx.run(args.fits // group // 4, group, **run)  # warm-up
@@ -610,16 +631,24 @@ def main_variants(args):
             wall = time.perf_counter() - t0
             e, pts = ctx.last_counts()
             ms, nl = ctx.last_timing()
+            # the evaluator alone: one chi^2 per lane for 2^17 models near the
+            # truth (no simplex, every lane busy to the end)
+            near = truth * (1.0 + np.random.default_rng(1).uniform(-0.02, 0.02, (1 << 17, truth.size)))
+            ctx.chi2_batch(near[:1024])
+            ctx.chi2_batch(near)
+            ev_ms, _ = ctx.last_timing()
         split = truth[variants.param_positions(v)["Tt"]] if v.torque2 else None
         fl = flops.eval_flops(v, curve.mjd, split)
         peak, mix = peak_of(prec, v, curve.mjd)
         achieved = fl * e / (ms / 1e3) / 1e12
-        rows[name] = {"n_params": space.n_params, "value": pts / wall, "fits_per_s": args.fits / wall,
+        rows[name] = {"n_params": space.n_params, "stage": "three (reopt)" if stage3 else "one", "value": pts / wall, "fits_per_s": args.fits / wall,
                       "evals_per_fit": round(e / args.fits, 1), "kernel_ms": round(ms, 2), "launches": nl,
                       "flop_per_model_point": round(fl / curve.n_data, 2), "achieved_tflops": round(achieved, 3),
+                      "chi2_batch_2p17_ms": round(ev_ms, 3),
+                      "chi2_batch_model_points_per_s": (1 << 17) * curve.n_data / (ev_ms / 1e3),
                       "peak_tflops": round(peak, 2), "frac": round(achieved / peak, 4)}
         log(f"{name}: {pts / wall:.3e} model-points/s, {achieved:.2f} TFLOP/s ({achieved / peak:.1%} of {peak:.1f})")
-    head = rows["ls_torque"]
+    head = rows[names[0]]
     out = base_line(head["value"], 1, 1, None, DTYPE[prec],
                     {"workload": "stage-one random search per variant (400 points, 3 days, TIME_STEP 0.01)",
                      "simplexes": args.fits, "evals_per_simplex": args.budget, "group_size": group,
@@ -637,6 +666,7 @@ def main():
                                            "variants"],
                     default="stage1")
     ap.add_argument("--plot-models", type=int, default=4096)
+    ap.add_argument("--variant-list", default="", help="comma-separated subset of the variants workload")
     ap.add_argument("--rmsd-samples", type=int, default=64)
     ap.add_argument("--dx", type=float, default=0.003)
     ap.add_argument("--precision", choices=["both", "mixed", "fp64", "f32simplex"], default="both")

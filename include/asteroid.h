@@ -40,7 +40,7 @@ enum {
   AST_HARD_RIGHT = -3,
   AST_HARD_BOTH = -4
 };
-enum { AST_MAX_PARAMS = 16, AST_MAX_TYPES = 16, AST_MAX_FILTERS = 8 };
+enum { AST_MAX_PARAMS = 16, AST_MAX_TYPES = 16, AST_MAX_FILTERS = 8, AST_MAX_MINIMA = 10 /* NOBS_MAX, asteroid.h:219 */ };
 
 /* Error codes. */
 enum {
@@ -67,9 +67,9 @@ enum {
  * chosen at run time here.  TORQUE2 implies TORQUE and RECT implies BC, as in
  * the reference (asteroid.h:32-38).  The kernels are built for LS, BC and
  * BW_BALL with and without torque, and for TREND with torque, TORQUE2,
- * ROTATE (with BC, with and without torque) and RECT (with and without
- * torque); other combinations return AST_E_UNSUPPORTED.  SEGMENT, P_PHI,
- * P_BOTH and NUDGE are not built. */
+ * ROTATE (with BC, with and without torque), RECT (with and without torque)
+ * and NUDGE (with torque, with and without BC); other combinations return
+ * AST_E_UNSUPPORTED.  SEGMENT, P_PHI and P_BOTH are not built. */
 typedef struct {
   int torque;     /* -DTORQUE : 6-ODE constant-torque rotation                 */
   int bc;         /* -DBC     : photometric b,c independent of kinematic ones  */
@@ -82,6 +82,9 @@ typedef struct {
   int torque2;    /* -DTORQUE2: a second torque after the split time Tt (cuda.c:288-317)       */
   int rotate;     /* -DROTATE : brightness ellipsoid rotated by theta_R, phi_R, psi_R (cuda.c:475-516) */
   int rect;       /* -DRECT   : rectangular prism brightness, phase 0 (cuda.c:609-661)          */
+  int nudge;      /* -DNUDGE  : chi^2 nudged by model minima near observed ones (cuda.c:716-758, 912-992);
+                   *            needs ast_set_minima                                               */
+  int v1s;        /* -DV1S    : with NUDGE, deeper model minima are not penalised (cuda.c:940-945) */
 } ast_variant;
 
 /* Parameter type ids of a variant: the reference's __COUNTER__ order
@@ -195,6 +198,16 @@ int ast_set_model_space(ast_ctx* ctx, int n_params, const int* property,
  * at most AST_MAX_FILTERS filters.  The data are copied. */
 int ast_upload_curve(ast_ctx* ctx, const ast_obs* obs, int n_data, int n_filters,
                      const ast_ephem3* eph);
+
+/* NUDGE: the observed brightness minima, replacing prepare_chi2_params'
+ * d_chi2_params upload (misc.c:279-323).  t_obs are times on the curve's
+ * clock (the reference's MJD - hMJD0, kept as float) and V_obs magnitudes
+ * in the converted system; 1 <= n_obs <= AST_MAX_MINIMA, else AST_E_ARG.
+ * The caller keeps only minima inside the data range, as misc.c:294-296
+ * does.  AST_E_STATE on a context whose variant is not NUDGE.  Every
+ * evaluation of a NUDGE context needs it (AST_E_STATE until set); it may
+ * come before or after ast_upload_curve. */
+int ast_set_minima(ast_ctx* ctx, const float* t_obs, const float* V_obs, int n_obs);
 
 /* Seed the per-simplex random streams and reset each group's best chi^2 to
  * 1e30: setup_kernel(..., seed, d_f, 1) (cuda.c:2039-2054, asteroid.c:629-634).

@@ -17,7 +17,8 @@ class OrcProblem(C.Structure):
                 ("property", C.c_int * (abi.MAX_PARAMS * abi.N_COLUMNS)),
                 ("types", C.c_int * abi.MAX_TYPES), ("limits", C.c_double * (2 * abi.MAX_TYPES)),
                 ("x2", abi.X2), ("n_data", C.c_int), ("n_filters", C.c_int),
-                ("obs", C.POINTER(abi.Obs)), ("eph", abi.Ephem3)]
+                ("obs", C.POINTER(abi.Obs)), ("eph", abi.Ephem3), ("n_min", C.c_int),
+                ("t_min", C.c_float * abi.MAX_MINIMA), ("V_min", C.c_float * abi.MAX_MINIMA)]
 
 
 def _has_fma():
@@ -71,8 +72,9 @@ def _p(a, ct=C.c_double):
 class Problem:
     """A model space plus a curve, held in the layout the oracle reads."""
 
-    def __init__(self, space, curve):
-        self.space, self.curve = space, curve
+    def __init__(self, space, curve, minima=None):
+        """minima = (t_obs, V_obs) for NUDGE variants (times on the curve's clock)."""
+        self.space, self.curve, self.minima = space, curve, minima
         pb = OrcProblem()
         pb.v = space.variant
         pb.n_params = space.n_params
@@ -89,10 +91,16 @@ class Problem:
         pb.n_filters = curve.n_filters
         pb.obs = C.cast(self._obs, C.POINTER(abi.Obs))
         pb.eph = curve.ephem3()
+        if minima is not None:
+            t_obs, V_obs = minima
+            pb.n_min = len(t_obs)
+            for k in range(len(t_obs)):
+                pb.t_min[k] = float(t_obs[k])
+                pb.V_min[k] = float(V_obs[k])
         self.pb = pb
 
     def with_space(self, space):
-        return Problem(space, self.curve)
+        return Problem(space, self.curve, self.minima)
 
     def chi2(self, params, form=CANON, want_vmod=False, want_traj=False):
         params = np.ascontiguousarray(np.atleast_2d(params), np.float64)
@@ -193,7 +201,7 @@ def periodogram(t, vm, n_clusters=5):
 
 
 MATH = {"sin": 0, "cos": 1, "tan": 2, "atan2": 3, "log": 4, "log10": 5, "exp": 6, "asin": 7,
-        "atan": 8, "expf": 9, "sin_mx": 10, "cos_mx": 11, "acos": 12}
+        "atan": 8, "expf": 9, "sin_mx": 10, "cos_mx": 11, "acos": 12, "powf": 13}
 
 
 def math(fn, x, y=None):

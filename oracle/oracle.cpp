@@ -61,6 +61,9 @@ struct Problem {
   int n_data, n_filters;
   const ast_obs* obs;
   ast_ephem3 eph;
+  // NUDGE's observed minima (chi2_struct, asteroid.h:319-337)
+  int n_min;
+  float t_min[AST_MAX_MINIMA], V_min[AST_MAX_MINIMA];
   // TORQUE2 implies TORQUE and RECT implies BC (asteroid.h:32-38)
   bool tq() const { return v.torque || v.torque2; }
   bool bc() const { return v.bc || v.rect; }
@@ -139,9 +142,119 @@ inline void rk4_interval(double t1, double t2, double time_step, int* nsteps, do
 }
 
 // ============================================================ literal chi2one
+// ------------------------------------------------------------------ NUDGE
+// The model-minima bookkeeping of chi2one (cuda.c:146-151, 716-758) and the
+// merit that uses it (cuda.c:912-992), with the reference's declared types:
+// the expressions are written as there, so C's promotions decide which
+// parts run in float and which in double.  Constants: asteroid.h:216-230.
+namespace nudge {
+const int M_MAX = 30;
+const float M_MAX2 = 20;
+const float DT_MAX = 0.12;
+const float DV_MAX = 24;
+const float DT_MAX2 = 1.5 * DT_MAX;
+const float P_MIN = 0.01;
+const float CHI2_0 = 10;
+const float CHI2_1 = 30;
+const float L_RC = 0.05;
+const float L_RC2 = L_RC * L_RC;
+const float L_A = 1.0 / (1.0 - L_RC2 / (1.0 + L_RC2));
+const float P_MIN2 = 1 / P_MIN - 1;
+}  // namespace nudge
+
+struct Nudge {
+  int M = 0;
+  bool failed = false;  // the reference returns 1e30 at once (cuda.c:735-738)
+  float t_mod[nudge::M_MAX], V_mod[nudge::M_MAX];
+  float t_old[2], V_old[2];
+
+  // called once per point i (0-based within the curve) with its model magnitude
+  void point(const Problem& P, int i, double MJD, double Vmod) {
+    using namespace nudge;
+    if (failed) return;
+    if (i < 2) {
+      t_old[i] = MJD;
+      V_old[i] = Vmod;
+      return;
+    }
+    if (V_old[1] > V_old[0] && V_old[1] >= Vmod) {
+      bool local = 0;
+      for (int ii = 0; ii < P.n_min; ii++)
+        if (std::fabs(t_old[1] - P.t_min[ii]) < DT_MAX2) local = 1;
+      if (local) {
+        M++;
+        if (M > M_MAX) {
+          failed = true;
+          return;
+        }
+        double a = ((Vmod - V_old[1]) / (MJD - t_old[1]) - (V_old[1] - V_old[0]) / (t_old[1] - t_old[0])) /
+                   (MJD - t_old[0]);
+        double b = (V_old[1] - V_old[0]) / (t_old[1] - t_old[0]) - a * (t_old[1] + t_old[0]);
+        double c = V_old[1] - a * t_old[1] * t_old[1] - b * t_old[1];
+        t_mod[M - 1] = -b / 2.0 / a;
+        V_mod[M - 1] = a * t_mod[M - 1] * t_mod[M - 1] + b * t_mod[M - 1] + c;
+      }
+    }
+    t_old[0] = t_old[1];
+    V_old[0] = V_old[1];
+    t_old[1] = MJD;
+    V_old[1] = Vmod;
+  }
+
+  // powf: glibc's in the literal form, the project's in the canonical one
+  static float powf_form(int form, float x, float y) { return form == 0 ? ::powf(x, y) : astm::powf(x, y); }
+
+  template <class CF>
+  CF merit(const Problem& P, int form, CF chi2a, const CF* delta_V) const {
+    using namespace nudge;
+    float S_M = 0.0;
+    float P_tot = 1.0;
+    for (int imod = 0; imod < M; imod++) {
+      for (int iobs = 0; iobs < P.n_min; iobs++) {
+        float dt = std::fabs(t_mod[imod] - P.t_min[iobs]);
+        if (dt < DT_MAX2) {
+          if (dt > DT_MAX) {
+            float x = (DT_MAX2 - dt) / (DT_MAX2 - DT_MAX);
+            S_M = S_M + x * x * (-2.0 * x + 3.0);
+          } else {
+            S_M = S_M + 1.0;
+            float dV = V_mod[imod] + delta_V[0] - P.V_min[iobs];
+            if (P.v.v1s && dV > 0.0) dV = 0.0;
+            float x = std::sqrt(dt * dt / DT_MAX / DT_MAX + dV * dV / DV_MAX / DV_MAX);
+            if (x < 1.0) {
+              float P_i = L_A * x * x / (x * x + L_RC2);
+              P_tot = P_tot * (P_MIN * (1.0 + P_MIN2 * P_i));
+            }
+          }
+        }
+      }
+    }
+    P_tot = powf_form(form, P_tot, 1.0 / P.n_min);
+    if (P_tot < P_MIN) P_tot = P_MIN;
+    if (chi2a > CHI2_1) {
+      P_tot = 1.0;
+    } else if (chi2a > CHI2_0) {
+      float x = (CHI2_1 - chi2a) / (CHI2_1 - CHI2_0);
+      float beta = x * x * (-2 * x + 3);
+      P_tot = powf_form(form, P_tot, beta);
+    }
+    float P_M;
+    if (S_M < M_MAX2) {
+      P_M = 1.0;
+    } else if (S_M < M_MAX) {
+      float x = (S_M - M_MAX2) / (M_MAX - M_MAX2);
+      P_M = 1.0 + 3 * x * x * (-2 * x + 3);
+    } else {
+      P_M = 4.0;
+    }
+    return chi2a * P_tot * P_M;
+  }
+};
+
 struct ModelOut {   // optional per-point outputs
   double* vmod;     // n_data
   double* traj;     // n_data*6: phi, theta, psi, Omega_i, Omega_s, Omega_l
+  Nudge* nudge;     // NUDGE's model minima (the magnitudes after a failure are NaN)
 };
 
 inline void ode_literal(bool torque, const double* y, double* f, const double* mu) {
@@ -347,7 +460,8 @@ inline void model_literal(const Problem& P, const double* params, double* sy2, d
       double alpha = std::acos(Sp_b * Ep_b + Sp_c * Ep_c + Sp_a * Ep_a);
       Vmod = Vmod - params[P.types[T.A]] * alpha;
     }
-    if (out && out->vmod) out->vmod[i] = Vmod;
+    if (out && out->vmod) out->vmod[i] = (out->nudge && out->nudge->failed) ? std::nan("") : Vmod;
+    if (out && out->nudge) out->nudge->point(P, i, P.obs[i].mjd, Vmod);
     int m = P.obs[i].filter;  // cuda.c:708-714
     double y = P.obs[i].V - Vmod;
     sy2[m] = sy2[m] + y * y * P.obs[i].w;
@@ -633,7 +747,8 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
       const double alpha = astm::acos(std::fma(Sd[2], Ed[2], std::fma(Sd[1], Ed[1], Sd[0] * Ed[0])));
       Vmod = p_fma(-R(params[P.types[T.A]]), R(alpha), Vmod);
     }
-    if (out && out->vmod) out->vmod[i] = p_val(Vmod);
+    if (out && out->vmod) out->vmod[i] = (out->nudge && out->nudge->failed) ? std::nan("") : p_val(Vmod);
+    if (out && out->nudge) out->nudge->point(P, i, P.obs[i].mjd, p_val(Vmod));
     int m = P.obs[i].filter;
     R w = R((double)P.obs[i].w);
     R y = R((double)P.obs[i].V) - Vmod;
@@ -656,12 +771,21 @@ inline CF chi2_from_sums(const Problem& P, const double* sy2, const double* sy, 
   return chi2a;
 }
 
+// chi^2 of one model; under NUDGE a failed model returns 1e30 with delta_V
+// untouched (cuda.c:737), else the nudged chi^2.
 template <class CF>
 inline CF chi2_eval(const Problem& P, int form, const double* params, CF* dV, ModelOut* out = nullptr) {
   double sy2[AST_MAX_FILTERS], sy[AST_MAX_FILTERS], sw[AST_MAX_FILTERS];
-  if (form == 0) model_literal(P, params, sy2, sy, sw, out);
-  else model_canon<double>(P, params, sy2, sy, sw, out);
-  return chi2_from_sums<CF>(P, sy2, sy, sw, dV);
+  Nudge nu;
+  ModelOut o{nullptr, nullptr, nullptr};
+  if (out) o = *out;
+  if (P.v.nudge) o.nudge = &nu;
+  if (form == 0) model_literal(P, params, sy2, sy, sw, &o);
+  else model_canon<double>(P, params, sy2, sy, sw, &o);
+  if (P.v.nudge && nu.failed) return CF(1e30);
+  CF c = chi2_from_sums<CF>(P, sy2, sy, sw, dV);
+  if (P.v.nudge) c = nu.merit<CF>(P, form, c, dV);
+  return c;
 }
 
 // ================================================================ x2params
@@ -1039,9 +1163,14 @@ inline CF rmsd_from_sums(const Problem& P, const double* sy2, const double* sy, 
 template <class CF>
 inline CF rmsd_eval(const Problem& P, int form, const double* params, CF* dV) {
   double sy2[AST_MAX_FILTERS], sy[AST_MAX_FILTERS], sw[AST_MAX_FILTERS];
-  if (form == 0) model_literal(P, params, sy2, sy, sw, nullptr);
-  else model_canon<double>(P, params, sy2, sy, sw, nullptr);
-  return rmsd_from_sums<CF>(P, sy2, sy, sw, dV);
+  Nudge nu;
+  ModelOut o{nullptr, nullptr, P.v.nudge ? &nu : nullptr};
+  if (form == 0) model_literal(P, params, sy2, sy, sw, &o);
+  else model_canon<double>(P, params, sy2, sy, sw, &o);
+  if (P.v.nudge && nu.failed) return CF(1e30);
+  CF c = rmsd_from_sums<CF>(P, sy2, sy, sw, dV);
+  if (P.v.nudge) c = nu.merit<CF>(P, form, c, dV);
+  return c;
 }
 
 struct RmsdCfg {
@@ -1193,7 +1322,8 @@ inline CF model_curve(const Problem& P, int form, const double* params, const st
   G.n_data = (int)grid.size();
   G.n_filters = 1;
   double sy2[AST_MAX_FILTERS], sy[AST_MAX_FILTERS], sw[AST_MAX_FILTERS];
-  ModelOut out{vmod, nullptr};
+  Nudge nu;  // the grid pass finds model minima too, and stops the curve on failure
+  ModelOut out{vmod, nullptr, P.v.nudge ? &nu : nullptr};
   if (form == 0) model_literal(G, params, sy2, sy, sw, &out);
   else model_canon<double>(G, params, sy2, sy, sw, &out);
   for (int i = 0; i < G.n_data; ++i) vmod[i] = vmod[i] + dV[0];
@@ -1319,6 +1449,9 @@ typedef struct {
   int n_filters;
   const ast_obs* obs;
   ast_ephem3 eph;
+  int n_min;                      /* NUDGE: observed minima (misc.c:279-323) */
+  float t_min[AST_MAX_MINIMA];
+  float V_min[AST_MAX_MINIMA];
 } orc_problem;
 
 static int load_problem(const orc_problem* in, orc::Problem* P) {
@@ -1341,6 +1474,12 @@ static int load_problem(const orc_problem* in, orc::Problem* P) {
   P->n_filters = in->n_filters;
   P->obs = in->obs;
   P->eph = in->eph;
+  if (in->v.nudge && (in->n_min < 1 || in->n_min > AST_MAX_MINIMA)) return AST_E_ARG;
+  P->n_min = in->v.nudge ? in->n_min : 0;
+  for (int k = 0; k < AST_MAX_MINIMA; ++k) {
+    P->t_min[k] = in->t_min[k];
+    P->V_min[k] = in->V_min[k];
+  }
   return 0;
 }
 
@@ -1357,10 +1496,12 @@ int orc_chi2(const orc_problem* pb, int form, const double* params, int n, doubl
     const double* p = params + (size_t)k * P.n_params;
     if (simplex_float(P.v)) {
       float d[AST_MAX_FILTERS];
+      std::fill(d, d + AST_MAX_FILTERS, std::nanf(""));  // stays NaN for a failed NUDGE model
       chi2[k] = orc::chi2_eval<float>(P, form, p, d, &out);
       for (int m = 0; m < P.n_filters; ++m) dV[(size_t)k * P.n_filters + m] = d[m];
     } else {
       double d[AST_MAX_FILTERS];
+      std::fill(d, d + AST_MAX_FILTERS, std::nan(""));
       chi2[k] = orc::chi2_eval<double>(P, form, p, d, &out);
       for (int m = 0; m < P.n_filters; ++m) dV[(size_t)k * P.n_filters + m] = d[m];
     }
@@ -1383,10 +1524,12 @@ int orc_model_curve(const orc_problem* pb, int form, const double* params, int n
     double* vm = vmod + (size_t)k * n_plot;
     if (simplex_float(P.v)) {
       float d[AST_MAX_FILTERS];
+      std::fill(d, d + AST_MAX_FILTERS, std::nanf(""));  // stays NaN for a failed NUDGE model
       chi2[k] = orc::model_curve<float>(P, form, p, grid, d, vm);
       for (int m = 0; m < P.n_filters; ++m) dV[(size_t)k * P.n_filters + m] = d[m];
     } else {
       double d[AST_MAX_FILTERS];
+      std::fill(d, d + AST_MAX_FILTERS, std::nan(""));
       chi2[k] = orc::model_curve<double>(P, form, p, grid, d, vm);
       for (int m = 0; m < P.n_filters; ++m) dV[(size_t)k * P.n_filters + m] = d[m];
     }
@@ -1528,6 +1671,7 @@ int orc_math(int fn, const double* x, const double* y, double* out, int n) {
       case 10: { double sn, cs; astm::sincos_mx(a, &sn, &cs); out[k] = sn; break; }
       case 11: { double sn, cs; astm::sincos_mx(a, &sn, &cs); out[k] = cs; break; }
       case 12: out[k] = astm::acos(a); break;
+      case 13: out[k] = astm::powf((float)a, (float)y[k]); break;
       default: return AST_E_ARG;
     }
   }

@@ -121,3 +121,17 @@ def test_mixed_form_sincos(name, ref):
     want = np.array([float(ref(mp.mpf(v))) for v in x])
     err = np.abs(got - want) / np.maximum(np.abs(want), 1.0)
     assert err.max() < 1e-13
+
+
+def test_powf_is_rounded_double_pow():
+    """NUDGE's powf: exp(y log x) in double rounded to float, within one float
+    ulp of the exact power over the merit's range (x in [1e-40, 1], y in
+    (0, 1]); 1 at x = 1, 0 at x = 0."""
+    r = np.random.default_rng(4)
+    x = np.exp(r.uniform(np.log(1e-38), 0.0, 2000)).astype(np.float32).astype(np.float64)
+    y = r.uniform(0.05, 1.0, 2000).astype(np.float32).astype(np.float64)
+    got = oracle.math("powf", x, y)
+    want = np.array([float(mp.power(mp.mpf(a), mp.mpf(b))) for a, b in zip(x, y)])
+    assert np.all(got == got.astype(np.float32))
+    assert np.max(np.abs(got - want) / np.spacing(np.float32(want))) <= 1.0
+    assert oracle.math("powf", np.array([1.0, 0.0]), np.array([0.3, 0.5])).tolist() == [1.0, 0.0]

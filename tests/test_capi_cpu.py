@@ -38,6 +38,9 @@ def test_strerror_and_validation_without_device():
     assert lib.ast_ctx_create(C.byref(h), C.byref(bad), 0) == -6
     bad = variants.make_variant(True, False, False, trend=True, torque2=True)  # no kernel for this mix
     assert lib.ast_ctx_create(C.byref(h), C.byref(bad), 0) == -6
+    bad = variants.ls_torque_nudge(abi.PREC_MIXED)  # NUDGE is offered in the FP64 and float-simplex forms
+    assert lib.ast_ctx_create(C.byref(h), C.byref(bad), 0) == -6
+    assert lib.ast_set_minima(None, None, None, 1) == -1
     assert lib.ast_run(None, None) == -1
     assert lib.ast_fetch(None, None, None, None) == -1
 
@@ -45,7 +48,8 @@ def test_strerror_and_validation_without_device():
 VARS = [variants.ls_torque(), variants.bc_free(), variants.bw_ball_torque(),
         variants.make_variant(False, False, False, True, True), variants.make_variant(True, True, False, True, True),
         variants.make_variant(False, False, True, False, False), variants.ls_torque_trend(), variants.ls_torque2(),
-        variants.bc_rotate(), variants.bc_rotate(torque=True), variants.rect(), variants.rect(torque=True)]
+        variants.bc_rotate(), variants.bc_rotate(torque=True), variants.rect(), variants.rect(torque=True),
+        variants.ls_torque_nudge(), variants.ls_torque_nudge(bc=True)]
 
 
 @pytest.mark.parametrize("v", VARS)

@@ -195,3 +195,36 @@ def test_driver_rmsd_matches_oracle(tmp_path):
         ranges = "".join("%13.6e %13.6e ; " % (a, b) for a, b in zip(pmin, pmax))
         assert lines[k] == "%e " % (nbad / ntot) + ranges
         assert file_lines[k] == ranges
+
+
+def test_driver_nudge_reads_observed_min(tmp_path):
+    """-DNUDGE: observed.min (absolute MJD, converted magnitude) read as floats,
+    minima outside the data range dropped (misc.c:279-323); the chi2_plot
+    line is the oracle's nudged chi^2 on the driver's own ingested data."""
+    sc = _scene(tmp_path, n_obs=80, filters="R")
+    curve = _ingested(tmp_path)
+    mjd0 = float(open(tmp_path / "dump.txt").readline().split()[2])
+    v = variants.ls_torque_nudge(abi.PREC_FP64)
+    space = variants.default_model_space(v)
+    p, rc = oracle.Problem(variants.default_model_space(variants.ls_torque(abi.PREC_FP64)), curve).x2params(
+        np.full((1, space.n_params), 0.43))
+    params = p[0]
+    lines = [(mjd0 + curve.mjd[20], 21.0), (mjd0 + curve.mjd[50], 20.5), (mjd0 - 3.0, 20.0)]
+    with open(tmp_path / "observed.min", "w") as f:
+        for t, V in lines:
+            f.write(f"{t:.6f} {V:.3f}\n")
+    f32 = np.float32
+    kept = [(f32(f32(t) - mjd0), f32(V)) for t, V in lines if mjd0 - 0.05 <= f32(t) <= curve.mjd[-1] + mjd0 + 0.05]
+    minima = (np.array([k[0] for k in kept], f32), np.array([k[1] for k in kept], f32))
+    cmd = [EXE, "-DP_PSI", "-DTORQUE", "-DINTERP", "-DNUDGE", "-DACC", "-i", "lc.txt", "-plot", "-m"] + \
+        ["%.17g" % x for x in params]
+    r = subprocess.run(cmd, cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout
+    assert "2 minima in observed.min" in r.stdout.splitlines()
+    _, _, chi2, dv = oracle.Problem(space, curve, minima=minima).model_curve(params, n_plot=20000)
+    line = [l for l in r.stdout.splitlines() if l.startswith("chi2_plot = ")]
+    assert len(line) == 1 and line[0].startswith("chi2_plot = %13.6e, lsq = " % chi2[0])
+    # no observed.min: an error, not a crash
+    (tmp_path / "observed.min").unlink()
+    r = subprocess.run(cmd, cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=60)
+    assert r.returncode != 0 and "observed.min" in r.stdout
