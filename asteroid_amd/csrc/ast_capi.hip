@@ -205,12 +205,6 @@ void interp_canon(const ast_ephem3& e, double t, double E[3], double S[3]) {
   for (int k = 0; k < 3; ++k) { E[k] = E[k] / En; S[k] = S[k] / Sn; }
 }
 
-// Phase angle of unit Sun and Earth directions, acos(S.E) in the canonical
-// form (TREND, cuda.c:694).
-double phase_angle(const double E[3], const double S[3]) {
-  return astm::acos(std::fma(S[2], E[2], std::fma(S[1], E[1], S[0] * E[0])));
-}
-
 struct CurveH {  // a validated light curve as the kernels read it
   std::vector<PointRec> pts;  // n_data observations, then the minima record (zero unless NUDGE)
   double sum_w[AST_MAX_FILTERS] = {0};
@@ -250,13 +244,14 @@ int make_curve(const ast_variant& v, const ast_obs* obs, int n_data, int n_filte
     p.w = o.w;
     p.filter = o.filter;
     p.mjd = o.mjd;
-    p.alpha = phase_angle(p.E, p.S);
-    p.ts = v.time_step;
     C.sum_w[o.filter] = C.sum_w[o.filter] + (double)o.w;
     seen[o.filter] = 1;
   }
   for (int m = 0; m < n_filters; ++m)
     if (!seen[m]) return AST_E_DATA;
+  TailRec tail{};
+  tail.ts = v.time_step;
+  std::memcpy(&C.pts[n_data], &tail, sizeof tail);
   C.n_data = n_data;
   C.n_filters = n_filters;
   C.t_first = obs[0].mjd;
@@ -290,8 +285,6 @@ int make_grid(const ast_variant& v, const CurveH& C, const ast_ephem3& eph, int 
       interp_canon(eph, t[i], p.E, p.S);
     }
     p.mjd = t[i];
-    p.alpha = phase_angle(p.E, p.S);
-    p.ts = v.time_step;
     if (i > 0) {
       const double t1 = t[i - 1], t2 = t[i];
       const int nsteps = (int)((t2 - t1) / v.time_step + 1);
@@ -795,7 +788,7 @@ struct ast_ctx {
   CurveH curve;
   PointRec* pts = nullptr;   // for ast_chi2_batch
   double* sum_w = nullptr;
-  MinimaRec minima{};         // NUDGE's observed minima (the curve table's last record)
+  TailRec minima{};           // NUDGE's observed minima (in the curve table's tail record)
   bool minima_set = false;
   double params0[AST_MAX_PARAMS]{};
   uint64_t seed = 0;
@@ -1573,7 +1566,11 @@ int ast_upload_curve(ast_ctx* c, const ast_obs* obs, int n_data, int n_filters, 
   if (!c) return AST_E_ARG;
   CurveH C;
   RETIF(make_curve(c->v, obs, n_data, n_filters, eph, &C));
-  std::memcpy(&C.pts[n_data], &c->minima, sizeof(MinimaRec));
+  {  // the tail record: the context's minima with this curve's TIME_STEP
+    TailRec tail = c->minima;
+    tail.ts = c->v.time_step;
+    std::memcpy(&C.pts[n_data], &tail, sizeof tail);
+  }
   RETIF(set_device(c));
   dev_free(c->pts);
   dev_free(c->sum_w);
@@ -1590,7 +1587,7 @@ int ast_set_minima(ast_ctx* c, const float* t_obs, const float* V_obs, int n_obs
   if (!c) return AST_E_ARG;
   if (!c->v.nudge) return AST_E_STATE;
   if (!t_obs || !V_obs || n_obs < 1 || n_obs > AST_MAX_MINIMA) return AST_E_ARG;
-  MinimaRec r{};
+  TailRec r{};
   r.n_obs = n_obs;
   r.v1s = c->v.v1s ? 1 : 0;
   for (int k = 0; k < n_obs; ++k) {
@@ -1602,6 +1599,7 @@ int ast_set_minima(ast_ctx* c, const float* t_obs, const float* V_obs, int n_obs
   c->minima_set = true;
   if (c->curve_set) {
     const int n = c->curve.n_data;
+    r.ts = c->v.time_step;
     std::memcpy(&c->curve.pts[n], &r, sizeof r);
     RETIF(set_device(c));
     HIPCHK(hipMemcpy(c->pts + n, &c->curve.pts[n], sizeof(PointRec), hipMemcpyHostToDevice));

@@ -27,9 +27,9 @@ constexpr int MAXF = AST_MAX_FILTERS;
 
 // One observation as the kernels read it: unit E and S at the observation,
 // the RK4 interval from the previous observation (cuda.c:321-323; h, h/2,
-// h/6 and the step count), V, w, the filter, the time, the phase angle and
-// TIME_STEP.  112 bytes.  A table of n_data records carries one more record
-// after them: the observed minima (MinimaRec) for NUDGE, zero otherwise.
+// h/6 and the step count), V, w, the filter and the time.  96 bytes.  A table
+// of n_data records carries one more record after them (TailRec): TIME_STEP
+// and, for NUDGE, the observed minima.
 struct PointRec {
   double E[3];
   double S[3];
@@ -37,24 +37,23 @@ struct PointRec {
   float V, w;
   int filter, nsteps;
   double mjd;    // time of the observation (for an interval split inside a model, TORQUE2)
-  double alpha;  // phase angle acos(S.E) (TREND)
-  double ts;     // the variant's TIME_STEP (for the split interval's steps)
 };
-static_assert(sizeof(PointRec) == 112, "PointRec layout");
+static_assert(sizeof(PointRec) == 96, "PointRec layout");
 
-// NUDGE (asteroid.h:216-230, 319-337; misc.c:279-323): the observed
-// brightness minima, times relative to the curve's first observation and
-// magnitudes in the converted system, as floats; v1s is the V1S macro
+// The record after a table's points: the variant's TIME_STEP (for TORQUE2's
+// split interval) and NUDGE's observed brightness minima (asteroid.h:216-230,
+// 319-337; misc.c:279-323): times relative to the curve's first observation
+// and magnitudes in the converted system, as floats; v1s is the V1S macro
 // (one-sided magnitude distance, cuda.c:940-945).
 constexpr int NOBS_MAX = 10;  // asteroid.h:219
-struct MinimaRec {
+struct TailRec {
   int n_obs;
   int v1s;
   float t_obs[NOBS_MAX];
   float V_obs[NOBS_MAX];
-  int pad[6];
+  double ts;
 };
-static_assert(sizeof(MinimaRec) == sizeof(PointRec), "the minima record takes one PointRec slot");
+static_assert(sizeof(TailRec) == sizeof(PointRec), "the tail record takes one PointRec slot");
 
 // Model variant as one compile-time code: the reference's macros
 // (makefile:3-36) as bit flags.  TORQUE2 implies TORQUE and RECT implies BC
@@ -330,7 +329,7 @@ __device__ inline void ode(const double* y, double* f, const double* mu, double 
 }
 
 typedef __attribute__((address_space(4))) const PointRec* PointPtr;
-typedef __attribute__((address_space(4))) const MinimaRec* MinimaPtr;
+typedef __attribute__((address_space(4))) const TailRec* TailPtr;
 
 // ------------------------------------------------------------------ NUDGE
 // Constants of asteroid.h:216-230, evaluated with the reference's types
@@ -362,7 +361,7 @@ struct NudgeState {
   int M = 0;
   bool failed = false;
 
-  __device__ void point(int i, double mjd, double Vmod, MinimaPtr mr) {
+  __device__ void point(int i, double mjd, double Vmod, TailPtr mr) {
     if (failed) return;
     if (i == 0) {
       t0 = (float)mjd;
@@ -402,7 +401,7 @@ struct NudgeState {
   // The merit (cuda.c:912-992): chi2 times the reward for model minima
   // close to observed ones in (t, V) and the penalty for too many of them.
   template <class CF>
-  __device__ CF merit(CF chi2a, CF dv0, MinimaPtr mr) const {
+  __device__ CF merit(CF chi2a, CF dv0, TailPtr mr) const {
     float S_M = 0.0f, P_tot = 1.0f;
     for (int imod = 0; imod < M; ++imod) {
       for (int iobs = 0; iobs < mr->n_obs; ++iobs) {
@@ -762,7 +761,7 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
   NudgeState nu;
   nu.t_mod = nu_t;
   nu.V_mod = nu_V;
-  MinimaPtr mr = (MinimaPtr)(pts + n_data);
+  TailPtr mr = (TailPtr)(pts + n_data);
   double trendA = 0.0, t_split = 0.0;
   if constexpr (TR) trendA = params[P::A];
   bool switched = false;
@@ -781,7 +780,7 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
         const double t1 = pts[i - 1].mjd;
         nsub = 2;
         t2_end = pr.mjd;
-        nsteps = (int)((t_split - t1) / pr.ts + 1);
+        nsteps = (int)((t_split - t1) / mr->ts + 1);
         h = (t_split - t1) / nsteps;
         hh = 0.5 * h;
         h6 = 1 / 6.0 * h;
@@ -790,7 +789,7 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
     for (int isub = 0; isub < nsub; ++isub) {
     if constexpr (T2) {
       if (isub == 1) {
-        nsteps = (int)((t2_end - t_split) / pr.ts + 1);
+        nsteps = (int)((t2_end - t_split) / mr->ts + 1);
         h = (t2_end - t_split) / nsteps;
         hh = 0.5 * h;
         h6 = 1 / 6.0 * h;
@@ -843,7 +842,10 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
         Vmod = ls_tail_f64(u, K.bcp);
       }
     }
-    if constexpr (TR) Vmod = __builtin_fma(-trendA, pr.alpha, Vmod);  // cuda.c:692-697
+    if constexpr (TR) {  // cuda.c:692-697, the phase angle from the inertial S.E
+      const double alpha = astm::acos(dot3(pr.S[0], pr.S[1], pr.S[2], pr.E[0], pr.E[1], pr.E[2]));
+      Vmod = __builtin_fma(-trendA, alpha, Vmod);
+    }
     if constexpr (FORM == FORM_PLOT) {
       // Under NUDGE the reference stops the curve at a failed model
       // (cuda.c:735-738); the rest of the curve is NaN here.
