@@ -57,6 +57,13 @@ AST_HD double p_flipsign(double x, int flag) {
   return of_bits(bits_of(x) ^ ((uint64_t)(uint32_t)(flag & 2) << 62));
 }
 
+// Low word of an integer-valued double k, |k| < 2^51: k + 1.5*2^52 is exact
+// and holds k in two's complement in its low mantissa bits.  One add, where a
+// range-checked float-to-int conversion takes three instructions.
+AST_HD uint32_t low_word_of_int(double k) { return (uint32_t)bits_of(k + 6755399441055744.0); }
+// x with its sign bit xored by bit 31 of m (exact, not counted).
+AST_HD double p_xorsign(double x, uint32_t m) { return of_bits(bits_of(x) ^ ((uint64_t)m << 32)); }
+
 // 2^k as a double, for -1022 <= k <= 1023.
 AST_HD double pow2i(int k) { return of_bits((uint64_t)(k + 1023) << 52); }
 
@@ -134,12 +141,14 @@ template <class R> AST_HD void sincos(R x, R* s, R* c) {
   R hz = R(0.5) * z;
   R w = R(1.0) - hz;
   R cs = w + p_fma(z * z, k_cos(z), (R(1.0) - w) - hz);
-  double kv = p_val(k);
-  int q = (p_fabs(kv) < 1073741824.0) ? ((int)kv & 3) : 0;  // |k| < 2^30: exact 32-bit conversion
+  // quadrant q = k mod 4: sin takes the sign of bit 1 of q, cos that of bit 1
+  // of q + 1, which is bit 1 xor bit 0
+  const uint32_t q = low_word_of_int(p_val(k));
   R s0 = (q & 1) ? cs : sn;
   R c0 = (q & 1) ? sn : cs;
-  *s = p_flipsign(s0, q & 2);
-  *c = p_flipsign(c0, (q + 1) & 2);
+  const uint32_t ms = (q << 30) & 0x80000000u;
+  *s = p_xorsign(s0, ms);
+  *c = p_xorsign(c0, ms ^ (q << 31));
 }
 
 // sin and cos for the mixed-precision form's body axes (relative error about
@@ -158,12 +167,14 @@ template <class R> AST_HD void sincos_mx(R x, R* s, R* c) {
   R hz = R(0.5) * z;
   R w = R(1.0) - hz;
   R cs = w + p_fma(z * z, pc, (R(1.0) - w) - hz);
-  double kv = p_val(k);
-  int q = (p_fabs(kv) < 1073741824.0) ? ((int)kv & 3) : 0;
+  // quadrant q = k mod 4: sin takes the sign of bit 1 of q, cos that of bit 1
+  // of q + 1, which is bit 1 xor bit 0
+  const uint32_t q = low_word_of_int(p_val(k));
   R s0 = (q & 1) ? cs : sn;
   R c0 = (q & 1) ? sn : cs;
-  *s = p_flipsign(s0, q & 2);
-  *c = p_flipsign(c0, (q + 1) & 2);
+  const uint32_t ms = (q << 30) & 0x80000000u;
+  *s = p_xorsign(s0, ms);
+  *c = p_xorsign(c0, ms ^ (q << 31));
 }
 
 template <class R> AST_HD R sin(R x) { R s, c; sincos(x, &s, &c); return s; }

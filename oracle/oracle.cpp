@@ -1428,6 +1428,7 @@ inline R p_floor(R a) { K.rnd++; return R(__builtin_floor(a.v)); }
 inline R p_fabs(R a) { return R(__builtin_fabs(a.v)); }
 inline R p_copysign(R a, R b) { return R(__builtin_copysign(a.v, b.v)); }
 inline R p_flipsign(R a, int flag) { return R(flag ? -a.v : a.v); }
+inline R p_xorsign(R a, uint32_t m) { return R(astm::p_xorsign(a.v, m)); }
 inline double p_val(R a) { return a.v; }
 }  // namespace cnt
 
