@@ -213,6 +213,12 @@ struct CurveH {  // a validated light curve as the kernels read it
   double t_first = 0, t_last = 0;  // times of the first and last observation
 };
 
+// Phase angle of unit Sun and Earth directions, acos(S.E) in the canonical
+// form (TREND, cuda.c:694): a property of the point, computed once.
+double phase_angle(const double E[3], const double S[3]) {
+  return astm::acos(std::fma(S[2], E[2], std::fma(S[1], E[1], S[0] * E[0])));
+}
+
 int make_curve(const ast_variant& v, const ast_obs* obs, int n_data, int n_filters, const ast_ephem3* eph,
                CurveH* out) {
   if (!obs || n_data <= 0 || n_filters <= 0) return AST_E_ARG;
@@ -243,7 +249,7 @@ int make_curve(const ast_variant& v, const ast_obs* obs, int n_data, int n_filte
     p.V = o.V;
     p.w = o.w;
     p.filter = o.filter;
-    p.mjd = o.mjd;
+    p.aux = v.trend ? phase_angle(p.E, p.S) : o.mjd;
     C.sum_w[o.filter] = C.sum_w[o.filter] + (double)o.w;
     seen[o.filter] = 1;
   }
@@ -284,7 +290,7 @@ int make_grid(const ast_variant& v, const CurveH& C, const ast_ephem3& eph, int 
       t[i] = C.t_first + (double)i * h;
       interp_canon(eph, t[i], p.E, p.S);
     }
-    p.mjd = t[i];
+    p.aux = v.trend ? phase_angle(p.E, p.S) : t[i];
     if (i > 0) {
       const double t1 = t[i - 1], t2 = t[i];
       const int nsteps = (int)((t2 - t1) / v.time_step + 1);

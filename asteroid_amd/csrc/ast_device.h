@@ -27,7 +27,8 @@ constexpr int MAXF = AST_MAX_FILTERS;
 
 // One observation as the kernels read it: unit E and S at the observation,
 // the RK4 interval from the previous observation (cuda.c:321-323; h, h/2,
-// h/6 and the step count), V, w, the filter and the time.  96 bytes.  A table
+// h/6 and the step count), V, w, the filter and one per-point value (aux):
+// the phase angle acos(S.E) under TREND, else the time.  96 bytes.  A table
 // of n_data records carries one more record after them (TailRec): TIME_STEP
 // and, for NUDGE, the observed minima.
 struct PointRec {
@@ -36,7 +37,7 @@ struct PointRec {
   double h, hh, h6;
   float V, w;
   int filter, nsteps;
-  double mjd;    // time of the observation (for an interval split inside a model, TORQUE2)
+  double aux;    // TREND: phase angle acos(S.E); otherwise the time (TORQUE2's split, NUDGE's minima)
 };
 static_assert(sizeof(PointRec) == 96, "PointRec layout");
 
@@ -679,6 +680,7 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
                          double dv0 = 0.0) {
   using P = Pos<VC>;
   AST_VFLAGS(VC);
+  static_assert(!(TR && (T2 || NU)), "PointRec.aux is the phase angle under TREND and the time otherwise");
   double sy2[NF], sy[NF];
 #pragma unroll
   for (int m = 0; m < NF; ++m) { sy2[m] = 0.0; sy[m] = 0.0; }
@@ -766,7 +768,7 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
   if constexpr (TR) trendA = params[P::A];
   bool switched = false;
   if constexpr (T2)  // cuda.c:291: the torque changes at this fraction of the span
-    t_split = __builtin_fma(params[P::Tt], pts[n_data - 1].mjd - pts[0].mjd, pts[0].mjd);
+    t_split = __builtin_fma(params[P::Tt], pts[n_data - 1].aux - pts[0].aux, pts[0].aux);
   for (int i = 0; i < n_data; ++i) {
     const auto& pr = pts[i];
     int nsteps = pr.nsteps;  // 0 for the first observation
@@ -776,10 +778,10 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
     if constexpr (T2) {
       // The interval holding t_split runs as two sub-intervals, the second
       // under the second torque (cuda.c:288-317); the torque then stays.
-      if (i > 0 && !switched && t_split >= pts[i - 1].mjd && t_split < pr.mjd) {
-        const double t1 = pts[i - 1].mjd;
+      if (i > 0 && !switched && t_split >= pts[i - 1].aux && t_split < pr.aux) {
+        const double t1 = pts[i - 1].aux;
         nsub = 2;
-        t2_end = pr.mjd;
+        t2_end = pr.aux;
         nsteps = (int)((t_split - t1) / mr->ts + 1);
         h = (t_split - t1) / nsteps;
         hh = 0.5 * h;
@@ -842,15 +844,12 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
         Vmod = ls_tail_f64(u, K.bcp);
       }
     }
-    if constexpr (TR) {  // cuda.c:692-697, the phase angle from the inertial S.E
-      const double alpha = astm::acos(dot3(pr.S[0], pr.S[1], pr.S[2], pr.E[0], pr.E[1], pr.E[2]));
-      Vmod = __builtin_fma(-trendA, alpha, Vmod);
-    }
+    if constexpr (TR) Vmod = __builtin_fma(-trendA, pr.aux, Vmod);  // cuda.c:692-697
     if constexpr (FORM == FORM_PLOT) {
       // Under NUDGE the reference stops the curve at a failed model
       // (cuda.c:735-738); the rest of the curve is NaN here.
       vplot[(size_t)i * vstride] = (NU && nu.failed) ? __builtin_nan("") : Vmod + dv0;
-      if constexpr (NU) nu.point(i, pr.mjd, Vmod, mr);
+      if constexpr (NU) nu.point(i, pr.aux, Vmod, mr);
       continue;
     }
     const double w = (double)pr.w;
@@ -867,7 +866,7 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
           sy[k] = __builtin_fma(yv, w, sy[k]);
         }
     }
-    if constexpr (NU) nu.point(i, pr.mjd, Vmod, mr);
+    if constexpr (NU) nu.point(i, pr.aux, Vmod, mr);
   }
   if constexpr (FORM == FORM_PLOT) return CF(0.0);
   // A failed NUDGE model returns before delta_V is written (cuda.c:737).

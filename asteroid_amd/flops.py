@@ -30,10 +30,9 @@ TABLE = {
                        per_rk4=(38, 60, 180, 4, 0, 8)),
     (False, "bw"): dict(per_model=(35, 44, 107, 9, 3, 6), per_point=(18, 49, 69, 1, 0, 2),
                         per_rk4=(39, 60, 142, 0, 0, 8)),
-    # TREND: the LS torque row plus the phase angle per point (a dot product,
-    # acos's branch-free part and the de-trending fma); acos's octant branches
-    # add per point what _acos_branch_ops counts
-    (True, "ls+trend"): dict(per_model=(28, 41, 77, 11, 3, 4), per_point=(24, 72, 87, 6, 5, 1),
+    # TREND: the LS torque row plus the de-trending fma (the phase angle is a
+    # property of the point, computed once on the host)
+    (True, "ls+trend"): dict(per_model=(28, 41, 77, 11, 3, 4), per_point=(22, 68, 74, 5, 4, 1),
                              per_rk4=(38, 60, 180, 4, 0, 8)),
     (True, "ls+rot"): dict(per_model=(40, 56, 122, 11, 3, 7), per_point=(22, 86, 91, 5, 4, 1),
                            per_rk4=(38, 60, 180, 4, 0, 8)),
@@ -70,47 +69,23 @@ def rk4_steps(mjd, time_step, split=None):
     return n
 
 
-_TAN_PI_8 = 0.41421356237309503
-
-
-def _acos_branch_ops(c):
-    """Operations of astm::acos(c) on its data-dependent branches, summed over
-    the points' cos(phase angle) c: atan2's reduction above tan(pi/8) (4 adds,
-    1 division), its swap of the arguments (2 adds) and its x < 0 fold (2 adds)."""
-    c = np.asarray(c, float)
-    y = np.sqrt((1.0 - c) * (1.0 + c))
-    ax, ay = np.abs(c), np.abs(y)
-    sw = ay > ax
-    num, den = np.where(sw, ax, ay), np.where(sw, ay, ax)
-    t = np.where(den == 0, 0.0, num / np.where(den == 0, 1.0, den))
-    big = t > _TAN_PI_8
-    out = np.zeros(6)
-    out[0] = 4 * big.sum() + 2 * sw.sum() + 2 * (c < 0).sum()
-    out[3] = big.sum()
-    return out
-
-
-def eval_ops(variant, mjd, split=None, phase_cos=None):
-    """Operation counts (6 columns) of one evaluation on a curve with times mjd.
-    TORQUE2: split = Tt; TREND: phase_cos = S.E per point (the phase angle's
-    acos branches)."""
+def eval_ops(variant, mjd, split=None):
+    """Operation counts (6 columns) of one evaluation on a curve with times mjd
+    (TORQUE2: split = Tt, its split interval counted as two stretches)."""
     t = TABLE[table_key(variant)]
     split = split if variant.torque2 else None
-    ops = (np.array(t["per_model"], float) + len(mjd) * np.array(t["per_point"], float)
-           + rk4_steps(mjd, variant.time_step, split) * np.array(t["per_rk4"], float))
-    if variant.trend and phase_cos is not None:
-        ops = ops + _acos_branch_ops(phase_cos)
-    return ops
+    return (np.array(t["per_model"], float) + len(mjd) * np.array(t["per_point"], float)
+            + rk4_steps(mjd, variant.time_step, split) * np.array(t["per_rk4"], float))
 
 
-def eval_flops(variant, mjd, split=None, phase_cos=None):
+def eval_flops(variant, mjd, split=None):
     """FLOPs of one evaluation; for the mixed form, its FP64 plus FP32 FLOPs.
     TORQUE2 without `split` counts its split interval as one stretch (at most
-    one RK4 step short); TREND without `phase_cos` leaves out acos's branches."""
+    one RK4 step short)."""
     if variant.precision == MIXED:
-        f64, f32 = eval_flops_mixed(variant, mjd, split, phase_cos)
+        f64, f32 = eval_flops_mixed(variant, mjd, split)
         return f64 + f32
-    return float(eval_ops(variant, mjd, split, phase_cos) @ _W)
+    return float(eval_ops(variant, mjd, split) @ _W)
 
 
 # ------------------------------------------------------------ mixed form
@@ -139,7 +114,7 @@ _MX_SINCOS_SAVED = np.array([0, 0, 3, 0, 0, 0], float)  # per sincos_mx call
 _W4 = np.array([1, 1, 2, 1], float)
 
 
-def eval_flops_mixed(variant, mjd, split=None, phase_cos=None):
+def eval_flops_mixed(variant, mjd, split=None):
     """(FP64 FLOPs, FP32 FLOPs) of one mixed-form evaluation."""
     key = table_key(variant)
     torque, kind = key
@@ -154,6 +129,4 @@ def eval_flops_mixed(variant, mjd, split=None, phase_cos=None):
     split = split if variant.torque2 else None
     f64 = (np.array(t["per_model"], float) @ _W
            + rk4_steps(mjd, variant.time_step, split) * (np.array(t["per_rk4"], float) @ _W) + n * (f64_pt @ _W))
-    if variant.trend and phase_cos is not None:
-        f64 += float(_acos_branch_ops(phase_cos) @ _W)
     return float(f64), float(n * f32_pt)

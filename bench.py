@@ -638,11 +638,7 @@ def main_variants(args):
             ctx.chi2_batch(near)
             ev_ms, _ = ctx.last_timing()
         split = truth[variants.param_positions(v)["Tt"]] if v.torque2 else None
-        pc = None
-        if v.trend:  # S.E per observation as the evaluator sees it (interpolated, normalised)
-            E, S = synth.quad_interp(curve.eph, curve.mjd)
-            pc = np.sum(E * S, axis=1) / (np.linalg.norm(E, axis=1) * np.linalg.norm(S, axis=1))
-        fl = flops.eval_flops(v, curve.mjd, split, pc)
+        fl = flops.eval_flops(v, curve.mjd, split)
         peak, mix = peak_of(prec, v, curve.mjd)
         achieved = fl * e / (ms / 1e3) / 1e12
         rows[name] = {"n_params": space.n_params, "stage": "three (reopt)" if stage3 else "one", "value": pts / wall, "fits_per_s": args.fits / wall,

@@ -743,9 +743,9 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
       R inner = p_fma(sl * sd, lg, cd + cl);
       Vmod = R(-2.5) * astm::log10((((bcp * sS) * sE) * isc) * inner);
     }
-    if (P.v.trend) {  // cuda.c:692-697: the phase angle acos(S.E), S.E in the inertial frame
-      const R alpha = astm::acos(dot3(R(Sd[0]), R(Sd[1]), R(Sd[2]), E0, E1, E2));
-      Vmod = p_fma(-R(params[P.types[T.A]]), alpha, Vmod);
+    if (P.v.trend) {  // cuda.c:692-697: acos(S.E) in the inertial frame (a property of the point)
+      const double alpha = astm::acos(std::fma(Sd[2], Ed[2], std::fma(Sd[1], Ed[1], Sd[0] * Ed[0])));
+      Vmod = p_fma(-R(params[P.types[T.A]]), R(alpha), Vmod);
     }
     if (out && out->vmod) out->vmod[i] = (out->nudge && out->nudge->failed) ? std::nan("") : p_val(Vmod);
     if (out && out->nudge) out->nudge->point(P, i, P.obs[i].mjd, p_val(Vmod));

@@ -19,15 +19,6 @@ def _count(v, curve, params):
     return out
 
 
-def _phase_cos(curve):
-    """S.E at each observation as the evaluator sees it: the interpolated and
-    normalised directions (the INTERP quadratic, read_data.c:345-355)."""
-    E, S = synth.quad_interp(curve.eph, curve.mjd)
-    E /= np.linalg.norm(E, axis=1, keepdims=True)
-    S /= np.linalg.norm(S, axis=1, keepdims=True)
-    return np.sum(E * S, axis=1)
-
-
 @pytest.mark.parametrize("v", [variants.ls_torque(), variants.bc_free(), variants.bw_ball_torque(),
                                variants.make_variant(False, False, False, True, True),
                                variants.make_variant(False, False, True, True, True), variants.ls_torque_trend(),
@@ -37,8 +28,7 @@ def _phase_cos(curve):
 def test_table_matches_counting_oracle(v, n, span):
     curve, truth = synth.make_curve(17, v, n, span)
     split = truth[variants.param_positions(v)["Tt"]] if v.torque2 else None
-    got = _count(v, curve, truth)
-    table = flops.eval_ops(v, curve.mjd, split, phase_cos=_phase_cos(curve) if v.trend else None)
+    got, table = _count(v, curve, truth), flops.eval_ops(v, curve.mjd, split)
     # The per-point and per-step work is branch-free; the once-per-model
     # asin(...) of theta_0 goes through atan2, whose octant reduction costs
     # up to two adds, one division and two adds more on some arguments.

@@ -20,6 +20,10 @@ def _jobs():
         (variants.ls_torque(abi.PREC_MIXED), 1, 200, 3.0, 256, 1, 30, 0),
         (variants.ls_torque(abi.PREC_FP64), 4, 50, 1.0, 256, 1, 25, 1),
         (variants.bc_free(abi.PREC_MIXED), 2, 120, 2.5, 512, 1, 30, 0),
+        (variants.ls_torque_trend(abi.PREC_FP64), 2, 80, 1.5, 256, 1, 25, 0),
+        (variants.ls_torque2(abi.PREC_F32SIMPLEX), 1, 70, 1.2, 256, 1, 25, 0),
+        (variants.bc_rotate(abi.PREC_FP64, torque=True), 1, 90, 1.5, 256, 1, 25, 1),
+        (variants.rect(abi.PREC_F32SIMPLEX), 2, 60, 1.0, 256, 1, 25, 0),
     ]
     jobs = []
     for k, (v, nf, n_obs, span, n_fits, nst, nsteps, reopt) in enumerate(specs):
