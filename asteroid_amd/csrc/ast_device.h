@@ -769,10 +769,20 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
   bool switched = false;
   if constexpr (T2)  // cuda.c:291: the torque changes at this fraction of the span
     t_split = __builtin_fma(params[P::Tt], pts[n_data - 1].aux - pts[0].aux, pts[0].aux);
+  // The next interval's RK4 constants are read one point ahead, so their
+  // scalar loads complete behind this point's work instead of stalling the
+  // next integration (pts[n_data], the tail record, keeps the look-ahead
+  // inside the table; its values are never used).
+  int nx_steps = pts[0].nsteps;  // 0 for the first observation
+  double nx_h = pts[0].h, nx_hh = pts[0].hh, nx_h6 = pts[0].h6;
   for (int i = 0; i < n_data; ++i) {
     const auto& pr = pts[i];
-    int nsteps = pr.nsteps;  // 0 for the first observation
-    double h = pr.h, hh = pr.hh, h6 = pr.h6;
+    int nsteps = nx_steps;
+    double h = nx_h, hh = nx_hh, h6 = nx_h6;
+    nx_steps = pts[i + 1].nsteps;
+    nx_h = pts[i + 1].h;
+    nx_hh = pts[i + 1].hh;
+    nx_h6 = pts[i + 1].h6;
     int nsub = 1;
     double t2_end = 0.0;
     if constexpr (T2) {
