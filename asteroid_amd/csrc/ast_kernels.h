@@ -445,8 +445,12 @@ __global__ void fit_init_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
 // One lane's evaluation inside the fitting kernel: parameters from the
 // lane's params[] buffer, delta_V to its last-dV slot.  Kept out of line so
 // the evaluator's register allocation does not compete with the simplex
-// bookkeeping live across it.
-template <class CF, int VC, int NF, bool MX>
+// bookkeeping live across it.  W (the calling kernel's waves per SIMD) makes
+// one copy per register budget: a function shared by kernels of different
+// budgets is compiled for the smallest, 128 VGPRs, where the FP64 evaluator
+// spills.  The mixed form keeps the one 128-VGPR copy, which measured faster
+// than copies given more registers.
+template <class CF, int VC, int NF, bool MX, int W>
 __device__ __attribute__((noinline)) CF eval_lane(const double* PRM, CF* DV, size_t nf, size_t fit, size_t rs,
                                                    size_t fo, const PointRec* pts_g, int n_data, int n_filters, int dof,
                                                    const double* sum_w) {
@@ -530,7 +534,7 @@ void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
   for (int tick = 0; tick < ticks && !done; ++tick) {
     Machine<CF, VC> M{sp, B, SC, fit, nf, fo, rs, jfit, reopt};
     if (phase_pending(R.phase)) {
-      CF fv = eval_lane<CF, VC, NF, MX>(B.PRM, B.DV, nf, fit, rs, fo, D.pts, D.n_data, D.n_filters, D.dof,
+      CF fv = eval_lane<CF, VC, NF, MX, MX ? 4 : W>(B.PRM, B.DV, nf, fit, rs, fo, D.pts, D.n_data, D.n_filters, D.dof,
                                             D.sum_w);
       R.nev++;
       R.nev_tot++;
