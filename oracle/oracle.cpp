@@ -8,8 +8,11 @@
 // What it restates (reference = pulsar123/Asteroid):
 //   ode_*            ODE_func, cuda.c:13-77
 //   chi2_literal     chi2one, cuda.c:81-1011 (in-scope branches: TORQUE or the
-//                    3-ODE form, INTERP, LS ellipsoid cuda.c:663-690, BW_BALL
-//                    cuda.c:596-607, BC cuda.c:568-574, chi^2 cuda.c:884-910),
+//                    3-ODE form, TORQUE2's split interval cuda.c:288-317,
+//                    INTERP, ROTATE cuda.c:475-516, LS ellipsoid cuda.c:663-690,
+//                    BW_BALL cuda.c:596-607, RECT cuda.c:609-661, BC
+//                    cuda.c:568-574, TREND cuda.c:692-697, chi^2 cuda.c:884-910,
+//                    NUDGE's model minima and merit cuda.c:716-758, 912-992),
 //                    written with the reference's expression order and the C
 //                    library's elementary functions, no FMA.
 //   chi2_canon       the same model in the project's canonical evaluation
@@ -26,6 +29,18 @@
 //                    call sites, block argmin, stage reseed, best-per-block
 //                    write.  Random draws come from astrng.h instead of cuRAND.
 //   quadratic interpolation of E,S (INTERP)  cuda.c:522-545
+//   plot_grid, model_curve, periodogram   the -plot path: read_data.c:304-360,
+//                    chi2_plot cuda.c:1865-1966, minima() misc.c:65-276
+//   rmsd_group       chi2_gpu_rms, cuda.c:2148-2389
+// How each part is pinned (tests/test_oracle_*.py, tests/test_golden.py):
+// closed forms (the Lommel-Seeliger sphere, a prism's projected area, an
+// observed minimum on a model minimum multiplying chi^2 by P_MIN),
+// invariants (torque-free |L| and energy, Sun-Earth reciprocity), an
+// independent scipy model of the rotation and brightness (every variant),
+// RK4's fourth order, brute-force least squares for delta_V and chi^2, the
+// P_psi period, and the literal form against the canonical one.  The random
+// streams (astrng.h, replacing cuRAND) are pinned by the Philox4x32-10
+// known-answer vectors; cuRAND's own streams are not reproduced.
 // Behaviour the reference leaves undefined is fixed here and in the kernels
 // the same way: coordinates of frozen parameters in reflected, expanded and
 // contracted points equal the worst vertex's (cuda.c:1638, 1663, 1696 leave

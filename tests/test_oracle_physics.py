@@ -12,6 +12,9 @@ index, a transposed operand) fails it:
   velocity obeys I_i W_i = L sin(theta) sin(psi), I_s W_s = L sin(theta) cos(psi),
   W_l = L cos(theta) at every observation;
 * the 3-ODE torque-free form and the 6-ODE form with zero torque agree;
+* torque-free angular velocities follow the Jacobi elliptic closed form
+  (half period 2K(k)/w of the sn component, the three amplitudes), in both
+  rotation modes;
 * RK4 converges with order 4 under step refinement;
 * an independent model (scipy DOP853 integration, z-x-z rotation matrices)
   reproduces the oracle's magnitudes to the RK4 truncation error;
@@ -330,3 +333,55 @@ def test_canonical_form_equals_literal(name, v, nf):
     assert np.max(np.abs(vm_l[ok] - vm_c[ok])) < 1e-9
     rtol = 1e-6 if v.precision != abi.PREC_FP64 else 1e-10
     assert np.max(np.abs(c_l[ok] - c_c[ok]) / np.abs(c_l[ok])) < rtol
+
+
+@pytest.mark.parametrize("seed", [21, 22, 23, 24])
+def test_torque_free_motion_is_jacobi_elliptic(seed):
+    """Torque-free rotation in closed form (Landau & Lifshitz, Mechanics §37):
+    with (l, i, s) as axes 1, 2, 3 (moments 1 < Ii < Is, a cyclic order of
+    the Euler equations), the angular velocity components are A1 cn, A2 sn
+    and A3 dn of tau = w t (1 and 3 exchanged when L^2 < 2 E Ii), with w and
+    the modulus k fixed by L^2 and 2E.  The oracle's integration must show
+    the sn component's half period 2 K(k) / w (scipy's ellipk) and the three
+    amplitudes."""
+    from scipy.special import ellipk
+    v, truth, ids = _torque_free_truth(seed)
+    pb, curve, _ = _problem(variants.ls_torque(time_step=1e-3), n=4000, span=4.0, truth=truth, seed=seed)
+    _, _, tr = pb.chi2(truth, want_traj=True)
+    t = curve.mjd
+    wi, ws, wl = tr[0, :, 3], tr[0, :, 4], tr[0, :, 5]
+    bt, ct = truth[ids["b_tumb"]], truth[ids["c_tumb"]]
+    Is = (1 + bt * bt) / (bt * bt + ct * ct)
+    Ii = (1 + ct * ct) / (bt * bt + ct * ct)
+    I1, I2, I3 = 1.0, Ii, Is
+    w1, w2, w3 = wl, wi, ws
+    L2 = I1 * I1 * w1[0] ** 2 + I2 * I2 * w2[0] ** 2 + I3 * I3 * w3[0] ** 2
+    E2 = I1 * w1[0] ** 2 + I2 * w2[0] ** 2 + I3 * w3[0] ** 2
+    if L2 > E2 * I2:  # rotation about the axis of largest inertia
+        A = (np.sqrt((E2 * I3 - L2) / (I1 * (I3 - I1))), np.sqrt((E2 * I3 - L2) / (I2 * (I3 - I2))),
+             np.sqrt((L2 - E2 * I1) / (I3 * (I3 - I1))))
+        w = np.sqrt((I3 - I2) * (L2 - E2 * I1) / (I1 * I2 * I3))
+        m = (I2 - I1) * (E2 * I3 - L2) / ((I3 - I2) * (L2 - E2 * I1))
+        cn_c, dn_c = w1, w3
+    else:
+        A = (np.sqrt((E2 * I3 - L2) / (I1 * (I3 - I1))), np.sqrt((L2 - E2 * I1) / (I2 * (I2 - I1))),
+             np.sqrt((L2 - E2 * I1) / (I3 * (I3 - I1))))
+        w = np.sqrt((I2 - I1) * (E2 * I3 - L2) / (I1 * I2 * I3))
+        m = (I3 - I2) * (L2 - E2 * I1) / ((I2 - I1) * (E2 * I3 - L2))
+        cn_c, dn_c = w3, w1
+    assert 0 < m < 1
+    half = 2 * ellipk(m) / w
+    # zero crossings of the sn component, linearly interpolated
+    s = np.sign(w2)
+    k = np.nonzero(s[:-1] * s[1:] < 0)[0]
+    tz = t[k] - w2[k] * (t[k + 1] - t[k]) / (w2[k + 1] - w2[k])
+    assert len(tz) >= 3, "too few crossings in the span"
+    gaps = np.diff(tz)
+    assert np.max(np.abs(gaps - half)) < 1e-5 * half, (gaps, half)
+    # amplitudes: the sn and cn components peak at A2 and A1; dn runs between
+    # A3 sqrt(1 - m) and A3 (and stays of one sign)
+    assert abs(np.max(np.abs(w2)) - A[1]) < 1e-4 * A[1]
+    assert abs(np.max(np.abs(cn_c)) - (A[0] if cn_c is w1 else A[2])) < 1e-4 * np.max(np.abs(cn_c))
+    dn_amp = A[2] if dn_c is w3 else A[0]
+    assert np.all(np.abs(dn_c) <= dn_amp * (1 + 1e-9)) and np.all(np.abs(dn_c) >= dn_amp * np.sqrt(1 - m) * (1 - 1e-9))
+    assert abs(np.max(np.abs(dn_c)) - dn_amp) < 1e-4 * dn_amp
