@@ -185,18 +185,25 @@ class Context:
     # ------------------------------------------------------------- batches
     @staticmethod
     def _job(space, curve, n_fits, nstages=1, n_steps=5000, eval_budget=0, size_min=0.0, seed=1, params0=None,
-             tag=0):
-        """A ctypes ast_job plus the arrays it points into (kept alive by the caller)."""
+             tag=0, minima=None):
+        """A ctypes ast_job plus the arrays it points into (kept alive by the caller).
+        minima = (t_obs, V_obs) for NUDGE variants."""
         keep = dict(prop=np.ascontiguousarray(space.property.ravel(), np.int32),
                     types=np.ascontiguousarray(space.types, np.int32),
                     lim=np.ascontiguousarray(space.limits, np.float64),
                     obs=curve.obs_array(), eph=curve.ephem3(),
-                    p0=None if params0 is None else np.ascontiguousarray(params0, np.float64))
+                    p0=None if params0 is None else np.ascontiguousarray(params0, np.float64),
+                    tmin=None if minima is None else np.ascontiguousarray(minima[0], np.float32),
+                    vmin=None if minima is None else np.ascontiguousarray(minima[1], np.float32))
         job = abi.Job(space.variant, space.n_params, keep["prop"].ctypes.data_as(C.POINTER(C.c_int)),
                       space.n_types, keep["types"].ctypes.data_as(C.POINTER(C.c_int)), _dp(keep["lim"]),
                       space.x2(), C.cast(keep["obs"], C.POINTER(abi.Obs)), curve.n_data, curve.n_filters,
                       C.pointer(keep["eph"]), int(n_fits), int(nstages), int(n_steps), int(eval_budget),
                       float(size_min), int(seed), _dp(keep["p0"]) if keep["p0"] is not None else None, int(tag))
+        if minima is not None:
+            job.n_minima = len(keep["tmin"])
+            job.t_minima = keep["tmin"].ctypes.data_as(C.POINTER(C.c_float))
+            job.V_minima = keep["vmin"].ctypes.data_as(C.POINTER(C.c_float))
         return job, keep
 
     @staticmethod
@@ -240,9 +247,10 @@ class Context:
         self._keep = {}
 
     def submit(self, space, curve, n_fits, nstages=1, n_steps=5000, eval_budget=0, size_min=0.0, seed=1,
-               params0=None, tag=0):
+               params0=None, tag=0, minima=None):
         """Submit a fit job (its own variant, model space and curve); returns its id."""
-        job, keep = self._job(space, curve, n_fits, nstages, n_steps, eval_budget, size_min, seed, params0, tag)
+        job, keep = self._job(space, curve, n_fits, nstages, n_steps, eval_budget, size_min, seed, params0, tag,
+                              minima)
         jid = C.c_uint64()
         _chk(self.lib.ast_submit(self.h, C.byref(job), C.byref(jid)), "ast_submit")
         return jid.value

@@ -306,6 +306,31 @@ int make_grid(const ast_variant& v, const CurveH& C, const ast_ephem3& eph, int 
   return AST_OK;
 }
 
+// NUDGE's observed minima as a tail record (misc.c:279-323 fills the same
+// table): 1..AST_MAX_MINIMA finite times and magnitudes.
+int make_minima(const ast_variant& v, const float* t_obs, const float* V_obs, int n_obs, TailRec* out) {
+  if (!t_obs || !V_obs || n_obs < 1 || n_obs > AST_MAX_MINIMA) return AST_E_ARG;
+  TailRec r{};
+  r.n_obs = n_obs;
+  r.v1s = v.v1s ? 1 : 0;
+  for (int k = 0; k < n_obs; ++k) {
+    if (!std::isfinite(t_obs[k]) || !std::isfinite(V_obs[k])) return AST_E_ARG;
+    r.t_obs[k] = t_obs[k];
+    r.V_obs[k] = V_obs[k];
+  }
+  *out = r;
+  return AST_OK;
+}
+
+// The curve's tail record with a job's minima (TIME_STEP kept).
+int set_tail_minima(const ast_variant& v, const float* t_obs, const float* V_obs, int n_obs, CurveH* C) {
+  TailRec r{};
+  RETIF(make_minima(v, t_obs, V_obs, n_obs, &r));
+  r.ts = v.time_step;
+  std::memcpy(&C->pts[C->n_data], &r, sizeof r);
+  return AST_OK;
+}
+
 int nf_class(int n_filters) { return n_filters <= 1 ? 1 : (n_filters <= 4 ? 4 : 8); }
 
 hipError_t dispatch(const ast_variant& v, LaunchArgs& a) {
@@ -1111,7 +1136,9 @@ int make_job(ast_ctx* c, const ast_job* job, int G, std::unique_ptr<DevJob>* out
              std::vector<double>* p0) {
   if (job->n_fits <= 0 || job->n_fits % G || job->nstages <= 0 || job->n_steps < 0 || job->eval_budget < 0)
     return AST_E_ARG;
-  if (job->variant.nudge) return AST_E_UNSUPPORTED;  // a job carries no observed minima
+  if (job->variant.nudge &&
+      (job->n_minima < 1 || job->n_minima > AST_MAX_MINIMA || !job->t_minima || !job->V_minima))
+    return AST_E_ARG;
   std::unique_ptr<DevJob> j(new DevJob());
   j->uid = c->next_uid++;
   RETIF(make_space(job->variant, job->n_params, job->property, job->n_types, job->types, job->limits, &job->x2,
@@ -1119,6 +1146,7 @@ int make_job(ast_ctx* c, const ast_job* job, int G, std::unique_ptr<DevJob>* out
   CurveH C;
   RETIF(make_curve(job->variant, job->obs, job->n_data, job->n_filters, job->eph, &C));
   if (C.n_data - j->space.n_params - C.n_filters <= 0) return AST_E_DATA;
+  if (job->variant.nudge) RETIF(set_tail_minima(job->variant, job->t_minima, job->V_minima, job->n_minima, &C));
   j->rp.n_groups = job->n_fits / G;
   j->rp.group_size = G;
   j->rp.nstages = job->nstages;
@@ -1592,15 +1620,8 @@ int ast_upload_curve(ast_ctx* c, const ast_obs* obs, int n_data, int n_filters, 
 int ast_set_minima(ast_ctx* c, const float* t_obs, const float* V_obs, int n_obs) {
   if (!c) return AST_E_ARG;
   if (!c->v.nudge) return AST_E_STATE;
-  if (!t_obs || !V_obs || n_obs < 1 || n_obs > AST_MAX_MINIMA) return AST_E_ARG;
   TailRec r{};
-  r.n_obs = n_obs;
-  r.v1s = c->v.v1s ? 1 : 0;
-  for (int k = 0; k < n_obs; ++k) {
-    if (!std::isfinite(t_obs[k]) || !std::isfinite(V_obs[k])) return AST_E_ARG;
-    r.t_obs[k] = t_obs[k];
-    r.V_obs[k] = V_obs[k];
-  }
+  RETIF(make_minima(c->v, t_obs, V_obs, n_obs, &r));
   c->minima = r;
   c->minima_set = true;
   if (c->curve_set) {

@@ -352,6 +352,9 @@ typedef struct {
   uint64_t seed;
   const double* params0;    /* start model (reoptimisation), or NULL */
   uint64_t tag;             /* returned with the result */
+  int n_minima;             /* NUDGE: observed minima, as ast_set_minima (0 otherwise) */
+  const float* t_minima;
+  const float* V_minima;
 } ast_job;
 
 typedef struct {

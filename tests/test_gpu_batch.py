@@ -102,3 +102,28 @@ def test_batch_schedules_agree():
     for k in range(len(jobs)):
         _same(runs[0][k], runs[1][k])
         _same(runs[0][k], runs[2][k])
+
+
+def test_batch_nudge_job_matches_alone_and_oracle():
+    """A Stage Three NUDGE job carries its observed minima in the batch; it
+    equals the same job alone and the oracle's run, beside jobs of other
+    variants."""
+    v = variants.ls_torque_nudge(abi.PREC_FP64)
+    curve, truth = synth.make_curve(77, v, 100, 1.5)
+    space = variants.default_model_space(v, reopt=1)
+    vm = synth.truth_vmod(v, truth, curve.mjd, curve.E, curve.S)
+    idx = [i - 1 for i in range(2, len(vm)) if vm[i - 1] > vm[i - 2] and vm[i - 1] >= vm[i]][:3]
+    minima = (np.array([curve.mjd[i] for i in idx], np.float32), np.array([vm[i] for i in idx], np.float32))
+    nj = dict(space=space, curve=curve, n_fits=256, n_steps=25, seed=9, params0=truth * 1.01, tag=99,
+              minima=minima)
+    jobs = _jobs()[:3] + [nj]
+    with api.Context(variants.ls_torque()) as ctx:
+        batch = ctx.run_jobs(jobs, work_points_per_launch=3000)
+        alone = ctx.run_jobs([nj], work_points_per_launch=5000)[0]
+    _same(batch[3], alone)
+    pb = oracle.Problem(space, curve, minima=minima)
+    want = pb.run(1, 256, nstages=1, n_steps=25, seed=9, reopt=1, params0=truth * 1.01, n_threads=8)
+    r = batch[3]
+    assert r["status"] == 0 and r["best_f"] == want["f"][0]
+    assert np.array_equal(r["best_params"], want["params"][0]) and np.array_equal(r["best_dV"], want["dV"][0])
+    assert r["n_evals"] == want["lane_nev"].sum()
