@@ -38,7 +38,9 @@
 // invariants (torque-free |L| and energy, Sun-Earth reciprocity), an
 // independent scipy model of the rotation and brightness (every variant),
 // RK4's fourth order, brute-force least squares for delta_V and chi^2, the
-// P_psi period, and the literal form against the canonical one.  The random
+// P_psi period, and the literal form against the canonical one; run_group
+// against a second, pure-Python restatement of chi2_gpu (bit for bit on every
+// lane, every simplex move and both stopping rules reached).  The random
 // streams (astrng.h, replacing cuRAND) are pinned by the Philox4x32-10
 // known-answer vectors; cuRAND's own streams are not reproduced.
 // Behaviour the reference leaves undefined is fixed here and in the kernels
