@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(LIB_DIR, "libasteroid_hip.so")
 P_TYPE, P_INDEPENDENT, P_FROZEN, P_ISEG, P_MULTI_SEGMENT, P_PERIODIC = range(6)
 N_COLUMNS = 6
 PERIODIC, PERIODIC_LAM, SOFT_BOTH, HARD_LEFT, HARD_RIGHT, HARD_BOTH = 1, 0, -1, -2, -3, -4
-MAX_PARAMS, MAX_TYPES, MAX_FILTERS, MAX_MINIMA = 16, 16, 8, 10
+MAX_PARAMS, MAX_TYPES, MAX_FILTERS, MAX_MINIMA, MAX_SEGMENTS = 16, 16, 8, 10, 4
 
 PREC_FP64, PREC_F32SIMPLEX, PREC_MIXED = 0, 1, 2
 
@@ -29,7 +29,8 @@ E_CANCELLED = -8
 class Variant(C.Structure):
     _fields_ = [("torque", C.c_int), ("bc", C.c_int), ("bw_ball", C.c_int), ("p_psi", C.c_int),
                 ("interp", C.c_int), ("precision", C.c_int), ("time_step", C.c_double), ("trend", C.c_int),
-                ("torque2", C.c_int), ("rotate", C.c_int), ("rect", C.c_int), ("nudge", C.c_int), ("v1s", C.c_int)]
+                ("torque2", C.c_int), ("rotate", C.c_int), ("rect", C.c_int), ("nudge", C.c_int), ("v1s", C.c_int),
+                ("n_seg", C.c_int), ("one_le", C.c_int)]
 
 
 class TypeIds(C.Structure):
@@ -147,6 +148,9 @@ def type_ids(variant):
     ids["n_types"] = k
     ids["n_params"] = (8 + 2 * bc + 3 * bool(variant.rotate) + bool(variant.trend)
                        + (7 if variant.torque2 else 3 * torque) + 3 * bool(variant.bw_ball))
+    if variant.n_seg > 1:  # asteroid.h:104-108
+        ids["n_params"] += ((6 + (7 if variant.torque2 else 3 * torque) - 2 * bool(variant.one_le))
+                            * (variant.n_seg - 1))
     return ids
 
 
@@ -171,6 +175,7 @@ def load():
         "ast_set_model_space": (I, [P, I, pI, I, pI, pD, C.POINTER(X2)]),
         "ast_upload_curve": (I, [P, C.POINTER(Obs), I, I, C.POINTER(Ephem3)]),
         "ast_set_minima": (I, [P, C.POINTER(C.c_float), C.POINTER(C.c_float), I]),
+        "ast_set_segments": (I, [P, C.POINTER(C.c_double), I]),
         "ast_rng_seed": (I, [P, C.c_uint64]),
         "ast_reset_best": (I, [P]),
         "ast_set_start": (I, [P, pD]),
@@ -210,5 +215,5 @@ EXPORTED = (
     "ast_last_timing", "ast_last_counts", "ast_default_model_space", "ast_strerror",
     "ast_last_hip_error", "ast_serve_start", "ast_submit", "ast_poll", "ast_cancel", "ast_serve_stats",
     "ast_serve_stop", "ast_run_jobs", "ast_autotune", "ast_set_waves", "ast_model_curve", "ast_periodogram",
-    "ast_rmsd", "ast_set_minima",
+    "ast_rmsd", "ast_set_minima", "ast_set_segments",
 )

@@ -83,6 +83,12 @@ class Context:
         _chk(self.lib.ast_set_minima(self.h, t.ctypes.data_as(C.POINTER(C.c_float)),
                                      v.ctypes.data_as(C.POINTER(C.c_float)), len(t)), "ast_set_minima")
 
+    def set_segments(self, t_start):
+        """SEGMENT's start times on the curve's clock (before upload_curve)."""
+        t = np.ascontiguousarray(t_start, np.float64)
+        _chk(self.lib.ast_set_segments(self.h, t.ctypes.data_as(C.POINTER(C.c_double)), len(t)),
+             "ast_set_segments")
+
     def rng_seed(self, seed):
         _chk(self.lib.ast_rng_seed(self.h, int(seed)), "ast_rng_seed")
 

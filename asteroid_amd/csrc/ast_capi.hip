@@ -50,6 +50,7 @@ extern template hipError_t launch_variant<VF_RECT>(const LaunchArgs&);
 extern template hipError_t launch_variant<VF_TQ | VF_RECT>(const LaunchArgs&);
 extern template hipError_t launch_variant<VF_TQ | VF_NUDGE>(const LaunchArgs&);
 extern template hipError_t launch_variant<VF_TQ | VF_BC | VF_NUDGE>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_SEG3>(const LaunchArgs&);
 }  // namespace ast
 
 using namespace ast;
@@ -97,14 +98,16 @@ void dev_free(T*& p) {
 // The variant codes built (one kernel family each), indexed by variant_index.
 constexpr int VARIANT_CODES[] = {0, VF_TQ, VF_BC, VF_TQ | VF_BC, VF_BW, VF_TQ | VF_BW,
                                  VF_TQ | VF_TREND, VF_TQ2, VF_BC | VF_ROT, VF_TQ | VF_BC | VF_ROT,
-                                 VF_RECT, VF_TQ | VF_RECT, VF_TQ | VF_NUDGE, VF_TQ | VF_BC | VF_NUDGE};
+                                 VF_RECT, VF_TQ | VF_RECT, VF_TQ | VF_NUDGE, VF_TQ | VF_BC | VF_NUDGE,
+                                 VF_SEG3};
 constexpr int N_VARIANTS = (int)(sizeof VARIANT_CODES / sizeof VARIANT_CODES[0]);
 
 // Variant code of the macros (TORQUE2 implies TORQUE, RECT implies BC).
 int variant_code(const ast_variant& v) {
   const bool t2 = v.torque2, tq = v.torque || t2, rc = v.rect, bc = v.bc || rc;
   return (tq && !t2 ? VF_TQ : 0) | (t2 ? VF_TQ2 : 0) | (bc && !rc ? VF_BC : 0) | (v.bw_ball ? VF_BW : 0) |
-         (v.trend ? VF_TREND : 0) | (v.rotate ? VF_ROT : 0) | (rc ? VF_RECT : 0) | (v.nudge ? VF_NUDGE : 0);
+         (v.trend ? VF_TREND : 0) | (v.rotate ? VF_ROT : 0) | (rc ? VF_RECT : 0) | (v.nudge ? VF_NUDGE : 0) |
+         (v.n_seg > 1 ? VF_SEG3 : 0);
 }
 
 // Index of the variant's kernel family, or -1 if it is not built.
@@ -121,6 +124,8 @@ bool variant_supported(const ast_variant& v) {
   // mixed form's 1e-6 chi^2 tolerance cannot hold for it; the reference's
   // Stage Three runs -DACC (README.md:89-93).
   if (v.nudge && v.precision == AST_PREC_MIXED) return false;
+  // SEGMENT is built for three segments with L and Es shared (16 parameters)
+  if (v.n_seg < 0 || v.n_seg > AST_MAX_SEGMENTS || (v.n_seg > 1 && (v.n_seg != 3 || !v.one_le))) return false;
   if (v.precision != AST_PREC_FP64 && v.precision != AST_PREC_F32SIMPLEX && v.precision != AST_PREC_MIXED)
     return false;
   if (!(v.time_step > 0)) return false;
@@ -158,6 +163,17 @@ int make_space(const ast_variant& v, int n_params, const int* property, int n_ty
     if (fr < -1 || fr > 1) return AST_E_ARG;
     pos++;
   }
+  // SEGMENT: theta_M, phi_M, phi_0, psi_0 of segments 1 and 2 (asteroid.c:128-150)
+  for (int k = 1; v.n_seg > 1 && k < v.n_seg; ++k) {
+    const int seg_types[] = {S.T.theta_M, S.T.phi_M, S.T.phi_0, S.T.psi_0};
+    for (int t : seg_types) {
+      const int* r = property + pos * AST_N_COLUMNS;
+      if (r[AST_P_TYPE] != t || r[AST_P_ISEG] != k || r[AST_P_MULTI_SEGMENT] != 0) return AST_E_UNSUPPORTED;
+      if (r[AST_P_FROZEN] < -1 || r[AST_P_FROZEN] > 1) return AST_E_ARG;
+      pos++;
+    }
+  }
+  if (pos != n_params) return AST_E_UNSUPPORTED;
   for (int i = 0; i < n_params; ++i)
     for (int k = 0; k < AST_N_COLUMNS; ++k) S.prop[i][k] = property[i * AST_N_COLUMNS + k];
   for (int t = 0; t < n_types; ++t) {
@@ -209,6 +225,7 @@ struct CurveH {  // a validated light curve as the kernels read it
   std::vector<PointRec> pts;  // n_data observations, then the minima record (zero unless NUDGE)
   double sum_w[AST_MAX_FILTERS] = {0};
   int n_data = 0, n_filters = 0;
+  int n_all = 0;          // observations given (SEGMENT leaves those before the first segment out of the table)
   int64_t rk4_steps = 0;  // RK4 steps of one evaluation
   double t_first = 0, t_last = 0;  // times of the first and last observation
 };
@@ -219,8 +236,42 @@ double phase_angle(const double E[3], const double S[3]) {
   return astm::acos(std::fma(S[2], E[2], std::fma(S[1], E[1], S[0] * E[0])));
 }
 
+// SEGMENT's split (read_data.c:244-250): observation i opens segment s when
+// its time is at or after seg_t[s].  The table keeps the observations from
+// the first segment on; the first record of segment k > 0 carries the step
+// count -k, where the evaluator starts that segment's rotation, and the
+// first record none.  sum_w covers the table; the degrees of freedom keep
+// every observation (n_all, cuda.c:907).
+int split_segments(int n_seg, const ast_obs* obs, const double* seg_t, CurveH* C) {
+  if (!seg_t) return AST_E_STATE;
+  int start[AST_MAX_SEGMENTS], k = 0;
+  for (int i = 0; i < C->n_data; ++i)
+    if (k < n_seg && obs[i].mjd >= seg_t[k]) start[k++] = i;
+  if (k < n_seg) return AST_E_DATA;
+  const int i0 = start[0], n = C->n_data - i0;
+  std::vector<PointRec> pts(C->pts.begin() + i0, C->pts.end());  // the tail record comes along
+  for (int m = 0; m < C->n_filters; ++m) C->sum_w[m] = 0.0;
+  C->rk4_steps = 0;
+  for (int i = 0; i < n; ++i) {
+    PointRec& p = pts[i];
+    for (int s = 0; s < n_seg; ++s)
+      if (start[s] == i + i0) {
+        p.nsteps = -s;
+        p.h = p.hh = p.h6 = 0.0;
+      }
+    if (p.nsteps > 0) C->rk4_steps += p.nsteps;
+    C->sum_w[p.filter] = C->sum_w[p.filter] + (double)p.w;
+  }
+  for (int m = 0; m < C->n_filters; ++m)
+    if (!(C->sum_w[m] > 0)) return AST_E_DATA;  // a filter with no observation in the segments
+  C->pts = std::move(pts);
+  C->n_data = n;
+  C->t_first = obs[i0].mjd;
+  return AST_OK;
+}
+
 int make_curve(const ast_variant& v, const ast_obs* obs, int n_data, int n_filters, const ast_ephem3* eph,
-               CurveH* out) {
+               CurveH* out, const double* seg_t = nullptr) {
   if (!obs || n_data <= 0 || n_filters <= 0) return AST_E_ARG;
   if (n_filters > AST_MAX_FILTERS) return AST_E_DATA;
   if (v.interp && !eph) return AST_E_ARG;
@@ -259,9 +310,11 @@ int make_curve(const ast_variant& v, const ast_obs* obs, int n_data, int n_filte
   tail.ts = v.time_step;
   std::memcpy(&C.pts[n_data], &tail, sizeof tail);
   C.n_data = n_data;
+  C.n_all = n_data;
   C.n_filters = n_filters;
   C.t_first = obs[0].mjd;
   C.t_last = obs[n_data - 1].mjd;
+  if (v.n_seg > 1) RETIF(split_segments(v.n_seg, obs, seg_t, &C));
   *out = std::move(C);
   return AST_OK;
 }
@@ -350,6 +403,7 @@ hipError_t dispatch(const ast_variant& v, LaunchArgs& a) {
     case VF_TQ | VF_RECT: return launch_variant<VF_TQ | VF_RECT>(a);
     case VF_TQ | VF_NUDGE: return launch_variant<VF_TQ | VF_NUDGE>(a);
     case VF_TQ | VF_BC | VF_NUDGE: return launch_variant<VF_TQ | VF_BC | VF_NUDGE>(a);
+    case VF_SEG3: return launch_variant<VF_SEG3>(a);
     default: return hipErrorInvalidValue;
   }
 }
@@ -448,6 +502,7 @@ struct DevJob {
   SpaceH space;
   RunParams rp;
   int n_data = 0, n_filters = 0;
+  int n_all = 0;               // observations in the degrees of freedom
   int cls = 0;                 // kernel class index (for streams and counters)
   double cost = 1;             // one evaluation, in observation-equivalents (eval_cost)
   int ticks = 1;               // evaluation rounds per launch
@@ -505,6 +560,7 @@ int job_upload(DevJob& j, const Pool& pool, const CurveH& C, const double* param
   const SpaceH& S = j.space;
   j.cf = S.v.precision == AST_PREC_FP64 ? 8 : 4;
   j.n_data = C.n_data;
+  j.n_all = C.n_all;
   j.n_filters = C.n_filters;
   const int N = S.n_params, G = j.rp.n_groups;
   const size_t fits = (size_t)G * j.rp.group_size;
@@ -568,7 +624,7 @@ int job_descs_typed(DevJob& j, int ticks) {
     D.size2_min = smc * smc;  // asteroid.h:245
     D.n_data = j.n_data;
     D.n_filters = j.n_filters;
-    D.dof = j.n_data - j.space.n_params - j.n_filters;
+    D.dof = j.n_all - j.space.n_params - j.n_filters;
     D.nstages = j.rp.nstages;
     D.n_steps = j.rp.n_steps;
     D.eval_budget = (int)(j.rp.eval_budget > 0x7fffffff ? 0x7fffffff : j.rp.eval_budget);
@@ -821,6 +877,8 @@ struct ast_ctx {
   double* sum_w = nullptr;
   TailRec minima{};           // NUDGE's observed minima (in the curve table's tail record)
   bool minima_set = false;
+  double seg_t[AST_MAX_SEGMENTS]{};  // SEGMENT's start times
+  bool seg_set = false;
   double params0[AST_MAX_PARAMS]{};
   uint64_t seed = 0;
   uint32_t* rng = nullptr;
@@ -1139,6 +1197,7 @@ int make_job(ast_ctx* c, const ast_job* job, int G, std::unique_ptr<DevJob>* out
   if (job->variant.nudge &&
       (job->n_minima < 1 || job->n_minima > AST_MAX_MINIMA || !job->t_minima || !job->V_minima))
     return AST_E_ARG;
+  if (job->variant.n_seg > 1) return AST_E_UNSUPPORTED;  // jobs carry no segment start times
   std::unique_ptr<DevJob> j(new DevJob());
   j->uid = c->next_uid++;
   RETIF(make_space(job->variant, job->n_params, job->property, job->n_types, job->types, job->limits, &job->x2,
@@ -1458,7 +1517,7 @@ int rmsd_run(ast_ctx* c, const ast_rmsd_cfg& cfg, RmsdOut* out) {
   A.sum_w = c->sum_w;
   A.n_data = c->curve.n_data;
   A.n_filters = c->curve.n_filters;
-  A.dof = c->curve.n_data - N - c->curve.n_filters;
+  A.dof = c->curve.n_all - N - c->curve.n_filters;
   A.sp = sp;
   A.reopt = c->space.x2.reopt;
   A.params0 = p0;
@@ -1598,12 +1657,13 @@ int ast_set_model_space(ast_ctx* c, int n_params, const int* property, int n_typ
 
 int ast_upload_curve(ast_ctx* c, const ast_obs* obs, int n_data, int n_filters, const ast_ephem3* eph) {
   if (!c) return AST_E_ARG;
+  if (c->v.n_seg > 1 && !c->seg_set) return AST_E_STATE;
   CurveH C;
-  RETIF(make_curve(c->v, obs, n_data, n_filters, eph, &C));
+  RETIF(make_curve(c->v, obs, n_data, n_filters, eph, &C, c->seg_set ? c->seg_t : nullptr));
   {  // the tail record: the context's minima with this curve's TIME_STEP
     TailRec tail = c->minima;
     tail.ts = c->v.time_step;
-    std::memcpy(&C.pts[n_data], &tail, sizeof tail);
+    std::memcpy(&C.pts[C.n_data], &tail, sizeof tail);  // after the table (SEGMENT may drop leading observations)
   }
   RETIF(set_device(c));
   dev_free(c->pts);
@@ -1631,6 +1691,17 @@ int ast_set_minima(ast_ctx* c, const float* t_obs, const float* V_obs, int n_obs
     RETIF(set_device(c));
     HIPCHK(hipMemcpy(c->pts + n, &c->curve.pts[n], sizeof(PointRec), hipMemcpyHostToDevice));
   }
+  return AST_OK;
+}
+
+int ast_set_segments(ast_ctx* c, const double* t_start, int n_seg) {
+  if (!c) return AST_E_ARG;
+  if (c->v.n_seg <= 1) return AST_E_STATE;
+  if (!t_start || n_seg != c->v.n_seg) return AST_E_ARG;
+  for (int k = 0; k < n_seg; ++k)
+    if (!std::isfinite(t_start[k])) return AST_E_ARG;
+  for (int k = 0; k < n_seg; ++k) c->seg_t[k] = t_start[k];
+  c->seg_set = true;
   return AST_OK;
 }
 
@@ -1663,7 +1734,7 @@ int ast_run(ast_ctx* c, const ast_run_cfg* rc) {
   if (rc->n_groups <= 0 || G < 64 || G > 1024 || (G & 63) || rc->nstages <= 0 || rc->n_steps < 0 ||
       rc->eval_budget < 0)
     return AST_E_ARG;
-  if (c->curve.n_data - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
+  if (c->curve.n_all - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
   RETIF(set_device(c));
   const size_t fits = (size_t)rc->n_groups * G;
   RETIF(ensure_streams(c, fits, rc->n_groups));
@@ -1777,7 +1848,7 @@ int ast_chi2_batch(ast_ctx* c, const double* params, int n, double* chi2, double
   if (!c || !params || n <= 0 || !chi2 || !dV) return AST_E_ARG;
   if (c->serve) return AST_E_BUSY;  // its streams and events belong to the scheduler
   if (!c->space_set || !c->curve_set || (c->v.nudge && !c->minima_set)) return AST_E_STATE;
-  if (c->curve.n_data - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
+  if (c->curve.n_all - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
   RETIF(set_device(c));
   double *pd = nullptr, *cd = nullptr, *dd = nullptr;
   int rc = dev_alloc(&pd, (size_t)n * c->space.n_params);
@@ -1794,7 +1865,7 @@ int ast_chi2_batch(ast_ctx* c, const double* params, int n, double* chi2, double
   a.sum_w = c->sum_w;
   a.n_data = c->curve.n_data;
   a.n_filters = c->curve.n_filters;
-  a.dof = c->curve.n_data - c->space.n_params - c->curve.n_filters;
+  a.dof = c->curve.n_all - c->space.n_params - c->curve.n_filters;
   a.params = pd;
   a.n = n;
   a.chi2 = cd;
@@ -1830,7 +1901,8 @@ int ast_rmsd(ast_ctx* c, const ast_rmsd_cfg* cfg, float* par_min, float* par_max
     return AST_E_ARG;
   if (c->serve) return AST_E_BUSY;
   if (!c->space_set || !c->curve_set || (c->v.nudge && !c->minima_set)) return AST_E_STATE;
-  if (c->curve.n_data - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
+  if (c->v.n_seg > 1) return AST_E_UNSUPPORTED;
+  if (c->curve.n_all - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
   RETIF(set_device(c));
   RmsdOut o{par_min, par_max, 0, 0, 0, 0, 0};
   const int rc = c->v.precision == AST_PREC_FP64 ? rmsd_run<double>(c, *cfg, &o) : rmsd_run<float>(c, *cfg, &o);
@@ -1851,7 +1923,8 @@ int ast_model_curve(ast_ctx* c, const double* params, int n, const ast_ephem3* e
   if (!c || !params || n <= 0 || !eph || n_plot < 2 || !vmod || !chi2 || !dV) return AST_E_ARG;
   if (c->serve) return AST_E_BUSY;
   if (!c->space_set || !c->curve_set || (c->v.nudge && !c->minima_set)) return AST_E_STATE;
-  if (c->curve.n_data - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
+  if (c->v.n_seg > 1) return AST_E_UNSUPPORTED;  // the plot's segment split is not built
+  if (c->curve.n_all - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
   std::vector<PointRec> grid;
   std::vector<double> tg;
   RETIF(make_grid(c->v, c->curve, *eph, n_plot, &grid, &tg));
@@ -1876,7 +1949,7 @@ int ast_model_curve(ast_ctx* c, const double* params, int n, const ast_ephem3* e
   a.sum_w = c->sum_w;
   a.n_data = c->curve.n_data;
   a.n_filters = nf;
-  a.dof = c->curve.n_data - np - nf;
+  a.dof = c->curve.n_all - np - nf;
   a.params = pd;
   a.n = n;
   a.chi2 = cd;

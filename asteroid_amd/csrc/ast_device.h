@@ -59,7 +59,8 @@ static_assert(sizeof(TailRec) == sizeof(PointRec), "the tail record takes one Po
 // Model variant as one compile-time code: the reference's macros
 // (makefile:3-36) as bit flags.  TORQUE2 implies TORQUE and RECT implies BC
 // (asteroid.h:32-38).
-enum : int { VF_TQ = 1, VF_BC = 2, VF_BW = 4, VF_TREND = 8, VF_TQ2 = 16, VF_ROT = 32, VF_RECT = 64, VF_NUDGE = 128 };
+enum : int { VF_TQ = 1, VF_BC = 2, VF_BW = 4, VF_TREND = 8, VF_TQ2 = 16, VF_ROT = 32, VF_RECT = 64, VF_NUDGE = 128,
+             VF_SEG3 = 256 };
 template <int VC>
 struct VarF {
   static constexpr bool TQ = (VC & (VF_TQ | VF_TQ2)) != 0;  // constant torque
@@ -70,10 +71,12 @@ struct VarF {
   static constexpr bool RO = (VC & VF_ROT) != 0;            // brightness frame rotated
   static constexpr bool RC = (VC & VF_RECT) != 0;           // rectangular prism brightness
   static constexpr bool NU = (VC & VF_NUDGE) != 0;          // model minima nudged to observed ones
+  static constexpr bool SG = (VC & VF_SEG3) != 0;           // SEGMENT, three segments, L and Es shared
 };
 #define AST_VFLAGS(VC)                                                                           \
   constexpr bool TQ = VarF<VC>::TQ, BC = VarF<VC>::BC, BW = VarF<VC>::BW, T2 = VarF<VC>::T2,   \
-                 TR = VarF<VC>::TR, RO = VarF<VC>::RO, RC = VarF<VC>::RC, NU = VarF<VC>::NU
+                 TR = VarF<VC>::TR, RO = VarF<VC>::RO, RC = VarF<VC>::RC, NU = VarF<VC>::NU,   \
+                 SG = VarF<VC>::SG
 
 // Parameter positions of a variant (the Property order, asteroid.c:86-125).
 template <int VC>
@@ -91,7 +94,14 @@ struct Pos {
   static constexpr int ob = o + (F::BC ? 2 : 0);
   static constexpr int theta_R = (F::BW || F::RO) ? 8 + ob : -1, phi_R = (F::BW || F::RO) ? 9 + ob : -1;
   static constexpr int psi_R = F::RO ? 10 + ob : -1, kappa = F::BW ? 10 + ob : -1;
-  static constexpr int N = 8 + o + (F::BC ? 2 : 0) + ((F::BW || F::RO) ? 3 : 0);
+  static constexpr int N0 = 8 + o + (F::BC ? 2 : 0) + ((F::BW || F::RO) ? 3 : 0);
+  // SEGMENT (LS, ONE_LE, three segments): segments 1 and 2 repeat theta_M,
+  // phi_M, phi_0 and psi_0 after the first segment's parameters
+  // (asteroid.c:128-150); everything else is shared.
+  static_assert(!F::SG || VC == VF_SEG3, "SEGMENT is built for LS without torque only");
+  static constexpr int seg(int k, int j) { return k == 0 ? (j == 3 ? 7 : j) : N0 + 4 * (k - 1) + j; }
+  static constexpr bool is_psi0(int i) { return i == psi_0 || (F::SG && (i == seg(1, 3) || i == seg(2, 3))); }
+  static constexpr int N = N0 + (F::SG ? 8 : 0);
 };
 
 // Model space by parameter position (the reference indexes limits by type;
@@ -155,7 +165,7 @@ __device__ int x2params(const CF* x, double* params, const Space<CF>& sp, int re
         psi_max = astm::atan(__builtin_sqrt(Ii * (Is - 1.0 / params[i]) / Is / (1.0 / params[i] - Ii)));
         psi_min = -psi_max;
       }
-    } else if (i == P::psi_0) {
+    } else if (P::is_psi0(i)) {
       params[i] = x[i] * (psi_max - psi_min) + psi_min;
     } else if (BC && i == P::b) {
       double log_b = log_c * (x[i] * (L1 - L0) + L0);
@@ -236,7 +246,7 @@ __device__ void params2x(CF* x, const double* params, const Space<CF>& sp) {
         LAM = params[i] > 1.0 / Ii;
         if (LAM) x[i] = 0.5 * ((params[i] - 1.0 / Ii) / (1.0 - 1.0 / Ii) + 1.0);
         else x[i] = 0.5 * (params[i] - 1.0 / Is) / (1.0 / Ii - 1.0 / Is);
-      } else if (i == P::psi_0) {
+      } else if (P::is_psi0(i)) {
         double psi_min, psi_max;
         if (LAM) { psi_min = 0.0; psi_max = 2.0 * PI; }
         else {
@@ -783,6 +793,37 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
     nx_h = pts[i + 1].h;
     nx_hh = pts[i + 1].hh;
     nx_h6 = pts[i + 1].h6;
+    if constexpr (SG) {
+      // The first observation of segment k > 0 (its record's step count is
+      // -k): the rotation starts again there from the segment's own Euler
+      // angles (cuda.c:163-252 inside the segment loop); L, Es and the
+      // shape are shared, so mu and the brightness constants stay.
+      if (nsteps < 0) {
+        const bool k1 = nsteps == -1;
+        const double tMk = k1 ? params[P::seg(1, 0)] : params[P::seg(2, 0)];
+        const double pMk = k1 ? params[P::seg(1, 1)] : params[P::seg(2, 1)];
+        const double p0k = k1 ? params[P::seg(1, 2)] : params[P::seg(2, 2)];
+        const double s0k = k1 ? params[P::seg(1, 3)] : params[P::seg(2, 3)];
+        astm::sincos(tMk, &sTM, &cTM);
+        astm::sincos(pMk, &sPM, &cPM);
+        Fd.M_x = sTM * cPM;
+        Fd.M_y = sTM * sPM;
+        Fd.M_z = cTM;
+        const double XMk = __builtin_sqrt(__builtin_fma(Fd.M_x, Fd.M_x, Fd.M_z * Fd.M_z));
+        Fd.XM_x = Fd.M_z / XMk;
+        Fd.XM_z = -Fd.M_x / XMk;
+        Fd.YM_x = Fd.M_y * Fd.XM_z;
+        Fd.YM_y = __builtin_fma(-Fd.M_x, Fd.XM_z, Fd.M_z * Fd.XM_x);
+        Fd.YM_z = -(Fd.M_y * Fd.XM_x);
+        astm::sincos(s0k, &sP0, &cP0);
+        y[1] = astm::asin(__builtin_sqrt((Es - 1.0) / (__builtin_fma(sP0 * sP0, Ii_inv - Is_inv, Is_inv) - 1.0)));
+        y[0] = p0k;
+        y[2] = s0k;
+        astm::sincos(y[1], &sc[0], &sc[1]);
+        sc[2] = sP0;
+        sc[3] = cP0;
+      }
+    }
     int nsub = 1;
     double t2_end = 0.0;
     if constexpr (T2) {

@@ -42,8 +42,8 @@ extern "C" int ast_default_model_space(const ast_variant* v, const double* ppsi_
   }
   add(T.c_tumb, 1, 1, AST_HARD_RIGHT);
   add(T.b_tumb, 0, 1, AST_HARD_BOTH);
-  add(T.Es, 0, 0, AST_HARD_BOTH);
-  add(T.L, 1, 0, AST_HARD_LEFT);
+  add(T.Es, 0, v->one_le ? 1 : 0, AST_HARD_BOTH);  // ONE_LE: shared by the segments (asteroid.c:68-72)
+  add(T.L, 1, v->one_le ? 1 : 0, AST_HARD_LEFT);
   add(T.psi_0, 0, 0, AST_PERIODIC_LAM);
   if (v->bc || v->rect) {
     add(T.c, 1, 1, AST_HARD_RIGHT);
@@ -94,12 +94,21 @@ extern "C" int ast_default_model_space(const ast_variant* v, const double* ppsi_
     lo[T.kappa] = std::log(0.01);
     hi[T.kappa] = std::log(0.1);
   }
+  const int n0 = n;  // SEGMENT: further segments repeat the unshared parameters (asteroid.c:128-150)
+  for (int k = 1; k < v->n_seg; ++k)
+    for (int j = 0; j < n0; ++j)
+      if (rows[j][AST_P_MULTI_SEGMENT] == 0) {
+        if (n >= AST_MAX_PARAMS) return AST_E_UNSUPPORTED;
+        for (int c = 0; c < AST_N_COLUMNS; ++c) rows[n][c] = rows[j][c];
+        rows[n++][AST_P_ISEG] = k;
+      }
+  if (n != T.n_params) return AST_E_UNSUPPORTED;
   *n_params = n;
   *n_types = nt;
   for (int i = 0; i < n; ++i)
     for (int k = 0; k < AST_N_COLUMNS; ++k) property[i * AST_N_COLUMNS + k] = rows[i][k];
   for (int t = 0; t < nt; ++t) types[t] = -1;
-  for (int i = 0; i < n; ++i) types[rows[i][AST_P_TYPE]] = i;
+  for (int i = 0; i < n0; ++i) types[rows[i][AST_P_TYPE]] = i;
   for (int t = 0; t < nt; ++t) {
     limits[t] = lo[t];
     limits[nt + t] = hi[t];

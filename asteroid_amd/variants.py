@@ -19,11 +19,11 @@ TMAX = 10.0                        # asteroid.c:507
 
 def make_variant(torque=True, bc=False, bw_ball=False, p_psi=True, interp=True,
                  precision=abi.PREC_FP64, time_step=1e-2, trend=False, torque2=False, rotate=False, rect=False,
-                 nudge=False, v1s=False):
+                 nudge=False, v1s=False, n_seg=0, one_le=False):
     """The reference's macros; TORQUE2 implies TORQUE and RECT implies BC (asteroid.h:32-38)."""
     return abi.Variant(int(torque or torque2), int(bc or rect), int(bw_ball), int(p_psi), int(interp),
                        int(precision), float(time_step), int(trend), int(torque2), int(rotate), int(rect),
-                       int(nudge), int(v1s))
+                       int(nudge), int(v1s), int(n_seg), int(one_le))
 
 
 def param_positions(variant):
@@ -45,7 +45,18 @@ def param_positions(variant):
         order += ["psi_R"]
     if variant.bw_ball:
         order += ["kappa"]
+    # SEGMENT: segment k > 0 repeats the parameters that are not shared, as
+    # "name:k", after the first segment's (asteroid.c:128-150)
+    for k in range(1, variant.n_seg):
+        order += [f"{n}:{k}" for n in order[:len(order)] if ":" not in n and not _shared(variant, n)]
     return {n: i for i, n in enumerate(order)}
+
+
+def _shared(variant, name):
+    """P_multi_segment of a parameter (asteroid.c:86-125; Es and L under ONE_LE)."""
+    if name in ("Es", "L"):
+        return bool(variant.one_le)
+    return name in ("A", "c_tumb", "b_tumb", "c", "b", "theta_R", "phi_R", "psi_R", "kappa")
 
 
 # Named variants of the benchmark workloads.
@@ -137,9 +148,20 @@ def default_model_space(variant, ppsi_hours=(2.0, 4800.0), reopt=0, frozen=None,
     if variant.bw_ball:
         rows += [(T["kappa"], 1, 0, 0, 1, abi.SOFT_BOTH)]
     rows = [list(r) for r in rows]
+    if variant.one_le:  # ONE_LE: Es and L shared by the segments (asteroid.c:68-72, 108-109)
+        for r in rows:
+            if r[abi.P_TYPE] in (T["Es"], T["L"]):
+                r[abi.P_MULTI_SEGMENT] = 1
+    n0 = len(rows)
+    for k in range(1, variant.n_seg):  # asteroid.c:128-150
+        for j in range(n0):
+            if rows[j][abi.P_MULTI_SEGMENT] == 0:
+                rows.append(list(rows[j]))
+                rows[-1][abi.P_ISEG] = k
+    assert len(rows) == T["n_params"]
     nt = T["n_types"]
     types = [-1] * nt
-    for i, r in enumerate(rows):
+    for i, r in enumerate(rows[:n0]):
         types[r[abi.P_TYPE]] = i
     lo = [0.0] * nt
     hi = [0.0] * nt
@@ -201,3 +223,8 @@ def ls_torque_nudge(precision=abi.PREC_FP64, time_step=1e-2, interp=True, bc=Fal
     """Stage Three (README.md:87-101): LS (or BC) with torque, chi^2 nudged by
     the observed minima (-DNUDGE), 11 (13 with BC) parameters."""
     return make_variant(True, bc, False, True, interp, precision, time_step, nudge=True, v1s=v1s)
+
+
+def ls_segments(precision=abi.PREC_FP64, time_step=1e-2, interp=True, n_seg=3, one_le=True):
+    """-DSEGMENT -DN_SEG=n_seg [-DONE_LE], LS without torque."""
+    return make_variant(False, False, False, True, interp, precision, time_step, n_seg=n_seg, one_le=one_le)

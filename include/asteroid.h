@@ -40,7 +40,8 @@ enum {
   AST_HARD_RIGHT = -3,
   AST_HARD_BOTH = -4
 };
-enum { AST_MAX_PARAMS = 16, AST_MAX_TYPES = 16, AST_MAX_FILTERS = 8, AST_MAX_MINIMA = 10 /* NOBS_MAX, asteroid.h:219 */ };
+enum { AST_MAX_PARAMS = 16, AST_MAX_TYPES = 16, AST_MAX_FILTERS = 8, AST_MAX_MINIMA = 10 /* NOBS_MAX, asteroid.h:219 */,
+       AST_MAX_SEGMENTS = 4 /* N_SEG, asteroid.h:40-46 */ };
 
 /* Error codes. */
 enum {
@@ -68,8 +69,9 @@ enum {
  * the reference (asteroid.h:32-38).  The kernels are built for LS, BC and
  * BW_BALL with and without torque, and for TREND with torque, TORQUE2,
  * ROTATE (with BC, with and without torque), RECT (with and without torque)
- * and NUDGE (with torque, with and without BC); other combinations return
- * AST_E_UNSUPPORTED.  SEGMENT, P_PHI and P_BOTH are not built. */
+ * NUDGE (with torque, with and without BC) and SEGMENT (LS without torque,
+ * three segments, ONE_LE: 16 parameters); other combinations return
+ * AST_E_UNSUPPORTED.  P_PHI and P_BOTH are not built. */
 typedef struct {
   int torque;     /* -DTORQUE : 6-ODE constant-torque rotation                 */
   int bc;         /* -DBC     : photometric b,c independent of kinematic ones  */
@@ -85,6 +87,9 @@ typedef struct {
   int nudge;      /* -DNUDGE  : chi^2 nudged by model minima near observed ones (cuda.c:716-758, 912-992);
                    *            needs ast_set_minima                                               */
   int v1s;        /* -DV1S    : with NUDGE, deeper model minima are not penalised (cuda.c:940-945) */
+  int n_seg;      /* -DSEGMENT -DN_SEG=n: the curve split at n start times, each segment its own
+                   *            rotation (cuda.c:162-813); 0 or 1 = one segment; needs ast_set_segments */
+  int one_le;     /* -DONE_LE : with SEGMENT, L and Es shared by all segments (asteroid.c:68-72) */
 } ast_variant;
 
 /* Parameter type ids of a variant: the reference's __COUNTER__ order
@@ -126,6 +131,8 @@ static inline int ast_variant_type_ids(const ast_variant* v, ast_type_ids* t) {
   t->n_types = id;
   t->n_params = 8 + (bc ? 2 : 0) + (v->rotate ? 3 : 0) + (v->trend ? 1 : 0) + (v->torque2 ? 7 : (torque ? 3 : 0)) +
                 (v->bw_ball ? 3 : 0);
+  if (v->n_seg > 1)  /* each further segment repeats the non-shared parameters (asteroid.h:104-108) */
+    t->n_params += (6 + (v->torque2 ? 7 : (torque ? 3 : 0)) - (v->one_le ? 2 : 0)) * (v->n_seg - 1);
   return AST_OK;
 }
 
@@ -208,6 +215,17 @@ int ast_upload_curve(ast_ctx* ctx, const ast_obs* obs, int n_data, int n_filters
  * evaluation of a NUDGE context needs it (AST_E_STATE until set); it may
  * come before or after ast_upload_curve. */
 int ast_set_minima(ast_ctx* ctx, const float* t_obs, const float* V_obs, int n_obs);
+
+/* SEGMENT: the segment start times, replacing the hard-coded T_START table
+ * (asteroid.h:40-46) and the split in read_data.c:244-250.  t_start are
+ * times on the curve's clock (the reference's T_START - hMJD0), n_seg of
+ * them, n_seg equal to the variant's; AST_E_ARG otherwise, AST_E_STATE on a
+ * context without SEGMENT.  Must come before ast_upload_curve.  As in the
+ * reference, observation i opens the next segment s when its time is at or
+ * after t_start[s]; observations before the first segment are left out of
+ * the sums but still count in the degrees of freedom.  A curve on which a
+ * segment gets no opening observation is refused (AST_E_DATA). */
+int ast_set_segments(ast_ctx* ctx, const double* t_start, int n_seg);
 
 /* Seed the per-simplex random streams and reset each group's best chi^2 to
  * 1e30: setup_kernel(..., seed, d_f, 1) (cuda.c:2039-2054, asteroid.c:629-634).

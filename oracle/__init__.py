@@ -18,7 +18,8 @@ class OrcProblem(C.Structure):
                 ("types", C.c_int * abi.MAX_TYPES), ("limits", C.c_double * (2 * abi.MAX_TYPES)),
                 ("x2", abi.X2), ("n_data", C.c_int), ("n_filters", C.c_int),
                 ("obs", C.POINTER(abi.Obs)), ("eph", abi.Ephem3), ("n_min", C.c_int),
-                ("t_min", C.c_float * abi.MAX_MINIMA), ("V_min", C.c_float * abi.MAX_MINIMA)]
+                ("t_min", C.c_float * abi.MAX_MINIMA), ("V_min", C.c_float * abi.MAX_MINIMA),
+                ("t_start", C.c_double * abi.MAX_SEGMENTS)]
 
 
 def _has_fma():
@@ -72,9 +73,10 @@ def _p(a, ct=C.c_double):
 class Problem:
     """A model space plus a curve, held in the layout the oracle reads."""
 
-    def __init__(self, space, curve, minima=None):
-        """minima = (t_obs, V_obs) for NUDGE variants (times on the curve's clock)."""
-        self.space, self.curve, self.minima = space, curve, minima
+    def __init__(self, space, curve, minima=None, t_start=None):
+        """minima = (t_obs, V_obs) for NUDGE variants, t_start = the segment start
+        times for SEGMENT variants (both on the curve's clock)."""
+        self.space, self.curve, self.minima, self.t_start = space, curve, minima, t_start
         pb = OrcProblem()
         pb.v = space.variant
         pb.n_params = space.n_params
@@ -97,10 +99,13 @@ class Problem:
             for k in range(len(t_obs)):
                 pb.t_min[k] = float(t_obs[k])
                 pb.V_min[k] = float(V_obs[k])
+        if t_start is not None:
+            for k, val in enumerate(t_start):
+                pb.t_start[k] = float(val)
         self.pb = pb
 
     def with_space(self, space):
-        return Problem(space, self.curve, self.minima)
+        return Problem(space, self.curve, self.minima, self.t_start)
 
     def chi2(self, params, form=CANON, want_vmod=False, want_traj=False):
         params = np.ascontiguousarray(np.atleast_2d(params), np.float64)

@@ -81,6 +81,12 @@ struct Problem {
   // NUDGE's observed minima (chi2_struct, asteroid.h:319-337)
   int n_min;
   float t_min[AST_MAX_MINIMA], V_min[AST_MAX_MINIMA];
+  // SEGMENT: first observation of each segment (read_data.c:244-250) and the
+  // parameter of each type in each segment (Types[N_TYPES][N_SEG],
+  // asteroid.c:152-168); n_seg = 0 without SEGMENT.
+  int n_seg;
+  int seg_start[AST_MAX_SEGMENTS];
+  int seg_types[AST_MAX_TYPES][AST_MAX_SEGMENTS];
   // TORQUE2 implies TORQUE and RECT implies BC (asteroid.h:32-38)
   bool tq() const { return v.torque || v.torque2; }
   bool bc() const { return v.bc || v.rect; }
@@ -291,10 +297,11 @@ inline void ode_literal(bool torque, const double* y, double* f, const double* m
 
 // Per-filter sums of one model (cuda.c:195-715).
 inline void model_literal(const Problem& P, const double* params, double* sy2, double* sy,
-                          double* sw, ModelOut* out) {
+                          double* sw, ModelOut* out, bool accumulate = false) {
   const ast_type_ids& T = P.T;
   const bool torque = P.tq();
-  for (int m = 0; m < P.n_filters; ++m) sy2[m] = sy[m] = sw[m] = 0.0;
+  if (!accumulate)
+    for (int m = 0; m < P.n_filters; ++m) sy2[m] = sy[m] = sw[m] = 0.0;
   double tM = params[P.types[T.theta_M]], pM = params[P.types[T.phi_M]];
   double M_x = std::sin(tM) * std::cos(pM);
   double M_y = std::sin(tM) * std::sin(pM);
@@ -522,13 +529,15 @@ template <class R> inline R dot3(R ax, R ay, R az, R bx, R by, R bz) {
 }
 
 template <class R>
-inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R* sw, ModelOut* out) {
+inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R* sw, ModelOut* out,
+                        bool accumulate = false) {
   using astm::p_fma;
   using astm::p_sqrt;
   using astm::p_val;
   const ast_type_ids& T = P.T;
   const bool torque = P.tq();
-  for (int m = 0; m < P.n_filters; ++m) sy2[m] = sy[m] = sw[m] = R(0.0);
+  if (!accumulate)
+    for (int m = 0; m < P.n_filters; ++m) sy2[m] = sy[m] = sw[m] = R(0.0);
   R sTM, cTM, sPM, cPM;
   astm::sincos(R(params[P.types[T.theta_M]]), &sTM, &cTM);
   astm::sincos(R(params[P.types[T.phi_M]]), &sPM, &cPM);
@@ -788,6 +797,41 @@ inline CF chi2_from_sums(const Problem& P, const double* sy2, const double* sy, 
   return chi2a;
 }
 
+// The model sums in either form.  Under SEGMENT (cuda.c:162-813) each
+// segment is an integration of its own from its first observation, with the
+// segment's parameters (params[Types[type][iseg]]), and the sums run on over
+// all segments in observation order; observations before the first segment
+// are not visited.  The segment's sub-problem is the plain model on its
+// observations with its parameters in the one-segment layout.
+inline void model_sums(const Problem& P, int form, const double* params, double* sy2, double* sy, double* sw,
+                       ModelOut* out) {
+  if (P.n_seg <= 1) {
+    if (form == 0) model_literal(P, params, sy2, sy, sw, out);
+    else model_canon<double>(P, params, sy2, sy, sw, out);
+    return;
+  }
+  for (int m = 0; m < P.n_filters; ++m) sy2[m] = sy[m] = sw[m] = 0.0;
+  for (int iseg = 0; iseg < P.n_seg; ++iseg) {
+    const int i1 = P.seg_start[iseg];
+    const int i2 = iseg < P.n_seg - 1 ? P.seg_start[iseg + 1] : P.n_data;  // cuda.c:260-268
+    Problem Q = P;
+    Q.n_seg = 0;
+    Q.obs = P.obs + i1;
+    Q.n_data = i2 - i1;
+    double ps[AST_MAX_PARAMS] = {0};
+    for (int t = 0; t < P.n_types; ++t)
+      if (P.types[t] >= 0) ps[P.types[t]] = params[P.seg_types[t][iseg]];
+    ModelOut o{nullptr, nullptr, nullptr};
+    if (out) {
+      o = *out;
+      if (o.vmod) o.vmod += i1;
+      if (o.traj) o.traj += 6 * (size_t)i1;
+    }
+    if (form == 0) model_literal(Q, ps, sy2, sy, sw, &o, true);
+    else model_canon<double>(Q, ps, sy2, sy, sw, &o, true);
+  }
+}
+
 // chi^2 of one model; under NUDGE a failed model returns 1e30 with delta_V
 // untouched (cuda.c:737), else the nudged chi^2.
 template <class CF>
@@ -797,8 +841,7 @@ inline CF chi2_eval(const Problem& P, int form, const double* params, CF* dV, Mo
   ModelOut o{nullptr, nullptr, nullptr};
   if (out) o = *out;
   if (P.v.nudge) o.nudge = &nu;
-  if (form == 0) model_literal(P, params, sy2, sy, sw, &o);
-  else model_canon<double>(P, params, sy2, sy, sw, &o);
+  model_sums(P, form, params, sy2, sy, sw, &o);
   if (P.v.nudge && nu.failed) return CF(1e30);
   CF c = chi2_from_sums<CF>(P, sy2, sy, sw, dV);
   if (P.v.nudge) c = nu.merit<CF>(P, form, c, dV);
@@ -1182,8 +1225,7 @@ inline CF rmsd_eval(const Problem& P, int form, const double* params, CF* dV) {
   double sy2[AST_MAX_FILTERS], sy[AST_MAX_FILTERS], sw[AST_MAX_FILTERS];
   Nudge nu;
   ModelOut o{nullptr, nullptr, P.v.nudge ? &nu : nullptr};
-  if (form == 0) model_literal(P, params, sy2, sy, sw, &o);
-  else model_canon<double>(P, params, sy2, sy, sw, &o);
+  model_sums(P, form, params, sy2, sy, sw, &o);
   if (P.v.nudge && nu.failed) return CF(1e30);
   CF c = rmsd_from_sums<CF>(P, sy2, sy, sw, dV);
   if (P.v.nudge) c = nu.merit<CF>(P, form, c, dV);
@@ -1470,6 +1512,7 @@ typedef struct {
   int n_min;                      /* NUDGE: observed minima (misc.c:279-323) */
   float t_min[AST_MAX_MINIMA];
   float V_min[AST_MAX_MINIMA];
+  double t_start[AST_MAX_SEGMENTS]; /* SEGMENT: segment start times on the curve's clock */
 } orc_problem;
 
 static int load_problem(const orc_problem* in, orc::Problem* P) {
@@ -1497,6 +1540,32 @@ static int load_problem(const orc_problem* in, orc::Problem* P) {
   for (int k = 0; k < AST_MAX_MINIMA; ++k) {
     P->t_min[k] = in->t_min[k];
     P->V_min[k] = in->V_min[k];
+  }
+  P->n_seg = 0;
+  if (in->v.n_seg > 1) {
+    const int ns = in->v.n_seg;
+    if (ns > AST_MAX_SEGMENTS || in->n_params != P->T.n_params) return AST_E_ARG;
+    int iseg = 0;
+    for (int i = 0; i < in->n_data; ++i)  // read_data.c:244-250
+      if (iseg < ns && in->obs[i].mjd >= in->t_start[iseg]) P->seg_start[iseg++] = i;
+    if (iseg < ns) return AST_E_DATA;  // a segment without an opening observation
+    for (int t = 0; t < AST_MAX_TYPES; ++t)
+      for (int k = 0; k < AST_MAX_SEGMENTS; ++k) P->seg_types[t][k] = -1;
+    for (int i = 0; i < in->n_params; ++i) {  // asteroid.c:152-168
+      const int t = P->prop[i][AST_P_TYPE];
+      if (t < 0 || t >= AST_MAX_TYPES) return AST_E_ARG;
+      if (P->prop[i][AST_P_MULTI_SEGMENT] == 0) {
+        const int k = P->prop[i][AST_P_ISEG];
+        if (k < 0 || k >= ns) return AST_E_ARG;
+        P->seg_types[t][k] = i;
+      } else {
+        for (int k = 0; k < ns; ++k) P->seg_types[t][k] = i;
+      }
+    }
+    for (int t = 0; t < P->n_types; ++t)
+      for (int k = 0; k < ns; ++k)
+        if (P->types[t] >= 0 && P->seg_types[t][k] < 0) return AST_E_ARG;
+    P->n_seg = ns;
   }
   return 0;
 }
@@ -1535,6 +1604,7 @@ int orc_model_curve(const orc_problem* pb, int form, const double* params, int n
   int rc = load_problem(pb, &P);
   if (rc) return rc;
   if (n_plot < 2) return AST_E_ARG;
+  if (P.n_seg > 1) return AST_E_UNSUPPORTED;  // the plot's segment split (read_data.c:357-363) is not restated
   const std::vector<ast_obs> grid = orc::plot_grid(P, form, n_plot);
   if (t) for (int i = 0; i < n_plot; ++i) t[i] = grid[i].mjd;
   auto work = [&](int k) {
@@ -1702,6 +1772,7 @@ int orc_count_ops(const orc_problem* pb, const double* params, double* out) {
   orc::Problem P;
   int rc = load_problem(pb, &P);
   if (rc) return rc;
+  if (P.n_seg > 1) return AST_E_UNSUPPORTED;  // counted per segment by the caller
   orc::cnt::K = orc::cnt::Counts{0, 0, 0, 0, 0, 0};
   orc::cnt::R sy2[AST_MAX_FILTERS], sy[AST_MAX_FILTERS], sw[AST_MAX_FILTERS];
   orc::model_canon<orc::cnt::R>(P, params, sy2, sy, sw, nullptr);

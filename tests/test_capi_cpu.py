@@ -40,7 +40,11 @@ def test_strerror_and_validation_without_device():
     assert lib.ast_ctx_create(C.byref(h), C.byref(bad), 0) == -6
     bad = variants.ls_torque_nudge(abi.PREC_MIXED)  # NUDGE is offered in the FP64 and float-simplex forms
     assert lib.ast_ctx_create(C.byref(h), C.byref(bad), 0) == -6
+    for bad in (variants.ls_segments(one_le=False), variants.ls_segments(n_seg=4),  # beyond 16 parameters
+                variants.make_variant(True, False, False, n_seg=3, one_le=True)):   # SEGMENT only without torque
+        assert lib.ast_ctx_create(C.byref(h), C.byref(bad), 0) == -6
     assert lib.ast_set_minima(None, None, None, 1) == -1
+    assert lib.ast_set_segments(None, None, 3) == -1
     assert lib.ast_run(None, None) == -1
     assert lib.ast_fetch(None, None, None, None) == -1
 
@@ -49,7 +53,7 @@ VARS = [variants.ls_torque(), variants.bc_free(), variants.bw_ball_torque(),
         variants.make_variant(False, False, False, True, True), variants.make_variant(True, True, False, True, True),
         variants.make_variant(False, False, True, False, False), variants.ls_torque_trend(), variants.ls_torque2(),
         variants.bc_rotate(), variants.bc_rotate(torque=True), variants.rect(), variants.rect(torque=True),
-        variants.ls_torque_nudge(), variants.ls_torque_nudge(bc=True)]
+        variants.ls_torque_nudge(), variants.ls_torque_nudge(bc=True), variants.ls_segments()]
 
 
 @pytest.mark.parametrize("v", VARS)
@@ -62,20 +66,24 @@ def test_c_tables_equal_python_tables(v):
     assert c["ppsi1"] == py.ppsi1 and c["ppsi2"] == py.ppsi2
 
 
-# (torque, bc, bw_ball, trend, torque2, rotate, rect): every flag combination the kernels build
-VARIANT_FLAGS = [(t, b, w, 0, 0, 0, 0) for t in (0, 1) for b in (0, 1) for w in (0, 1)] + [
-    (1, 0, 0, 1, 0, 0, 0), (1, 0, 0, 0, 1, 0, 0), (0, 1, 0, 0, 0, 1, 0), (1, 1, 0, 0, 0, 1, 0),
-    (0, 1, 0, 0, 0, 0, 1), (1, 1, 0, 0, 0, 0, 1), (1, 0, 0, 0, 1, 1, 0), (0, 1, 0, 1, 0, 1, 1)]
+# (torque, bc, bw_ball, trend, torque2, rotate, rect, n_seg, one_le): every flag combination the
+# kernels build, and parameter counts of SEGMENT builds that are not built
+VARIANT_FLAGS = [(t, b, w, 0, 0, 0, 0, 0, 0) for t in (0, 1) for b in (0, 1) for w in (0, 1)] + [
+    (1, 0, 0, 1, 0, 0, 0, 0, 0), (1, 0, 0, 0, 1, 0, 0, 0, 0), (0, 1, 0, 0, 0, 1, 0, 0, 0),
+    (1, 1, 0, 0, 0, 1, 0, 0, 0), (0, 1, 0, 0, 0, 0, 1, 0, 0), (1, 1, 0, 0, 0, 0, 1, 0, 0),
+    (1, 0, 0, 0, 1, 1, 0, 0, 0), (0, 1, 0, 1, 0, 1, 1, 0, 0), (0, 0, 0, 0, 0, 0, 0, 3, 1),
+    (0, 0, 0, 0, 0, 0, 0, 4, 0), (1, 0, 0, 0, 0, 0, 0, 3, 1), (0, 1, 0, 1, 1, 0, 0, 4, 1)]
 
 
 def test_header_type_ids_match_python(tmp_path):
     n = len(abi.TypeIds._fields_)
-    rows_c = ",".join("{%d,%d,%d,%d,%d,%d,%d}" % f for f in VARIANT_FLAGS)
+    rows_c = ",".join("{%d,%d,%d,%d,%d,%d,%d,%d,%d}" % f for f in VARIANT_FLAGS)
     src = tmp_path / "ids.c"
     src.write_text('#include <stdio.h>\n#include "asteroid.h"\nint main(void){\n'
-                   'static const int F[][7] = {%s};\n'
+                   'static const int F[][9] = {%s};\n'
                    'for (unsigned m = 0; m < sizeof F / sizeof F[0]; ++m) {\n'
-                   ' ast_variant v = {F[m][0], F[m][1], F[m][2], 1, 1, 0, 0.01, F[m][3], F[m][4], F[m][5], F[m][6]};\n'
+                   ' ast_variant v = {F[m][0], F[m][1], F[m][2], 1, 1, 0, 0.01, F[m][3], F[m][4], F[m][5], F[m][6],'
+                   ' 0, 0, F[m][7], F[m][8]};\n'
                    ' ast_type_ids t; ast_variant_type_ids(&v, &t); int* p = (int*)&t;\n'
                    ' for (int k = 0; k < %d; ++k) printf("%%d ", p[k]); printf("\\n"); }\nreturn 0; }\n'
                    % (rows_c, n))
@@ -83,8 +91,8 @@ def test_header_type_ids_match_python(tmp_path):
     subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
     rows = subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True, check=True).stdout.split("\n")
     assert C.sizeof(abi.TypeIds) == 4 * n
-    for m, (t, b, w, tr, t2, ro, rc) in enumerate(VARIANT_FLAGS):
-        v = variants.make_variant(t, b, w, trend=tr, torque2=t2, rotate=ro, rect=rc)
+    for m, (t, b, w, tr, t2, ro, rc, ns, le) in enumerate(VARIANT_FLAGS):
+        v = variants.make_variant(t, b, w, trend=tr, torque2=t2, rotate=ro, rect=rc, n_seg=ns, one_le=le)
         ids = abi.type_ids(v)
         got = [int(x) for x in rows[m].split()]
         assert got == [ids[f] for f, _ in abi.TypeIds._fields_], VARIANT_FLAGS[m]

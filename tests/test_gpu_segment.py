@@ -1,0 +1,121 @@
+"""SEGMENT (-DSEGMENT -DN_SEG=3 -DONE_LE, LS) on the GPU against the oracle:
+chi^2 and delta_V of the evaluator bit for bit (FP64 and float-simplex
+forms; the mixed form within its 1e-6 relative tolerance), whole fits lane
+by lane, observations before the first segment and an observation past two
+start times; the call rules of ast_set_segments and the refusals."""
+import numpy as np
+import pytest
+
+import oracle
+from asteroid_amd import abi, api, synth, variants
+
+pytestmark = pytest.mark.gpu
+
+PRECS = {"fp64": abi.PREC_FP64, "f32simplex": abi.PREC_F32SIMPLEX, "mixed": abi.PREC_MIXED}
+
+
+def _case(prec, n=240, span=3.0, seed=6, skip=0):
+    v8 = variants.make_variant(False, False, False, True, True, prec)
+    curve, truth8 = synth.make_curve(seed, v8, n, span)
+    v = variants.ls_segments(prec)
+    space = variants.default_model_space(v)
+    t0 = curve.mjd[skip] if skip else curve.mjd[0] - 0.01
+    t_start = np.array([t0, curve.mjd[0] + span / 3, curve.mjd[0] + 2 * span / 3])
+    pos8 = variants.param_positions(v8)
+    rng = np.random.default_rng(seed)
+    truth = np.array([truth8[pos8[name.split(":")[0]]] + (rng.uniform(-0.2, 0.2) if ":" in name else 0.0)
+                      for name in variants.param_positions(v)])
+    return v, curve, truth, space, t_start
+
+
+def _ctx(v, space, curve, t_start):
+    ctx = api.Context(v)
+    ctx.set_model_space(space)
+    ctx.set_segments(t_start)
+    ctx.upload_curve(curve)
+    return ctx
+
+
+def _models(pb, space, truth, n=300, seed=3):
+    x = np.random.default_rng(seed).uniform(0.02, 0.98, (n, space.n_params))
+    p, rc = pb.x2params(x, reopt=0)
+    near = truth * (1 + np.random.default_rng(seed + 1).uniform(-0.003, 0.003, (32, space.n_params)))
+    return np.vstack([truth, near, p[rc == 0]])
+
+
+@pytest.mark.parametrize("skip", [0, 7], ids=["all", "first7_out"])
+@pytest.mark.parametrize("prec", ["fp64", "f32simplex"])
+def test_segment_chi2_batch_bitwise(prec, skip):
+    v, curve, truth, space, t_start = _case(PRECS[prec], skip=skip)
+    pb = oracle.Problem(space, curve, t_start=t_start)
+    params = _models(pb, space, truth)
+    want_c, want_dv = pb.chi2(params, form=oracle.CANON)
+    print(prec, skip, "models", len(params), flush=True)
+    with _ctx(v, space, curve, t_start) as ctx:
+        got_c, got_dv = ctx.chi2_batch(params)
+    assert np.array_equal(got_c, want_c)
+    assert np.array_equal(got_dv, want_dv)
+
+
+def test_segment_mixed_within_tolerance():
+    v, curve, truth, space, t_start = _case(PRECS["mixed"])
+    pb = oracle.Problem(variants.default_model_space(variants.ls_segments(abi.PREC_FP64)), curve, t_start=t_start)
+    params = _models(pb, space, truth)
+    want_c, _ = pb.chi2(params, form=oracle.CANON)
+    with _ctx(v, space, curve, t_start) as ctx:
+        got_c, _ = ctx.chi2_batch(params)
+    ok = want_c < 1e29
+    rel = np.abs(got_c[ok] - want_c[ok]) / want_c[ok]
+    print("mixed max rel", rel.max(), flush=True)
+    assert rel.max() < 1e-6
+
+
+def test_segment_observation_past_two_starts():
+    v, curve, truth, space, _ = _case(abi.PREC_FP64, seed=8)
+    i = 100
+    t_start = np.array([curve.mjd[0], curve.mjd[i] - 1e-4, curve.mjd[i] - 5e-5])  # segment 2 opens at i + 1
+    pb = oracle.Problem(space, curve, t_start=t_start)
+    params = _models(pb, space, truth, n=64)
+    want_c, want_dv = pb.chi2(params)
+    with _ctx(v, space, curve, t_start) as ctx:
+        got_c, got_dv = ctx.chi2_batch(params)
+    assert np.array_equal(got_c, want_c) and np.array_equal(got_dv, want_dv)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "f32simplex"])
+def test_segment_fit_trajectories_identical(prec):
+    v, curve, truth, space, t_start = _case(PRECS[prec], n=150, span=2.0, seed=12, skip=3)
+    pb = oracle.Problem(space, curve, t_start=t_start)
+    G, gs, n_steps, seed = 2, 64, 60, 31
+    want = pb.run(G, gs, nstages=2, n_steps=n_steps, seed=seed, form=oracle.CANON, n_threads=8)
+    print(prec, "oracle f", want["f"], flush=True)
+    with _ctx(v, space, curve, t_start) as ctx:
+        ctx.rng_seed(seed)
+        ctx.run(G, gs, nstages=2, n_steps=n_steps, ticks_per_launch=37)
+        f, p, dv = ctx.fetch()
+        lf, lne, lns = ctx.fetch_lanes()
+    assert np.array_equal(lne, want["lane_nev"]) and np.array_equal(lns, want["lane_steps"])
+    assert np.array_equal(lf, want["lane_f"])
+    assert np.array_equal(f, want["f"]) and np.array_equal(p, want["params"]) and np.array_equal(dv, want["dV"])
+
+
+def test_segment_call_rules():
+    v, curve, truth, space, t_start = _case(abi.PREC_FP64, n=60)
+    with api.Context(v) as ctx:
+        ctx.set_model_space(space)
+        with pytest.raises(api.AsteroidError):
+            ctx.upload_curve(curve)  # start times first
+        with pytest.raises(api.AsteroidError):
+            ctx.set_segments(t_start[:2])  # three segments
+        with pytest.raises(api.AsteroidError):
+            ctx.set_segments([t_start[0], t_start[1], curve.mjd[-1] + 1.0])
+            ctx.upload_curve(curve)  # the third segment has no observation
+        ctx.set_segments(t_start)
+        ctx.upload_curve(curve)
+        with pytest.raises(api.AsteroidError):
+            ctx.model_curve(truth[None, :])  # not built for SEGMENT
+        with pytest.raises(api.AsteroidError):
+            ctx.rmsd(1, 64, 4, 0.003)
+    with api.Context(variants.ls_torque()) as ctx:
+        with pytest.raises(api.AsteroidError):
+            ctx.set_segments(t_start)  # not a SEGMENT context

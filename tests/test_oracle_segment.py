@@ -1,0 +1,137 @@
+"""SEGMENT (-DSEGMENT -DN_SEG=3 -DONE_LE) in the oracle: the split of the
+curve at the segment start times (read_data.c:244-250), the per-segment
+parameters (Types[type][iseg], asteroid.c:128-168) and the sums over all
+segments with the full N_data in the degrees of freedom (cuda.c:162-268,
+884-906).
+
+The pins: every segment's model magnitudes equal the plain LS model's on
+that segment's observations alone, with the segment's parameters picked by
+name; chi^2 and delta_V equal a brute-force weighted least-squares fit of
+those magnitudes; observations before the first start are not visited; an
+observation at or after two start times opens only one segment."""
+import numpy as np
+import pytest
+
+import oracle
+from asteroid_amd import synth, variants
+
+
+def _sub(curve, i1, i2):
+    return synth.Curve(curve.mjd[i1:i2], curve.V[i1:i2], curve.w[i1:i2], curve.filter[i1:i2], curve.n_filters,
+                       curve.E[i1:i2], curve.S[i1:i2], curve.eph)
+
+
+def _seg_params(v_seg, truth8, rng):
+    """A 16-parameter SEGMENT model around an LS model: each further segment
+    gets its own Euler angles and psi_0."""
+    pos8 = variants.param_positions(variants.make_variant(False, False, False, True, True))
+    pos = variants.param_positions(v_seg)
+    p = np.zeros(len(pos))
+    for name, i in pos.items():
+        base = name.split(":")[0]
+        p[i] = truth8[pos8[base]]
+        if ":" in name:
+            p[i] += rng.uniform(-0.3, 0.3)
+    return p
+
+
+def _setup(n=90, span=3.0, nf=2, seed=4, t_start=None):
+    v8 = variants.make_variant(False, False, False, True, True)
+    curve, truth8 = synth.make_curve(seed, v8, n, span, n_filters=nf)
+    vs = variants.ls_segments()
+    space = variants.default_model_space(vs)
+    if t_start is None:
+        t_start = [curve.mjd[0] - 0.1, curve.mjd[0] + span / 3, curve.mjd[0] + 2 * span / 3]
+    pb = oracle.Problem(space, curve, t_start=t_start)
+    pb8 = oracle.Problem(variants.default_model_space(v8), curve)
+    params = _seg_params(vs, truth8, np.random.default_rng(seed))
+    return pb, pb8, curve, truth8, params, np.asarray(t_start)
+
+
+def _starts(mjd, t_start):
+    s, k = [], 0
+    for i, t in enumerate(mjd):  # restated from read_data.c:244-250
+        if k < len(t_start) and t >= t_start[k]:
+            s.append(i)
+            k += 1
+    return s
+
+
+def _segment_vmod(pb8, curve, params, vs, starts, form):
+    """Magnitudes of each segment from the plain LS model on its observations."""
+    pos8 = variants.param_positions(pb8.space.variant)
+    pos = variants.param_positions(vs)
+    vm = np.zeros(curve.n_data)
+    for k, i1 in enumerate(starts):
+        i2 = starts[k + 1] if k + 1 < len(starts) else curve.n_data
+        p8 = np.zeros(8)
+        for name, j in pos8.items():
+            p8[j] = params[pos[name if k == 0 or variants._shared(vs, name) else f"{name}:{k}"]]
+        sub = oracle.Problem(pb8.space, _sub(curve, i1, i2))
+        _, _, v = sub.chi2(p8, form=form, want_vmod=True)
+        vm[i1:i2] = v[0]
+    return vm
+
+
+@pytest.mark.parametrize("form", [oracle.LITERAL, oracle.CANON])
+def test_segments_are_plain_models_on_their_observations(form):
+    pb, pb8, curve, _, params, t_start = _setup()
+    starts = _starts(curve.mjd, t_start)
+    assert len(starts) == 3 and starts[0] == 0 and 0 < starts[1] < starts[2] < curve.n_data
+    _, _, vm = pb.chi2(params, form=form, want_vmod=True)
+    ref = _segment_vmod(pb8, curve, params, pb.space.variant, starts, form)
+    assert np.array_equal(vm[0], ref)
+    # each segment restarts: the segment model differs from one long integration
+    _, _, v_one = pb8.chi2(params[:8], form=form, want_vmod=True)
+    assert np.array_equal(vm[0][:starts[1]], v_one[0][:starts[1]])
+    assert not np.allclose(vm[0][starts[1]:], v_one[0][starts[1]:])
+
+
+def test_chi2_is_the_least_squares_fit_over_all_segments():
+    pb, pb8, curve, _, params, t_start = _setup(t_start=None, seed=7)
+    # the first observations lie before the first start: left out of the sums
+    t_start = np.array([curve.mjd[5], curve.mjd[0] + 1.0, curve.mjd[0] + 2.0])
+    pb = oracle.Problem(pb.space, curve, t_start=t_start)
+    starts = _starts(curve.mjd, t_start)
+    assert starts[0] == 5
+    c, dV, vm = pb.chi2(params, want_vmod=True)
+    assert np.all(vm[0][:5] == 0)  # never visited
+    ref = _segment_vmod(pb8, curve, params, pb.space.variant, starts, oracle.CANON)
+    assert np.array_equal(vm[0][5:], ref[5:])
+    inc = np.arange(curve.n_data) >= 5
+    V, w, f = curve.V.astype(float), curve.w.astype(float), curve.filter
+    chi2 = 0.0
+    for m in range(curve.n_filters):
+        s = inc & (f == m)
+        d = V[s] - ref[s]
+        dv = np.sum(w[s] * d) / np.sum(w[s])  # the weighted mean minimises sum w (d - dv)^2
+        assert dV[0][m] == pytest.approx(dv, rel=1e-12, abs=1e-12)
+        chi2 += np.sum(w[s] * (d - dv) ** 2)
+    dof = curve.n_data - pb.space.n_params - curve.n_filters  # all observations count
+    assert c[0] == pytest.approx(chi2 / dof, rel=1e-10)
+
+
+def test_one_observation_past_two_starts_opens_one_segment():
+    pb, pb8, curve, _, params, _ = _setup(seed=9)
+    i = 30
+    t_start = np.array([curve.mjd[0], curve.mjd[i] - 1e-4, curve.mjd[i] - 5e-5])
+    pb = oracle.Problem(pb.space, curve, t_start=t_start)
+    starts = _starts(curve.mjd, t_start)
+    assert starts == [0, i, i + 1]  # the third segment opens at the next observation
+    _, _, vm = pb.chi2(params, want_vmod=True)
+    assert np.array_equal(vm[0], _segment_vmod(pb8, curve, params, pb.space.variant, starts, oracle.CANON))
+
+
+def test_a_segment_without_observations_is_refused():
+    pb, _, curve, _, params, _ = _setup()
+    bad = oracle.Problem(pb.space, curve, t_start=[curve.mjd[0], curve.mjd[-1], curve.mjd[-1] + 1.0])
+    with pytest.raises(AssertionError):
+        bad.chi2(params)
+
+
+def test_fit_runs_on_segments():
+    pb, _, curve, _, params, _ = _setup(n=60)
+    r = pb.run(2, 16, n_steps=60, seed=3)
+    assert np.all(np.isfinite(r["f"])) and np.all(r["f"] < 1e29)
+    c, _ = pb.chi2(r["params"][0])
+    assert c[0] == r["f"][0]
