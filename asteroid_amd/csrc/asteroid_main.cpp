@@ -6,7 +6,8 @@
 // stdout and result-file lines, on top of the C ABI of libasteroid_hip.so.
 // The reference chooses its model variant at compile time (makefile:3-36);
 // here the same macros are run-time flags (-DTORQUE, -DTORQUE2, -DP_PSI,
-// -DBC, -DRECT, -DROTATE, -DTREND, -DBW_BALL, -DNUDGE, -DV1S, -DINTERP, -DACC),
+// -DBC, -DRECT, -DROTATE, -DTREND, -DBW_BALL, -DNUDGE, -DV1S, -DSEGMENT, -DN_SEG=3|4,
+// -DONE_LE, -DINTERP, -DACC),
 // defaulting to -DP_PSI -DTORQUE -DINTERP; with -DNUDGE the observed minima
 // come from observed.min in the working directory;
 // -DMIXED (no counterpart in the reference) selects the mixed-precision form.
@@ -416,6 +417,10 @@ int main(int argc, char** argv) {
       else if (mac == "RECT") v.rect = v.bc = 1;            // asteroid.h:36-38
       else if (mac == "NUDGE") v.nudge = 1;
       else if (mac == "V1S") v.v1s = 1;
+      else if (mac == "SEGMENT") { if (v.n_seg < 2) v.n_seg = 4; }  // N_SEG defaults to 4 (asteroid.h:42-46)
+      else if (mac == "N_SEG=3") v.n_seg = 3;
+      else if (mac == "N_SEG=4") v.n_seg = 4;
+      else if (mac == "ONE_LE") v.one_le = 1;
       else if (mac == "INTERP") v.interp = 1;
       else if (mac == "ACC") acc = 1;
       else if (mac == "MIXED") acc = 2;
@@ -532,6 +537,13 @@ int main(int argc, char** argv) {
   if (rc == AST_E_NODEVICE) die("No HIP devices!");
   CHECK(rc);
   CHECK(ast_set_model_space(ctx, n_params, property, n_types, types, limits, &x2));
+  if (v.n_seg > 1) {  // the segment start times (asteroid.h:40-46) on the curve's clock
+    static const double T3[3] = {58051.044623, 58053.078872, 58055.234144};
+    static const double T4[4] = {58051.044623, 58053.078872, 58054.093273, 58055.234144};
+    double ts[4];
+    for (int k = 0; k < v.n_seg; ++k) ts[k] = (v.n_seg == 3 ? T3[k] : T4[k]) - D.mjd0;
+    CHECK(ast_set_segments(ctx, ts, v.n_seg));
+  }
   CHECK(ast_upload_curve(ctx, D.obs.data(), N_data, N_filters, &D.eph));
   if (v.nudge) load_minima(ctx, D);
   if (plot) {

@@ -69,23 +69,43 @@ def rk4_steps(mjd, time_step, split=None):
     return n
 
 
-def eval_ops(variant, mjd, split=None):
+def segments(variant, mjd, seg_t):
+    """SEGMENT: (one-segment variant, times) of each segment, by the split of
+    read_data.c:244-250; every segment is a whole evaluation of its own
+    (cuda.c:163-252 run per segment)."""
+    import copy
+    plain = copy.copy(variant)
+    plain.n_seg, plain.one_le = 0, 0
+    mjd = np.asarray(mjd, float)
+    start, k = [], 0
+    for i, t in enumerate(mjd):
+        if k < variant.n_seg and t >= seg_t[k]:
+            start.append(i)
+            k += 1
+    ends = start[1:] + [len(mjd)]
+    return [(plain, mjd[a:b]) for a, b in zip(start, ends)]
+
+
+def eval_ops(variant, mjd, split=None, seg_t=None):
     """Operation counts (6 columns) of one evaluation on a curve with times mjd
-    (TORQUE2: split = Tt, its split interval counted as two stretches)."""
+    (TORQUE2: split = Tt, its split interval counted as two stretches;
+    SEGMENT: seg_t = the segment start times)."""
+    if variant.n_seg > 1:
+        return sum(eval_ops(pv, m) for pv, m in segments(variant, mjd, seg_t))
     t = TABLE[table_key(variant)]
     split = split if variant.torque2 else None
     return (np.array(t["per_model"], float) + len(mjd) * np.array(t["per_point"], float)
             + rk4_steps(mjd, variant.time_step, split) * np.array(t["per_rk4"], float))
 
 
-def eval_flops(variant, mjd, split=None):
+def eval_flops(variant, mjd, split=None, seg_t=None):
     """FLOPs of one evaluation; for the mixed form, its FP64 plus FP32 FLOPs.
     TORQUE2 without `split` counts its split interval as one stretch (at most
     one RK4 step short)."""
     if variant.precision == MIXED:
-        f64, f32 = eval_flops_mixed(variant, mjd, split)
+        f64, f32 = eval_flops_mixed(variant, mjd, split, seg_t)
         return f64 + f32
-    return float(eval_ops(variant, mjd, split) @ _W)
+    return float(eval_ops(variant, mjd, split, seg_t) @ _W)
 
 
 # ------------------------------------------------------------ mixed form
@@ -114,8 +134,11 @@ _MX_SINCOS_SAVED = np.array([0, 0, 3, 0, 0, 0], float)  # per sincos_mx call
 _W4 = np.array([1, 1, 2, 1], float)
 
 
-def eval_flops_mixed(variant, mjd, split=None):
+def eval_flops_mixed(variant, mjd, split=None, seg_t=None):
     """(FP64 FLOPs, FP32 FLOPs) of one mixed-form evaluation."""
+    if variant.n_seg > 1:
+        parts = [eval_flops_mixed(pv, m) for pv, m in segments(variant, mjd, seg_t)]
+        return sum(a for a, _ in parts), sum(b for _, b in parts)
     key = table_key(variant)
     torque, kind = key
     t = TABLE[key]

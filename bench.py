@@ -77,12 +77,12 @@ def traffic_from_profile(form):
     return None
 
 
-def peak_of(prec, variant, mjd):
+def peak_of(prec, variant, mjd, seg_t=None):
     """Vector peak of the form's op mix: FP64 FLOPs at the FP64 rate, FP32 FLOPs at the FP32 rate."""
     from asteroid_amd import flops
     if prec != MIXED:
         return FP64_PEAK_TFLOPS, {}
-    f64, f32 = flops.eval_flops_mixed(variant, mjd)
+    f64, f32 = flops.eval_flops_mixed(variant, mjd, seg_t=seg_t)
     return ((f64 + f32) / (f64 / FP64_PEAK_TFLOPS + f32 / FP32_PEAK_TFLOPS),
             {"fp64_flop_per_model_point": round(f64 / len(mjd), 2),
              "fp32_flop_per_model_point": round(f32 / len(mjd), 2)})
@@ -578,6 +578,7 @@ VARIANT_RUNS = {
     "rect": (lambda p, ts: variants_mod().rect(p, time_step=ts), 1),
     "rect_torque": (lambda p, ts: variants_mod().rect(p, time_step=ts, torque=True), 1),
     "nudge_torque": (lambda p, ts: variants_mod().ls_torque_nudge(p, time_step=ts), 1),
+    "segment3": (lambda p, ts: variants_mod().ls_segments(p, time_step=ts), 1),
 }
 
 
@@ -611,13 +612,26 @@ def main_variants(args):
         v = mk(prec, 1e-2)
         if v.nudge and prec == MIXED:
             continue  # NUDGE runs in the FP64 and float-simplex forms only
-        curve, truth = synth.make_curve(2024, v, 400, 3.0, n_filters=nf, sigma_range=(0.05, 0.3))
+        seg_t = None
+        if v.n_seg > 1:
+            # SEGMENT: the LS curve split in thirds; each further segment's
+            # Euler angles and psi_0 a little away from the first one's
+            v1 = variants.make_variant(False, False, False, True, True, prec, 1e-2)
+            curve, t8 = synth.make_curve(2024, v1, 400, 3.0, n_filters=nf, sigma_range=(0.05, 0.3))
+            seg_t = curve.mjd[0] + np.array([0.0, 1.0, 2.0])
+            p8, off = variants.param_positions(v1), np.random.default_rng(7)
+            truth = np.array([t8[p8[k.split(":")[0]]] + (off.uniform(-0.2, 0.2) if ":" in k else 0.0)
+                              for k in variants.param_positions(v)])
+        else:
+            curve, truth = synth.make_curve(2024, v, 400, 3.0, n_filters=nf, sigma_range=(0.05, 0.3))
         # NUDGE is Stage Three (README.md:87-101): a reoptimisation from a model
         # (here the truth perturbed by 2%), not a random search
         stage3 = bool(v.nudge)
         space = variants.default_model_space(v, ppsi_hours=(2.0, 4800.0), reopt=1 if stage3 else 0)
         with api.Context(v) as ctx:
             ctx.set_model_space(space)
+            if seg_t is not None:
+                ctx.set_segments(seg_t)
             ctx.upload_curve(curve)
             if stage3:
                 ctx.set_minima(*minima_of(v, curve, truth))
@@ -638,8 +652,8 @@ def main_variants(args):
             ctx.chi2_batch(near)
             ev_ms, _ = ctx.last_timing()
         split = truth[variants.param_positions(v)["Tt"]] if v.torque2 else None
-        fl = flops.eval_flops(v, curve.mjd, split)
-        peak, mix = peak_of(prec, v, curve.mjd)
+        fl = flops.eval_flops(v, curve.mjd, split, seg_t)
+        peak, mix = peak_of(prec, v, curve.mjd, seg_t)
         achieved = fl * e / (ms / 1e3) / 1e12
         rows[name] = {"n_params": space.n_params, "stage": "three (reopt)" if stage3 else "one", "value": pts / wall, "fits_per_s": args.fits / wall,
                       "evals_per_fit": round(e / args.fits, 1), "kernel_ms": round(ms, 2), "launches": nl,

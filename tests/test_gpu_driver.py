@@ -228,3 +228,28 @@ def test_driver_nudge_reads_observed_min(tmp_path):
     (tmp_path / "observed.min").unlink()
     r = subprocess.run(cmd, cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=60)
     assert r.returncode != 0 and "observed.min" in r.stdout
+
+
+def test_driver_segment_matches_oracle(tmp_path):
+    # -DSEGMENT -DN_SEG=3 -DONE_LE: the curve split at the reference's T_START
+    # times (asteroid.h:40-46) less the first observation's MJD; the scene's
+    # first observations precede the first start and are left out of the sums
+    _scene(tmp_path, n_obs=80, filters="RG")
+    curve = _ingested(tmp_path)
+    mjd0 = float(open(tmp_path / "dump.txt").readline().split()[2])
+    t_start = np.array([58051.044623, 58053.078872, 58055.234144]) - mjd0
+    assert curve.mjd[0] < t_start[0] < curve.mjd[-1]
+    cmd = [EXE, "-DSEGMENT", "-DN_SEG=3", "-DONE_LE", "-DP_PSI", "-DINTERP", "-DACC", "-N", "1", "-i", "lc.txt",
+           "-o", "out.txt", "-seed", "9", "-blocks", "2", "-bsize", "64", "-steps", "30", "-Ppsi", "2", "4800"]
+    r = subprocess.run(cmd, cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout
+    v = variants.ls_segments(abi.PREC_FP64)
+    space = variants.default_model_space(v, ppsi_hours=(2.0, 4800.0))
+    want = oracle.Problem(space, curve, t_start=t_start).run(2, 64, nstages=1, n_steps=30, seed=9, n_threads=4)
+    best = [n for n in map(_numbers, r.stdout.splitlines()) if n is not None and len(n) == 1 + space.n_params]
+    assert len(best) == 1, r.stdout
+    g = int(np.argmin(want["f"]))
+    assert abs(best[0][0] - want["f"][g]) <= 1e-6 * want["f"][g]
+    np.testing.assert_allclose(best[0][1:], want["params"][g], rtol=0, atol=1e-11)
+    lines = [l.split() for l in open(tmp_path / "out.txt").read().splitlines() if l.strip()]
+    np.testing.assert_allclose([float(l[0]) for l in lines], want["f"], rtol=1e-6)
