@@ -74,3 +74,25 @@ def test_sincos_mx_saves_three_fma():
     n_full = body("sincos").count("p_fma") + body("k_sin").count("p_fma") + body("k_cos").count("p_fma")
     n_mx = body("sincos_mx").count("p_fma")
     assert n_full - n_mx == flops._MX_SINCOS_SAVED[2] == 3
+
+
+def test_segment_counts_are_the_sum_over_segments():
+    """SEGMENT: the counting oracle on each segment's observations (with that
+    segment's parameters) summed equals the table's count for the split
+    curve; observations before the first start and the RK4 steps into each
+    segment's first observation are not counted."""
+    v1 = variants.make_variant(False, False, False, True, True)
+    curve, truth = synth.make_curve(5, v1, 300, 3.0)
+    vs = variants.ls_segments()
+    t_start = curve.mjd[0] + np.array([0.2, 1.1, 2.05])
+    parts = flops.segments(vs, curve.mjd, t_start)
+    assert [len(m) for _, m in parts] == [int(np.sum((curve.mjd >= a) & (curve.mjd < b)))
+                                          for a, b in zip(t_start, list(t_start[1:]) + [np.inf])]
+    starts = np.searchsorted(curve.mjd, t_start)
+    total = np.zeros(6)
+    for k, (a, b) in enumerate(zip(starts, list(starts[1:]) + [curve.n_data])):
+        sub = synth.Curve(curve.mjd[a:b], curve.V[a:b], curve.w[a:b], curve.filter[a:b], 1, curve.E[a:b],
+                          curve.S[a:b], curve.eph)
+        total += _count(v1, sub, truth)
+    d = np.abs(total - flops.eval_ops(vs, curve.mjd, seg_t=t_start))
+    assert np.all(d <= 3 * np.array([4, 0, 0, 1, 0, 0])), (total, flops.eval_ops(vs, curve.mjd, seg_t=t_start))
