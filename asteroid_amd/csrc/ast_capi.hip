@@ -228,6 +228,7 @@ struct CurveH {  // a validated light curve as the kernels read it
   int n_all = 0;          // observations given (SEGMENT leaves those before the first segment out of the table)
   int64_t rk4_steps = 0;  // RK4 steps of one evaluation
   double t_first = 0, t_last = 0;  // times of the first and last observation
+  PointRec first{};                // the first observation (the plotting grid starts there)
 };
 
 // Phase angle of unit Sun and Earth directions, acos(S.E) in the canonical
@@ -236,18 +237,27 @@ double phase_angle(const double E[3], const double S[3]) {
   return astm::acos(std::fma(S[2], E[2], std::fma(S[1], E[1], S[0] * E[0])));
 }
 
-// SEGMENT's split (read_data.c:244-250): observation i opens segment s when
-// its time is at or after seg_t[s].  The table keeps the observations from
-// the first segment on; the first record of segment k > 0 carries the step
-// count -k, where the evaluator starts that segment's rotation, and the
-// first record none.  sum_w covers the table; the degrees of freedom keep
-// every observation (n_all, cuda.c:907).
+// SEGMENT's split (read_data.c:244-250, and 357-363 for the plotting grid):
+// point i opens segment s when its time is at or after seg_t[s].  Returns
+// the number of segments opened.
+int segment_starts(int n_seg, int n, const double* t, const double* seg_t, int* start) {
+  int k = 0;
+  for (int i = 0; i < n; ++i)
+    if (k < n_seg && t[i] >= seg_t[k]) start[k++] = i;
+  return k;
+}
+
+// The observation table keeps the observations from the first segment on;
+// the first record of segment k > 0 carries the step count -1 - k, where the
+// evaluator starts that segment's rotation, and the first record none.
+// sum_w covers the table; the degrees of freedom keep every observation
+// (n_all, cuda.c:907).
 int split_segments(int n_seg, const ast_obs* obs, const double* seg_t, CurveH* C) {
   if (!seg_t) return AST_E_STATE;
-  int start[AST_MAX_SEGMENTS], k = 0;
-  for (int i = 0; i < C->n_data; ++i)
-    if (k < n_seg && obs[i].mjd >= seg_t[k]) start[k++] = i;
-  if (k < n_seg) return AST_E_DATA;
+  int start[AST_MAX_SEGMENTS];
+  std::vector<double> t(C->n_data);
+  for (int i = 0; i < C->n_data; ++i) t[i] = obs[i].mjd;
+  if (segment_starts(n_seg, C->n_data, t.data(), seg_t, start) < n_seg) return AST_E_DATA;
   const int i0 = start[0], n = C->n_data - i0;
   std::vector<PointRec> pts(C->pts.begin() + i0, C->pts.end());  // the tail record comes along
   for (int m = 0; m < C->n_filters; ++m) C->sum_w[m] = 0.0;
@@ -256,7 +266,7 @@ int split_segments(int n_seg, const ast_obs* obs, const double* seg_t, CurveH* C
     PointRec& p = pts[i];
     for (int s = 0; s < n_seg; ++s)
       if (start[s] == i + i0) {
-        p.nsteps = -s;
+        p.nsteps = s == 0 ? 0 : -1 - s;
         p.h = p.hh = p.h6 = 0.0;
       }
     if (p.nsteps > 0) C->rk4_steps += p.nsteps;
@@ -266,7 +276,6 @@ int split_segments(int n_seg, const ast_obs* obs, const double* seg_t, CurveH* C
     if (!(C->sum_w[m] > 0)) return AST_E_DATA;  // a filter with no observation in the segments
   C->pts = std::move(pts);
   C->n_data = n;
-  C->t_first = obs[i0].mjd;
   return AST_OK;
 }
 
@@ -312,6 +321,7 @@ int make_curve(const ast_variant& v, const ast_obs* obs, int n_data, int n_filte
   C.n_data = n_data;
   C.n_all = n_data;
   C.n_filters = n_filters;
+  C.first = C.pts[0];
   C.t_first = obs[0].mjd;
   C.t_last = obs[n_data - 1].mjd;
   if (v.n_seg > 1) RETIF(split_segments(v.n_seg, obs, seg_t, &C));
@@ -326,7 +336,7 @@ int make_curve(const ast_variant& v, const ast_obs* obs, int n_data, int n_filte
 // normalised).  RK4 intervals between grid points as between observations
 // (cuda.c:321-323).  V, w and the filter are unused on a grid.
 int make_grid(const ast_variant& v, const CurveH& C, const ast_ephem3& eph, int n_plot, std::vector<PointRec>* out,
-              std::vector<double>* t_out) {
+              std::vector<double>* t_out, const double* seg_t = nullptr, int* g0 = nullptr) {
   if (n_plot < 2 || C.n_data < 1) return AST_E_ARG;
   std::vector<PointRec> g((size_t)n_plot + 1);
   g[n_plot] = C.pts[C.n_data];  // the minima record
@@ -336,7 +346,7 @@ int make_grid(const ast_variant& v, const CurveH& C, const ast_ephem3& eph, int 
     PointRec& p = g[i];
     std::memset(&p, 0, sizeof p);
     if (i == 0 || i == n_plot - 1) {
-      const PointRec& d = C.pts[i == 0 ? 0 : C.n_data - 1];
+      const PointRec& d = i == 0 ? C.first : C.pts[C.n_data - 1];
       t[i] = i == 0 ? C.t_first : C.t_last;
       for (int k = 0; k < 3; ++k) { p.E[k] = d.E[k]; p.S[k] = d.S[k]; }
     } else {
@@ -353,6 +363,19 @@ int make_grid(const ast_variant& v, const CurveH& C, const ast_ephem3& eph, int 
       p.hh = 0.5 * hs;
       p.h6 = 1 / 6.0 * hs;
     }
+  }
+  if (g0) *g0 = 0;
+  if (v.n_seg > 1) {  // SEGMENT: each segment's first grid point restarts the rotation
+    int start[AST_MAX_SEGMENTS];
+    if (!seg_t || !g0) return AST_E_STATE;
+    if (segment_starts(v.n_seg, n_plot, t.data(), seg_t, start) < v.n_seg) return AST_E_DATA;
+    for (int s = 0; s < v.n_seg; ++s)
+      if (start[s] > 0) {
+        PointRec& p = g[start[s]];
+        p.nsteps = -1 - s;
+        p.h = p.hh = p.h6 = 0.0;
+      }
+    *g0 = start[0];  // grid points before it are not computed (their magnitudes are NaN)
   }
   *out = std::move(g);
   if (t_out) *t_out = std::move(t);
@@ -1923,11 +1946,11 @@ int ast_model_curve(ast_ctx* c, const double* params, int n, const ast_ephem3* e
   if (!c || !params || n <= 0 || !eph || n_plot < 2 || !vmod || !chi2 || !dV) return AST_E_ARG;
   if (c->serve) return AST_E_BUSY;
   if (!c->space_set || !c->curve_set || (c->v.nudge && !c->minima_set)) return AST_E_STATE;
-  if (c->v.n_seg > 1) return AST_E_UNSUPPORTED;  // the plot's segment split is not built
   if (c->curve.n_all - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
   std::vector<PointRec> grid;
   std::vector<double> tg;
-  RETIF(make_grid(c->v, c->curve, *eph, n_plot, &grid, &tg));
+  int g0 = 0;
+  RETIF(make_grid(c->v, c->curve, *eph, n_plot, &grid, &tg, c->seg_t, &g0));
   RETIF(set_device(c));
   const int np = c->space.n_params, nf = c->curve.n_filters;
   PointRec* gd = nullptr;
@@ -1966,6 +1989,9 @@ int ast_model_curve(ast_ctx* c, const double* params, int n, const ast_ephem3* e
   if (he == hipSuccess) he = hipMemcpy(chi2, cd, sizeof(double) * n, hipMemcpyDeviceToHost);
   if (he == hipSuccess) he = hipMemcpy(dV, dd, sizeof(double) * n * nf, hipMemcpyDeviceToHost);
   if (he == hipSuccess) he = hipMemcpy(vmod, vd, sizeof(double) * n * (size_t)n_plot, hipMemcpyDeviceToHost);
+  if (he == hipSuccess)
+    for (int k = 0; k < n; ++k)
+      for (int i = 0; i < g0; ++i) vmod[(size_t)k * n_plot + i] = std::nan("");
   int result = AST_OK;
   if (he != hipSuccess) {
     (void)hipDeviceSynchronize();

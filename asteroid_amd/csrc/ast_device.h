@@ -794,16 +794,17 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
     nx_hh = pts[i + 1].hh;
     nx_h6 = pts[i + 1].h6;
     if constexpr (SG) {
-      // The first observation of segment k > 0 (its record's step count is
-      // -k): the rotation starts again there from the segment's own Euler
-      // angles (cuda.c:163-252 inside the segment loop); L, Es and the
-      // shape are shared, so mu and the brightness constants stay.
+      // The first point of segment k (its record's step count is -1 - k;
+      // a table that opens with segment 0 has no marker there): the
+      // rotation starts again from the segment's own Euler angles
+      // (cuda.c:163-252 inside the segment loop); L, Es and the shape are
+      // shared, so mu and the brightness constants stay.
       if (nsteps < 0) {
-        const bool k1 = nsteps == -1;
-        const double tMk = k1 ? params[P::seg(1, 0)] : params[P::seg(2, 0)];
-        const double pMk = k1 ? params[P::seg(1, 1)] : params[P::seg(2, 1)];
-        const double p0k = k1 ? params[P::seg(1, 2)] : params[P::seg(2, 2)];
-        const double s0k = k1 ? params[P::seg(1, 3)] : params[P::seg(2, 3)];
+        const bool k0 = nsteps == -1, k1 = nsteps == -2;
+        const double tMk = k0 ? params[P::seg(0, 0)] : k1 ? params[P::seg(1, 0)] : params[P::seg(2, 0)];
+        const double pMk = k0 ? params[P::seg(0, 1)] : k1 ? params[P::seg(1, 1)] : params[P::seg(2, 1)];
+        const double p0k = k0 ? params[P::seg(0, 2)] : k1 ? params[P::seg(1, 2)] : params[P::seg(2, 2)];
+        const double s0k = k0 ? params[P::seg(0, 3)] : k1 ? params[P::seg(1, 3)] : params[P::seg(2, 3)];
         astm::sincos(tMk, &sTM, &cTM);
         astm::sincos(pMk, &sPM, &cPM);
         Fd.M_x = sTM * cPM;

@@ -85,6 +85,7 @@ struct Problem {
   // parameter of each type in each segment (Types[N_TYPES][N_SEG],
   // asteroid.c:152-168); n_seg = 0 without SEGMENT.
   int n_seg;
+  double seg_t[AST_MAX_SEGMENTS];
   int seg_start[AST_MAX_SEGMENTS];
   int seg_types[AST_MAX_TYPES][AST_MAX_SEGMENTS];
   // TORQUE2 implies TORQUE and RECT implies BC (asteroid.h:32-38)
@@ -1383,8 +1384,18 @@ inline CF model_curve(const Problem& P, int form, const double* params, const st
   double sy2[AST_MAX_FILTERS], sy[AST_MAX_FILTERS], sw[AST_MAX_FILTERS];
   Nudge nu;  // the grid pass finds model minima too, and stops the curve on failure
   ModelOut out{vmod, nullptr, P.v.nudge ? &nu : nullptr};
-  if (form == 0) model_literal(G, params, sy2, sy, sw, &out);
-  else model_canon<double>(G, params, sy2, sy, sw, &out);
+  if (P.n_seg > 1) {  // the grid's own segment split (read_data.c:357-363, cuda.c:1943-1947)
+    int iseg = 0;
+    for (int i = 0; i < G.n_data; ++i)
+      if (iseg < P.n_seg && grid[i].mjd >= P.seg_t[iseg]) G.seg_start[iseg++] = i;
+    if (iseg < P.n_seg) return CF(NAN);
+    for (int i = 0; i < G.seg_start[0]; ++i) vmod[i] = std::nan("");  // never computed
+    model_sums(G, form, params, sy2, sy, sw, &out);
+  } else if (form == 0) {
+    model_literal(G, params, sy2, sy, sw, &out);
+  } else {
+    model_canon<double>(G, params, sy2, sy, sw, &out);
+  }
   for (int i = 0; i < G.n_data; ++i) vmod[i] = vmod[i] + dV[0];
   return chi2;
 }
@@ -1545,6 +1556,7 @@ static int load_problem(const orc_problem* in, orc::Problem* P) {
   if (in->v.n_seg > 1) {
     const int ns = in->v.n_seg;
     if (ns > AST_MAX_SEGMENTS || in->n_params != P->T.n_params) return AST_E_ARG;
+    for (int k = 0; k < ns; ++k) P->seg_t[k] = in->t_start[k];
     int iseg = 0;
     for (int i = 0; i < in->n_data; ++i)  // read_data.c:244-250
       if (iseg < ns && in->obs[i].mjd >= in->t_start[iseg]) P->seg_start[iseg++] = i;
@@ -1604,7 +1616,6 @@ int orc_model_curve(const orc_problem* pb, int form, const double* params, int n
   int rc = load_problem(pb, &P);
   if (rc) return rc;
   if (n_plot < 2) return AST_E_ARG;
-  if (P.n_seg > 1) return AST_E_UNSUPPORTED;  // the plot's segment split (read_data.c:357-363) is not restated
   const std::vector<ast_obs> grid = orc::plot_grid(P, form, n_plot);
   if (t) for (int i = 0; i < n_plot; ++i) t[i] = grid[i].mjd;
   auto work = [&](int k) {

@@ -2,7 +2,8 @@
 chi^2 and delta_V of the evaluator bit for bit (FP64 and float-simplex
 forms; the mixed form within its 1e-6 relative tolerance), whole fits lane
 by lane, observations before the first segment and an observation past two
-start times; the call rules of ast_set_segments and the refusals."""
+start times, the -plot curve (grid points before the first start NaN);
+the call rules of ast_set_segments and the refusals."""
 import numpy as np
 import pytest
 
@@ -113,9 +114,24 @@ def test_segment_call_rules():
         ctx.set_segments(t_start)
         ctx.upload_curve(curve)
         with pytest.raises(api.AsteroidError):
-            ctx.model_curve(truth[None, :])  # not built for SEGMENT
-        with pytest.raises(api.AsteroidError):
             ctx.rmsd(1, 64, 4, 0.003)
     with api.Context(variants.ls_torque()) as ctx:
         with pytest.raises(api.AsteroidError):
             ctx.set_segments(t_start)  # not a SEGMENT context
+
+
+@pytest.mark.parametrize("skip", [0, 9], ids=["all", "first9_out"])
+@pytest.mark.parametrize("prec", ["fp64", "f32simplex"])
+def test_segment_model_curve_bitwise(prec, skip):
+    v, curve, truth, space, t_start = _case(PRECS[prec], skip=skip)
+    pb = oracle.Problem(space, curve, t_start=t_start)
+    params = _models(pb, space, truth, n=16)[:8]
+    t_w, vm_w, c_w, dv_w = pb.model_curve(params, n_plot=3000, form=oracle.CANON, n_threads=4)
+    with _ctx(v, space, curve, t_start) as ctx:
+        t_g, vm_g, c_g, dv_g = ctx.model_curve(params, n_plot=3000)
+    g0 = int(np.searchsorted(t_w, t_start[0]))
+    print(prec, skip, "grid points before the first start", g0, flush=True)
+    assert (g0 > 0) == (skip > 0)
+    assert np.array_equal(t_g, t_w) and np.array_equal(c_g, c_w) and np.array_equal(dv_g, dv_w)
+    assert np.all(np.isnan(vm_g[:, :g0])) and np.all(np.isnan(vm_w[:, :g0]))
+    assert np.array_equal(vm_g[:, g0:], vm_w[:, g0:])

@@ -135,3 +135,26 @@ def test_fit_runs_on_segments():
     assert np.all(np.isfinite(r["f"])) and np.all(r["f"] < 1e29)
     c, _ = pb.chi2(r["params"][0])
     assert c[0] == r["f"][0]
+
+
+def test_plot_curve_restarts_at_each_segment():
+    # the -plot grid splits by the same rule (read_data.c:357-363): with every
+    # segment given the first segment's parameters, the curve equals the plain
+    # LS curve up to the second start and differs after it; grid points before
+    # the first start are never computed
+    pb, pb8, curve, truth8, params, _ = _setup(seed=11)
+    t_start = np.array([curve.mjd[4], curve.mjd[0] + 1.0, curve.mjd[0] + 2.0])
+    pb = oracle.Problem(pb.space, curve, t_start=t_start)
+    same = params.copy()
+    same[8:12] = same[[0, 1, 2, 7]]
+    same[12:16] = same[[0, 1, 2, 7]]
+    t, vm, c, dv = pb.model_curve(same, n_plot=2000)
+    t8, vm8, c8, dv8 = pb8.model_curve(same[:8], n_plot=2000)
+    assert np.array_equal(t, t8)  # the grid spans every observation
+    g = np.searchsorted(t, t_start)
+    assert np.all(np.isnan(vm[0][:g[0]]))
+    # segment 0 starts at its first grid point, and every segment restarts
+    assert np.all(np.isfinite(vm[0][g[0]:]))
+    assert not np.allclose(vm[0][g[1]:] - dv[0][0], vm8[0][g[1]:] - dv8[0][0])
+    c_obs, dv_obs = pb.chi2(same)
+    assert c[0] == c_obs[0] and np.array_equal(dv[0], dv_obs[0])
