@@ -224,7 +224,11 @@ int ast_set_minima(ast_ctx* ctx, const float* t_obs, const float* V_obs, int n_o
  * reference, observation i opens the next segment s when its time is at or
  * after t_start[s]; observations before the first segment are left out of
  * the sums but still count in the degrees of freedom.  A curve on which a
- * segment gets no opening observation is refused (AST_E_DATA). */
+ * segment gets no opening observation is refused (AST_E_DATA).
+ * ast_model_curve splits its grid the same way (read_data.c:357-363) and
+ * returns NaN for grid points before the first start, which the reference
+ * never computes; ast_rmsd is refused (AST_E_UNSUPPORTED): the reference's
+ * chi2_gpu_rms never loads the segment starts. */
 int ast_set_segments(ast_ctx* ctx, const double* t_start, int n_seg);
 
 /* Seed the per-simplex random streams and reset each group's best chi^2 to
