@@ -70,7 +70,8 @@ class Job(C.Structure):
                 ("eph", C.POINTER(Ephem3)), ("n_fits", C.c_int), ("nstages", C.c_int), ("n_steps", C.c_int),
                 ("eval_budget", C.c_int64), ("size_min", C.c_double), ("seed", C.c_uint64),
                 ("params0", C.POINTER(C.c_double)), ("tag", C.c_uint64), ("n_minima", C.c_int),
-                ("t_minima", C.POINTER(C.c_float)), ("V_minima", C.POINTER(C.c_float))]
+                ("t_minima", C.POINTER(C.c_float)), ("V_minima", C.POINTER(C.c_float)),
+                ("t_segments", C.POINTER(C.c_double))]
 
 
 class Result(C.Structure):

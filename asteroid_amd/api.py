@@ -191,16 +191,18 @@ class Context:
     # ------------------------------------------------------------- batches
     @staticmethod
     def _job(space, curve, n_fits, nstages=1, n_steps=5000, eval_budget=0, size_min=0.0, seed=1, params0=None,
-             tag=0, minima=None):
+             tag=0, minima=None, t_segments=None):
         """A ctypes ast_job plus the arrays it points into (kept alive by the caller).
-        minima = (t_obs, V_obs) for NUDGE variants."""
+        minima = (t_obs, V_obs) for NUDGE variants, t_segments = the start times
+        for SEGMENT variants."""
         keep = dict(prop=np.ascontiguousarray(space.property.ravel(), np.int32),
                     types=np.ascontiguousarray(space.types, np.int32),
                     lim=np.ascontiguousarray(space.limits, np.float64),
                     obs=curve.obs_array(), eph=curve.ephem3(),
                     p0=None if params0 is None else np.ascontiguousarray(params0, np.float64),
                     tmin=None if minima is None else np.ascontiguousarray(minima[0], np.float32),
-                    vmin=None if minima is None else np.ascontiguousarray(minima[1], np.float32))
+                    vmin=None if minima is None else np.ascontiguousarray(minima[1], np.float32),
+                    tseg=None if t_segments is None else np.ascontiguousarray(t_segments, np.float64))
         job = abi.Job(space.variant, space.n_params, keep["prop"].ctypes.data_as(C.POINTER(C.c_int)),
                       space.n_types, keep["types"].ctypes.data_as(C.POINTER(C.c_int)), _dp(keep["lim"]),
                       space.x2(), C.cast(keep["obs"], C.POINTER(abi.Obs)), curve.n_data, curve.n_filters,
@@ -210,6 +212,8 @@ class Context:
             job.n_minima = len(keep["tmin"])
             job.t_minima = keep["tmin"].ctypes.data_as(C.POINTER(C.c_float))
             job.V_minima = keep["vmin"].ctypes.data_as(C.POINTER(C.c_float))
+        if t_segments is not None:
+            job.t_segments = _dp(keep["tseg"])
         return job, keep
 
     @staticmethod
@@ -253,10 +257,10 @@ class Context:
         self._keep = {}
 
     def submit(self, space, curve, n_fits, nstages=1, n_steps=5000, eval_budget=0, size_min=0.0, seed=1,
-               params0=None, tag=0, minima=None):
+               params0=None, tag=0, minima=None, t_segments=None):
         """Submit a fit job (its own variant, model space and curve); returns its id."""
         job, keep = self._job(space, curve, n_fits, nstages, n_steps, eval_budget, size_min, seed, params0, tag,
-                              minima)
+                              minima, t_segments)
         jid = C.c_uint64()
         _chk(self.lib.ast_submit(self.h, C.byref(job), C.byref(jid)), "ast_submit")
         return jid.value

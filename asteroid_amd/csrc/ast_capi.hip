@@ -1220,13 +1220,17 @@ int make_job(ast_ctx* c, const ast_job* job, int G, std::unique_ptr<DevJob>* out
   if (job->variant.nudge &&
       (job->n_minima < 1 || job->n_minima > AST_MAX_MINIMA || !job->t_minima || !job->V_minima))
     return AST_E_ARG;
-  if (job->variant.n_seg > 1) return AST_E_UNSUPPORTED;  // jobs carry no segment start times
+  if (job->variant.n_seg > 1 && !job->t_segments) return AST_E_ARG;
+  if (job->variant.n_seg > 1)
+    for (int k = 0; k < job->variant.n_seg; ++k)
+      if (!std::isfinite(job->t_segments[k])) return AST_E_ARG;
   std::unique_ptr<DevJob> j(new DevJob());
   j->uid = c->next_uid++;
   RETIF(make_space(job->variant, job->n_params, job->property, job->n_types, job->types, job->limits, &job->x2,
                    &j->space));
   CurveH C;
-  RETIF(make_curve(job->variant, job->obs, job->n_data, job->n_filters, job->eph, &C));
+  RETIF(make_curve(job->variant, job->obs, job->n_data, job->n_filters, job->eph, &C,
+                   job->variant.n_seg > 1 ? job->t_segments : nullptr));
   if (C.n_data - j->space.n_params - C.n_filters <= 0) return AST_E_DATA;
   if (job->variant.nudge) RETIF(set_tail_minima(job->variant, job->t_minima, job->V_minima, job->n_minima, &C));
   j->rp.n_groups = job->n_fits / G;

@@ -377,6 +377,7 @@ typedef struct {
   int n_minima;             /* NUDGE: observed minima, as ast_set_minima (0 otherwise) */
   const float* t_minima;
   const float* V_minima;
+  const double* t_segments; /* SEGMENT: the variant's n_seg start times, as ast_set_segments (else NULL) */
 } ast_job;
 
 typedef struct {

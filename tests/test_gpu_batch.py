@@ -127,3 +127,25 @@ def test_batch_nudge_job_matches_alone_and_oracle():
     assert r["status"] == 0 and r["best_f"] == want["f"][0]
     assert np.array_equal(r["best_params"], want["params"][0]) and np.array_equal(r["best_dV"], want["dV"][0])
     assert r["n_evals"] == want["lane_nev"].sum()
+
+
+def test_batch_segment_job_matches_alone_and_oracle():
+    """A SEGMENT job carries its start times in the batch (observations before
+    the first start left out); it equals the same job alone and the oracle's
+    run, beside jobs of other variants."""
+    v1 = variants.make_variant(False, False, False, True, True, abi.PREC_FP64)
+    curve, t8 = synth.make_curve(81, v1, 150, 2.0)
+    v = variants.ls_segments(abi.PREC_FP64)
+    space = variants.default_model_space(v)
+    t_seg = np.array([curve.mjd[4], curve.mjd[0] + 0.7, curve.mjd[0] + 1.3])
+    sj = dict(space=space, curve=curve, n_fits=256, n_steps=30, seed=13, tag=7, t_segments=t_seg)
+    jobs = _jobs()[:3] + [sj]
+    with api.Context(variants.ls_torque()) as ctx:
+        batch = ctx.run_jobs(jobs, work_points_per_launch=3000)
+        alone = ctx.run_jobs([sj], work_points_per_launch=5000)[0]
+    _same(batch[3], alone)
+    want = oracle.Problem(space, curve, t_start=t_seg).run(1, 256, nstages=1, n_steps=30, seed=13, n_threads=8)
+    r = batch[3]
+    assert r["status"] == 0 and r["best_f"] == want["f"][0]
+    assert np.array_equal(r["best_params"], want["params"][0]) and np.array_equal(r["best_dV"], want["dV"][0])
+    assert r["n_evals"] == want["lane_nev"].sum()
