@@ -55,7 +55,7 @@ class X2(C.Structure):
 class RunCfg(C.Structure):
     _fields_ = [("n_groups", C.c_int), ("group_size", C.c_int), ("nstages", C.c_int),
                 ("n_steps", C.c_int), ("eval_budget", C.c_int64), ("size_min", C.c_double),
-                ("ticks_per_launch", C.c_int), ("use_graphs", C.c_int)]
+                ("ticks_per_launch", C.c_int), ("use_graphs", C.c_int), ("resident_groups", C.c_int)]
 
 
 class ServeCfg(C.Structure):
@@ -71,7 +71,7 @@ class Job(C.Structure):
                 ("eval_budget", C.c_int64), ("size_min", C.c_double), ("seed", C.c_uint64),
                 ("params0", C.POINTER(C.c_double)), ("tag", C.c_uint64), ("n_minima", C.c_int),
                 ("t_minima", C.POINTER(C.c_float)), ("V_minima", C.POINTER(C.c_float)),
-                ("t_segments", C.POINTER(C.c_double))]
+                ("t_segments", C.POINTER(C.c_double)), ("seg_first_obs", C.POINTER(C.c_int))]
 
 
 class Result(C.Structure):
@@ -177,10 +177,15 @@ def load():
         "ast_upload_curve": (I, [P, C.POINTER(Obs), I, I, C.POINTER(Ephem3)]),
         "ast_set_minima": (I, [P, C.POINTER(C.c_float), C.POINTER(C.c_float), I]),
         "ast_set_segments": (I, [P, C.POINTER(C.c_double), I]),
+        "ast_set_segment_starts": (I, [P, pI, C.POINTER(C.c_double), I]),
         "ast_rng_seed": (I, [P, C.c_uint64]),
         "ast_reset_best": (I, [P]),
         "ast_set_start": (I, [P, pD]),
         "ast_run": (I, [P, C.POINTER(RunCfg)]),
+        "ast_run_begin": (I, [P, C.POINTER(RunCfg)]),
+        "ast_run_continue": (I, [P, I, pI]),
+        "ast_run_status": (I, [P, pI64, pI64, pI]),
+        "ast_run_cancel": (I, [P]),
         "ast_fetch": (I, [P, pD, pD, pD]),
         "ast_fetch_lanes": (I, [P, pD, pI64, pI64]),
         "ast_chi2_batch": (I, [P, pD, I, pD, pD]),
@@ -216,5 +221,6 @@ EXPORTED = (
     "ast_last_timing", "ast_last_counts", "ast_default_model_space", "ast_strerror",
     "ast_last_hip_error", "ast_serve_start", "ast_submit", "ast_poll", "ast_cancel", "ast_serve_stats",
     "ast_serve_stop", "ast_run_jobs", "ast_autotune", "ast_set_waves", "ast_model_curve", "ast_periodogram",
-    "ast_rmsd", "ast_set_minima", "ast_set_segments",
+    "ast_rmsd", "ast_set_minima", "ast_set_segments", "ast_run_begin", "ast_run_continue", "ast_run_status",
+    "ast_set_segment_starts", "ast_run_cancel",
 )

@@ -89,6 +89,16 @@ class Context:
         _chk(self.lib.ast_set_segments(self.h, t.ctypes.data_as(C.POINTER(C.c_double)), len(t)),
              "ast_set_segments")
 
+    def set_segment_starts(self, first_obs, t_start):
+        """SEGMENT with the observations' split given as the first observation
+        index of each segment, and the plotting grid's as times on the curve's
+        clock (before upload_curve)."""
+        i = np.ascontiguousarray(first_obs, np.int32)
+        t = np.ascontiguousarray(t_start, np.float64)
+        _chk(self.lib.ast_set_segment_starts(self.h, i.ctypes.data_as(C.POINTER(C.c_int)),
+                                             t.ctypes.data_as(C.POINTER(C.c_double)), len(t)),
+             "ast_set_segment_starts")
+
     def rng_seed(self, seed):
         _chk(self.lib.ast_rng_seed(self.h, int(seed)), "ast_rng_seed")
 
@@ -100,12 +110,37 @@ class Context:
         _chk(self.lib.ast_set_start(self.h, _dp(p)), "ast_set_start")
 
     def run(self, n_groups, group_size=256, nstages=1, n_steps=5000, eval_budget=0, size_min=0.0,
-            ticks_per_launch=0, use_graphs=False):
+            ticks_per_launch=0, use_graphs=False, resident_groups=0):
         cfg = abi.RunCfg(n_groups, group_size, nstages, n_steps, eval_budget, size_min, ticks_per_launch,
-                         int(use_graphs))
+                         int(use_graphs), resident_groups)
         _chk(self.lib.ast_run(self.h, C.byref(cfg)), "ast_run")
         self._last_groups = n_groups
         self._last_fits = n_groups * group_size
+
+    def run_begin(self, n_groups, group_size=256, nstages=1, n_steps=5000, eval_budget=0, size_min=0.0,
+                  ticks_per_launch=0, use_graphs=False, resident_groups=0):
+        """Start a run (ast_run_begin); advance it with run_continue."""
+        cfg = abi.RunCfg(n_groups, group_size, nstages, n_steps, eval_budget, size_min, ticks_per_launch,
+                         int(use_graphs), resident_groups)
+        _chk(self.lib.ast_run_begin(self.h, C.byref(cfg)), "ast_run_begin")
+        self._last_groups = n_groups
+        self._last_fits = n_groups * group_size
+
+    def run_continue(self, max_cycles=0):
+        """Up to max_cycles launch cycles (0: to the end); True once every fit has ended."""
+        fin = C.c_int(0)
+        _chk(self.lib.ast_run_continue(self.h, max_cycles, C.byref(fin)), "ast_run_continue")
+        return bool(fin.value)
+
+    def run_cancel(self):
+        """End an unfinished run early (ast_run_cancel)."""
+        _chk(self.lib.ast_run_cancel(self.h), "ast_run_cancel")
+
+    def run_status(self):
+        """(fits started, fits ended, finished) of the current run."""
+        st, dn, fin = C.c_int64(0), C.c_int64(0), C.c_int(0)
+        _chk(self.lib.ast_run_status(self.h, C.byref(st), C.byref(dn), C.byref(fin)), "ast_run_status")
+        return st.value, dn.value, bool(fin.value)
 
     def fetch(self):
         G, N, NF = self._last_groups, self.space.n_params, self.curve.n_filters
@@ -191,7 +226,7 @@ class Context:
     # ------------------------------------------------------------- batches
     @staticmethod
     def _job(space, curve, n_fits, nstages=1, n_steps=5000, eval_budget=0, size_min=0.0, seed=1, params0=None,
-             tag=0, minima=None, t_segments=None):
+             tag=0, minima=None, t_segments=None, seg_first_obs=None):
         """A ctypes ast_job plus the arrays it points into (kept alive by the caller).
         minima = (t_obs, V_obs) for NUDGE variants, t_segments = the start times
         for SEGMENT variants."""
@@ -202,7 +237,8 @@ class Context:
                     p0=None if params0 is None else np.ascontiguousarray(params0, np.float64),
                     tmin=None if minima is None else np.ascontiguousarray(minima[0], np.float32),
                     vmin=None if minima is None else np.ascontiguousarray(minima[1], np.float32),
-                    tseg=None if t_segments is None else np.ascontiguousarray(t_segments, np.float64))
+                    tseg=None if t_segments is None else np.ascontiguousarray(t_segments, np.float64),
+                    sfirst=None if seg_first_obs is None else np.ascontiguousarray(seg_first_obs, np.int32))
         job = abi.Job(space.variant, space.n_params, keep["prop"].ctypes.data_as(C.POINTER(C.c_int)),
                       space.n_types, keep["types"].ctypes.data_as(C.POINTER(C.c_int)), _dp(keep["lim"]),
                       space.x2(), C.cast(keep["obs"], C.POINTER(abi.Obs)), curve.n_data, curve.n_filters,
@@ -214,6 +250,8 @@ class Context:
             job.V_minima = keep["vmin"].ctypes.data_as(C.POINTER(C.c_float))
         if t_segments is not None:
             job.t_segments = _dp(keep["tseg"])
+        if seg_first_obs is not None:
+            job.seg_first_obs = keep["sfirst"].ctypes.data_as(C.POINTER(C.c_int))
         return job, keep
 
     @staticmethod
@@ -257,10 +295,10 @@ class Context:
         self._keep = {}
 
     def submit(self, space, curve, n_fits, nstages=1, n_steps=5000, eval_budget=0, size_min=0.0, seed=1,
-               params0=None, tag=0, minima=None, t_segments=None):
+               params0=None, tag=0, minima=None, t_segments=None, seg_first_obs=None):
         """Submit a fit job (its own variant, model space and curve); returns its id."""
         job, keep = self._job(space, curve, n_fits, nstages, n_steps, eval_budget, size_min, seed, params0, tag,
-                              minima, t_segments)
+                              minima, t_segments, seg_first_obs)
         jid = C.c_uint64()
         _chk(self.lib.ast_submit(self.h, C.byref(job), C.byref(jid)), "ast_submit")
         return jid.value

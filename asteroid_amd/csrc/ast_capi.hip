@@ -17,6 +17,7 @@
 //    as they finish.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -252,12 +253,22 @@ int segment_starts(int n_seg, int n, const double* t, const double* seg_t, int* 
 // evaluator starts that segment's rotation, and the first record none.
 // sum_w covers the table; the degrees of freedom keep every observation
 // (n_all, cuda.c:907).
-int split_segments(int n_seg, const ast_obs* obs, const double* seg_t, CurveH* C) {
-  if (!seg_t) return AST_E_STATE;
+// With seg_first (the caller's split, e.g. on the observer's clock as
+// read_data.c:246 does, before the light-time correction) the segments open
+// at those observation indices, which must increase and lie in the table.
+int split_segments(int n_seg, const ast_obs* obs, const double* seg_t, const int* seg_first, CurveH* C) {
   int start[AST_MAX_SEGMENTS];
-  std::vector<double> t(C->n_data);
-  for (int i = 0; i < C->n_data; ++i) t[i] = obs[i].mjd;
-  if (segment_starts(n_seg, C->n_data, t.data(), seg_t, start) < n_seg) return AST_E_DATA;
+  if (seg_first) {
+    for (int k = 0; k < n_seg; ++k)
+      if (seg_first[k] < 0 || seg_first[k] >= C->n_data || (k > 0 && seg_first[k] <= seg_first[k - 1]))
+        return AST_E_DATA;
+    for (int k = 0; k < n_seg; ++k) start[k] = seg_first[k];
+  } else {
+    if (!seg_t) return AST_E_STATE;
+    std::vector<double> t(C->n_data);
+    for (int i = 0; i < C->n_data; ++i) t[i] = obs[i].mjd;
+    if (segment_starts(n_seg, C->n_data, t.data(), seg_t, start) < n_seg) return AST_E_DATA;
+  }
   const int i0 = start[0], n = C->n_data - i0;
   std::vector<PointRec> pts(C->pts.begin() + i0, C->pts.end());  // the tail record comes along
   for (int m = 0; m < C->n_filters; ++m) C->sum_w[m] = 0.0;
@@ -280,7 +291,7 @@ int split_segments(int n_seg, const ast_obs* obs, const double* seg_t, CurveH* C
 }
 
 int make_curve(const ast_variant& v, const ast_obs* obs, int n_data, int n_filters, const ast_ephem3* eph,
-               CurveH* out, const double* seg_t = nullptr) {
+               CurveH* out, const double* seg_t = nullptr, const int* seg_first = nullptr) {
   if (!obs || n_data <= 0 || n_filters <= 0) return AST_E_ARG;
   if (n_filters > AST_MAX_FILTERS) return AST_E_DATA;
   if (v.interp && !eph) return AST_E_ARG;
@@ -324,7 +335,7 @@ int make_curve(const ast_variant& v, const ast_obs* obs, int n_data, int n_filte
   C.first = C.pts[0];
   C.t_first = obs[0].mjd;
   C.t_last = obs[n_data - 1].mjd;
-  if (v.n_seg > 1) RETIF(split_segments(v.n_seg, obs, seg_t, &C));
+  if (v.n_seg > 1) RETIF(split_segments(v.n_seg, obs, seg_t, seg_first, &C));
   *out = std::move(C);
   return AST_OK;
 }
@@ -439,7 +450,7 @@ constexpr int MAX_CLASSES = N_VARIANTS * 3 * 3;  // variants x 3 filter capaciti
 constexpr int NSTREAMS = 4;      // classes share streams round-robin
 constexpr size_t SLOT_BYTES = 8 * ((AST_MAX_PARAMS + 1) * AST_MAX_PARAMS + (AST_MAX_PARAMS + 1) +
                                    4 * AST_MAX_PARAMS + AST_MAX_FILTERS + 2) +
-                              (AST_MAX_PARAMS + 1) + 4 * 5;
+                              (AST_MAX_PARAMS + 1) + 4 * 7;
 
 struct Pool {
   size_t nf = 0;        // fits
@@ -450,6 +461,7 @@ struct Pool {
   uint8_t* IND = nullptr;
   double* PRM = nullptr;
   int32_t *ctl = nullptr, *lstep = nullptr, *nev = nullptr, *nev_tot = nullptr, *steps_tot = nullptr;
+  int32_t *fitid = nullptr, *work = nullptr;
   int32_t *g_reopt = nullptr, *g_stage = nullptr, *g_done = nullptr;
   int32_t* job_done = nullptr;     // [n_slots]: a job's finished-group counter sits at its first slot
   std::vector<int32_t> job_done_h; // host copy, read back once per cycle
@@ -459,7 +471,8 @@ struct Pool {
 void pool_free(Pool& p) {
   dev_free(p.X); dev_free(p.F); dev_free(p.X0); dev_free(p.XR); dev_free(p.XP); dev_free(p.DV);
   dev_free(p.fr); dev_free(p.sf); dev_free(p.IND); dev_free(p.PRM); dev_free(p.ctl); dev_free(p.lstep);
-  dev_free(p.nev); dev_free(p.nev_tot); dev_free(p.steps_tot); dev_free(p.g_reopt); dev_free(p.g_stage);
+  dev_free(p.nev); dev_free(p.nev_tot); dev_free(p.steps_tot); dev_free(p.fitid); dev_free(p.work);
+  dev_free(p.g_reopt); dev_free(p.g_stage);
   dev_free(p.g_done); dev_free(p.job_done);
   p.nf = 0;
   p.n_slots = 0;
@@ -486,6 +499,8 @@ int pool_alloc(Pool& p, size_t nf, int G) {
   ALLOC(p.nev, int32_t, nf);
   ALLOC(p.nev_tot, int32_t, nf);
   ALLOC(p.steps_tot, int32_t, nf);
+  ALLOC(p.fitid, int32_t, nf);
+  ALLOC(p.work, int32_t, nf);
   ALLOC(p.g_reopt, int32_t, nf / G);
   ALLOC(p.g_stage, int32_t, nf / G);
   ALLOC(p.g_done, int32_t, nf / G);
@@ -506,7 +521,8 @@ SlotBufs<CF> pool_bufs(const Pool& p, int cls) {
   SlotBufs<CF> B;
   B.X = (CF*)p.X; B.F = (CF*)p.F; B.IND = p.IND; B.X0 = (CF*)p.X0; B.XR = (CF*)p.XR; B.XP = (CF*)p.XP;
   B.PRM = p.PRM; B.DV = (CF*)p.DV; B.ctl = p.ctl; B.lstep = p.lstep; B.nev = p.nev; B.nev_tot = p.nev_tot;
-  B.steps_tot = p.steps_tot; B.fr = (CF*)p.fr; B.sf = (CF*)p.sf; B.g_reopt = p.g_reopt; B.g_stage = p.g_stage;
+  B.steps_tot = p.steps_tot; B.fr = (CF*)p.fr; B.sf = (CF*)p.sf; B.fitid = p.fitid; B.work = p.work;
+  B.g_reopt = p.g_reopt; B.g_stage = p.g_stage;
   B.g_done = p.g_done; B.nf = p.nf;
   return B;
 }
@@ -530,6 +546,12 @@ struct DevJob {
   double cost = 1;             // one evaluation, in observation-equivalents (eval_cost)
   int ticks = 1;               // evaluation rounds per launch
   int cf = 8;                  // simplex float bytes
+  // Workgroups the job runs on (its slots): every group (n_workers =
+  // rp.n_groups), or fewer for a single-stage job, whose lanes then take the
+  // job's fits from a device queue as their fits end (refill).
+  int n_workers = 0;
+  bool refill = false;
+  bool reduced = false;        // refill: the groups' results have been formed
   // device memory (owned unless borrowed)
   PointRec* pts = nullptr;
   double* sum_w = nullptr;
@@ -540,7 +562,13 @@ struct DevJob {
   double* best_params = nullptr;
   double* best_dV = nullptr;
   int32_t* done = nullptr;     // finished-group counter
-  void* descs = nullptr;       // GroupDesc<CF>[n_groups] (device, for the init kernel)
+  void* fit_f = nullptr;       // per fit: stage-final best chi^2 (CF), evaluations, steps
+  int32_t* fit_nev = nullptr;
+  int32_t* fit_steps = nullptr;
+  double* fit_params = nullptr;  // refill: per fit, its best point and last delta_V
+  double* fit_dV = nullptr;
+  int32_t* queue = nullptr;    // refill: next fit to start, fits finished
+  void* descs = nullptr;       // GroupDesc<CF>[n_workers] (device, for the init kernel)
   std::vector<unsigned char> desc_host;  // the same descriptors on the host
   bool borrowed = false;       // rng and best arrays belong to the context
   std::vector<int> slots;
@@ -554,6 +582,8 @@ struct DevJob {
 
 void job_free(DevJob& j) {
   dev_free(j.pts); dev_free(j.sum_w); dev_free(j.sp); dev_free(j.params0);
+  dev_free(j.fit_f); dev_free(j.fit_nev); dev_free(j.fit_steps); dev_free(j.fit_params); dev_free(j.fit_dV);
+  dev_free(j.queue);
   j.done = nullptr;  // the pool's
   dev_free(j.descs);
   if (!j.borrowed) { dev_free(j.rng); dev_free(j.best_f); dev_free(j.best_params); dev_free(j.best_dV); }
@@ -575,11 +605,24 @@ double eval_cost(const ast_variant& v, const CurveH& C) {
   return (double)C.n_data + w * (double)C.rk4_steps;
 }
 
+// Device bytes of a job: its slots (simplex state), random-stream counters,
+// per-fit results, per-group bests and its curve table.
+size_t job_bytes(int n_groups, int group_size, int n_workers, int n_params, int n_filters, size_t n_records) {
+  const size_t fits = (size_t)n_groups * group_size;
+  const bool refill = n_workers < n_groups;
+  return (size_t)n_workers * group_size * SLOT_BYTES + fits * (4 + 8 + 8 + (refill ? 8 * (n_params + n_filters) : 0)) +
+         (size_t)n_groups * 256 + sizeof(PointRec) * n_records;
+}
+
 // Device memory of one job (its own streams and bests unless borrowed).  The
 // clears run on `s`, the stream the job's kernels run on, so they are ordered
 // before its first launch.
 int job_upload(DevJob& j, const Pool& pool, const CurveH& C, const double* params0, hipStream_t s) {
   if (j.slots.empty() || j.slots[0] < 0 || j.slots[0] >= pool.n_slots) return AST_E_STATE;
+  if (j.n_workers <= 0) j.n_workers = j.rp.n_groups;
+  if ((int)j.slots.size() != j.n_workers || j.n_workers > j.rp.n_groups) return AST_E_STATE;
+  j.refill = j.n_workers < j.rp.n_groups;
+  if (j.refill && j.rp.nstages != 1) return AST_E_ARG;  // a stage reseed needs the whole group
   const SpaceH& S = j.space;
   j.cf = S.v.precision == AST_PREC_FP64 ? 8 : 4;
   j.n_data = C.n_data;
@@ -621,13 +664,26 @@ int job_upload(DevJob& j, const Pool& pool, const CurveH& C, const double* param
     HIPCHK(hipMemsetAsync(j.best_params, 0, sizeof(double) * G * AST_MAX_PARAMS, s));
     HIPCHK(hipMemsetAsync(j.best_dV, 0, sizeof(double) * G * AST_MAX_FILTERS, s));
   }
-  j.bytes = sizeof(PointRec) * C.pts.size() + fits * (SLOT_BYTES + 4) + (size_t)G * 256;
+  RETIF(dev_alloc((char**)&j.fit_f, (size_t)j.cf * fits));
+  RETIF(dev_alloc(&j.fit_nev, fits));
+  RETIF(dev_alloc(&j.fit_steps, fits));
+  HIPCHK(hipMemsetAsync(j.fit_nev, 0, fits * sizeof(int32_t), s));
+  HIPCHK(hipMemsetAsync(j.fit_steps, 0, fits * sizeof(int32_t), s));
+  if (j.refill) {
+    RETIF(dev_alloc(&j.fit_params, fits * (size_t)N));
+    RETIF(dev_alloc(&j.fit_dV, fits * (size_t)C.n_filters));
+    RETIF(dev_alloc(&j.queue, (size_t)2));
+    const int32_t q[2] = {(int32_t)((size_t)j.n_workers * j.rp.group_size), 0};
+    HIPCHK(hipMemcpy(j.queue, q, sizeof q, hipMemcpyHostToDevice));
+  }
+  j.reduced = false;
+  j.bytes = job_bytes(j.rp.n_groups, j.rp.group_size, j.n_workers, N, C.n_filters, C.pts.size());
   return 0;
 }
 
 template <class CF>
 int job_descs_typed(DevJob& j, int ticks) {
-  const int G = j.rp.n_groups;
+  const int G = j.n_workers;
   std::vector<GroupDesc<CF>> d(G);
   double sm = j.rp.size_min > 0 ? j.rp.size_min : (j.cf == 8 ? 1e-10 : 1e-5);  // asteroid.h:233-237
   CF smc = (CF)sm;
@@ -655,6 +711,14 @@ int job_descs_typed(DevJob& j, int ticks) {
     D.slot = j.slots[g];
     D.group_in_job = g;
     D.ticks = ticks;
+    D.fit_f = (CF*)j.fit_f;
+    D.fit_nev = j.fit_nev;
+    D.fit_steps = j.fit_steps;
+    D.refill = j.refill ? 1 : 0;
+    D.n_fits = j.rp.n_groups * j.rp.group_size;
+    D.queue = j.queue;
+    D.fit_params = j.fit_params;
+    D.fit_dV = j.fit_dV;
   }
   if (!j.descs) RETIF(dev_alloc((char**)&j.descs, sizeof(GroupDesc<CF>) * G));
   HIPCHK(hipMemcpy(j.descs, d.data(), sizeof(GroupDesc<CF>) * G, hipMemcpyHostToDevice));
@@ -672,7 +736,9 @@ int job_descs(DevJob& j, int ticks) {
 // groups.  Checked on the host before every launch.
 bool job_fits_pool(const DevJob& j, const Pool& pool) {
   if (j.rp.group_size != pool.G || j.rp.n_groups <= 0) return false;
-  if ((int)j.slots.size() != j.rp.n_groups || !j.descs) return false;
+  if (j.n_workers <= 0 || j.n_workers > j.rp.n_groups) return false;
+  if ((int)j.slots.size() != j.n_workers || !j.descs || !j.fit_f || !j.fit_nev || !j.fit_steps) return false;
+  if (j.refill && (j.rp.nstages != 1 || !j.queue || !j.fit_params || !j.fit_dV)) return false;
   if (j.space.n_params > AST_MAX_PARAMS || j.n_filters > AST_MAX_FILTERS || j.n_filters <= 0) return false;
   for (int s : j.slots)
     if (s < 0 || s >= pool.n_slots) return false;
@@ -693,11 +759,33 @@ hipError_t job_launch(DevJob& j, const Pool& pool, int op, hipStream_t s) {
   SlotBufs<float> bf;
   if (j.cf == 8) { bd = pool_bufs<double>(pool, j.cls); a.bufs = &bd; }
   else { bf = pool_bufs<float>(pool, j.cls); a.bufs = &bf; }
-  a.n_groups = j.rp.n_groups;
+  a.n_groups = j.n_workers;
   a.group_size = j.rp.group_size;
   a.n_filters = j.n_filters;
   return dispatch(j.space.v, a);
 }
+
+// A job whose workgroups have all finished: with refill, form each group's
+// result from its fits' results (group_best_kernel) and wait for it.
+int job_reduce(DevJob& j, hipStream_t s) {
+  if (!j.refill || j.reduced) return 0;
+  const int G = j.rp.group_size, N = j.space.n_params;
+  if (j.cf == 8)
+    hipLaunchKernelGGL((group_best_kernel<double>), dim3(j.rp.n_groups), dim3(64), 0, s, (const double*)j.fit_f,
+                       j.fit_params, j.fit_dV, G, N, j.n_filters, j.rp.n_groups, (double*)j.best_f, j.best_params,
+                       j.best_dV);
+  else
+    hipLaunchKernelGGL((group_best_kernel<float>), dim3(j.rp.n_groups), dim3(64), 0, s, (const float*)j.fit_f,
+                       j.fit_params, j.fit_dV, G, N, j.n_filters, j.rp.n_groups, (float*)j.best_f, j.best_params,
+                       j.best_dV);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  j.reduced = true;
+  return 0;
+}
+
+// All fits of the job have ended.
+bool job_complete(const DevJob& j) { return j.groups_done >= j.n_workers; }
 
 // Kernel class of a job: one fit-kernel instantiation (variant, filter
 // capacity, precision form).
@@ -901,7 +989,8 @@ struct ast_ctx {
   TailRec minima{};           // NUDGE's observed minima (in the curve table's tail record)
   bool minima_set = false;
   double seg_t[AST_MAX_SEGMENTS]{};  // SEGMENT's start times
-  bool seg_set = false;
+  int seg_first[AST_MAX_SEGMENTS]{};  // and the first observation of each segment, when given
+  bool seg_set = false, seg_first_set = false;
   double params0[AST_MAX_PARAMS]{};
   uint64_t seed = 0;
   uint32_t* rng = nullptr;
@@ -915,6 +1004,9 @@ struct ast_ctx {
   DevJob* last_job = nullptr;  // lanes of the last ast_run (slot rows 0..fits-1)
   std::unique_ptr<DevJob> ref_job;
   int last_groups = 0, last_group_size = 0;
+  bool run_active = false;     // an ast_run_begin whose job has not finished
+  bool run_graphs = false;
+  long run_cycles = 0, run_max_cycles = 0;
   double last_ms = 0;
   int last_launches = 0;
   int64_t last_evals = 0, last_points = 0;
@@ -958,7 +1050,7 @@ int class_batch_update(ast_ctx* c, int k, const std::vector<DevJob*>& members) {
   int groups = 0;
   for (DevJob* j : members) {
     bytes += j->desc_host.size();
-    groups += j->rp.n_groups;
+    groups += j->n_workers;
   }
   if (bytes > cb.cap) {
     dev_free(cb.d_descs);
@@ -1042,7 +1134,7 @@ int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, in
   std::vector<DevJob*> members[MAX_CLASSES];
   bool used_s[NSTREAMS] = {false};
   for (DevJob* j : jobs) {
-    if (j->groups_done >= j->rp.n_groups) continue;
+    if (job_complete(*j)) continue;
     if (!job_fits_pool(*j, c->pool)) {
       set_err("job does not fit the slot pool");
       return AST_E_STATE;
@@ -1099,10 +1191,14 @@ int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, in
     HIPCHK(hipMemcpy(P.job_done_h.data(), P.job_done, sizeof(int32_t) * P.n_slots, hipMemcpyDeviceToHost));
   for (size_t i = 0; i < jobs.size(); ++i) done[i] = P.job_done_h[(size_t)jobs[i]->slots[0]];
   *kernel_ms += ms_max;
-  const Clock::time_point now = Clock::now();
   for (size_t i = 0; i < jobs.size(); ++i) {
-    if (jobs[i]->groups_done < jobs[i]->rp.n_groups && done[i] >= jobs[i]->rp.n_groups) jobs[i]->t_done = now;
-    jobs[i]->groups_done = done[i];
+    DevJob& j = *jobs[i];
+    const bool was = job_complete(j);
+    j.groups_done = done[i];
+    if (!was && job_complete(j)) {
+      RETIF(job_reduce(j, c->streams[stream_of(j.cls)]));
+      j.t_done = Clock::now();
+    }
   }
   return 0;
 }
@@ -1197,14 +1293,20 @@ int job_result(DevJob& j, ast_result* r) {
   return 0;
 }
 
-// Slot-state counters of a job's groups -> evaluation count.
+// Evaluations a job has done so far: the lanes' work counters over its slots
+// (consecutive slots are read in one copy).
 int job_evals(const ast_ctx* c, const DevJob& j, int64_t* evals) {
-  const int G = j.rp.group_size;
-  std::vector<int32_t> tmp(G);
+  const size_t G = (size_t)j.rp.group_size;
+  std::vector<int32_t> tmp;
   int64_t s = 0;
-  for (int slot : j.slots) {
-    HIPCHK(hipMemcpy(tmp.data(), c->pool.nev_tot + (size_t)slot * G, sizeof(int32_t) * G, hipMemcpyDeviceToHost));
-    for (int k = 0; k < G; ++k) s += tmp[k];
+  for (size_t a = 0; a < j.slots.size();) {
+    size_t b = a + 1;
+    while (b < j.slots.size() && j.slots[b] == j.slots[b - 1] + 1) ++b;
+    tmp.resize((b - a) * G);
+    HIPCHK(hipMemcpy(tmp.data(), c->pool.work + (size_t)j.slots[a] * G, sizeof(int32_t) * tmp.size(),
+                     hipMemcpyDeviceToHost));
+    for (int32_t w : tmp) s += w;
+    a = b;
   }
   *evals = s;
   return 0;
@@ -1230,8 +1332,9 @@ int make_job(ast_ctx* c, const ast_job* job, int G, std::unique_ptr<DevJob>* out
                    &j->space));
   CurveH C;
   RETIF(make_curve(job->variant, job->obs, job->n_data, job->n_filters, job->eph, &C,
-                   job->variant.n_seg > 1 ? job->t_segments : nullptr));
-  if (C.n_data - j->space.n_params - C.n_filters <= 0) return AST_E_DATA;
+                   job->variant.n_seg > 1 ? job->t_segments : nullptr,
+                   job->variant.n_seg > 1 ? job->seg_first_obs : nullptr));
+  if (C.n_all - j->space.n_params - C.n_filters <= 0) return AST_E_DATA;  // dof over every observation
   if (job->variant.nudge) RETIF(set_tail_minima(job->variant, job->t_minima, job->V_minima, job->n_minima, &C));
   j->rp.n_groups = job->n_fits / G;
   j->rp.group_size = G;
@@ -1329,7 +1432,7 @@ int batch_run(ast_ctx* c, const std::vector<const ast_job*>& jobs, int G, double
   }
   for (long L = 0; !rc; ++L) {
     bool all = true;
-    for (DevJob* j : run) all = all && j->groups_done >= j->rp.n_groups;
+    for (DevJob* j : run) all = all && job_complete(*j);
     if (all || (cycle_limit > 0 && L >= cycle_limit)) break;
     if (L > max_cycles) {
       set_err("fit did not terminate");
@@ -1418,16 +1521,23 @@ void serve_loop(ast_ctx* c) {
           if (j->id == id) j->cancelled = true;
       }
       S.cancels.clear();
-      // Admission: FIFO, while the job's groups fit the free slots and its
-      // footprint fits the HBM budget.  The slots are taken off the free
-      // list here, so later jobs of the same pass see what is left.
+      // Admission: FIFO, while the job fits the free slots and its footprint
+      // fits the HBM budget.  A multi-stage job needs a slot per group (its
+      // stage reseeds need whole groups); a single-stage job starts on as many
+      // of its groups' worth of slots as are free, at least one, and its lanes
+      // take the rest of its fits as their fits end (refill), so free slots
+      // never wait behind a job that does not fit whole.  The slots are taken
+      // off the free list here, so later jobs of the same pass see what is
+      // left.
       while (!S.pending.empty()) {
         DevJob& j = *S.pending.front();
         auto in = S.inputs.find(j.id);
-        const int64_t need = (int64_t)j.rp.n_groups;
-        const int64_t bytes = (int64_t)(j.rp.n_groups * (int64_t)j.rp.group_size) * (int64_t)(SLOT_BYTES + 4) +
-                              (int64_t)j.n_data * (int64_t)sizeof(PointRec);
-        if ((int64_t)c->pool.free_slots.size() < need) break;
+        const int64_t nfree = (int64_t)c->pool.free_slots.size();
+        const int64_t need = j.rp.nstages == 1 ? std::min<int64_t>((int64_t)j.rp.n_groups, std::max<int64_t>(nfree, 1))
+                                               : (int64_t)j.rp.n_groups;
+        const int64_t bytes = (int64_t)job_bytes(j.rp.n_groups, j.rp.group_size, (int)need, j.space.n_params,
+                                                 j.n_filters, in->second.first.pts.size());
+        if (nfree < need) break;
         if (S.cfg.hbm_budget_bytes > 0 && S.resident_bytes + bytes > S.cfg.hbm_budget_bytes &&
             !(S.resident.empty() && admitted.empty()))
           break;
@@ -1436,6 +1546,8 @@ void serve_loop(ast_ctx* c) {
           j.slots.push_back(c->pool.free_slots.back());
           c->pool.free_slots.pop_back();
         }
+        j.n_workers = (int)need;
+        j.bytes = (size_t)bytes;
         S.resident_bytes += bytes;
         admitted_in.push_back(std::move(in->second));
         S.inputs.erase(in);
@@ -1478,7 +1590,7 @@ void serve_loop(ast_ctx* c) {
       std::lock_guard<std::mutex> lk(S.mu);
       for (auto it = S.resident.begin(); it != S.resident.end();) {
         DevJob& j = **it;
-        if (j.cancelled || j.status != AST_OK || j.groups_done >= j.rp.n_groups) {
+        if (j.cancelled || j.status != AST_OK || job_complete(j)) {
           finished.push_back(std::move(*it));
           it = S.resident.erase(it);
         } else {
@@ -1506,8 +1618,7 @@ void serve_loop(ast_ctx* c) {
       r.run_ms = j.run_ms;
       std::lock_guard<std::mutex> lk(S.mu);
       for (int s : j.slots) c->pool.free_slots.push_back(s);
-      S.resident_bytes -= (int64_t)(j.rp.n_groups * (int64_t)j.rp.group_size) * (int64_t)(SLOT_BYTES + 4) +
-                          (int64_t)j.n_data * (int64_t)sizeof(PointRec);
+      S.resident_bytes -= (int64_t)j.bytes;
       S.evals += ev;
       S.points += ev * j.n_data;
       job_free(j);
@@ -1686,7 +1797,8 @@ int ast_upload_curve(ast_ctx* c, const ast_obs* obs, int n_data, int n_filters, 
   if (!c) return AST_E_ARG;
   if (c->v.n_seg > 1 && !c->seg_set) return AST_E_STATE;
   CurveH C;
-  RETIF(make_curve(c->v, obs, n_data, n_filters, eph, &C, c->seg_set ? c->seg_t : nullptr));
+  RETIF(make_curve(c->v, obs, n_data, n_filters, eph, &C, c->seg_set ? c->seg_t : nullptr,
+                   c->seg_first_set ? c->seg_first : nullptr));
   {  // the tail record: the context's minima with this curve's TIME_STEP
     TailRec tail = c->minima;
     tail.ts = c->v.time_step;
@@ -1729,11 +1841,29 @@ int ast_set_segments(ast_ctx* c, const double* t_start, int n_seg) {
     if (!std::isfinite(t_start[k])) return AST_E_ARG;
   for (int k = 0; k < n_seg; ++k) c->seg_t[k] = t_start[k];
   c->seg_set = true;
+  c->seg_first_set = false;
+  return AST_OK;
+}
+
+int ast_set_segment_starts(ast_ctx* c, const int* first_obs, const double* t_start, int n_seg) {
+  if (!c) return AST_E_ARG;
+  if (c->v.n_seg <= 1) return AST_E_STATE;
+  if (!first_obs || !t_start || n_seg != c->v.n_seg) return AST_E_ARG;
+  for (int k = 0; k < n_seg; ++k)
+    if (!std::isfinite(t_start[k]) || first_obs[k] < 0 || (k > 0 && first_obs[k] <= first_obs[k - 1]))
+      return AST_E_ARG;
+  for (int k = 0; k < n_seg; ++k) {
+    c->seg_t[k] = t_start[k];
+    c->seg_first[k] = first_obs[k];
+  }
+  c->seg_set = true;
+  c->seg_first_set = true;
   return AST_OK;
 }
 
 int ast_rng_seed(ast_ctx* c, uint64_t seed) {
   if (!c) return AST_E_ARG;
+  if (c->run_active) return AST_E_STATE;  // the running job counts on its streams
   RETIF(set_device(c));
   c->seed = seed;
   if (c->rng) HIPCHK(hipMemset(c->rng, 0, c->rng_cap * sizeof(uint32_t)));
@@ -1742,6 +1872,7 @@ int ast_rng_seed(ast_ctx* c, uint64_t seed) {
 
 int ast_reset_best(ast_ctx* c) {
   if (!c) return AST_E_ARG;
+  if (c->run_active) return AST_E_STATE;
   RETIF(set_device(c));
   return reset_best_f(c);
 }
@@ -1753,19 +1884,24 @@ int ast_set_start(ast_ctx* c, const double* params0) {
   return AST_OK;
 }
 
-int ast_run(ast_ctx* c, const ast_run_cfg* rc) {
+int ast_run_begin(ast_ctx* c, const ast_run_cfg* rc) {
   if (!c || !rc) return AST_E_ARG;
   if (c->serve) return AST_E_BUSY;
   if (!c->space_set || !c->curve_set || (c->v.nudge && !c->minima_set)) return AST_E_STATE;
   const int G = rc->group_size;
   if (rc->n_groups <= 0 || G < 64 || G > 1024 || (G & 63) || rc->nstages <= 0 || rc->n_steps < 0 ||
-      rc->eval_budget < 0)
+      rc->eval_budget < 0 || rc->resident_groups < 0)
     return AST_E_ARG;
+  const int workers = rc->resident_groups > 0 && rc->resident_groups < rc->n_groups ? rc->resident_groups
+                                                                                     : rc->n_groups;
+  if (workers < rc->n_groups && rc->nstages != 1) return AST_E_ARG;  // stage reseeds need whole groups
   if (c->curve.n_all - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
   RETIF(set_device(c));
-  const size_t fits = (size_t)rc->n_groups * G;
+  c->run_active = false;
+  c->last_groups = 0;
+  const size_t fits = (size_t)rc->n_groups * G, lanes = (size_t)workers * G;
   RETIF(ensure_streams(c, fits, rc->n_groups));
-  if (c->pool.nf < fits || c->pool.G != G) RETIF(pool_alloc(c->pool, fits, G));
+  if (c->pool.nf < lanes || c->pool.G != G) RETIF(pool_alloc(c->pool, lanes, G));
   // The reference path is one job that borrows the context's streams and bests.
   if (c->ref_job) job_free(*c->ref_job);
   c->ref_job.reset(new DevJob());
@@ -1779,6 +1915,7 @@ int ast_run(ast_ctx* c, const ast_run_cfg* rc) {
   j.rp.eval_budget = rc->eval_budget;
   j.rp.size_min = rc->size_min;
   j.rp.seed = c->seed;
+  j.n_workers = workers;
   j.borrowed = true;
   j.rng = c->rng;
   j.best_f = c->best_f;
@@ -1786,7 +1923,7 @@ int ast_run(ast_ctx* c, const ast_run_cfg* rc) {
   j.best_dV = c->best_dV;
   j.cls = class_of(c->v, c->curve.n_filters);
   const hipStream_t s = c->streams[stream_of(j.cls)];
-  for (int g = 0; g < rc->n_groups; ++g) j.slots.push_back(g);
+  for (int g = 0; g < workers; ++g) j.slots.push_back(g);
   RETIF(job_upload(j, c->pool, c->curve, c->params0, s));
   // The context's streams and bests were cleared by synchronous-API calls on
   // the null stream, which the job's non-blocking stream does not wait for.
@@ -1795,33 +1932,100 @@ int ast_run(ast_ctx* c, const ast_run_cfg* rc) {
   if (rc->ticks_per_launch > 0) {
     ticks = rc->ticks_per_launch > 4096 ? 4096 : rc->ticks_per_launch;
   } else {  // about 100 ms of kernel per launch when the GPU is full
-    double tot = (double)fits * c->curve.n_data;
+    double tot = (double)lanes * c->curve.n_data;
     double t = 3.0e9 / (tot > 1 ? tot : 1);
     ticks = (int)(t < 16 ? 16 : (t > 4096 ? 4096 : t));
   }
+  j.ticks = ticks;
   RETIF(job_descs(j, ticks));
   HIPCHK(job_launch(j, c->pool, 0, s));
   c->last_ms = 0;
   c->last_launches = 0;
+  c->last_evals = 0;
+  c->last_points = 0;
   // Every tick completes one evaluation of each running simplex; a stage
-  // needs at most (N+1) + n_steps*(N+2) of them.
+  // needs at most (N+1) + n_steps*(N+2) of them, and with refill a lane runs
+  // its share of the fits one after another.
   const double per_stage = (double)(c->space.n_params + 1) + (double)rc->n_steps * (c->space.n_params + 2);
-  const long max_cycles = (long)((per_stage * rc->nstages + 16.0) / ticks) + 4;
+  const double rounds = std::ceil((double)fits / (double)lanes) + 1.0;
+  c->run_max_cycles = (long)((per_stage * rc->nstages * rounds + 16.0) / ticks) + 4;
+  c->run_cycles = 0;
+  c->run_graphs = rc->use_graphs != 0;
+  c->run_active = true;
+  return AST_OK;
+}
+
+int ast_run_continue(ast_ctx* c, int max_cycles, int* finished) {
+  if (!c) return AST_E_ARG;
+  if (c->serve) return AST_E_BUSY;
+  if (!c->run_active || !c->ref_job) return AST_E_STATE;
+  RETIF(set_device(c));
+  DevJob& j = *c->ref_job;
   std::vector<DevJob*> jobs{&j};
-  for (long L = 0; j.groups_done < rc->n_groups; ++L) {
-    if (L > max_cycles) {
+  for (long L = 0; !job_complete(j) && (max_cycles <= 0 || L < max_cycles); ++L) {
+    if (c->run_cycles > c->run_max_cycles) {
+      c->run_active = false;
       set_err("fit did not terminate");
       return AST_E_STATE;
     }
-    RETIF(run_cycle(c, jobs, &c->last_ms, &c->last_launches, rc->use_graphs != 0));
+    int e = run_cycle(c, jobs, &c->last_ms, &c->last_launches, c->run_graphs);
+    if (e) {
+      c->run_active = false;
+      return e;
+    }
+    c->run_cycles++;
   }
   int64_t ev = 0;
   RETIF(job_evals(c, j, &ev));
   c->last_evals = ev;
   c->last_points = ev * c->curve.n_data;
-  c->last_groups = rc->n_groups;
-  c->last_group_size = G;
+  const bool fin = job_complete(j);
+  if (fin) {
+    c->run_active = false;
+    c->last_groups = j.rp.n_groups;
+    c->last_group_size = j.rp.group_size;
+  }
+  if (finished) *finished = fin ? 1 : 0;
   return AST_OK;
+}
+
+int ast_run_status(ast_ctx* c, int64_t* fits_started, int64_t* fits_done, int* finished) {
+  if (!c) return AST_E_ARG;
+  if (!c->ref_job) return AST_E_STATE;
+  RETIF(set_device(c));
+  const DevJob& j = *c->ref_job;
+  const int64_t all = (int64_t)j.rp.n_groups * j.rp.group_size;
+  int64_t st = 0, dn = 0;
+  if (j.refill) {  // the queue's counters (a finished lane takes one more start than there are fits)
+    int32_t q[2] = {0, 0};
+    HIPCHK(hipMemcpy(q, j.queue, sizeof q, hipMemcpyDeviceToHost));
+    st = std::min<int64_t>(q[0], all);
+    dn = std::min<int64_t>(q[1], all);
+  } else {  // whole groups end together
+    st = all;
+    dn = (int64_t)j.groups_done * j.rp.group_size;
+  }
+  if (fits_started) *fits_started = st;
+  if (fits_done) *fits_done = dn;
+  if (finished) *finished = job_complete(j) ? 1 : 0;
+  return AST_OK;
+}
+
+int ast_run_cancel(ast_ctx* c) {
+  if (!c) return AST_E_ARG;
+  if (c->serve) return AST_E_BUSY;
+  if (!c->run_active) return AST_OK;
+  RETIF(set_device(c));
+  HIPCHK(hipDeviceSynchronize());  // nothing of the run is in flight after a cycle; drain anyway
+  c->run_active = false;
+  return AST_OK;
+}
+
+int ast_run(ast_ctx* c, const ast_run_cfg* rc) {
+  RETIF(ast_run_begin(c, rc));
+  int fin = 0;
+  RETIF(ast_run_continue(c, 0, &fin));
+  return fin ? AST_OK : AST_E_STATE;
 }
 
 int ast_fetch(ast_ctx* c, double* f, double* params, double* dV) {
@@ -1845,27 +2049,27 @@ int ast_fetch(ast_ctx* c, double* f, double* params, double* dV) {
 
 int ast_fetch_lanes(ast_ctx* c, double* f_last, int64_t* n_evals, int64_t* n_steps) {
   if (!c) return AST_E_ARG;
-  if (c->serve) return AST_E_BUSY;  // the slot pool belongs to the scheduler
-  if (!c->last_groups) return AST_E_STATE;
+  if (c->serve) return AST_E_BUSY;
+  if (!c->last_groups || !c->ref_job) return AST_E_STATE;
   RETIF(set_device(c));
-  const size_t fits = (size_t)c->last_groups * c->last_group_size;
-  if (f_last) {  // 8-byte cells; a float group fills the first half of its slot's cells
-    if (c->best_cf == 8) {
-      HIPCHK(hipMemcpy(f_last, c->pool.sf, sizeof(double) * fits, hipMemcpyDeviceToHost));
+  const DevJob& j = *c->ref_job;
+  const size_t fits = (size_t)j.rp.n_groups * j.rp.group_size;
+  if (f_last) {
+    if (j.cf == 8) {
+      HIPCHK(hipMemcpy(f_last, j.fit_f, sizeof(double) * fits, hipMemcpyDeviceToHost));
     } else {
-      const size_t G = (size_t)c->last_group_size;
-      std::vector<float> tmp(2 * fits);
-      HIPCHK(hipMemcpy(tmp.data(), c->pool.sf, sizeof(float) * 2 * fits, hipMemcpyDeviceToHost));
-      for (size_t k = 0; k < fits; ++k) f_last[k] = tmp[2 * G * (k / G) + k % G];
+      std::vector<float> tmp(fits);
+      HIPCHK(hipMemcpy(tmp.data(), j.fit_f, sizeof(float) * fits, hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < fits; ++k) f_last[k] = tmp[k];
     }
   }
   std::vector<int32_t> tmp(fits);
   if (n_evals) {
-    HIPCHK(hipMemcpy(tmp.data(), c->pool.nev_tot, sizeof(int32_t) * fits, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(tmp.data(), j.fit_nev, sizeof(int32_t) * fits, hipMemcpyDeviceToHost));
     for (size_t k = 0; k < fits; ++k) n_evals[k] = tmp[k];
   }
   if (n_steps) {
-    HIPCHK(hipMemcpy(tmp.data(), c->pool.steps_tot, sizeof(int32_t) * fits, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(tmp.data(), j.fit_steps, sizeof(int32_t) * fits, hipMemcpyDeviceToHost));
     for (size_t k = 0; k < fits; ++k) n_steps[k] = tmp[k];
   }
   return AST_OK;
@@ -1927,6 +2131,7 @@ int ast_rmsd(ast_ctx* c, const ast_rmsd_cfg* cfg, float* par_min, float* par_max
       !(cfg->dx >= 0))
     return AST_E_ARG;
   if (c->serve) return AST_E_BUSY;
+  if (c->run_active) return AST_E_STATE;  // it shares the context's random streams
   if (!c->space_set || !c->curve_set || (c->v.nudge && !c->minima_set)) return AST_E_STATE;
   if (c->v.n_seg > 1) return AST_E_UNSUPPORTED;
   if (c->curve.n_all - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
@@ -2050,7 +2255,7 @@ int ast_last_counts(ast_ctx* c, int64_t* evals, int64_t* model_points) {
 // ------------------------------------------------------------------ serving
 int ast_serve_start(ast_ctx* c, const ast_serve_cfg* cfg) {
   if (!c || !cfg) return AST_E_ARG;
-  if (c->serve) return AST_E_STATE;
+  if (c->serve || c->run_active) return AST_E_STATE;
   const int G = cfg->group_size ? cfg->group_size : 256;
   if (G < 64 || G > 1024 || (G & 63) || cfg->max_resident_fits < G) return AST_E_ARG;
   RETIF(set_device(c));
@@ -2072,7 +2277,12 @@ int ast_submit(ast_ctx* c, const ast_job* job, uint64_t* id) {
   CurveH C;
   std::vector<double> p0;
   RETIF(make_job(c, job, S.cfg.group_size, &j, &C, &p0));
-  if (j->rp.n_groups > c->pool.n_slots) return AST_E_NOMEM;
+  // a multi-stage job needs a slot per group; a single-stage one can run on
+  // fewer (its lanes take the rest of its fits from its queue)
+  if (j->rp.nstages != 1 && j->rp.n_groups > c->pool.n_slots) {
+    set_err("a multi-stage job has more groups than the pool has slots");
+    return AST_E_NOMEM;
+  }
   std::lock_guard<std::mutex> lk(S.mu);
   j->id = S.next_id++;
   *id = j->id;
@@ -2085,6 +2295,7 @@ int ast_submit(ast_ctx* c, const ast_job* job, uint64_t* id) {
 int ast_run_jobs(ast_ctx* c, const ast_job* jobs, int n_jobs, const ast_batch_cfg* cfg, ast_result* results) {
   if (!c || !jobs || n_jobs <= 0 || !results) return AST_E_ARG;
   if (c->serve) return AST_E_BUSY;
+  if (c->run_active) return AST_E_STATE;  // the slot pool holds a running ast_run_begin job
   const int G = (cfg && cfg->group_size) ? cfg->group_size : 256;
   if (G < 64 || G > 1024 || (G & 63)) return AST_E_ARG;
   const double work = (cfg && cfg->work_points_per_launch > 0) ? cfg->work_points_per_launch : 16000.0;
@@ -2108,6 +2319,7 @@ int ast_autotune(ast_ctx* c, const ast_job* jobs, int n_jobs, const ast_batch_cf
                  ast_tune_entry* out, int max_out, int* n_out) {
   if (!c || !jobs || n_jobs <= 0 || (max_out > 0 && !out) || !n_out) return AST_E_ARG;
   if (c->serve) return AST_E_BUSY;
+  if (c->run_active) return AST_E_STATE;
   *n_out = 0;
   const int G = (cfg && cfg->group_size) ? cfg->group_size : 256;
   if (G < 64 || G > 1024 || (G & 63)) return AST_E_ARG;

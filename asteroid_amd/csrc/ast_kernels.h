@@ -68,6 +68,8 @@ struct SlotBufs {
   int32_t* steps_tot; // steps, all stages of this job
   CF* fr;             // f_r kept for the expansion test
   CF* sf;             // stage-final best chi^2 (s_f, cuda.c:1739-1742)
+  int32_t* fitid;     // the job's index of the fit the lane runs (its random stream)
+  int32_t* work;      // evaluations the lane did since its group started (all fits)
   int32_t* g_reopt;   // per group slot: s_x2_params.reopt
   int32_t* g_stage;   // per group slot: current stage
   int32_t* g_done;    // per group slot: all stages finished
@@ -94,13 +96,29 @@ struct GroupDesc {
   int n_data, n_filters, dof;
   int nstages, n_steps, eval_budget, reopt;
   int slot;               // group slot
-  int group_in_job;
+  int group_in_job;       // the group (with refill: the worker) index within the job
   int ticks;              // evaluation rounds in this launch
+  // Per-fit results of the job, written by each fit when it ends: its
+  // stage-final best chi^2 (1e30 if it failed), evaluations and simplex steps
+  // over all its stages.
+  CF* fit_f;
+  int32_t* fit_nev;
+  int32_t* fit_steps;
+  // Refill (single-stage jobs run by fewer workgroups than the job has
+  // groups): a lane whose fit ended takes the next unstarted fit of the job
+  // from queue[0] (queue[1] counts finished fits) and keeps the ended fit's
+  // best point (x2params of its best vertex) and its last delta_V, from which
+  // group_best_kernel forms each group's result once every fit has ended.
+  int refill;
+  int n_fits;
+  int32_t* queue;
+  double* fit_params;     // [n_fits][N]
+  double* fit_dV;         // [n_fits][n_filters]
 };
 
 template <class CF>
 struct LaneRegs {
-  int phase, j, failed, bad, l, nev, nev_tot, steps_tot;
+  int phase, j, failed, bad, l, nev, nev_tot, steps_tot, work;
   uint32_t rng;
   CF fr, sf;
 };
@@ -219,6 +237,54 @@ struct Machine {
     R.sf = R.failed ? (CF)1e30 : F(IND(0));
     R.steps_tot += R.l;
     R.phase = PH_STAGE_DONE;
+  }
+
+  // A fit's own results, when it ends (refill): stage-final best chi^2,
+  // counts, the physical parameters of its best vertex (x2params through the
+  // lane's params[] buffer, as the group's writer does at cuda.c:1810-1823)
+  // and the delta_V of its last evaluation (cuda.c:1816-1822).
+  __device__ void end_fit(const LaneRegs<CF>& R, const GroupDesc<CF>& D) const {
+    D.fit_f[jfit] = R.sf;
+    D.fit_nev[jfit] = R.nev_tot;
+    D.fit_steps[jfit] = R.steps_tot;
+    D.rng[jfit] = R.rng;
+    if (!R.failed) {
+      CF xb[N];
+      double prm[N];
+      const int b = IND(0);
+#pragma unroll
+      for (int i = 0; i < N; ++i) { xb[i] = X(b, i); prm[i] = PRM(i); }
+      x2params<CF, VC>(xb, prm, sp, reopt);
+#pragma unroll
+      for (int i = 0; i < N; ++i) D.fit_params[jfit * N + i] = prm[i];
+    }
+    for (int m = 0; m < D.n_filters; ++m) D.fit_dV[jfit * D.n_filters + m] = B.DV[(size_t)m * rs + fo];
+  }
+
+  // The lane's state for a new fit (as fit_init_kernel leaves a group's
+  // lanes), then its stage start.
+  __device__ void begin_fit(LaneRegs<CF>& R, const GroupDesc<CF>& D) const {
+#pragma unroll
+    for (int v = 0; v <= N; ++v) IND(v) = (uint8_t)v;
+    double prm[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) prm[i] = reopt ? D.params0[i] : 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) PRM(i) = prm[i];
+    CF x[N];
+    if (reopt) params2x<CF, VC>(x, prm, sp);
+    else
+      for (int i = 0; i < N; ++i) x[i] = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) X(0, i) = x[i];
+    for (int m = 0; m < MAXF; ++m) B.DV[(size_t)m * rs + fo] = 0;
+    R.phase = PH_START;
+    R.nev_tot = 0;
+    R.steps_tot = 0;
+    R.rng = D.rng[jfit];
+    R.fr = 0;
+    R.sf = (CF)1e30;
+    advance(R, (CF)0, false);
   }
 
   __device__ void replace_worst_with_xr(CF fv) const {
@@ -433,6 +499,8 @@ __global__ void fit_init_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
   B.nev[fit] = 0;
   B.nev_tot[fit] = 0;
   B.steps_tot[fit] = 0;
+  B.fitid[fit] = D.group_in_job * (int)blockDim.x + (int)threadIdx.x;
+  B.work[fit] = 0;
   B.fr[fo] = 0;
   B.sf[fo] = (CF)1e30;
   if (threadIdx.x == 0) {
@@ -481,6 +549,50 @@ __device__ __attribute__((noinline)) CF eval_lane(const double* PRM, CF* DV, siz
   return fv;
 }
 
+// The next `1` from a counter shared by many lanes: one atomic per
+// wavefront for all of its lanes that ask in this pass, each getting its own
+// value in lane order.  queue[1] counts the requests (each follows a fit's
+// end), queue[0] hands out the fit indices.
+__device__ inline int take_fit(int32_t* queue) {
+  const uint64_t mask = __ballot(1);
+  const int lane = (int)__lane_id();
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+  int base = 0;
+  if (lane == leader) {
+    const int cnt = __popcll(mask);
+    base = atomicAdd(&queue[0], cnt);
+    atomicAdd(&queue[1], cnt);
+  }
+  base = __shfl(base, leader);
+  return base + rank;
+}
+
+// Each group's result from its fits' own results, once every fit of the job
+// has ended (refill): the block argmin and the best-per-group write of
+// cuda.c:1739-1742, 1810-1823 -- the lowest chi^2 of the group's fits, the
+// first one on ties, replaces the stored best only when strictly lower.
+template <class CF>
+__global__ __launch_bounds__(64) void group_best_kernel(const CF* fit_f, const double* fit_params,
+                                                        const double* fit_dV, int G, int N, int n_filters,
+                                                        int n_groups, CF* best_f, double* best_params,
+                                                        double* best_dV) {
+  const int g = blockIdx.x;
+  if (threadIdx.x != 0 || g >= n_groups) return;
+  int tm = 0;
+  CF smin = (CF)1e30;
+  for (int k = 0; k < G; ++k) {
+    const CF f = fit_f[(size_t)g * G + k];
+    if (f < smin) { smin = f; tm = k; }
+  }
+  if (smin < best_f[g]) {
+    const size_t j = (size_t)g * G + tm;
+    best_f[g] = smin;
+    for (int i = 0; i < N; ++i) best_params[(size_t)g * N + i] = fit_params[j * N + i];
+    for (int m = 0; m < n_filters; ++m) best_dV[(size_t)g * n_filters + m] = fit_dV[j * n_filters + m];
+  }
+}
+
 // ---------------------------------------------------------------- fit kernel
 // One workgroup = one group (the reference's block); it runs D.ticks rounds
 // of "evaluate the pending points, advance every lane", then saves its state.
@@ -506,8 +618,8 @@ void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
   const size_t nf = B.nf, rs = KC * nf;
   const size_t fit = (size_t)slot * G + t;
   const size_t fo = (size_t)slot * G * KC + t;
-  const uint64_t jfit = (uint64_t)D.group_in_job * G + t;
   if (B.g_done[slot]) return;  // same value for every thread of the group
+  uint64_t jfit = (uint64_t)B.fitid[fit];  // group_in_job * G + t unless the lane has refilled
   const Space<CF>& sp = *D.sp;
   const StageCfg<CF> SC{D.seed, D.size2_min, D.n_steps, D.eval_budget};
   const int nstages = D.nstages, ticks = D.ticks;
@@ -523,9 +635,11 @@ void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
   R.nev = B.nev[fit];
   R.nev_tot = B.nev_tot[fit];
   R.steps_tot = B.steps_tot[fit];
+  R.work = B.work[fit];
   R.rng = D.rng[jfit];
   R.fr = B.fr[fo];
   R.sf = B.sf[fo];
+  const bool refill = D.refill != 0;
   bool done = false;
   {
     Machine<CF, VC> M{sp, B, SC, fit, nf, fo, rs, jfit, reopt};
@@ -538,7 +652,28 @@ void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
                                             D.sum_w);
       R.nev++;
       R.nev_tot++;
+      R.work++;
       M.advance(R, fv, true);
+    }
+    if (refill) {
+      // A lane whose fit ended keeps its results and starts the job's next
+      // unstarted fit (one that fails at its start ends at once and the lane
+      // takes another); with the queue empty it idles.  The group is done
+      // when every lane idles.
+      while (R.phase == PH_STAGE_DONE) {
+        Machine<CF, VC> Me{sp, B, SC, fit, nf, fo, rs, jfit, reopt};
+        Me.end_fit(R, D);
+        const int nj = take_fit(D.queue);
+        if (nj >= D.n_fits) {
+          R.phase = PH_FIT_DONE;
+          break;
+        }
+        jfit = (uint64_t)nj;
+        Machine<CF, VC> Mb{sp, B, SC, fit, nf, fo, rs, jfit, reopt};
+        Mb.begin_fit(R, D);
+      }
+      if (__syncthreads_count(R.phase != PH_FIT_DONE) == 0) done = true;
+      continue;
     }
     const int running = __syncthreads_count(phase_pending(R.phase));
     if (running) continue;
@@ -600,6 +735,9 @@ void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
         for (int m = 0; m < D.n_filters; ++m)
           D.best_dV[(size_t)g * D.n_filters + m] = B.DV[(size_t)m * rs + fo];
       }
+      D.fit_f[jfit] = R.sf;
+      D.fit_nev[jfit] = R.nev_tot;
+      D.fit_steps[jfit] = R.steps_tot;
       stage++;
       R.phase = PH_FIT_DONE;
       done = true;
@@ -610,6 +748,8 @@ void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
   B.nev[fit] = R.nev;
   B.nev_tot[fit] = R.nev_tot;
   B.steps_tot[fit] = R.steps_tot;
+  B.work[fit] = R.work;
+  B.fitid[fit] = (int32_t)jfit;
   D.rng[jfit] = R.rng;
   B.fr[fo] = R.fr;
   B.sf[fo] = R.sf;

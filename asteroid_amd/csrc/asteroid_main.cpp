@@ -50,7 +50,21 @@ struct Ingested {
   std::vector<char> filters;
   ast_ephem3 eph{};
   double mjd0 = 0;  // hMJD0
+  std::vector<double> mjd_raw;  // observer's MJD of each observation, before the light-time correction
 };
+
+// SEGMENT's start times (asteroid.h:40-46) and the first observation of each
+// segment: read_data.c:244-250 compares the observer's MJD (MJD_obs[i],
+// before the light-time correction) with T_START.  Returns the number of
+// segments opened.
+int segment_first_obs(const std::vector<double>& mjd_raw, int n_seg, const double* T, int* first) {
+  int k = 0;
+  for (size_t i = 0; i < mjd_raw.size(); ++i)
+    if (k < n_seg && mjd_raw[i] >= T[k]) first[k++] = (int)i;
+  return k;
+}
+const double T_START3[3] = {58051.044623, 58053.078872, 58055.234144};
+const double T_START4[4] = {58051.044623, 58053.078872, 58054.093273, 58055.234144};
 
 // 3-point Lagrange interpolation of E and S at time t (misc.c:20-35).
 void quad_interp(const double M[3], const double E[3][3], const double S[3][3], double t, double* e, double* s) {
@@ -180,6 +194,7 @@ Ingested read_data(const char* data_file, bool interp) {
     o.filter = filt[k];
     double delay = E / LIGHT_SPEED;
     o.mjd = mjd_obs[k];
+    D.mjd_raw.push_back(mjd_obs[k]);
     if (filt[k] != W_filter && filt[k] != D_filter) o.mjd = o.mjd - delay;
     if (k == 0) D.mjd0 = o.mjd;
     o.mjd = o.mjd - D.mjd0;
@@ -528,6 +543,13 @@ int main(int argc, char** argv) {
     for (int k = 0; k < 3; ++k)
       fprintf(fd, "# eph %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", D.eph.mjd[k], D.eph.E[0][k], D.eph.E[1][k],
               D.eph.E[2][k], D.eph.S[0][k], D.eph.S[1][k], D.eph.S[2][k]);
+    if (v.n_seg > 1) {  // the segments' first observations
+      int first[4] = {0, 0, 0, 0};
+      const int n = segment_first_obs(D.mjd_raw, v.n_seg, v.n_seg == 3 ? T_START3 : T_START4, first);
+      fprintf(fd, "# seg_first");
+      for (int k = 0; k < n; ++k) fprintf(fd, " %d", first[k]);
+      fprintf(fd, "\n");
+    }
     fclose(fd);
     return 0;
   }
@@ -537,12 +559,15 @@ int main(int argc, char** argv) {
   if (rc == AST_E_NODEVICE) die("No HIP devices!");
   CHECK(rc);
   CHECK(ast_set_model_space(ctx, n_params, property, n_types, types, limits, &x2));
-  if (v.n_seg > 1) {  // the segment start times (asteroid.h:40-46) on the curve's clock
-    static const double T3[3] = {58051.044623, 58053.078872, 58055.234144};
-    static const double T4[4] = {58051.044623, 58053.078872, 58054.093273, 58055.234144};
+  if (v.n_seg > 1) {
+    // the observations split on the observer's clock (read_data.c:244-250),
+    // the plotting grid at the start times on the curve's clock (read_data.c:357)
+    const double* T = v.n_seg == 3 ? T_START3 : T_START4;
     double ts[4];
-    for (int k = 0; k < v.n_seg; ++k) ts[k] = (v.n_seg == 3 ? T3[k] : T4[k]) - D.mjd0;
-    CHECK(ast_set_segments(ctx, ts, v.n_seg));
+    int first[4] = {0, 0, 0, 0};
+    for (int k = 0; k < v.n_seg; ++k) ts[k] = T[k] - D.mjd0;
+    if (segment_first_obs(D.mjd_raw, v.n_seg, T, first) < v.n_seg) die("Error: a data segment has no observations!");
+    CHECK(ast_set_segment_starts(ctx, first, ts, v.n_seg));
   }
   CHECK(ast_upload_curve(ctx, D.obs.data(), N_data, N_filters, &D.eph));
   if (v.nudge) load_minima(ctx, D);

@@ -180,6 +180,10 @@ typedef struct {
   double size_min;     /* SIZE_MIN; 0 = the precision's default (asteroid.h:233-237) */
   int ticks_per_launch;/* evaluation rounds per kernel launch; 0 = automatic   */
   int use_graphs;      /* 1: replay each launch cycle from a captured HIP graph (same results) */
+  int resident_groups; /* nstages = 1 only: run the n_groups*group_size fits on this many workgroups,
+                          each lane starting the next unstarted fit when its fit ends (a device work
+                          queue); 0 = one workgroup per group.  Results, random streams and per-fit
+                          trajectories are the same either way. */
 } ast_run_cfg;
 
 typedef struct ast_ctx ast_ctx;
@@ -231,6 +235,19 @@ int ast_set_minima(ast_ctx* ctx, const float* t_obs, const float* V_obs, int n_o
  * chi2_gpu_rms never loads the segment starts. */
 int ast_set_segments(ast_ctx* ctx, const double* t_start, int n_seg);
 
+/* SEGMENT with the observations' split given by the caller: first_obs[k] is
+ * the index of the first observation of segment k (increasing).  The
+ * reference splits the observations on the observer's clock, before the
+ * light-time correction (read_data.c:244-250: MJD_obs[i] >= T_START[iseg]),
+ * and the plotting grid on the light-time-corrected clock (read_data.c:357:
+ * hPlot.MJD + hMJD0 >= T_START); so an observation within one light time
+ * after a start opens that segment in the reference although its corrected
+ * time is before the start.  t_start (on the curve's clock, as for
+ * ast_set_segments) splits the plotting grid.  A segment whose first
+ * observation is past the table is refused at ast_upload_curve
+ * (AST_E_DATA).  Replaces an earlier ast_set_segments. */
+int ast_set_segment_starts(ast_ctx* ctx, const int* first_obs, const double* t_start, int n_seg);
+
 /* Seed the per-simplex random streams and reset each group's best chi^2 to
  * 1e30: setup_kernel(..., seed, d_f, 1) (cuda.c:2039-2054, asteroid.c:629-634).
  * The project's counter-based generator replaces cuRAND. */
@@ -248,6 +265,23 @@ int ast_set_start(ast_ctx* ctx, const double* params0);
  * groups finish their nstages stages): chi2_gpu (cuda.c:1325-1834) with the
  * launch at asteroid.c:675 and the sync at asteroid.c:688. */
 int ast_run(ast_ctx* ctx, const ast_run_cfg* cfg);
+
+/* ast_run in pieces, for callers that interleave other work or time a run
+ * launch by launch (the reference's host loop, asteroid.c:655-688, launches
+ * and waits on chi2_gpu once per pass).  ast_run_begin uploads the job and
+ * starts its groups; ast_run_continue runs up to max_cycles launch cycles
+ * (0 = until the job ends) and sets *finished once every fit has ended;
+ * ast_run_status reports the fits started and ended so far.  While a run is
+ * unfinished, ast_rng_seed, ast_reset_best, ast_rmsd, ast_run_jobs,
+ * ast_autotune and ast_serve_start return AST_E_STATE; a new ast_run_begin
+ * abandons it.  ast_last_counts and ast_last_timing cover the cycles since
+ * ast_run_begin.  ast_run is ast_run_begin + ast_run_continue(ctx, 0, ...). */
+int ast_run_begin(ast_ctx* ctx, const ast_run_cfg* cfg);
+int ast_run_continue(ast_ctx* ctx, int max_cycles, int* finished);
+int ast_run_status(ast_ctx* ctx, int64_t* fits_started, int64_t* fits_done, int* finished);
+/* End an unfinished run early: its fits stop where they are; per-group
+ * results are those written by groups that had finished. */
+int ast_run_cancel(ast_ctx* ctx);
 
 /* Per-group results of the runs since the last seed/reset: d_f, d_params,
  * d_dV (asteroid.c:728-730).  f has n_groups entries, params
@@ -378,6 +412,8 @@ typedef struct {
   const float* t_minima;
   const float* V_minima;
   const double* t_segments; /* SEGMENT: the variant's n_seg start times, as ast_set_segments (else NULL) */
+  const int* seg_first_obs; /* SEGMENT: first observation of each segment, as ast_set_segment_starts
+                               (NULL: split the observations by t_segments) */
 } ast_job;
 
 typedef struct {

@@ -19,7 +19,8 @@ class OrcProblem(C.Structure):
                 ("x2", abi.X2), ("n_data", C.c_int), ("n_filters", C.c_int),
                 ("obs", C.POINTER(abi.Obs)), ("eph", abi.Ephem3), ("n_min", C.c_int),
                 ("t_min", C.c_float * abi.MAX_MINIMA), ("V_min", C.c_float * abi.MAX_MINIMA),
-                ("t_start", C.c_double * abi.MAX_SEGMENTS)]
+                ("t_start", C.c_double * abi.MAX_SEGMENTS), ("has_seg_first", C.c_int),
+                ("seg_first", C.c_int * abi.MAX_SEGMENTS)]
 
 
 def _has_fma():
@@ -73,10 +74,13 @@ def _p(a, ct=C.c_double):
 class Problem:
     """A model space plus a curve, held in the layout the oracle reads."""
 
-    def __init__(self, space, curve, minima=None, t_start=None):
+    def __init__(self, space, curve, minima=None, t_start=None, seg_first=None):
         """minima = (t_obs, V_obs) for NUDGE variants, t_start = the segment start
-        times for SEGMENT variants (both on the curve's clock)."""
+        times for SEGMENT variants (both on the curve's clock); seg_first = the
+        first observation of each segment, when the observations are split by
+        index rather than by t_start."""
         self.space, self.curve, self.minima, self.t_start = space, curve, minima, t_start
+        self.seg_first = seg_first
         pb = OrcProblem()
         pb.v = space.variant
         pb.n_params = space.n_params
@@ -102,10 +106,14 @@ class Problem:
         if t_start is not None:
             for k, val in enumerate(t_start):
                 pb.t_start[k] = float(val)
+        if seg_first is not None:
+            pb.has_seg_first = 1
+            for k, val in enumerate(seg_first):
+                pb.seg_first[k] = int(val)
         self.pb = pb
 
     def with_space(self, space):
-        return Problem(space, self.curve, self.minima, self.t_start)
+        return Problem(space, self.curve, self.minima, self.t_start, self.seg_first)
 
     def chi2(self, params, form=CANON, want_vmod=False, want_traj=False):
         params = np.ascontiguousarray(np.atleast_2d(params), np.float64)

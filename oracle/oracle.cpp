@@ -1524,6 +1524,9 @@ typedef struct {
   float t_min[AST_MAX_MINIMA];
   float V_min[AST_MAX_MINIMA];
   double t_start[AST_MAX_SEGMENTS]; /* SEGMENT: segment start times on the curve's clock */
+  int has_seg_first;                /* SEGMENT: 1 = the observations split at seg_first[] (the caller's
+                                       split on the observer's clock, read_data.c:244-250), 0 = at t_start */
+  int seg_first[AST_MAX_SEGMENTS];
 } orc_problem;
 
 static int load_problem(const orc_problem* in, orc::Problem* P) {
@@ -1558,8 +1561,16 @@ static int load_problem(const orc_problem* in, orc::Problem* P) {
     if (ns > AST_MAX_SEGMENTS || in->n_params != P->T.n_params) return AST_E_ARG;
     for (int k = 0; k < ns; ++k) P->seg_t[k] = in->t_start[k];
     int iseg = 0;
-    for (int i = 0; i < in->n_data; ++i)  // read_data.c:244-250
-      if (iseg < ns && in->obs[i].mjd >= in->t_start[iseg]) P->seg_start[iseg++] = i;
+    if (in->has_seg_first) {
+      for (int k = 0; k < ns; ++k) {
+        if (in->seg_first[k] < 0 || in->seg_first[k] >= in->n_data || (k > 0 && in->seg_first[k] <= in->seg_first[k - 1]))
+          return AST_E_DATA;
+        P->seg_start[iseg++] = in->seg_first[k];
+      }
+    } else {
+      for (int i = 0; i < in->n_data; ++i)  // read_data.c:244-250
+        if (iseg < ns && in->obs[i].mjd >= in->t_start[iseg]) P->seg_start[iseg++] = i;
+    }
     if (iseg < ns) return AST_E_DATA;  // a segment without an opening observation
     for (int t = 0; t < AST_MAX_TYPES; ++t)
       for (int k = 0; k < AST_MAX_SEGMENTS; ++k) P->seg_types[t][k] = -1;

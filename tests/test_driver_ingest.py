@@ -126,3 +126,29 @@ def test_ingest_rejects_bad_input(tmp_path):
     assert r.returncode == 1 and "require -m" in r.stdout
     r = subprocess.run([EXE, "-i", "lc.txt", "-m"] + ["0.5"] * 11, cwd=tmp_path, stdout=subprocess.PIPE, text=True)
     assert r.returncode == 1 and "-m can only be used together with -reopt or -plot" in r.stdout
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="driver not built")
+def test_segment_split_on_observer_clock(tmp_path):
+    # -DSEGMENT: an observation 1e-5 d after a start time T_START (observer's
+    # clock) opens that segment, as read_data.c:246 compares MJD_obs with
+    # T_START before the light-time correction (about 1e-3 d here) moves its
+    # time before the start
+    sc = _scene(tmp_path, n_obs=30, filters="R")
+    T = np.array([58051.044623, 58053.078872, 58055.234144])
+    t = np.sort(np.concatenate([sc["t"], T + 1e-5]))
+    with open(tmp_path / "lc.txt", "w") as f:
+        for k in range(len(t)):
+            f.write(f"R {t[k]:.8f} 20.0 0.1 x\n")
+    r = subprocess.run([EXE, "-DSEGMENT", "-DN_SEG=3", "-DONE_LE", "-DP_PSI", "-DINTERP", "-i", "lc.txt", "-dump_ingest",
+                        "dump.txt"],
+                       cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout
+    lines = open(tmp_path / "dump.txt").read().splitlines()
+    mjd0 = float(lines[0].split()[2])
+    got = [int(x) for x in [l for l in lines if l.startswith("# seg_first")][0].split()[2:]]
+    want = [int(np.argmax(t >= T[k])) for k in range(3)]
+    assert got == want
+    rel = np.array([float(l.split()[0]) for l in lines if not l.startswith("#")])
+    on_corrected_clock = [int(np.argmax(rel >= T[k] - mjd0)) for k in range(3)]
+    assert all(c == w + 1 for c, w in zip(on_corrected_clock, want))  # the corrected clock would split later

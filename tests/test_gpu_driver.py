@@ -231,13 +231,22 @@ def test_driver_nudge_reads_observed_min(tmp_path):
 
 
 def test_driver_segment_matches_oracle(tmp_path):
-    # -DSEGMENT -DN_SEG=3 -DONE_LE: the curve split at the reference's T_START
-    # times (asteroid.h:40-46) less the first observation's MJD; the scene's
-    # first observations precede the first start and are left out of the sums
-    _scene(tmp_path, n_obs=80, filters="RG")
+    # -DSEGMENT -DN_SEG=3 -DONE_LE: the observations split at the reference's
+    # T_START times (asteroid.h:40-46) on the observer's clock (read_data.c:246),
+    # with one observation 1e-5 d after the second start, which its light-time
+    # correction moves before that start; the scene's first observations
+    # precede the first start and are left out of the sums
+    sc = _scene(tmp_path, n_obs=80, filters="RG")
+    T = np.array([58051.044623, 58053.078872, 58055.234144])
+    t = np.sort(np.concatenate([sc["t"], [T[1] + 1e-5]]))
+    with open(tmp_path / "lc.txt", "w") as f:
+        for k in range(len(t)):
+            f.write(f"{'RG'[k % 2]} {t[k]:.8f} {20.0 + 0.3 * np.sin(7 * k):.5f} 0.1 x\n")
     curve = _ingested(tmp_path)
     mjd0 = float(open(tmp_path / "dump.txt").readline().split()[2])
-    t_start = np.array([58051.044623, 58053.078872, 58055.234144]) - mjd0
+    t_start = T - mjd0
+    seg_first = [int(np.argmax(t >= T[k])) for k in range(3)]
+    assert curve.mjd[seg_first[1]] < t_start[1]  # the observation the observer's clock puts in segment 1
     assert curve.mjd[0] < t_start[0] < curve.mjd[-1]
     cmd = [EXE, "-DSEGMENT", "-DN_SEG=3", "-DONE_LE", "-DP_PSI", "-DINTERP", "-DACC", "-N", "1", "-i", "lc.txt",
            "-o", "out.txt", "-seed", "9", "-blocks", "2", "-bsize", "64", "-steps", "30", "-Ppsi", "2", "4800"]
@@ -245,7 +254,8 @@ def test_driver_segment_matches_oracle(tmp_path):
     assert r.returncode == 0, r.stdout
     v = variants.ls_segments(abi.PREC_FP64)
     space = variants.default_model_space(v, ppsi_hours=(2.0, 4800.0))
-    want = oracle.Problem(space, curve, t_start=t_start).run(2, 64, nstages=1, n_steps=30, seed=9, n_threads=4)
+    want = oracle.Problem(space, curve, t_start=t_start, seg_first=seg_first).run(2, 64, nstages=1, n_steps=30,
+                                                                                 seed=9, n_threads=4)
     best = [n for n in map(_numbers, r.stdout.splitlines()) if n is not None and len(n) == 1 + space.n_params]
     assert len(best) == 1, r.stdout
     g = int(np.argmin(want["f"]))

@@ -135,3 +135,27 @@ def test_segment_model_curve_bitwise(prec, skip):
     assert np.array_equal(t_g, t_w) and np.array_equal(c_g, c_w) and np.array_equal(dv_g, dv_w)
     assert np.all(np.isnan(vm_g[:, :g0])) and np.all(np.isnan(vm_w[:, :g0]))
     assert np.array_equal(vm_g[:, g0:], vm_w[:, g0:])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "f32simplex"])
+def test_segment_starts_by_index_bitwise(prec):
+    # ast_set_segment_starts: the observations split at given indices (here
+    # each segment opening two observations before its start time, as a split
+    # on the observer's clock can), the grid at the start times
+    v, curve, truth, space, t_start = _case(PRECS[prec], skip=5)
+    by_time = [int(np.argmax(curve.mjd >= t)) for t in t_start]
+    first = [by_time[0]] + [k - 2 for k in by_time[1:]]
+    pb = oracle.Problem(space, curve, t_start=t_start, seg_first=first)
+    params = _models(pb, space, truth, n=120)
+    want_c, want_dv = pb.chi2(params, form=oracle.CANON)
+    assert not np.array_equal(want_c, oracle.Problem(space, curve, t_start=t_start).chi2(params)[0])
+    with api.Context(v) as ctx:
+        ctx.set_model_space(space)
+        ctx.set_segment_starts(first, t_start)
+        ctx.upload_curve(curve)
+        got_c, got_dv = ctx.chi2_batch(params)
+        res = ctx.run_jobs([dict(space=space, curve=curve, n_fits=256, nstages=1, n_steps=20, seed=3,
+                                 t_segments=t_start, seg_first_obs=first)])
+    assert np.array_equal(got_c, want_c) and np.array_equal(got_dv, want_dv)
+    want = pb.run(1, 256, nstages=1, n_steps=20, seed=3, n_threads=8)
+    assert res[0]["best_f"] == want["f"][0] and np.array_equal(res[0]["best_params"], want["params"][0])
