@@ -15,6 +15,7 @@ P_TYPE, P_INDEPENDENT, P_FROZEN, P_ISEG, P_MULTI_SEGMENT, P_PERIODIC = range(6)
 N_COLUMNS = 6
 PERIODIC, PERIODIC_LAM, SOFT_BOTH, HARD_LEFT, HARD_RIGHT, HARD_BOTH = 1, 0, -1, -2, -3, -4
 MAX_PARAMS, MAX_TYPES, MAX_FILTERS, MAX_MINIMA, MAX_SEGMENTS = 16, 16, 8, 10, 4
+NOALLOC_MAX_MODELS = 16384
 
 PREC_FP64, PREC_F32SIMPLEX, PREC_MIXED = 0, 1, 2
 
@@ -189,6 +190,7 @@ def load():
         "ast_fetch": (I, [P, pD, pD, pD]),
         "ast_fetch_lanes": (I, [P, pD, pI64, pI64]),
         "ast_chi2_batch": (I, [P, pD, I, pD, pD]),
+        "ast_chi2_batch_noalloc": (I, [P, pD, I, pD, pD]),
         "ast_model_curve": (I, [P, pD, I, C.POINTER(Ephem3), I, pD, pD, pD, pD]),
         "ast_periodogram": (I, [P, pD, pD, I, I, pD, pD, pI]),
         "ast_rmsd": (I, [P, C.POINTER(RmsdCfg), C.POINTER(C.c_float), C.POINTER(C.c_float), pI64, pI64, pD, pD]),
@@ -222,5 +224,5 @@ EXPORTED = (
     "ast_last_hip_error", "ast_serve_start", "ast_submit", "ast_poll", "ast_cancel", "ast_serve_stats",
     "ast_serve_stop", "ast_run_jobs", "ast_autotune", "ast_set_waves", "ast_model_curve", "ast_periodogram",
     "ast_rmsd", "ast_set_minima", "ast_set_segments", "ast_run_begin", "ast_run_continue", "ast_run_status",
-    "ast_set_segment_starts", "ast_run_cancel",
+    "ast_set_segment_starts", "ast_run_cancel", "ast_chi2_batch_noalloc",
 )

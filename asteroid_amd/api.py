@@ -159,6 +159,15 @@ class Context:
                                       ns.ctypes.data_as(C.POINTER(C.c_int64))), "ast_fetch_lanes")
         return f, ne, ns
 
+    def chi2_batch_noalloc(self, params):
+        """chi2_batch through ast_chi2_batch_noalloc (no device allocation)."""
+        params = np.ascontiguousarray(np.atleast_2d(params), np.float64)
+        n = params.shape[0]
+        chi2 = np.zeros(n)
+        dv = np.zeros((n, self.curve.n_filters))
+        _chk(self.lib.ast_chi2_batch_noalloc(self.h, _dp(params), n, _dp(chi2), _dp(dv)), "ast_chi2_batch_noalloc")
+        return chi2, dv
+
     def chi2_batch(self, params):
         p = np.ascontiguousarray(np.atleast_2d(params), np.float64)
         n = p.shape[0]

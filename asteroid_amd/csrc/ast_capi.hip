@@ -58,6 +58,20 @@ using namespace ast;
 
 static thread_local char g_hip_err[256] = "";
 
+// Device memory of ast_chi2_batch_noalloc: the models, chi^2 and delta_V of
+// up to AST_NOALLOC_MAX_MODELS models (3.2 MB), one copy per GPU, held until
+// the process ends.  In host code `scratch` names a host-side stand-in, not
+// the device array: its address on each device comes from
+// hipGetSymbolAddress (scratch_setup, called by ast_ctx_create with the
+// device current) and only that address is passed to kernels and copies.
+constexpr size_t NOALLOC_PARAMS = (size_t)AST_NOALLOC_MAX_MODELS * AST_MAX_PARAMS;
+constexpr size_t NOALLOC_CHI2 = (size_t)AST_NOALLOC_MAX_MODELS;
+constexpr size_t NOALLOC_DV = (size_t)AST_NOALLOC_MAX_MODELS * AST_MAX_FILTERS;
+static __device__ __align__(256) double scratch[NOALLOC_PARAMS + NOALLOC_CHI2 + NOALLOC_DV];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static double* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {nullptr};
+static std::mutex g_scratch_mu;
+
 static void set_err(const char* msg) { snprintf(g_hip_err, sizeof g_hip_err, "%s", msg); }
 
 #define HIPCHK(call)                                                                                   \
@@ -1730,6 +1744,20 @@ const char* ast_strerror(int code) {
 
 const char* ast_last_hip_error(void) { return g_hip_err; }
 
+// The device address of `scratch` on `device` (current), looked up once per
+// device for the process.
+static hipError_t scratch_setup(int device) {
+  if (device < 0 || device >= SCRATCH_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!ptr) return hipErrorInvalidSymbol;
+  g_scratch_ptr[device] = (double*)ptr;
+  return hipSuccess;
+}
+
 int ast_ctx_create(ast_ctx** out, const ast_variant* v, int device) {
   if (!out || !v) return AST_E_ARG;
   *out = nullptr;
@@ -1749,6 +1777,7 @@ int ast_ctx_create(ast_ctx** out, const ast_variant* v, int device) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming);
   }
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = scratch_setup(device);
   if (e != hipSuccess) {
     set_err(hipGetErrorString(e));
     ast_ctx_destroy(c);
@@ -2122,6 +2151,56 @@ int ast_chi2_batch(ast_ctx* c, const double* params, int n, double* chi2, double
   dev_free(cd);
   dev_free(dd);
   return result;
+}
+
+int ast_chi2_batch_noalloc(ast_ctx* c, const double* params, int n, double* chi2, double* dV) {
+  if (!c || !params || n <= 0 || !chi2 || !dV) return AST_E_ARG;
+  if (n > AST_NOALLOC_MAX_MODELS) {
+    snprintf(g_hip_err, sizeof g_hip_err, "ast_chi2_batch_noalloc takes at most %d models per call (got %d)",
+             (int)AST_NOALLOC_MAX_MODELS, n);
+    return AST_E_ARG;
+  }
+  if (c->serve) return AST_E_BUSY;
+  if (!c->space_set || !c->curve_set || (c->v.nudge && !c->minima_set)) return AST_E_STATE;
+  if (c->curve.n_all - c->space.n_params - c->curve.n_filters <= 0) return AST_E_DATA;
+  double* const base = c->device >= 0 && c->device < SCRATCH_MAX_DEVICES ? g_scratch_ptr[c->device] : nullptr;
+  if (!base) return AST_E_STATE;
+  RETIF(set_device(c));
+  double* pd = base;                     // models, n * n_params
+  double* cd = base + NOALLOC_PARAMS;    // chi^2, n
+  double* dd = cd + NOALLOC_CHI2;        // delta_V, n * n_filters
+  const hipStream_t s = c->streams[0];
+  const int N = c->space.n_params, NF = c->curve.n_filters;
+  LaunchArgs a{};
+  a.op = 2;
+  a.stream = s;
+  a.pts = c->pts;
+  a.sum_w = c->sum_w;
+  a.n_data = c->curve.n_data;
+  a.n_filters = NF;
+  a.dof = c->curve.n_all - N - NF;
+  a.params = pd;
+  a.n = n;
+  a.chi2 = cd;
+  a.dV = dd;
+  hipError_t he = hipMemcpyAsync(pd, params, sizeof(double) * (size_t)n * N, hipMemcpyHostToDevice, s);
+  if (he == hipSuccess) he = hipEventRecord(c->ev0[0], s);
+  if (he == hipSuccess) he = dispatch(c->v, a);
+  if (he == hipSuccess) he = hipEventRecord(c->ev1[0], s);
+  if (he == hipSuccess) he = hipMemcpyAsync(chi2, cd, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, s);
+  if (he == hipSuccess) he = hipMemcpyAsync(dV, dd, sizeof(double) * (size_t)n * NF, hipMemcpyDeviceToHost, s);
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  float ms = 0;
+  if (he == hipSuccess) he = hipEventElapsedTime(&ms, c->ev0[0], c->ev1[0]);
+  if (he != hipSuccess) {
+    snprintf(g_hip_err, sizeof g_hip_err, "%s (chi2 batch, no allocation)", hipGetErrorString(he));
+    return AST_E_HIP;
+  }
+  c->last_ms = ms;
+  c->last_launches = 1;
+  c->last_evals = n;
+  c->last_points = (int64_t)n * c->curve.n_data;
+  return AST_OK;
 }
 
 int ast_rmsd(ast_ctx* c, const ast_rmsd_cfg* cfg, float* par_min, float* par_max, int64_t* n_tot, int64_t* n_bad,

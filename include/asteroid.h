@@ -41,7 +41,8 @@ enum {
   AST_HARD_BOTH = -4
 };
 enum { AST_MAX_PARAMS = 16, AST_MAX_TYPES = 16, AST_MAX_FILTERS = 8, AST_MAX_MINIMA = 10 /* NOBS_MAX, asteroid.h:219 */,
-       AST_MAX_SEGMENTS = 4 /* N_SEG, asteroid.h:40-46 */ };
+       AST_MAX_SEGMENTS = 4 /* N_SEG, asteroid.h:40-46 */,
+       AST_NOALLOC_MAX_MODELS = 16384 /* models per ast_chi2_batch_noalloc call */ };
 
 /* Error codes. */
 enum {
@@ -299,6 +300,19 @@ int ast_fetch_lanes(ast_ctx* ctx, double* f_last, int64_t* n_evals, int64_t* n_s
  * delta_V of n parameter vectors (physical units, n*n_params doubles),
  * computed one model per GPU lane.  chi2 has n entries, dV n*n_filters. */
 int ast_chi2_batch(ast_ctx* ctx, const double* params, int n, double* chi2, double* dV);
+
+/* ast_chi2_batch without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The models, chi^2 and delta_V are staged in a fixed device
+ * array of the library (one per GPU, 3.2 MB, held until the process ends)
+ * instead of buffers allocated per call; results are those of
+ * ast_chi2_batch.  At most AST_NOALLOC_MAX_MODELS models per call (more is
+ * AST_E_ARG).  All contexts on one GPU share that array: calls that overlap
+ * on one GPU (from several threads or contexts) race and must be
+ * serialised by the caller.  The context must have its model space and
+ * curve (ast_set_model_space, ast_upload_curve, which do allocate). */
+int ast_chi2_batch_noalloc(ast_ctx* ctx, const double* params, int n, double* chi2, double* dV);
 
 /* The -plot evaluation (chi2_plot, cuda.c:1865-1966, launched at
  * asteroid.c:880; its grid is built at read_data.c:304-360).  For each of n
