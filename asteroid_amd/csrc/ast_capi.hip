@@ -1029,6 +1029,7 @@ struct ast_ctx {
   std::atomic<uint64_t> next_uid{1};
   int waves[MAX_CLASSES];  // fit-kernel register budget per class (4, 3 or 2 waves per SIMD)
   hipEvent_t ev_fork = nullptr, ev_join[NSTREAMS] = {nullptr};  // graph capture fork/join
+  hipEvent_t ev_pre[NSTREAMS] = {nullptr};  // work queued on the class streams before a graph launch
   hipGraphExec_t cg_exec = nullptr;  // the captured cycle
   std::vector<uint64_t> cg_key;
   ast_ctx() { for (int k = 0; k < MAX_CLASSES; ++k) waves[k] = 4; }
@@ -1168,6 +1169,14 @@ int run_cycle_once(ast_ctx* c, std::vector<DevJob*>& jobs, double* kernel_ms, in
     // one graph launch on stream 0; every class stream joins back into it
     RETIF(cycle_graph(c, members, used_s));
     const hipStream_t s0 = c->streams[0];
+    // The graph runs on stream 0, but a job's init kernel and clears were
+    // queued on its class's stream: stream 0 waits for them, or the first
+    // fit launch could read slot state before it is written.
+    for (int k = 1; k < NSTREAMS; ++k)
+      if (used_s[k]) {
+        HIPCHK(hipEventRecord(c->ev_pre[k], c->streams[k]));
+        HIPCHK(hipStreamWaitEvent(s0, c->ev_pre[k], 0));
+      }
     HIPCHK(hipEventRecord(c->ev0[0], s0));
     HIPCHK(hipGraphLaunch(c->cg_exec, s0));
     HIPCHK(hipEventRecord(c->ev1[0], s0));
@@ -1258,6 +1267,9 @@ int ensure_streams(ast_ctx* c, size_t fits, size_t groups) {
     c->best_cap = groups;
     c->best_cf = cf;
   }
+  // the clears and copies above ran on the null stream, which the context's
+  // non-blocking streams do not wait for
+  HIPCHK(hipDeviceSynchronize());
   return 0;
 }
 
@@ -1775,6 +1787,7 @@ int ast_ctx_create(ast_ctx** out, const ast_variant* v, int device) {
     if (e == hipSuccess) e = hipEventCreate(&c->ev0[k]);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1[k]);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_pre[k], hipEventDisableTiming);
   }
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
   if (e == hipSuccess) e = scratch_setup(device);
@@ -1806,6 +1819,7 @@ int ast_ctx_destroy(ast_ctx* c) {
     if (c->ev0[k]) (void)hipEventDestroy(c->ev0[k]);
     if (c->ev1[k]) (void)hipEventDestroy(c->ev1[k]);
     if (c->ev_join[k]) (void)hipEventDestroy(c->ev_join[k]);
+    if (c->ev_pre[k]) (void)hipEventDestroy(c->ev_pre[k]);
   }
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   delete c;
@@ -1896,7 +1910,9 @@ int ast_rng_seed(ast_ctx* c, uint64_t seed) {
   RETIF(set_device(c));
   c->seed = seed;
   if (c->rng) HIPCHK(hipMemset(c->rng, 0, c->rng_cap * sizeof(uint32_t)));
-  return reset_best_f(c);
+  RETIF(reset_best_f(c));
+  HIPCHK(hipDeviceSynchronize());  // before any kernel on the non-blocking streams reads them
+  return AST_OK;
 }
 
 int ast_reset_best(ast_ctx* c) {

@@ -707,9 +707,6 @@ def main_variants(args):
     evals/s and the fit kernel's fraction of the op-mix peak per variant."""
     from asteroid_amd import api, flops, synth, variants
     prec = {"both": MIXED, "mixed": MIXED, "fp64": FP64, "f32simplex": F32SIMPLEX}[args.precision]
-    if prec == MIXED:
-        raise SystemExit("--workload variants in the mixed form is disabled: an earlier run of it left the GPU "
-                         "with a memory fault whose cause is not yet found; use --precision fp64 or f32simplex")
     group = 256
     rows = {}
     names = args.variant_list.split(",") if args.variant_list else list(VARIANT_RUNS)
