@@ -774,13 +774,72 @@ def main_variants(args):
     return out
 
 
+def main_counters(args):
+    """One short pass over every kept kernel, for rocprofv3 counter runs
+    (each PMC pass reruns the program): three 192-round launches of the
+    Stage One fitting kernel over 2^18 resident simplexes (mixed and FP64),
+    a 2^17-model chi^2 batch in each form, 4096 model curves of 20000 grid
+    points, 2^18 RMSD lanes x 16 models (float-simplex and mixed), and a
+    1 GiB device copy whose FETCH_SIZE and WRITE_SIZE calibrate the traffic
+    counters."""
+    from asteroid_amd import api, variants, workloads
+    t0 = time.perf_counter()
+    out = {}
+    for prec in (MIXED, FP64):
+        v, curve, truth, space = workloads.stage_one(prec)
+        with api.Context(v) as ctx:
+            ctx.set_model_space(space)
+            ctx.upload_curve(curve)
+            ctx.set_waves(v, 1, 3)
+            ctx.rng_seed(1)
+            ctx.run_begin(4096, 256, nstages=1, n_steps=5000, eval_budget=args.budget, size_min=SIZE_MIN,
+                          ticks_per_launch=args.ticks, use_graphs=False, resident_groups=1024)
+            ctx.run_continue(3)
+            e, _ = ctx.last_counts()
+            ms, nl = ctx.last_timing()
+            ctx.run_cancel()
+            near = truth * (1.0 + np.random.default_rng(1).uniform(-0.02, 0.02, (1 << 17, truth.size)))
+            ctx.chi2_batch(near)
+            bms, _ = ctx.last_timing()
+        out[PREC_NAMES[prec]] = {"fit_evals": e, "fit_ms": ms, "fit_launches": nl, "chi2_batch_ms": bms}
+        log(f"{PREC_NAMES[prec]}: fit {nl} launches {ms:.1f} ms, chi2 batch {bms:.1f} ms")
+    v, curve, truth, space = workloads.stage_one(F32SIMPLEX)
+    with api.Context(v) as ctx:
+        ctx.set_model_space(space)
+        ctx.upload_curve(curve)
+        ctx.model_curve(truth * (1.0 + np.random.default_rng(5).uniform(-0.001, 0.001, (4096, truth.size))),
+                        n_plot=20000)
+        out["model_curve_ms"], _ = ctx.last_timing()
+    for prec in (F32SIMPLEX, MIXED):
+        v, curve, truth, _ = workloads.stage_one(prec)
+        sp = variants.default_model_space(v, ppsi_hours=(2.0, 4800.0), reopt=1)
+        with api.Context(v) as ctx:
+            ctx.set_model_space(sp)
+            ctx.upload_curve(curve)
+            ctx.rng_seed(21)
+            ctx.set_start(truth)
+            r = ctx.rmsd(1024, 256, 16, args.dx)
+            out[f"rmsd_{PREC_NAMES[prec]}_ms"], _ = ctx.last_timing()
+            out[f"rmsd_{PREC_NAMES[prec]}_models"] = r["n_tot"]
+    import torch  # a plain device copy of known size for the traffic counters
+    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    out["copy_bytes"] = 1 << 30
+    line = base_line(None, 1, 0, None, "mixed", {"workload": "counters: one pass over every kept kernel"})
+    line.update(out)
+    line["bench_wall_s"] = round(time.perf_counter() - t0, 1)
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["stage1", "small", "reopt", "batched", "serving", "plot", "rmsd",
-                                           "variants"],
+                                           "variants", "counters"],
                     default="stage1")
     ap.add_argument("--plot-models", type=int, default=4096)
     ap.add_argument("--variant-list", default="", help="comma-separated subset of the variants workload")
@@ -810,7 +869,8 @@ def main():
               "without communication; run it with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     fn = {"stage1": main_stage_one, "small": main_small, "reopt": main_reopt, "batched": main_batched,
-          "serving": main_serving, "plot": main_plot, "rmsd": main_rmsd, "variants": main_variants}[args.workload]
+          "serving": main_serving, "plot": main_plot, "rmsd": main_rmsd, "variants": main_variants,
+          "counters": main_counters}[args.workload]
     print(json.dumps(fn(args)), flush=True)
 
 
