@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(LIB_DIR, "libasteroid_hip.so")
 P_TYPE, P_INDEPENDENT, P_FROZEN, P_ISEG, P_MULTI_SEGMENT, P_PERIODIC = range(6)
 N_COLUMNS = 6
 PERIODIC, PERIODIC_LAM, SOFT_BOTH, HARD_LEFT, HARD_RIGHT, HARD_BOTH = 1, 0, -1, -2, -3, -4
-MAX_PARAMS, MAX_TYPES, MAX_FILTERS, MAX_MINIMA, MAX_SEGMENTS = 16, 16, 8, 10, 4
+MAX_PARAMS, MAX_TYPES, MAX_FILTERS, MAX_MINIMA, MAX_SEGMENTS = 38, 16, 8, 10, 4
 NOALLOC_MAX_MODELS = 16384
 
 PREC_FP64, PREC_F32SIMPLEX, PREC_MIXED = 0, 1, 2
@@ -61,7 +61,7 @@ class RunCfg(C.Structure):
 
 class ServeCfg(C.Structure):
     _fields_ = [("max_resident_fits", C.c_int64), ("group_size", C.c_int), ("hbm_budget_bytes", C.c_int64),
-                ("work_points_per_launch", C.c_double)]
+                ("work_points_per_launch", C.c_double), ("max_params", C.c_int)]
 
 
 class Job(C.Structure):

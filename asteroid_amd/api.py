@@ -297,9 +297,10 @@ class Context:
         _chk(self.lib.ast_set_waves(self.h, C.byref(variant), int(n_filters), int(waves)), "ast_set_waves")
 
     # ------------------------------------------------------------- serving
-    def serve_start(self, max_resident_fits, group_size=256, hbm_budget_bytes=0, work_points_per_launch=0.0):
+    def serve_start(self, max_resident_fits, group_size=256, hbm_budget_bytes=0, work_points_per_launch=0.0,
+                    max_params=0):
         cfg = abi.ServeCfg(int(max_resident_fits), int(group_size), int(hbm_budget_bytes),
-                           float(work_points_per_launch))
+                           float(work_points_per_launch), int(max_params))
         _chk(self.lib.ast_serve_start(self.h, C.byref(cfg)), "ast_serve_start")
         self._keep = {}
 

@@ -51,7 +51,14 @@ extern template hipError_t launch_variant<VF_RECT>(const LaunchArgs&);
 extern template hipError_t launch_variant<VF_TQ | VF_RECT>(const LaunchArgs&);
 extern template hipError_t launch_variant<VF_TQ | VF_NUDGE>(const LaunchArgs&);
 extern template hipError_t launch_variant<VF_TQ | VF_BC | VF_NUDGE>(const LaunchArgs&);
-extern template hipError_t launch_variant<VF_SEG3>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_SEG>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_SEG | VF_SEG4>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_SEG | VF_SEGLE>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_SEG | VF_SEG4 | VF_SEGLE>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_TQ | VF_SEG>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_TQ | VF_SEG | VF_SEG4>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_TQ | VF_SEG | VF_SEGLE>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_TQ | VF_SEG | VF_SEG4 | VF_SEGLE>(const LaunchArgs&);
 }  // namespace ast
 
 using namespace ast;
@@ -59,7 +66,7 @@ using namespace ast;
 static thread_local char g_hip_err[256] = "";
 
 // Device memory of ast_chi2_batch_noalloc: the models, chi^2 and delta_V of
-// up to AST_NOALLOC_MAX_MODELS models (3.2 MB), one copy per GPU, held until
+// up to AST_NOALLOC_MAX_MODELS models (6.2 MB), one copy per GPU, held until
 // the process ends.  In host code `scratch` names a host-side stand-in, not
 // the device array: its address on each device comes from
 // hipGetSymbolAddress (scratch_setup, called by ast_ctx_create with the
@@ -114,7 +121,9 @@ void dev_free(T*& p) {
 constexpr int VARIANT_CODES[] = {0, VF_TQ, VF_BC, VF_TQ | VF_BC, VF_BW, VF_TQ | VF_BW,
                                  VF_TQ | VF_TREND, VF_TQ2, VF_BC | VF_ROT, VF_TQ | VF_BC | VF_ROT,
                                  VF_RECT, VF_TQ | VF_RECT, VF_TQ | VF_NUDGE, VF_TQ | VF_BC | VF_NUDGE,
-                                 VF_SEG3};
+                                 VF_SEG, VF_SEG | VF_SEG4, VF_SEG | VF_SEGLE, VF_SEG | VF_SEG4 | VF_SEGLE,
+                                 VF_TQ | VF_SEG, VF_TQ | VF_SEG | VF_SEG4, VF_TQ | VF_SEG | VF_SEGLE,
+                                 VF_TQ | VF_SEG | VF_SEG4 | VF_SEGLE};
 constexpr int N_VARIANTS = (int)(sizeof VARIANT_CODES / sizeof VARIANT_CODES[0]);
 
 // Variant code of the macros (TORQUE2 implies TORQUE, RECT implies BC).
@@ -122,7 +131,7 @@ int variant_code(const ast_variant& v) {
   const bool t2 = v.torque2, tq = v.torque || t2, rc = v.rect, bc = v.bc || rc;
   return (tq && !t2 ? VF_TQ : 0) | (t2 ? VF_TQ2 : 0) | (bc && !rc ? VF_BC : 0) | (v.bw_ball ? VF_BW : 0) |
          (v.trend ? VF_TREND : 0) | (v.rotate ? VF_ROT : 0) | (rc ? VF_RECT : 0) | (v.nudge ? VF_NUDGE : 0) |
-         (v.n_seg > 1 ? VF_SEG3 : 0);
+         (v.n_seg > 1 ? VF_SEG | (v.n_seg == 4 ? VF_SEG4 : 0) | (v.one_le ? 0 : VF_SEGLE) : 0);
 }
 
 // Index of the variant's kernel family, or -1 if it is not built.
@@ -139,8 +148,11 @@ bool variant_supported(const ast_variant& v) {
   // mixed form's 1e-6 chi^2 tolerance cannot hold for it; the reference's
   // Stage Three runs -DACC (README.md:89-93).
   if (v.nudge && v.precision == AST_PREC_MIXED) return false;
-  // SEGMENT is built for three segments with L and Es shared (16 parameters)
-  if (v.n_seg < 0 || v.n_seg > AST_MAX_SEGMENTS || (v.n_seg > 1 && (v.n_seg != 3 || !v.one_le))) return false;
+  // SEGMENT is built for three or four segments (the reference's T_START
+  // tables, asteroid.h:40-46), ONE_LE or not, on the LS ellipsoid with or
+  // without constant torque
+  if (v.n_seg < 0 || v.n_seg > AST_MAX_SEGMENTS || v.n_seg == 2) return false;
+  if (v.n_seg > 1 && (v.bc || v.bw_ball || v.trend || v.torque2 || v.rotate || v.rect || v.nudge)) return false;
   if (v.precision != AST_PREC_FP64 && v.precision != AST_PREC_F32SIMPLEX && v.precision != AST_PREC_MIXED)
     return false;
   if (!(v.time_step > 0)) return false;
@@ -178,15 +190,26 @@ int make_space(const ast_variant& v, int n_params, const int* property, int n_ty
     if (fr < -1 || fr > 1) return AST_E_ARG;
     pos++;
   }
-  // SEGMENT: theta_M, phi_M, phi_0, psi_0 of segments 1 and 2 (asteroid.c:128-150)
-  for (int k = 1; v.n_seg > 1 && k < v.n_seg; ++k) {
-    const int seg_types[] = {S.T.theta_M, S.T.phi_M, S.T.phi_0, S.T.psi_0};
-    for (int t : seg_types) {
-      const int* r = property + pos * AST_N_COLUMNS;
-      if (r[AST_P_TYPE] != t || r[AST_P_ISEG] != k || r[AST_P_MULTI_SEGMENT] != 0) return AST_E_UNSUPPORTED;
-      if (r[AST_P_FROZEN] < -1 || r[AST_P_FROZEN] > 1) return AST_E_ARG;
-      pos++;
+  // SEGMENT (asteroid.c:128-150): the first segment's shape parameters (and
+  // under ONE_LE its Es and L) are multi-segment, the rest repeat for each
+  // further segment in the same order
+  if (v.n_seg > 1) {
+    std::vector<int> rep;
+    for (int i = 0; i < pos; ++i) {
+      const int t = property[i * AST_N_COLUMNS + AST_P_TYPE];
+      const bool shared = t == S.T.c_tumb || t == S.T.b_tumb || (v.one_le && (t == S.T.Es || t == S.T.L));
+      if (property[i * AST_N_COLUMNS + AST_P_MULTI_SEGMENT] != (shared ? 1 : 0)) return AST_E_UNSUPPORTED;
+      if (property[i * AST_N_COLUMNS + AST_P_ISEG] != 0) return AST_E_UNSUPPORTED;
+      if (!shared) rep.push_back(t);
     }
+    for (int k = 1; k < v.n_seg; ++k)
+      for (int t : rep) {
+        if (pos >= n_params) return AST_E_UNSUPPORTED;
+        const int* r = property + pos * AST_N_COLUMNS;
+        if (r[AST_P_TYPE] != t || r[AST_P_ISEG] != k || r[AST_P_MULTI_SEGMENT] != 0) return AST_E_UNSUPPORTED;
+        if (r[AST_P_FROZEN] < -1 || r[AST_P_FROZEN] > 1) return AST_E_ARG;
+        pos++;
+      }
   }
   if (pos != n_params) return AST_E_UNSUPPORTED;
   for (int i = 0; i < n_params; ++i)
@@ -451,7 +474,14 @@ hipError_t dispatch(const ast_variant& v, LaunchArgs& a) {
     case VF_TQ | VF_RECT: return launch_variant<VF_TQ | VF_RECT>(a);
     case VF_TQ | VF_NUDGE: return launch_variant<VF_TQ | VF_NUDGE>(a);
     case VF_TQ | VF_BC | VF_NUDGE: return launch_variant<VF_TQ | VF_BC | VF_NUDGE>(a);
-    case VF_SEG3: return launch_variant<VF_SEG3>(a);
+    case VF_SEG: return launch_variant<VF_SEG>(a);
+    case VF_SEG | VF_SEG4: return launch_variant<VF_SEG | VF_SEG4>(a);
+    case VF_SEG | VF_SEGLE: return launch_variant<VF_SEG | VF_SEGLE>(a);
+    case VF_SEG | VF_SEG4 | VF_SEGLE: return launch_variant<VF_SEG | VF_SEG4 | VF_SEGLE>(a);
+    case VF_TQ | VF_SEG: return launch_variant<VF_TQ | VF_SEG>(a);
+    case VF_TQ | VF_SEG | VF_SEG4: return launch_variant<VF_TQ | VF_SEG | VF_SEG4>(a);
+    case VF_TQ | VF_SEG | VF_SEGLE: return launch_variant<VF_TQ | VF_SEG | VF_SEGLE>(a);
+    case VF_TQ | VF_SEG | VF_SEG4 | VF_SEGLE: return launch_variant<VF_TQ | VF_SEG | VF_SEG4 | VF_SEGLE>(a);
     default: return hipErrorInvalidValue;
   }
 }
@@ -459,17 +489,20 @@ hipError_t dispatch(const ast_variant& v, LaunchArgs& a) {
 // ------------------------------------------------------------- slot pool
 // Device state of up to `nf` resident simplexes.  Element storage is 8 bytes
 // wide so that float and double simplexes share the pool; arrays have room
-// for AST_MAX_PARAMS parameters.
+// for P parameters, the largest parameter count of what the pool was sized
+// for (a kernel class packs its rows by its own count, so a class of N
+// parameters touches (N + 1) N simplex rows whatever P is).
 constexpr int MAX_CLASSES = N_VARIANTS * 3 * 3;  // variants x 3 filter capacities x 3 precision forms
 constexpr int NSTREAMS = 4;      // classes share streams round-robin
-constexpr size_t SLOT_BYTES = 8 * ((AST_MAX_PARAMS + 1) * AST_MAX_PARAMS + (AST_MAX_PARAMS + 1) +
-                                   4 * AST_MAX_PARAMS + AST_MAX_FILTERS + 2) +
-                              (AST_MAX_PARAMS + 1) + 4 * 7;
+constexpr size_t slot_bytes(size_t P) {
+  return 8 * ((P + 1) * P + (P + 1) + 4 * P + AST_MAX_FILTERS + 2) + (P + 1) + 4 * 7;
+}
 
 struct Pool {
   size_t nf = 0;        // fits
   int G = 0;            // group size the slots are cut for
   int n_slots = 0;      // nf / G
+  int P = 0;            // parameter capacity of a slot
   void *X = nullptr, *F = nullptr, *X0 = nullptr, *XR = nullptr, *XP = nullptr, *DV = nullptr;
   void *fr = nullptr, *sf = nullptr;
   uint8_t* IND = nullptr;
@@ -490,12 +523,14 @@ void pool_free(Pool& p) {
   dev_free(p.g_done); dev_free(p.job_done);
   p.nf = 0;
   p.n_slots = 0;
+  p.P = 0;
   p.free_slots.clear();
 }
 
-int pool_alloc(Pool& p, size_t nf, int G) {
+int pool_alloc(Pool& p, size_t nf, int G, int n_params) {
   pool_free(p);
-  const size_t P = AST_MAX_PARAMS;
+  if (n_params <= 0 || n_params > AST_MAX_PARAMS) return AST_E_ARG;
+  const size_t P = (size_t)n_params;
   int rc = 0;
 #define ALLOC(ptr, type, count) if (!rc) rc = dev_alloc((type**)&(ptr), (count))
   ALLOC(p.X, double, (P + 1) * P * nf);
@@ -523,6 +558,7 @@ int pool_alloc(Pool& p, size_t nf, int G) {
   if (rc) { pool_free(p); return rc; }
   p.nf = nf;
   p.G = G;
+  p.P = n_params;
   p.n_slots = (int)(nf / G);
   p.job_done_h.assign((size_t)p.n_slots, 0);
   p.free_slots.clear();
@@ -624,7 +660,7 @@ double eval_cost(const ast_variant& v, const CurveH& C) {
 size_t job_bytes(int n_groups, int group_size, int n_workers, int n_params, int n_filters, size_t n_records) {
   const size_t fits = (size_t)n_groups * group_size;
   const bool refill = n_workers < n_groups;
-  return (size_t)n_workers * group_size * SLOT_BYTES + fits * (4 + 8 + 8 + (refill ? 8 * (n_params + n_filters) : 0)) +
+  return (size_t)n_workers * group_size * slot_bytes((size_t)n_params) + fits * (4 + 8 + 8 + (refill ? 8 * (n_params + n_filters) : 0)) +
          (size_t)n_groups * 256 + sizeof(PointRec) * n_records;
 }
 
@@ -753,7 +789,7 @@ bool job_fits_pool(const DevJob& j, const Pool& pool) {
   if (j.n_workers <= 0 || j.n_workers > j.rp.n_groups) return false;
   if ((int)j.slots.size() != j.n_workers || !j.descs || !j.fit_f || !j.fit_nev || !j.fit_steps) return false;
   if (j.refill && (j.rp.nstages != 1 || !j.queue || !j.fit_params || !j.fit_dV)) return false;
-  if (j.space.n_params > AST_MAX_PARAMS || j.n_filters > AST_MAX_FILTERS || j.n_filters <= 0) return false;
+  if (j.space.n_params > pool.P || j.n_filters > AST_MAX_FILTERS || j.n_filters <= 0) return false;
   for (int s : j.slots)
     if (s < 0 || s >= pool.n_slots) return false;
   return (size_t)pool.n_slots * (size_t)pool.G <= pool.nf;
@@ -1392,13 +1428,16 @@ int batch_run(ast_ctx* c, const std::vector<const ast_job*>& jobs, int G, double
   std::vector<CurveH> curves(n_jobs);
   std::vector<std::vector<double>> p0(n_jobs);
   size_t groups = 0;
+  int n_params = 1;
   for (int i = 0; i < n_jobs; ++i) {
     RETIF(make_job(c, jobs[i], G, &J[i], &curves[i], &p0[i]));
     J[i]->id = (uint64_t)i;
     groups += (size_t)J[i]->rp.n_groups;
+    n_params = std::max(n_params, J[i]->space.n_params);
   }
   RETIF(set_device(c));
-  if ((size_t)c->pool.n_slots < groups || c->pool.G != G) RETIF(pool_alloc(c->pool, groups * G, G));
+  if ((size_t)c->pool.n_slots < groups || c->pool.G != G || c->pool.P < n_params)
+    RETIF(pool_alloc(c->pool, groups * G, G, n_params));
   c->last_groups = 0;  // the pool's rows no longer hold an ast_run's lanes
   // Work of one simplex of each job: an upper bound of its evaluations (the
   // budget, or the step limit) times the cost of one.
@@ -1946,7 +1985,8 @@ int ast_run_begin(ast_ctx* c, const ast_run_cfg* rc) {
   c->last_groups = 0;
   const size_t fits = (size_t)rc->n_groups * G, lanes = (size_t)workers * G;
   RETIF(ensure_streams(c, fits, rc->n_groups));
-  if (c->pool.nf < lanes || c->pool.G != G) RETIF(pool_alloc(c->pool, lanes, G));
+  if (c->pool.nf < lanes || c->pool.G != G || c->pool.P < c->space.n_params)
+    RETIF(pool_alloc(c->pool, lanes, G, c->space.n_params));
   // The reference path is one job that borrows the context's streams and bests.
   if (c->ref_job) job_free(*c->ref_job);
   c->ref_job.reset(new DevJob());
@@ -2352,10 +2392,12 @@ int ast_serve_start(ast_ctx* c, const ast_serve_cfg* cfg) {
   if (!c || !cfg) return AST_E_ARG;
   if (c->serve || c->run_active) return AST_E_STATE;
   const int G = cfg->group_size ? cfg->group_size : 256;
-  if (G < 64 || G > 1024 || (G & 63) || cfg->max_resident_fits < G) return AST_E_ARG;
+  const int P = cfg->max_params ? cfg->max_params : AST_MAX_PARAMS;
+  if (G < 64 || G > 1024 || (G & 63) || cfg->max_resident_fits < G || P < 0 || P > AST_MAX_PARAMS)
+    return AST_E_ARG;
   RETIF(set_device(c));
   size_t fits = (size_t)(cfg->max_resident_fits / G) * G;
-  RETIF(pool_alloc(c->pool, fits, G));
+  RETIF(pool_alloc(c->pool, fits, G, P));
   c->serve.reset(new Serve());
   c->serve->cfg = *cfg;
   c->serve->cfg.group_size = G;
@@ -2377,6 +2419,10 @@ int ast_submit(ast_ctx* c, const ast_job* job, uint64_t* id) {
   if (j->rp.nstages != 1 && j->rp.n_groups > c->pool.n_slots) {
     set_err("a multi-stage job has more groups than the pool has slots");
     return AST_E_NOMEM;
+  }
+  if (j->space.n_params > c->pool.P) {
+    set_err("the job has more parameters than the serving slots hold (ast_serve_cfg.max_params)");
+    return AST_E_ARG;
   }
   std::lock_guard<std::mutex> lk(S.mu);
   j->id = S.next_id++;

@@ -59,8 +59,11 @@ static_assert(sizeof(TailRec) == sizeof(PointRec), "the tail record takes one Po
 // Model variant as one compile-time code: the reference's macros
 // (makefile:3-36) as bit flags.  TORQUE2 implies TORQUE and RECT implies BC
 // (asteroid.h:32-38).
+// VF_SEG is SEGMENT (cuda.c:162-813) with three segments and ONE_LE; VF_SEG4
+// makes it four segments, VF_SEGLE gives every segment its own L and Es (no
+// ONE_LE).
 enum : int { VF_TQ = 1, VF_BC = 2, VF_BW = 4, VF_TREND = 8, VF_TQ2 = 16, VF_ROT = 32, VF_RECT = 64, VF_NUDGE = 128,
-             VF_SEG3 = 256 };
+             VF_SEG = 256, VF_SEG4 = 512, VF_SEGLE = 1024 };
 template <int VC>
 struct VarF {
   static constexpr bool TQ = (VC & (VF_TQ | VF_TQ2)) != 0;  // constant torque
@@ -71,7 +74,9 @@ struct VarF {
   static constexpr bool RO = (VC & VF_ROT) != 0;            // brightness frame rotated
   static constexpr bool RC = (VC & VF_RECT) != 0;           // rectangular prism brightness
   static constexpr bool NU = (VC & VF_NUDGE) != 0;          // model minima nudged to observed ones
-  static constexpr bool SG = (VC & VF_SEG3) != 0;           // SEGMENT, three segments, L and Es shared
+  static constexpr bool SG = (VC & VF_SEG) != 0;            // SEGMENT: the curve split into segments
+  static constexpr int NSEG = SG ? ((VC & VF_SEG4) ? 4 : 3) : 1;
+  static constexpr bool ONE_LE = SG && !(VC & VF_SEGLE);    // L and Es shared by the segments
 };
 #define AST_VFLAGS(VC)                                                                           \
   constexpr bool TQ = VarF<VC>::TQ, BC = VarF<VC>::BC, BW = VarF<VC>::BW, T2 = VarF<VC>::T2,   \
@@ -95,13 +100,33 @@ struct Pos {
   static constexpr int theta_R = (F::BW || F::RO) ? 8 + ob : -1, phi_R = (F::BW || F::RO) ? 9 + ob : -1;
   static constexpr int psi_R = F::RO ? 10 + ob : -1, kappa = F::BW ? 10 + ob : -1;
   static constexpr int N0 = 8 + o + (F::BC ? 2 : 0) + ((F::BW || F::RO) ? 3 : 0);
-  // SEGMENT (LS, ONE_LE, three segments): segments 1 and 2 repeat theta_M,
-  // phi_M, phi_0 and psi_0 after the first segment's parameters
-  // (asteroid.c:128-150); everything else is shared.
-  static_assert(!F::SG || VC == VF_SEG3, "SEGMENT is built for LS without torque only");
-  static constexpr int seg(int k, int j) { return k == 0 ? (j == 3 ? 7 : j) : N0 + 4 * (k - 1) + j; }
-  static constexpr bool is_psi0(int i) { return i == psi_0 || (F::SG && (i == seg(1, 3) || i == seg(2, 3))); }
-  static constexpr int N = N0 + (F::SG ? 8 : 0);
+  // SEGMENT (asteroid.c:128-150, asteroid.h:104-108): after the first
+  // segment's parameters, each further segment repeats, in the same order,
+  // those that are not multi-segment: theta_M, phi_M, phi_0, the torque
+  // components, Es and L (unless ONE_LE) and psi_0; c_tumb and b_tumb are
+  // shared.  rep(r) is the first-segment position of the r-th repeated
+  // parameter, base(i) the first-segment position of parameter i (its role)
+  // and segof(i) its segment.
+  static_assert(!F::SG || (VC & ~(VF_SEG | VF_SEG4 | VF_SEGLE | VF_TQ)) == 0,
+                "SEGMENT is built for the LS ellipsoid with or without torque");
+  static constexpr int NSEG = F::NSEG;
+  static constexpr int R = F::SG ? 4 + (F::TQ ? 3 : 0) + (F::ONE_LE ? 0 : 2) : 0;
+  static constexpr int rep(int r) {
+    return r < 3 ? r : (F::TQ && r < 6) ? r : (!F::ONE_LE && r == R - 3) ? Es : (!F::ONE_LE && r == R - 2) ? L : psi_0;
+  }
+  static constexpr int rank(int bp) {
+    for (int r = 0; r < R; ++r)
+      if (rep(r) == bp) return r;
+    return -1;
+  }
+  static constexpr int base(int i) { return i < N0 ? i : rep((i - N0) % R); }
+  static constexpr int segof(int i) { return i < N0 ? 0 : (i - N0) / R + 1; }
+  static constexpr int seg(int k, int bp) { return (k == 0 || rank(bp) < 0) ? bp : N0 + (k - 1) * R + rank(bp); }
+  static constexpr bool is_Es(int i) { return base(i) == Es; }
+  static constexpr bool is_L(int i) { return base(i) == L; }
+  static constexpr bool is_psi0(int i) { return base(i) == psi_0; }
+  static constexpr int N = N0 + R * (NSEG - 1);
+  static_assert(N <= MAXP && N < 64, "parameter count beyond the slot layout (and the 6-bit vertex index)");
 };
 
 // Model space by parameter position (the reference indexes limits by type;
@@ -140,7 +165,7 @@ __device__ int x2params(const CF* x, double* params, const Space<CF>& sp, int re
   int LAM = 0;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    if (i == P::Es) LAM = x[i] >= 0.5;
+    if (P::is_Es(i)) LAM = x[i] >= 0.5;
     int per = sp.periodic[i];
     if ((x[i] < 0.0 && hard_left(per, LAM)) || (x[i] > 1.0 && hard_right(per, LAM))) return 1;
   }
@@ -155,7 +180,7 @@ __device__ int x2params(const CF* x, double* params, const Space<CF>& sp, int re
       double c_tumb = params[P::c_tumb];
       Is = (1.0 + b_tumb * b_tumb) / (b_tumb * b_tumb + c_tumb * c_tumb);
       Ii = (1.0 + c_tumb * c_tumb) / (b_tumb * b_tumb + c_tumb * c_tumb);
-    } else if (i == P::Es) {
+    } else if (P::is_Es(i)) {
       if (LAM) {
         params[i] = 2.0 * (x[i] - 0.5) * (1.0 - 1.0 / Ii) + 1.0 / Ii;
         psi_min = 0.0;
@@ -174,7 +199,7 @@ __device__ int x2params(const CF* x, double* params, const Space<CF>& sp, int re
     } else if (sp.periodic[i] == AST_PERIODIC) {
       params[i] = x[i] * 2.0 * PI;
     } else if (sp.independent[i] == 1) {
-      if (!sp.p_psi || i != P::L || reopt) params[i] = x[i] * (L1 - L0) + L0;
+      if (!sp.p_psi || !P::is_L(i) || reopt) params[i] = x[i] * (L1 - L0) + L0;
       if (i == P::c_tumb) {
         log_c_tumb = params[i];
         params[i] = astm::exp(log_c_tumb);
@@ -184,10 +209,10 @@ __device__ int x2params(const CF* x, double* params, const Space<CF>& sp, int re
         params[i] = astm::exp(log_c);
       } else if (BW && i == P::kappa) {
         params[i] = astm::exp(params[i]);
-      } else if (i == P::L && !reopt && sp.p_psi) {
+      } else if (P::is_L(i) && !reopt && sp.p_psi) {
         float dP = sp.ppsi2 - sp.ppsi1;
         params[i] = x[i] * (CF)dP + (CF)sp.ppsi1;
-        double EsV = params[P::Es];
+        double EsV = params[P::seg(P::segof(i), P::Es)];  // the segment's Es (cuda.c:1263)
         double Einv = 1.0 / EsV;
         double k2;
         if (LAM) k2 = (Is - Ii) * (Einv - 1.0) / ((Ii - 1.0) * (Is - Einv));
@@ -205,6 +230,22 @@ __device__ int x2params(const CF* x, double* params, const Space<CF>& sp, int re
     }
   }
   return 0;
+}
+
+// Segment builds call the map through one out-of-line copy.  With it inlined
+// at every simplex proposal site, the builds with torque and per-segment L
+// and Es (29 and 38 parameters) gave fit trajectories that left the CPU
+// restatement's at some lanes (hipcc of ROCm 7.2 for gfx950, at -O1, -O2 and
+// -O3 alike), while the same map evaluated on its own matched it bitwise; the
+// out-of-line copy matches bit for bit.  Other builds keep it inlined.
+template <class CF, int VC>
+__device__ __attribute__((noinline)) int x2params_ool(const CF* x, double* params, const Space<CF>& sp, int reopt) {
+  return x2params<CF, VC>(x, params, sp, reopt);
+}
+template <class CF, int VC>
+__device__ inline int x2params_at(const CF* x, double* params, const Space<CF>& sp, int reopt) {
+  if constexpr (VarF<VC>::SG) return x2params_ool<CF, VC>(x, params, sp, reopt);
+  else return x2params<CF, VC>(x, params, sp, reopt);
 }
 
 // ------------------------------------------------------------------ params2x
@@ -241,7 +282,7 @@ __device__ void params2x(CF* x, const double* params, const Space<CF>& sp) {
       if (i == P::b_tumb) {
         double par = astm::log(b_tumb) / astm::log(c_tumb);
         x[i] = (par - L0) / (L1 - L0);
-      } else if (i == P::Es) {
+      } else if (P::is_Es(i)) {
         i_Es = i;
         LAM = params[i] > 1.0 / Ii;
         if (LAM) x[i] = 0.5 * ((params[i] - 1.0 / Ii) / (1.0 - 1.0 / Ii) + 1.0);
@@ -672,6 +713,77 @@ __device__ inline double ls_tail_mx(const ScaledDirs& ud, float bcp) {
   return M2P5_LOG10_2 * log2_f32x((((bcp * sS) * sE) * isc) * inner);
 }
 
+// Parameter BP (a first-segment position) of segment k: its own copy if it
+// is one that each segment repeats, the shared one otherwise.
+template <int VC, int BP>
+AST_AI double segval(const double* params, int k) {
+  using P = Pos<VC>;
+  if constexpr (P::NSEG == 1 || P::rank(BP) < 0) {
+    return params[BP];
+  } else {
+    double v = params[P::seg(0, BP)];
+    if (k == 1) v = params[P::seg(1, BP)];
+    if (k == 2) v = params[P::seg(2, BP)];
+    if constexpr (P::NSEG > 3)
+      if (k == 3) v = params[P::seg(3, BP)];
+    return v;
+  }
+}
+
+// The rotation at the start of segment k (k = 0: the start of the model;
+// cuda.c:190-252): the spin frame M, XM, YM (XM_y = 0) from theta_M and
+// phi_M, theta from Es and psi_0, the initial Euler angles (and, with
+// torque, the angular velocity from L) and the ODE constants.
+template <int VC>
+AST_AI void begin_rotation(const double* params, int k, double Is_inv, double Ii_inv, double Is, double Ii,
+                           SpinFrame<double>& Fd, double* y, double* mu, double& mu02, double* sc) {
+  using P = Pos<VC>;
+  constexpr bool TQ = VarF<VC>::TQ;
+  double sTM, cTM, sPM, cPM;
+  astm::sincos(segval<VC, P::theta_M>(params, k), &sTM, &cTM);
+  astm::sincos(segval<VC, P::phi_M>(params, k), &sPM, &cPM);
+  Fd.M_x = sTM * cPM;
+  Fd.M_y = sTM * sPM;
+  Fd.M_z = cTM;
+  const double XM = __builtin_sqrt(__builtin_fma(Fd.M_x, Fd.M_x, Fd.M_z * Fd.M_z));
+  Fd.XM_x = Fd.M_z / XM;
+  Fd.XM_z = -Fd.M_x / XM;
+  Fd.YM_x = Fd.M_y * Fd.XM_z;
+  Fd.YM_y = __builtin_fma(-Fd.M_x, Fd.XM_z, Fd.M_z * Fd.XM_x);
+  Fd.YM_z = -(Fd.M_y * Fd.XM_x);
+  const double Es = segval<VC, P::Es>(params, k), psi0 = segval<VC, P::psi_0>(params, k);
+  const double L = segval<VC, P::L>(params, k);
+  double sP0, cP0;
+  astm::sincos(psi0, &sP0, &cP0);
+  const double theta = astm::asin(__builtin_sqrt((Es - 1.0) / (__builtin_fma(sP0 * sP0, Ii_inv - Is_inv, Is_inv) - 1.0)));
+  const double phi = segval<VC, P::phi_0>(params, k);
+  astm::sincos(theta, &sc[0], &sc[1]);
+  sc[2] = sP0;
+  sc[3] = cP0;
+  if constexpr (TQ) {
+    mu[0] = (Is - 1.0) * Ii_inv;
+    mu[1] = (1.0 - Ii) * Is_inv;
+    mu[2] = Ii - Is;
+    mu[3] = segval<VC, P::Ti>(params, k);
+    mu[4] = segval<VC, P::Ts>(params, k);
+    mu[5] = segval<VC, P::Tl>(params, k);
+    y[0] = ((L * Ii_inv) * sc[0]) * sP0;
+    y[1] = ((L * Is_inv) * sc[0]) * cP0;
+    y[2] = L * sc[1];
+    y[3] = phi;
+    y[4] = theta;
+    y[5] = psi0;
+  } else {
+    mu[0] = L;
+    mu[1] = 0.5 * (Ii_inv + Is_inv);
+    mu[2] = 0.5 * (Ii_inv - Is_inv);
+    mu02 = mu[0] * mu[2];
+    y[0] = phi;
+    y[1] = theta;
+    y[2] = psi0;
+  }
+}
+
 // chi^2 of one model (chi2one, cuda.c:81-1011, in-scope branches), summed
 // per filter; NF is the compile-time filter capacity (>= n_filters).  With MX
 // (the mixed-precision form) the Euler angles and angular velocity are
@@ -694,63 +806,22 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
   double sy2[NF], sy[NF];
 #pragma unroll
   for (int m = 0; m < NF; ++m) { sy2[m] = 0.0; sy[m] = 0.0; }
-  double sTM, cTM, sPM, cPM;
-  astm::sincos(params[P::theta_M], &sTM, &cTM);
-  astm::sincos(params[P::phi_M], &sPM, &cPM);
-  SpinFrame<double> Fd;
-  Fd.M_x = sTM * cPM;
-  Fd.M_y = sTM * sPM;
-  Fd.M_z = cTM;
-  const double XM = __builtin_sqrt(__builtin_fma(Fd.M_x, Fd.M_x, Fd.M_z * Fd.M_z));
-  Fd.XM_x = Fd.M_z / XM;
-  Fd.XM_z = -Fd.M_x / XM;
-  Fd.YM_x = Fd.M_y * Fd.XM_z;
-  Fd.YM_y = __builtin_fma(-Fd.M_x, Fd.XM_z, Fd.M_z * Fd.XM_x);
-  Fd.YM_z = -(Fd.M_y * Fd.XM_x);
   const double bt = params[P::b_tumb], ct = params[P::c_tumb];
   const double den = __builtin_fma(ct, ct, bt * bt);
   const double Is = __builtin_fma(bt, bt, 1.0) / den;
   const double Ii = __builtin_fma(ct, ct, 1.0) / den;
   const double Is_inv = 1.0 / Is;
   const double Ii_inv = 1.0 / Ii;
-  const double Es = params[P::Es], psi0 = params[P::psi_0], L = params[P::L];
-  double sP0, cP0;
-  astm::sincos(psi0, &sP0, &cP0);
-  double theta = astm::asin(__builtin_sqrt((Es - 1.0) / (__builtin_fma(sP0 * sP0, Ii_inv - Is_inv, Is_inv) - 1.0)));
-  double phi = params[P::phi_0];
-  double psi = psi0;
   constexpr int NODE = TQ ? 6 : 3;
+  SpinFrame<double> Fd;
   double mu[6];
   double mu02 = 0.0;
   double y[6];
   // sin/cos of the current theta and psi, shared by the next RK4 step's
   // first stage and the body axes at the observation.
   double sc[4];
-  astm::sincos(theta, &sc[0], &sc[1]);
-  sc[2] = sP0;
-  sc[3] = cP0;
-  if (TQ) {
-    mu[0] = (Is - 1.0) * Ii_inv;
-    mu[1] = (1.0 - Ii) * Is_inv;
-    mu[2] = Ii - Is;
-    mu[3] = params[P::Ti];
-    mu[4] = params[P::Ts];
-    mu[5] = params[P::Tl];
-    y[0] = ((L * Ii_inv) * sc[0]) * sP0;
-    y[1] = ((L * Is_inv) * sc[0]) * cP0;
-    y[2] = L * sc[1];
-    y[3] = phi;
-    y[4] = theta;
-    y[5] = psi;
-  } else {
-    mu[0] = L;
-    mu[1] = 0.5 * (Ii_inv + Is_inv);
-    mu[2] = 0.5 * (Ii_inv - Is_inv);
-    mu02 = mu[0] * mu[2];
-    y[0] = phi;
-    y[1] = theta;
-    y[2] = psi;
-  }
+  begin_rotation<VC>(params, 0, Is_inv, Ii_inv, Is, Ii, Fd, y, mu, mu02, sc);
+  double phi, theta, psi;
   ModelConsts K{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   if constexpr (RO) {
     astm::sincos(params[P::phi_R], &K.sR, &K.cR);
@@ -796,34 +867,10 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
     if constexpr (SG) {
       // The first point of segment k (its record's step count is -1 - k;
       // a table that opens with segment 0 has no marker there): the
-      // rotation starts again from the segment's own Euler angles
-      // (cuda.c:163-252 inside the segment loop); L, Es and the shape are
-      // shared, so mu and the brightness constants stay.
-      if (nsteps < 0) {
-        const bool k0 = nsteps == -1, k1 = nsteps == -2;
-        const double tMk = k0 ? params[P::seg(0, 0)] : k1 ? params[P::seg(1, 0)] : params[P::seg(2, 0)];
-        const double pMk = k0 ? params[P::seg(0, 1)] : k1 ? params[P::seg(1, 1)] : params[P::seg(2, 1)];
-        const double p0k = k0 ? params[P::seg(0, 2)] : k1 ? params[P::seg(1, 2)] : params[P::seg(2, 2)];
-        const double s0k = k0 ? params[P::seg(0, 3)] : k1 ? params[P::seg(1, 3)] : params[P::seg(2, 3)];
-        astm::sincos(tMk, &sTM, &cTM);
-        astm::sincos(pMk, &sPM, &cPM);
-        Fd.M_x = sTM * cPM;
-        Fd.M_y = sTM * sPM;
-        Fd.M_z = cTM;
-        const double XMk = __builtin_sqrt(__builtin_fma(Fd.M_x, Fd.M_x, Fd.M_z * Fd.M_z));
-        Fd.XM_x = Fd.M_z / XMk;
-        Fd.XM_z = -Fd.M_x / XMk;
-        Fd.YM_x = Fd.M_y * Fd.XM_z;
-        Fd.YM_y = __builtin_fma(-Fd.M_x, Fd.XM_z, Fd.M_z * Fd.XM_x);
-        Fd.YM_z = -(Fd.M_y * Fd.XM_x);
-        astm::sincos(s0k, &sP0, &cP0);
-        y[1] = astm::asin(__builtin_sqrt((Es - 1.0) / (__builtin_fma(sP0 * sP0, Ii_inv - Is_inv, Is_inv) - 1.0)));
-        y[0] = p0k;
-        y[2] = s0k;
-        astm::sincos(y[1], &sc[0], &sc[1]);
-        sc[2] = sP0;
-        sc[3] = cP0;
-      }
+      // rotation starts again from the segment's own parameters (cuda.c:163-252
+      // inside the segment loop); the shape, and so the brightness
+      // constants, are shared.
+      if (nsteps < 0) begin_rotation<VC>(params, -1 - nsteps, Is_inv, Ii_inv, Is, Ii, Fd, y, mu, mu02, sc);
     }
     int nsub = 1;
     double t2_end = 0.0;

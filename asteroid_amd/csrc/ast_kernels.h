@@ -61,7 +61,7 @@ struct SlotBufs {
   CF* XP;       // [N]        expanded / contracted point
   double* PRM;  // [N]        the thread's params[] buffer (cuda.c:1344)
   CF* DV;       // [MAXF]     delta_V of the last evaluation
-  int32_t* ctl;       // phase | j<<4 | failed<<9 | bad<<10
+  int32_t* ctl;       // phase | j<<4 | failed<<10 | bad<<11 (j <= N <= 63)
   int32_t* lstep;     // simplex steps in this stage (l, cuda.c:1436)
   int32_t* nev;       // chi^2 evaluations in this stage
   int32_t* nev_tot;   // evaluations, all stages of this job
@@ -157,7 +157,7 @@ struct Machine {
     double prm[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) prm[i] = PRM(i);
-    int rc = x2params<CF, VC>(xv, prm, sp, reopt);
+    int rc = x2params_at<CF, VC>(xv, prm, sp, reopt);
 #pragma unroll
     for (int i = 0; i < N; ++i) PRM(i) = prm[i];
     return rc == 0;
@@ -209,7 +209,7 @@ struct Machine {
         xi = xmin + r * (xmax - xmin);
       }
       x0v[i] = xi;
-      if (i == P::Es) LAM = xi >= 0.5;
+      if (P::is_Es(i)) LAM = xi >= 0.5;
     }
 #pragma unroll
     for (int i = 0; i < N; ++i) X(0, i) = x0v[i];
@@ -254,7 +254,7 @@ struct Machine {
       const int b = IND(0);
 #pragma unroll
       for (int i = 0; i < N; ++i) { xb[i] = X(b, i); prm[i] = PRM(i); }
-      x2params<CF, VC>(xb, prm, sp, reopt);
+      x2params_at<CF, VC>(xb, prm, sp, reopt);
 #pragma unroll
       for (int i = 0; i < N; ++i) D.fit_params[jfit * N + i] = prm[i];
     }
@@ -628,9 +628,9 @@ void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
   LaneRegs<CF> R;
   const int ctl = B.ctl[fit];
   R.phase = ctl & 15;
-  R.j = (ctl >> 4) & 31;
-  R.failed = (ctl >> 9) & 1;
-  R.bad = (ctl >> 10) & 1;
+  R.j = (ctl >> 4) & 63;
+  R.failed = (ctl >> 10) & 1;
+  R.bad = (ctl >> 11) & 1;
   R.l = B.lstep[fit];
   R.nev = B.nev[fit];
   R.nev_tot = B.nev_tot[fit];
@@ -701,7 +701,7 @@ void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
           double prm[N];
 #pragma unroll
           for (int i = 0; i < N; ++i) prm[i] = M.PRM(i);
-          x2params<CF, VC>(xb, prm, sp, reopt);
+          x2params_at<CF, VC>(xb, prm, sp, reopt);
           params2x<CF, VC>(xb, prm, sp);
 #pragma unroll
           for (int i = 0; i < N; ++i) { M.PRM(i) = prm[i]; M.X(b, i) = xb[i]; }
@@ -726,7 +726,7 @@ void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
         const int b = M.IND(0);
 #pragma unroll
         for (int i = 0; i < N; ++i) { xb[i] = M.X(b, i); prm[i] = M.PRM(i); }
-        x2params<CF, VC>(xb, prm, sp, reopt);
+        x2params_at<CF, VC>(xb, prm, sp, reopt);
 #pragma unroll
         for (int i = 0; i < N; ++i) {
           M.PRM(i) = prm[i];
@@ -743,7 +743,7 @@ void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
       done = true;
     }
   }
-  B.ctl[fit] = R.phase | (R.j << 4) | (R.failed << 9) | (R.bad << 10);
+  B.ctl[fit] = R.phase | (R.j << 4) | (R.failed << 10) | (R.bad << 11);
   B.lstep[fit] = R.l;
   B.nev[fit] = R.nev;
   B.nev_tot[fit] = R.nev_tot;
@@ -906,9 +906,9 @@ __global__ __launch_bounds__(W == 4 ? 1024 : 256) __attribute__((amdgpu_waves_pe
       if (xx < SMALL && hard_left(per, LAM)) xx = (CF)SMALL;
       else if (xx > 1.0 - SMALL && hard_right(per, LAM)) xx = (CF)(1.0 - SMALL);
       x1[i] = xx;
-      if (i == P::Es) LAM = x1[i] >= (CF)0.5;
+      if (P::is_Es(i)) LAM = x1[i] >= (CF)0.5;
     }
-    if (x2params<CF, VC>(x1, prm, sp, A.reopt)) continue;
+    if (x2params_at<CF, VC>(x1, prm, sp, A.reopt)) continue;
     CF dv[NF];
     const CF f = chi2_model<CF, VC, NF, MX, FORM_RMSD>(prm, (PointPtr)(uintptr_t)A.pts, A.n_data,
                                                                A.n_filters, A.dof, A.sum_w, dv);

@@ -40,7 +40,7 @@ enum {
   AST_HARD_RIGHT = -3,
   AST_HARD_BOTH = -4
 };
-enum { AST_MAX_PARAMS = 16, AST_MAX_TYPES = 16, AST_MAX_FILTERS = 8, AST_MAX_MINIMA = 10 /* NOBS_MAX, asteroid.h:219 */,
+enum { AST_MAX_PARAMS = 38 /* SEGMENT with torque, four segments, no ONE_LE (asteroid.h:104-106) */, AST_MAX_TYPES = 16, AST_MAX_FILTERS = 8, AST_MAX_MINIMA = 10 /* NOBS_MAX, asteroid.h:219 */,
        AST_MAX_SEGMENTS = 4 /* N_SEG, asteroid.h:40-46 */,
        AST_NOALLOC_MAX_MODELS = 16384 /* models per ast_chi2_batch_noalloc call */ };
 
@@ -70,9 +70,10 @@ enum {
  * the reference (asteroid.h:32-38).  The kernels are built for LS, BC and
  * BW_BALL with and without torque, and for TREND with torque, TORQUE2,
  * ROTATE (with BC, with and without torque), RECT (with and without torque)
- * NUDGE (with torque, with and without BC) and SEGMENT (LS without torque,
- * three segments, ONE_LE: 16 parameters); other combinations return
- * AST_E_UNSUPPORTED.  P_PHI and P_BOTH are not built. */
+ * NUDGE (with torque, with and without BC) and SEGMENT (LS with and without
+ * torque, three or four segments, with and without ONE_LE: 16 to 38
+ * parameters); other combinations return AST_E_UNSUPPORTED.  P_PHI and
+ * P_BOTH are not built. */
 typedef struct {
   int torque;     /* -DTORQUE : 6-ODE constant-torque rotation                 */
   int bc;         /* -DBC     : photometric b,c independent of kinematic ones  */
@@ -305,7 +306,7 @@ int ast_chi2_batch(ast_ctx* ctx, const double* params, int n, double* chi2, doub
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  The models, chi^2 and delta_V are staged in a fixed device
- * array of the library (one per GPU, 3.2 MB, held until the process ends)
+ * array of the library (one per GPU, 6.2 MB, held until the process ends)
  * instead of buffers allocated per call; results are those of
  * ast_chi2_batch.  At most AST_NOALLOC_MAX_MODELS models per call (more is
  * AST_E_ARG).  All contexts on one GPU share that array: calls that overlap
@@ -400,6 +401,8 @@ typedef struct {
   int64_t hbm_budget_bytes;      /* admission limit on resident job bytes; 0 = slots only */
   double work_points_per_launch; /* evaluation work per simplex per launch, in observation-
                                     equivalents (an RK4 step counts 1.4-2.8); 0 = 16000 */
+  int max_params;                /* parameters a slot holds (the largest job's count);
+                                    0 = AST_MAX_PARAMS; larger jobs are refused (AST_E_ARG) */
 } ast_serve_cfg;
 
 typedef struct {

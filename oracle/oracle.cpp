@@ -906,7 +906,8 @@ int x2params(const Problem& P, const CF* x, double* params, int reopt) {
       } else if (t == T.L && !reopt && P.v.p_psi) {
         float dP = P.x2.ppsi2 - P.x2.ppsi1;
         params[i] = x[i] * (CF)dP + (CF)P.x2.ppsi1;
-        double EsV = params[P.types[T.Es]];
+        // the Es of L's own segment, params[sTypes[T_Es][iseg]] (cuda.c:1263)
+        double EsV = params[P.n_seg > 1 ? P.seg_types[T.Es][P.prop[i][AST_P_ISEG]] : P.types[T.Es]];
         double Einv = 1.0 / EsV;
         double k2;
         if (LAM) k2 = (Is - Ii) * (Einv - 1.0) / ((Ii - 1.0) * (Is - Einv));

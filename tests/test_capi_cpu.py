@@ -40,8 +40,10 @@ def test_strerror_and_validation_without_device():
     assert lib.ast_ctx_create(C.byref(h), C.byref(bad), 0) == -6
     bad = variants.ls_torque_nudge(abi.PREC_MIXED)  # NUDGE is offered in the FP64 and float-simplex forms
     assert lib.ast_ctx_create(C.byref(h), C.byref(bad), 0) == -6
-    for bad in (variants.ls_segments(one_le=False), variants.ls_segments(n_seg=4),  # beyond 16 parameters
-                variants.make_variant(True, False, False, n_seg=3, one_le=True)):   # SEGMENT only without torque
+    for bad in (variants.ls_segments(n_seg=2),                                  # T_START has 3 or 4 entries
+                variants.make_variant(True, True, False, n_seg=3, one_le=True),    # SEGMENT on the LS ellipsoid only
+                variants.make_variant(True, False, True, n_seg=4),
+                variants.make_variant(True, False, False, torque2=True, n_seg=3)):
         assert lib.ast_ctx_create(C.byref(h), C.byref(bad), 0) == -6
     assert lib.ast_set_minima(None, None, None, 1) == -1
     assert lib.ast_set_segments(None, None, 3) == -1
@@ -53,7 +55,9 @@ VARS = [variants.ls_torque(), variants.bc_free(), variants.bw_ball_torque(),
         variants.make_variant(False, False, False, True, True), variants.make_variant(True, True, False, True, True),
         variants.make_variant(False, False, True, False, False), variants.ls_torque_trend(), variants.ls_torque2(),
         variants.bc_rotate(), variants.bc_rotate(torque=True), variants.rect(), variants.rect(torque=True),
-        variants.ls_torque_nudge(), variants.ls_torque_nudge(bc=True), variants.ls_segments()]
+        variants.ls_torque_nudge(), variants.ls_torque_nudge(bc=True), variants.ls_segments()] + [
+        variants.ls_segments(n_seg=n, one_le=le, torque=tq) for n in (3, 4) for le in (True, False) for tq in (False, True)
+        if (n, le, tq) != (3, True, False)]
 
 
 @pytest.mark.parametrize("v", VARS)
@@ -96,6 +100,33 @@ def test_header_type_ids_match_python(tmp_path):
         ids = abi.type_ids(v)
         got = [int(x) for x in rows[m].split()]
         assert got == [ids[f] for f, _ in abi.TypeIds._fields_], VARIANT_FLAGS[m]
+
+
+STRUCTS = [("ast_variant", abi.Variant), ("ast_type_ids", abi.TypeIds), ("ast_obs", abi.Obs),
+           ("ast_ephem3", abi.Ephem3), ("ast_x2", abi.X2), ("ast_run_cfg", abi.RunCfg),
+           ("ast_rmsd_cfg", abi.RmsdCfg), ("ast_serve_cfg", abi.ServeCfg), ("ast_job", abi.Job),
+           ("ast_result", abi.Result), ("ast_serve_stats_t", abi.ServeStats), ("ast_batch_cfg", abi.BatchCfg),
+           ("ast_tune_entry", abi.TuneEntry)]
+
+
+def test_struct_layouts_match_header(tmp_path):
+    # every ctypes mirror has the size and field offsets the C compiler gives the header's struct
+    lines = []
+    for cname, py in STRUCTS:
+        lines.append(' printf("%%zu", sizeof(%s));' % cname)
+        for f in py._fields_:
+            lines.append(' printf(" %%zu", offsetof(%s, %s));' % (cname, f[0]))
+        lines.append(' printf("\\n");')
+    src = tmp_path / "lay.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "asteroid.h"\nint main(void){\n%s\nreturn 0; }\n'
+                   % "\n".join(lines))
+    exe = tmp_path / "lay"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    rows = subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True, check=True).stdout.split("\n")
+    for (cname, py), row in zip(STRUCTS, rows):
+        got = [int(x) for x in row.split()]
+        want = [C.sizeof(py)] + [getattr(py, f[0]).offset for f in py._fields_]
+        assert got == want, cname
 
 
 def test_no_device_is_reported():

@@ -158,3 +158,107 @@ def test_plot_curve_restarts_at_each_segment():
     assert not np.allclose(vm[0][g[1]:] - dv[0][0], vm8[0][g[1]:] - dv8[0][0])
     c_obs, dv_obs = pb.chi2(same)
     assert c[0] == c_obs[0] and np.array_equal(dv[0], dv_obs[0])
+
+
+SEG_BUILDS = [(n, le, tq) for n in (3, 4) for le in (True, False) for tq in (False, True)]
+
+
+def _ids(b):
+    return f"seg{b[0]}{'_onele' if b[1] else ''}{'_torque' if b[2] else ''}"
+
+
+def _build(n_seg, one_le, torque, seed=7, n=120, span=4.0):
+    v0 = variants.make_variant(torque, False, False, True, True)
+    vs = variants.ls_segments(n_seg=n_seg, one_le=one_le, torque=torque)
+    curve, truth0 = synth.make_curve(seed, v0, n, span, n_filters=2)
+    t_start = curve.mjd[0] - 0.05 + np.arange(n_seg) * span / n_seg
+    pos0, pos = variants.param_positions(v0), variants.param_positions(vs)
+    rng = np.random.default_rng(seed)
+    params = np.zeros(len(pos))
+    for name, i in pos.items():
+        b = name.split(":")[0]
+        params[i] = truth0[pos0[b]]
+        if ":" in name:  # each further segment its own rotation (and L, Es)
+            params[i] *= 1.0 + rng.uniform(-0.02, 0.02) if b in ("L", "Es", "Ti", "Ts", "Tl") else 1.0
+            if b not in ("L", "Es", "Ti", "Ts", "Tl"):
+                params[i] += rng.uniform(-0.3, 0.3)
+    return v0, vs, curve, params, t_start
+
+
+@pytest.mark.parametrize("build", SEG_BUILDS, ids=_ids)
+def test_every_segment_build_is_plain_models_on_their_observations(build):
+    n_seg, one_le, torque = build
+    v0, vs, curve, params, t_start = _build(n_seg, one_le, torque)
+    space = variants.default_model_space(vs)
+    assert space.n_params == 8 + 3 * torque + (6 + 3 * torque - 2 * one_le) * (n_seg - 1)  # asteroid.h:104-106
+    pb = oracle.Problem(space, curve, t_start=t_start)
+    starts = _starts(curve.mjd, t_start)
+    assert len(starts) == n_seg
+    pos0, pos = variants.param_positions(v0), variants.param_positions(vs)
+    space0 = variants.default_model_space(v0)
+    vm = np.zeros(curve.n_data)
+    for k, i1 in enumerate(starts):
+        i2 = starts[k + 1] if k + 1 < n_seg else curve.n_data
+        p0 = np.array([params[pos[n if k == 0 or variants._shared(vs, n) else f"{n}:{k}"]] for n in pos0])
+        _, _, v = oracle.Problem(space0, _sub(curve, i1, i2)).chi2(p0, form=oracle.CANON, want_vmod=True)
+        vm[i1:i2] = v[0]
+    c, dv, got = pb.chi2(params, form=oracle.CANON, want_vmod=True)
+    assert np.array_equal(got[0][starts[0]:], vm[starts[0]:])
+    # chi^2 and delta_V: the weighted least squares of those magnitudes, with
+    # every observation in the degrees of freedom
+    sel = np.arange(curve.n_data) >= starts[0]
+    chi2, dvs = 0.0, []
+    for m in range(curve.n_filters):
+        f = sel & (curve.filter == m)
+        y, w = curve.V[f].astype(float) - vm[f], curve.w[f].astype(float)
+        d = np.sum(w * y) / np.sum(w)
+        dvs.append(d)
+        chi2 += np.sum(w * (y - d) ** 2)
+    chi2 /= curve.n_data - space.n_params - curve.n_filters
+    assert c[0] == pytest.approx(chi2, rel=1e-10) and np.allclose(dv[0], dvs, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("build", SEG_BUILDS, ids=_ids)
+def test_segment_x2params_maps_each_segment_like_the_plain_model(build):
+    # with every Es on one side of 1/2, segment k's parameters are the plain
+    # model's map of segment k's coordinates (L from 1/P_psi with the
+    # segment's own Es, cuda.c:1263)
+    n_seg, one_le, torque = build
+    v0, vs, curve, params, t_start = _build(n_seg, one_le, torque)
+    space, space0 = variants.default_model_space(vs), variants.default_model_space(v0)
+    pb = oracle.Problem(space, curve, t_start=t_start)
+    pb0 = oracle.Problem(space0, curve)
+    pos0, pos = variants.param_positions(v0), variants.param_positions(vs)
+    rng = np.random.default_rng(n_seg * 10 + one_le * 2 + torque)
+    for side in (0.25, 0.75):
+        x = rng.uniform(0.05, 0.95, (50, space.n_params))
+        for name, i in pos.items():
+            if name.split(":")[0] == "Es":
+                x[:, i] = side + rng.uniform(-0.2, 0.2, 50)
+        p, rc = pb.x2params(x, reopt=0)
+        for k in range(n_seg):
+            idx = [pos[n if k == 0 or variants._shared(vs, n) else f"{n}:{k}"] for n in pos0]
+            p0, rc0 = pb0.x2params(x[:, idx], reopt=0)
+            ok = (rc == 0) & (rc0 == 0)
+            assert ok.sum() > 20
+            assert np.array_equal(p[ok][:, idx], p0[ok])
+
+
+def test_segment_lam_is_the_last_segments():
+    # the reference's x2params takes LAM from the last Es in the parameter
+    # order (its first loop, cuda.c:1128-1147) and maps every segment's Es
+    # with it (cuda.c:1170-1184): a SAM coordinate of the first segment is
+    # mapped on the LAM branch when the last segment's Es is LAM
+    v0, vs, curve, params, t_start = _build(3, False, False)
+    space = variants.default_model_space(vs)
+    pb = oracle.Problem(space, curve, t_start=t_start)
+    pos = variants.param_positions(vs)
+    x = np.full(space.n_params, 0.5)
+    x[pos["c_tumb"]], x[pos["b_tumb"]] = 0.4, 0.6
+    x[pos["Es"]], x[pos["Es:1"]], x[pos["Es:2"]] = 0.2, 0.3, 0.8
+    p, rc = pb.x2params(x, reopt=1)
+    assert rc[0] == 0
+    c, b = p[0, pos["c_tumb"]], p[0, pos["b_tumb"]]
+    Ii = (1 + c * c) / (b * b + c * c)
+    for name, xe in (("Es", 0.2), ("Es:1", 0.3), ("Es:2", 0.8)):
+        assert p[0, pos[name]] == pytest.approx(2 * (xe - 0.5) * (1 - 1 / Ii) + 1 / Ii, rel=1e-14)
