@@ -201,6 +201,7 @@ def load():
         "ast_autotune": (I, [P, C.POINTER(Job), I, C.POINTER(BatchCfg), I, C.POINTER(TuneEntry), I, pI]),
         "ast_set_waves": (I, [P, C.POINTER(Variant), I, I]),
         "ast_serve_start": (I, [P, C.POINTER(ServeCfg)]),
+        "ast_copy_bandwidth": (I, [P, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
         "ast_submit": (I, [P, C.POINTER(Job), C.POINTER(C.c_uint64)]),
         "ast_poll": (I, [P, C.POINTER(Result), I, pI, I]),
         "ast_cancel": (I, [P, C.c_uint64]),
@@ -224,5 +225,5 @@ EXPORTED = (
     "ast_last_hip_error", "ast_serve_start", "ast_submit", "ast_poll", "ast_cancel", "ast_serve_stats",
     "ast_serve_stop", "ast_run_jobs", "ast_autotune", "ast_set_waves", "ast_model_curve", "ast_periodogram",
     "ast_rmsd", "ast_set_minima", "ast_set_segments", "ast_run_begin", "ast_run_continue", "ast_run_status",
-    "ast_set_segment_starts", "ast_run_cancel", "ast_chi2_batch_noalloc",
+    "ast_set_segment_starts", "ast_run_cancel", "ast_chi2_batch_noalloc", "ast_copy_bandwidth",
 )

@@ -296,6 +296,12 @@ class Context:
     def set_waves(self, variant, n_filters, waves):
         _chk(self.lib.ast_set_waves(self.h, C.byref(variant), int(n_filters), int(waves)), "ast_set_waves")
 
+    def copy_bandwidth(self, n_bytes, reps=1):
+        """Bytes read plus written per second of a plain device copy of n_bytes."""
+        out = C.c_double()
+        _chk(self.lib.ast_copy_bandwidth(self.h, int(n_bytes), int(reps), C.byref(out)), "ast_copy_bandwidth")
+        return out.value
+
     # ------------------------------------------------------------- serving
     def serve_start(self, max_resident_fits, group_size=256, hbm_budget_bytes=0, work_points_per_launch=0.0,
                     max_params=0):

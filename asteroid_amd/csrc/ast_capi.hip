@@ -2209,6 +2209,52 @@ int ast_chi2_batch(ast_ctx* c, const double* params, int n, double* chi2, double
   return result;
 }
 
+// A plain streaming copy of n 16-byte words (one load and one store per
+// word, grid-stride) and a fill, for ast_copy_bandwidth.
+__global__ void __launch_bounds__(256) copy_words_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                         size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+__global__ void __launch_bounds__(256) fill_words_kernel(uint4* __restrict__ dst, size_t n, uint32_t v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = make_uint4(v, v ^ 1u, v ^ 2u, v ^ 3u);
+}
+
+int ast_copy_bandwidth(ast_ctx* c, int64_t bytes, int reps, double* bytes_per_s) {
+  if (!c || bytes < 16 || reps <= 0 || !bytes_per_s) return AST_E_ARG;
+  if (c->serve) return AST_E_BUSY;  // its streams and events belong to the scheduler
+  RETIF(set_device(c));
+  const size_t n = (size_t)bytes / 16;
+  uint4 *src = nullptr, *dst = nullptr;
+  int rc = dev_alloc(&src, n);
+  if (!rc) rc = dev_alloc(&dst, n);
+  if (rc) { dev_free(src); dev_free(dst); return rc; }
+  hipStream_t s = c->streams[0];
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(fill_words_kernel, dim3(blocks), dim3(256), 0, s, src, n, 0x5a5a5a5au);
+  hipError_t he = hipGetLastError();
+  if (he == hipSuccess) he = hipEventRecord(c->ev0[0], s);
+  for (int r = 0; r < reps && he == hipSuccess; ++r) {
+    hipLaunchKernelGGL(copy_words_kernel, dim3(blocks), dim3(256), 0, s, src, dst, n);
+    he = hipGetLastError();
+  }
+  if (he == hipSuccess) he = hipEventRecord(c->ev1[0], s);
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  float ms = 0;
+  if (he == hipSuccess) he = hipEventElapsedTime(&ms, c->ev0[0], c->ev1[0]);
+  dev_free(src);
+  dev_free(dst);
+  if (he != hipSuccess) {
+    snprintf(g_hip_err, sizeof g_hip_err, "%s (copy bandwidth)", hipGetErrorString(he));
+    return AST_E_HIP;
+  }
+  c->last_ms = ms;
+  c->last_launches = reps;
+  *bytes_per_s = ms > 0 ? 2.0 * 16.0 * (double)n * reps / (ms * 1e-3) : 0.0;
+  return AST_OK;
+}
+
 int ast_chi2_batch_noalloc(ast_ctx* c, const double* params, int n, double* chi2, double* dV) {
   if (!c || !params || n <= 0 || !chi2 || !dV) return AST_E_ARG;
   if (n > AST_NOALLOC_MAX_MODELS) {

@@ -780,8 +780,8 @@ def main_counters(args):
     Stage One fitting kernel over 2^18 resident simplexes (mixed and FP64),
     a 2^17-model chi^2 batch in each form, 4096 model curves of 20000 grid
     points, 2^18 RMSD lanes x 16 models (float-simplex and mixed), and a
-    1 GiB device copy whose FETCH_SIZE and WRITE_SIZE calibrate the traffic
-    counters."""
+    plain copy kernel reading and writing 1 GiB whose FETCH_SIZE and
+    WRITE_SIZE calibrate the traffic counters."""
     from asteroid_amd import api, variants, workloads
     t0 = time.perf_counter()
     out = {}
@@ -821,12 +821,11 @@ def main_counters(args):
             r = ctx.rmsd(1024, 256, 16, args.dx)
             out[f"rmsd_{PREC_NAMES[prec]}_ms"], _ = ctx.last_timing()
             out[f"rmsd_{PREC_NAMES[prec]}_models"] = r["n_tot"]
-    import torch  # a plain device copy of known size for the traffic counters
-    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    b.copy_(a)
-    torch.cuda.synchronize()
-    out["copy_bytes"] = 1 << 30
+    # A kernel of known traffic for the traffic counters: the library's plain
+    # copy of 1 GiB (1 GiB read and 1 GiB written per launch).
+    with api.Context(variants.ls_torque(MIXED)) as ctx:
+        out["copy_bytes_per_s"] = ctx.copy_bandwidth(1 << 30, 1)
+    out["copy_kernel_bytes_read"] = out["copy_kernel_bytes_written"] = 1 << 30
     line = base_line(None, 1, 0, None, "mixed", {"workload": "counters: one pass over every kept kernel"})
     line.update(out)
     line["bench_wall_s"] = round(time.perf_counter() - t0, 1)

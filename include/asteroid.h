@@ -315,6 +315,16 @@ int ast_chi2_batch(ast_ctx* ctx, const double* params, int n, double* chi2, doub
  * curve (ast_set_model_space, ast_upload_curve, which do allocate). */
 int ast_chi2_batch_noalloc(ast_ctx* ctx, const double* params, int n, double* chi2, double* dV);
 
+/* Device copy bandwidth: copies bytes (rounded down to 16-byte words)
+ * between two device buffers that the call allocates and frees, reps times,
+ * with a plain streaming copy kernel, and returns the bytes read plus
+ * written per second of the timed copies (ast_last_timing gives their
+ * milliseconds).  No counterpart in the reference: a measurement aid, the
+ * achievable HBM rate beside the fitting kernel's roofline and a kernel of
+ * known traffic that calibrates the HBM traffic counters.  AST_E_BUSY while
+ * serving. */
+int ast_copy_bandwidth(ast_ctx* ctx, int64_t bytes, int reps, double* bytes_per_s);
+
 /* The -plot evaluation (chi2_plot, cuda.c:1865-1966, launched at
  * asteroid.c:880; its grid is built at read_data.c:304-360).  For each of n
  * models (physical parameters, n*n_params doubles): chi^2 and delta_V
