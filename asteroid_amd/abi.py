@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(LIB_DIR, "libasteroid_hip.so")
 P_TYPE, P_INDEPENDENT, P_FROZEN, P_ISEG, P_MULTI_SEGMENT, P_PERIODIC = range(6)
 N_COLUMNS = 6
 PERIODIC, PERIODIC_LAM, SOFT_BOTH, HARD_LEFT, HARD_RIGHT, HARD_BOTH = 1, 0, -1, -2, -3, -4
-MAX_PARAMS, MAX_TYPES, MAX_FILTERS, MAX_MINIMA, MAX_SEGMENTS = 38, 16, 8, 10, 4
+MAX_PARAMS, MAX_TYPES, MAX_FILTERS, MAX_MINIMA, MAX_SEGMENTS = 40, 16, 8, 10, 4
 NOALLOC_MAX_MODELS = 16384
 
 PREC_FP64, PREC_F32SIMPLEX, PREC_MIXED = 0, 1, 2

@@ -26,7 +26,9 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 VARIANT_TUS = ["ast_k_ls", "ast_k_bc", "ast_k_bw", "ast_k_tq_ls", "ast_k_tq_bc", "ast_k_tq_bw", "ast_k_tq_trend",
                "ast_k_tq2", "ast_k_bc_rot", "ast_k_tq_bc_rot", "ast_k_rect", "ast_k_tq_rect", "ast_k_tq_nudge",
                "ast_k_tq_bc_nudge", "ast_k_seg", "ast_k_seg4", "ast_k_seg_le", "ast_k_seg4_le", "ast_k_tq_seg",
-               "ast_k_tq_seg4", "ast_k_tq_seg_le", "ast_k_tq_seg4_le"]
+               "ast_k_tq_seg4", "ast_k_tq_seg_le", "ast_k_tq_seg4_le", "ast_k_bc_seg", "ast_k_bc_seg4",
+               "ast_k_bc_seg_le", "ast_k_bc_seg4_le", "ast_k_tq_bc_seg", "ast_k_tq_bc_seg4", "ast_k_tq_bc_seg_le",
+               "ast_k_tq_bc_seg4_le"]
 
 
 def _run(cmd):

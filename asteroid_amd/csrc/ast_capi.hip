@@ -59,6 +59,14 @@ extern template hipError_t launch_variant<VF_TQ | VF_SEG>(const LaunchArgs&);
 extern template hipError_t launch_variant<VF_TQ | VF_SEG | VF_SEG4>(const LaunchArgs&);
 extern template hipError_t launch_variant<VF_TQ | VF_SEG | VF_SEGLE>(const LaunchArgs&);
 extern template hipError_t launch_variant<VF_TQ | VF_SEG | VF_SEG4 | VF_SEGLE>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_BC | VF_SEG>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_BC | VF_SEG | VF_SEG4>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_BC | VF_SEG | VF_SEGLE>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_BC | VF_SEG | VF_SEG4 | VF_SEGLE>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_TQ | VF_BC | VF_SEG>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_TQ | VF_BC | VF_SEG | VF_SEG4>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_TQ | VF_BC | VF_SEG | VF_SEGLE>(const LaunchArgs&);
+extern template hipError_t launch_variant<VF_TQ | VF_BC | VF_SEG | VF_SEG4 | VF_SEGLE>(const LaunchArgs&);
 }  // namespace ast
 
 using namespace ast;
@@ -66,7 +74,7 @@ using namespace ast;
 static thread_local char g_hip_err[256] = "";
 
 // Device memory of ast_chi2_batch_noalloc: the models, chi^2 and delta_V of
-// up to AST_NOALLOC_MAX_MODELS models (6.2 MB), one copy per GPU, held until
+// up to AST_NOALLOC_MAX_MODELS models (6.4 MB), one copy per GPU, held until
 // the process ends.  In host code `scratch` names a host-side stand-in, not
 // the device array: its address on each device comes from
 // hipGetSymbolAddress (scratch_setup, called by ast_ctx_create with the
@@ -123,7 +131,11 @@ constexpr int VARIANT_CODES[] = {0, VF_TQ, VF_BC, VF_TQ | VF_BC, VF_BW, VF_TQ | 
                                  VF_RECT, VF_TQ | VF_RECT, VF_TQ | VF_NUDGE, VF_TQ | VF_BC | VF_NUDGE,
                                  VF_SEG, VF_SEG | VF_SEG4, VF_SEG | VF_SEGLE, VF_SEG | VF_SEG4 | VF_SEGLE,
                                  VF_TQ | VF_SEG, VF_TQ | VF_SEG | VF_SEG4, VF_TQ | VF_SEG | VF_SEGLE,
-                                 VF_TQ | VF_SEG | VF_SEG4 | VF_SEGLE};
+                                 VF_TQ | VF_SEG | VF_SEG4 | VF_SEGLE,
+                                 VF_BC | VF_SEG, VF_BC | VF_SEG | VF_SEG4, VF_BC | VF_SEG | VF_SEGLE,
+                                 VF_BC | VF_SEG | VF_SEG4 | VF_SEGLE, VF_TQ | VF_BC | VF_SEG,
+                                 VF_TQ | VF_BC | VF_SEG | VF_SEG4, VF_TQ | VF_BC | VF_SEG | VF_SEGLE,
+                                 VF_TQ | VF_BC | VF_SEG | VF_SEG4 | VF_SEGLE};
 constexpr int N_VARIANTS = (int)(sizeof VARIANT_CODES / sizeof VARIANT_CODES[0]);
 
 // Variant code of the macros (TORQUE2 implies TORQUE, RECT implies BC).
@@ -150,9 +162,9 @@ bool variant_supported(const ast_variant& v) {
   if (v.nudge && v.precision == AST_PREC_MIXED) return false;
   // SEGMENT is built for three or four segments (the reference's T_START
   // tables, asteroid.h:40-46), ONE_LE or not, on the LS ellipsoid with or
-  // without constant torque
+  // without constant torque and photometric b, c
   if (v.n_seg < 0 || v.n_seg > AST_MAX_SEGMENTS || v.n_seg == 2) return false;
-  if (v.n_seg > 1 && (v.bc || v.bw_ball || v.trend || v.torque2 || v.rotate || v.rect || v.nudge)) return false;
+  if (v.n_seg > 1 && (v.bw_ball || v.trend || v.torque2 || v.rotate || v.rect || v.nudge)) return false;
   if (v.precision != AST_PREC_FP64 && v.precision != AST_PREC_F32SIMPLEX && v.precision != AST_PREC_MIXED)
     return false;
   if (!(v.time_step > 0)) return false;
@@ -190,14 +202,16 @@ int make_space(const ast_variant& v, int n_params, const int* property, int n_ty
     if (fr < -1 || fr > 1) return AST_E_ARG;
     pos++;
   }
-  // SEGMENT (asteroid.c:128-150): the first segment's shape parameters (and
-  // under ONE_LE its Es and L) are multi-segment, the rest repeat for each
+  // SEGMENT (asteroid.c:128-150): the first segment's shape parameters
+  // (c_tumb, b_tumb, and the photometric c, b under BC) and under ONE_LE its
+  // Es and L are multi-segment (asteroid.c:88-120), the rest repeat for each
   // further segment in the same order
   if (v.n_seg > 1) {
     std::vector<int> rep;
     for (int i = 0; i < pos; ++i) {
       const int t = property[i * AST_N_COLUMNS + AST_P_TYPE];
-      const bool shared = t == S.T.c_tumb || t == S.T.b_tumb || (v.one_le && (t == S.T.Es || t == S.T.L));
+      const bool shared = t == S.T.c_tumb || t == S.T.b_tumb || (S.T.c >= 0 && t == S.T.c) ||
+                          (S.T.b >= 0 && t == S.T.b) || (v.one_le && (t == S.T.Es || t == S.T.L));
       if (property[i * AST_N_COLUMNS + AST_P_MULTI_SEGMENT] != (shared ? 1 : 0)) return AST_E_UNSUPPORTED;
       if (property[i * AST_N_COLUMNS + AST_P_ISEG] != 0) return AST_E_UNSUPPORTED;
       if (!shared) rep.push_back(t);
@@ -482,6 +496,14 @@ hipError_t dispatch(const ast_variant& v, LaunchArgs& a) {
     case VF_TQ | VF_SEG | VF_SEG4: return launch_variant<VF_TQ | VF_SEG | VF_SEG4>(a);
     case VF_TQ | VF_SEG | VF_SEGLE: return launch_variant<VF_TQ | VF_SEG | VF_SEGLE>(a);
     case VF_TQ | VF_SEG | VF_SEG4 | VF_SEGLE: return launch_variant<VF_TQ | VF_SEG | VF_SEG4 | VF_SEGLE>(a);
+    case VF_BC | VF_SEG: return launch_variant<VF_BC | VF_SEG>(a);
+    case VF_BC | VF_SEG | VF_SEG4: return launch_variant<VF_BC | VF_SEG | VF_SEG4>(a);
+    case VF_BC | VF_SEG | VF_SEGLE: return launch_variant<VF_BC | VF_SEG | VF_SEGLE>(a);
+    case VF_BC | VF_SEG | VF_SEG4 | VF_SEGLE: return launch_variant<VF_BC | VF_SEG | VF_SEG4 | VF_SEGLE>(a);
+    case VF_TQ | VF_BC | VF_SEG: return launch_variant<VF_TQ | VF_BC | VF_SEG>(a);
+    case VF_TQ | VF_BC | VF_SEG | VF_SEG4: return launch_variant<VF_TQ | VF_BC | VF_SEG | VF_SEG4>(a);
+    case VF_TQ | VF_BC | VF_SEG | VF_SEGLE: return launch_variant<VF_TQ | VF_BC | VF_SEG | VF_SEGLE>(a);
+    case VF_TQ | VF_BC | VF_SEG | VF_SEG4 | VF_SEGLE: return launch_variant<VF_TQ | VF_BC | VF_SEG | VF_SEG4 | VF_SEGLE>(a);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1588,18 +1610,21 @@ void serve_loop(ast_ctx* c) {
       S.cancels.clear();
       // Admission: FIFO, while the job fits the free slots and its footprint
       // fits the HBM budget.  A multi-stage job needs a slot per group (its
-      // stage reseeds need whole groups); a single-stage job starts on as many
-      // of its groups' worth of slots as are free, at least one, and its lanes
-      // take the rest of its fits as their fits end (refill), so free slots
-      // never wait behind a job that does not fit whole.  The slots are taken
-      // off the free list here, so later jobs of the same pass see what is
-      // left.
+      // stage reseeds need whole groups).  A single-stage job starts on as
+      // many of its groups' worth of slots as are free once at least half of
+      // its groups' worth is (or the whole pool, for a job larger than twice
+      // the pool), and its lanes take the rest of its fits as their fits end
+      // (refill): free slots do not wait behind a job that does not fit
+      // whole, and no job runs on so few slots that it trails the others by
+      // more than twice its time on all of them.  The slots are taken off the
+      // free list here, so later jobs of the same pass see what is left.
       while (!S.pending.empty()) {
         DevJob& j = *S.pending.front();
         auto in = S.inputs.find(j.id);
         const int64_t nfree = (int64_t)c->pool.free_slots.size();
-        const int64_t need = j.rp.nstages == 1 ? std::min<int64_t>((int64_t)j.rp.n_groups, std::max<int64_t>(nfree, 1))
-                                               : (int64_t)j.rp.n_groups;
+        const int64_t G_job = (int64_t)j.rp.n_groups;
+        const int64_t least = std::min<int64_t>((G_job + 1) / 2, (int64_t)c->pool.n_slots);
+        const int64_t need = j.rp.nstages == 1 ? (nfree >= least ? std::min<int64_t>(G_job, nfree) : least) : G_job;
         const int64_t bytes = (int64_t)job_bytes(j.rp.n_groups, j.rp.group_size, (int)need, j.space.n_params,
                                                  j.n_filters, in->second.first.pts.size());
         if (nfree < need) break;

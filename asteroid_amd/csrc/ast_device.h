@@ -103,12 +103,12 @@ struct Pos {
   // SEGMENT (asteroid.c:128-150, asteroid.h:104-108): after the first
   // segment's parameters, each further segment repeats, in the same order,
   // those that are not multi-segment: theta_M, phi_M, phi_0, the torque
-  // components, Es and L (unless ONE_LE) and psi_0; c_tumb and b_tumb are
-  // shared.  rep(r) is the first-segment position of the r-th repeated
+  // components, Es and L (unless ONE_LE) and psi_0; c_tumb, b_tumb and the
+  // photometric c, b are shared.  rep(r) is the first-segment position of the r-th repeated
   // parameter, base(i) the first-segment position of parameter i (its role)
   // and segof(i) its segment.
-  static_assert(!F::SG || (VC & ~(VF_SEG | VF_SEG4 | VF_SEGLE | VF_TQ)) == 0,
-                "SEGMENT is built for the LS ellipsoid with or without torque");
+  static_assert(!F::SG || (VC & ~(VF_SEG | VF_SEG4 | VF_SEGLE | VF_TQ | VF_BC)) == 0,
+                "SEGMENT is built for the LS ellipsoid with or without torque and photometric b, c");
   static constexpr int NSEG = F::NSEG;
   static constexpr int R = F::SG ? 4 + (F::TQ ? 3 : 0) + (F::ONE_LE ? 0 : 2) : 0;
   static constexpr int rep(int r) {

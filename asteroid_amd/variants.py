@@ -225,6 +225,6 @@ def ls_torque_nudge(precision=abi.PREC_FP64, time_step=1e-2, interp=True, bc=Fal
     return make_variant(True, bc, False, True, interp, precision, time_step, nudge=True, v1s=v1s)
 
 
-def ls_segments(precision=abi.PREC_FP64, time_step=1e-2, interp=True, n_seg=3, one_le=True, torque=False):
-    """-DSEGMENT -DN_SEG=n_seg [-DONE_LE] [-DTORQUE], LS."""
-    return make_variant(torque, False, False, True, interp, precision, time_step, n_seg=n_seg, one_le=one_le)
+def ls_segments(precision=abi.PREC_FP64, time_step=1e-2, interp=True, n_seg=3, one_le=True, torque=False, bc=False):
+    """-DSEGMENT -DN_SEG=n_seg [-DONE_LE] [-DTORQUE] [-DBC], LS."""
+    return make_variant(torque, bc, False, True, interp, precision, time_step, n_seg=n_seg, one_le=one_le)

@@ -444,6 +444,26 @@ def main_small(args):
     return out
 
 
+def jobs_vs_fp64(res_mx, res_64):
+    """Each job's answer in the mixed form beside its answer in the FP64 form:
+    the relative difference of the two best chi^2 and the largest relative
+    difference of the two best parameter vectors, summarised over the jobs
+    (the two forms' simplexes take different paths, so the searches are
+    compared by their answers)."""
+    dchi, dpar = [], []
+    for a, b in zip(res_mx, res_64):
+        if a["best_f"] >= 1e29 or b["best_f"] >= 1e29:
+            continue
+        dchi.append(abs(a["best_f"] - b["best_f"]) / b["best_f"])
+        dpar.append(float(np.max(np.abs(a["best_params"] - b["best_params"]) /
+                                 np.maximum(np.abs(b["best_params"]), 1e-3))))
+    dchi, dpar = np.array(dchi), np.array(dpar)
+    q = lambda x: {"median": float(np.median(x)), "p90": float(np.percentile(x, 90)), "max": float(x.max())}
+    return {"jobs": len(dchi), "best_rel_chi2_diff": q(dchi), "best_params_max_rel_diff": q(dpar),
+            "jobs_same_model_1e-3": float(np.mean(dpar <= 1e-3)),
+            "jobs_mixed_lower_chi2": float(np.mean([a["best_f"] < b["best_f"] for a, b in zip(res_mx, res_64)]))}
+
+
 def main_reopt(args):
     from asteroid_amd import workloads
     jobs_mx, truth = workloads.reopt_bc(MIXED)
@@ -462,7 +482,8 @@ def main_reopt(args):
     out.update({"fits_per_s": m_mx["fits_per_s"], "roofline": m_mx["roofline"], "mixed": m_mx, "fp64": m_64,
                 "best_chi2_mixed": b_mx["best_f"], "best_chi2_fp64": b_64["best_f"],
                 "best_rel_chi2_diff": abs(b_mx["best_f"] - b_64["best_f"]) / b_64["best_f"],
-                "best_param_max_rel_diff": float(dpar.max()), "precision_gate": gate_jobs(jobs_mx, res_mx)})
+                "best_param_max_rel_diff": float(dpar.max()), "per_job_vs_fp64": jobs_vs_fp64(res_mx, res_64),
+                "precision_gate": gate_jobs(jobs_mx, res_mx)})
     return out
 
 
@@ -482,7 +503,7 @@ def main_batched(args):
                                  "60%% LS+torque / 30%% BC / 10%% BW_BALL+torque, 1024 simplexes each" % pts,
                      "simplexes": 512 * 1024, "evals_per_simplex": args.batch_budget})
     out.update({"fits_per_s": m_mx["fits_per_s"], "roofline": m_mx["roofline"], "mixed": m_mx, "fp64": m_64,
-                "precision_gate": gate_jobs(jobs_mx, res_mx)})
+                "per_job_vs_fp64": jobs_vs_fp64(res_mx, res_64), "precision_gate": gate_jobs(jobs_mx, res_mx)})
     return out
 
 

@@ -40,7 +40,7 @@ enum {
   AST_HARD_RIGHT = -3,
   AST_HARD_BOTH = -4
 };
-enum { AST_MAX_PARAMS = 38 /* SEGMENT with torque, four segments, no ONE_LE (asteroid.h:104-106) */, AST_MAX_TYPES = 16, AST_MAX_FILTERS = 8, AST_MAX_MINIMA = 10 /* NOBS_MAX, asteroid.h:219 */,
+enum { AST_MAX_PARAMS = 40 /* SEGMENT with torque and BC, four segments, no ONE_LE (asteroid.h:104-106) */, AST_MAX_TYPES = 16, AST_MAX_FILTERS = 8, AST_MAX_MINIMA = 10 /* NOBS_MAX, asteroid.h:219 */,
        AST_MAX_SEGMENTS = 4 /* N_SEG, asteroid.h:40-46 */,
        AST_NOALLOC_MAX_MODELS = 16384 /* models per ast_chi2_batch_noalloc call */ };
 
@@ -71,7 +71,7 @@ enum {
  * BW_BALL with and without torque, and for TREND with torque, TORQUE2,
  * ROTATE (with BC, with and without torque), RECT (with and without torque)
  * NUDGE (with torque, with and without BC) and SEGMENT (LS with and without
- * torque, three or four segments, with and without ONE_LE: 16 to 38
+ * torque and BC, three or four segments, with and without ONE_LE: 16 to 40
  * parameters); other combinations return AST_E_UNSUPPORTED.  P_PHI and
  * P_BOTH are not built. */
 typedef struct {
@@ -306,7 +306,7 @@ int ast_chi2_batch(ast_ctx* ctx, const double* params, int n, double* chi2, doub
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  The models, chi^2 and delta_V are staged in a fixed device
- * array of the library (one per GPU, 6.2 MB, held until the process ends)
+ * array of the library (one per GPU, 6.4 MB, held until the process ends)
  * instead of buffers allocated per call; results are those of
  * ast_chi2_batch.  At most AST_NOALLOC_MAX_MODELS models per call (more is
  * AST_E_ARG).  All contexts on one GPU share that array: calls that overlap

@@ -182,7 +182,11 @@ class Problem:
         nf, N = self.curve.n_filters, self.space.n_params
         nfit = n_groups * group_size
         rng = np.zeros(nfit, np.uint32) if rng is None else rng
-        d_f = np.full(n_groups, 1e30) if d_f is None else d_f
+        # setup_kernel's 1e30 (cuda.c:2050) in d_f's CHI_FLOAT (asteroid.c:730):
+        # a float under the float-simplex forms, so a group whose every lane
+        # failed reports float(1e30)
+        sentinel = float(np.float32(1e30)) if self.space.variant.precision != abi.PREC_FP64 else 1e30
+        d_f = np.full(n_groups, sentinel) if d_f is None else d_f
         d_params = np.zeros((n_groups, N)) if d_params is None else d_params
         d_dV = np.zeros((n_groups, nf)) if d_dV is None else d_dV
         p0 = np.zeros(N) if params0 is None else np.ascontiguousarray(params0, np.float64)

@@ -41,7 +41,7 @@ def test_strerror_and_validation_without_device():
     bad = variants.ls_torque_nudge(abi.PREC_MIXED)  # NUDGE is offered in the FP64 and float-simplex forms
     assert lib.ast_ctx_create(C.byref(h), C.byref(bad), 0) == -6
     for bad in (variants.ls_segments(n_seg=2),                                  # T_START has 3 or 4 entries
-                variants.make_variant(True, True, False, n_seg=3, one_le=True),    # SEGMENT on the LS ellipsoid only
+                variants.make_variant(True, True, False, rect=True, n_seg=3, one_le=True),  # SEGMENT: LS (BC) only
                 variants.make_variant(True, False, True, n_seg=4),
                 variants.make_variant(True, False, False, torque2=True, n_seg=3)):
         assert lib.ast_ctx_create(C.byref(h), C.byref(bad), 0) == -6
@@ -56,8 +56,9 @@ VARS = [variants.ls_torque(), variants.bc_free(), variants.bw_ball_torque(),
         variants.make_variant(False, False, True, False, False), variants.ls_torque_trend(), variants.ls_torque2(),
         variants.bc_rotate(), variants.bc_rotate(torque=True), variants.rect(), variants.rect(torque=True),
         variants.ls_torque_nudge(), variants.ls_torque_nudge(bc=True), variants.ls_segments()] + [
-        variants.ls_segments(n_seg=n, one_le=le, torque=tq) for n in (3, 4) for le in (True, False) for tq in (False, True)
-        if (n, le, tq) != (3, True, False)]
+        variants.ls_segments(n_seg=n, one_le=le, torque=tq, bc=bc) for n in (3, 4) for le in (True, False)
+        for tq in (False, True) for bc in (False, True)
+        if (n, le, tq, bc) != (3, True, False, False)]
 
 
 @pytest.mark.parametrize("v", VARS)

@@ -106,9 +106,9 @@ def test_run_continue_refuses_conflicting_calls():
 
 
 def test_serving_partial_admission_equals_alone():
-    # a pool of 5 slots: the 2-group job takes 2, the 8-group job A starts on
-    # the 3 left and the second 8-group job on what its predecessors free;
-    # each takes the rest of its fits from its own queue
+    # a pool of 5 slots: the 2-group job takes 2; the 8-group job A waits for
+    # half its groups' worth (4) and the second 8-group job starts on what its
+    # predecessors free; each takes the rest of its fits from its own queue
     vA = variants.ls_torque(abi.PREC_FP64)
     vB = variants.ls_torque(abi.PREC_MIXED)
     A = _case(vA, n_obs=70, span=1.2, seed=31)

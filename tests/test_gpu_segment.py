@@ -4,8 +4,8 @@ forms; the mixed form within its 1e-6 relative tolerance), whole fits lane
 by lane, observations before the first segment and an observation past two
 start times, the -plot curve (grid points before the first start NaN), the
 call rules of ast_set_segments and the refusals; for all eight SEGMENT
-builds (three or four segments, with and without ONE_LE and TORQUE: 16 to
-38 parameters), chi^2, delta_V and two-stage fits bit for bit and the mixed
+builds (three or four segments, with and without ONE_LE, TORQUE and BC:
+16 to 40 parameters), chi^2, delta_V and two-stage fits bit for bit and the mixed
 form within tolerance; slot pools cut for the largest job's parameter
 count."""
 import numpy as np
@@ -165,12 +165,12 @@ def test_segment_starts_by_index_bitwise(prec):
     assert res[0]["best_f"] == want["f"][0] and np.array_equal(res[0]["best_params"], want["params"][0])
 
 
-SEG_BUILDS = [(n, le, tq) for n in (3, 4) for le in (True, False) for tq in (False, True)]
+SEG_BUILDS = [(n, le, tq, bc) for n in (3, 4) for le in (True, False) for tq in (False, True) for bc in (False, True)]
 
 
-def _build_case(prec, n_seg, one_le, torque, seed=9, n=150, span=4.0):
-    v0 = variants.make_variant(torque, False, False, True, True, prec)
-    v = variants.ls_segments(prec, n_seg=n_seg, one_le=one_le, torque=torque)
+def _build_case(prec, n_seg, one_le, torque, bc=False, seed=9, n=150, span=4.0):
+    v0 = variants.make_variant(torque, bc, False, True, True, prec)
+    v = variants.ls_segments(prec, n_seg=n_seg, one_le=one_le, torque=torque, bc=bc)
     curve, truth0 = synth.make_curve(seed, v0, n, span, n_filters=2)
     t_start = curve.mjd[0] + 0.3 + np.arange(n_seg) * (span - 0.3) / n_seg  # the first observations left out
     pos0, pos = variants.param_positions(v0), variants.param_positions(v)
@@ -183,7 +183,7 @@ def _build_case(prec, n_seg, one_le, torque, seed=9, n=150, span=4.0):
 
 
 def _bid(b):
-    return f"seg{b[0]}{'_onele' if b[1] else ''}{'_torque' if b[2] else ''}"
+    return f"seg{b[0]}{'_onele' if b[1] else ''}{'_torque' if b[2] else ''}{'_bc' if b[3] else ''}"
 
 
 @pytest.mark.parametrize("build", SEG_BUILDS, ids=_bid)
@@ -225,34 +225,34 @@ def test_segment_builds_mixed_within_tolerance(build):
 
 
 def test_pool_sized_for_largest_job():
-    """A batch holding a 10-parameter job and a 38-parameter one cuts its slots
-    for 38 parameters; each job equals the same job alone (on a pool cut for
-    its own count) and the 38-parameter job equals the oracle.  Serving slots
+    """A batch holding a 10-parameter job and a 40-parameter one cuts its slots
+    for 40 parameters; each job equals the same job alone (on a pool cut for
+    its own count) and the 40-parameter job equals the oracle.  Serving slots
     cut for fewer parameters than a job refuse it; a job that fits runs as
     alone."""
-    v, curve, truth, space, t_start = _build_case(abi.PREC_FP64, 4, False, True)
-    assert space.n_params == 38
+    v, curve, truth, space, t_start = _build_case(abi.PREC_FP64, 4, False, True, True)
+    assert space.n_params == abi.MAX_PARAMS == 40
     v10 = variants.ls_torque(abi.PREC_FP64)
     c10, _ = synth.make_curve(5, v10, 80, 1.5)
     s10 = variants.default_model_space(v10)
     j10 = dict(space=s10, curve=c10, n_fits=256, n_steps=30, seed=3)
-    j38 = dict(space=space, curve=curve, n_fits=256, n_steps=20, seed=4, t_segments=t_start)
+    j40 = dict(space=space, curve=curve, n_fits=256, n_steps=20, seed=4, t_segments=t_start)
     with api.Context(v10) as ctx:
         a10 = ctx.run_jobs([j10])[0]
-        a38 = ctx.run_jobs([j38])[0]
-        both = ctx.run_jobs([j10, j38])
+        a40 = ctx.run_jobs([j40])[0]
+        both = ctx.run_jobs([j10, j40])
         ctx.serve_start(max_resident_fits=512, max_params=s10.n_params)
         with pytest.raises(RuntimeError, match="AST_E_ARG"):
             ctx.submit(space, curve, n_fits=256, nstages=1, n_steps=20, seed=4, t_segments=t_start)
         sid = ctx.submit(s10, c10, n_fits=256, nstages=1, n_steps=30, seed=3)
         served = ctx.wait_all([sid])[sid]
         ctx.serve_stop()
-    for got, want in ((both[0], a10), (both[1], a38), (served, a10)):
+    for got, want in ((both[0], a10), (both[1], a40), (served, a10)):
         for key in ("status", "best_f", "best_group", "n_evals"):
             assert got[key] == want[key], key
         assert np.array_equal(got["best_params"], want["best_params"])
         assert np.array_equal(got["best_dV"], want["best_dV"])
     want = oracle.Problem(space, curve, t_start=t_start).run(1, 256, nstages=1, n_steps=20, seed=4, n_threads=8)
-    assert a38["status"] == 0 and a38["best_f"] == want["f"][0]
-    assert np.array_equal(a38["best_params"], want["params"][0]) and np.array_equal(a38["best_dV"], want["dV"][0])
-    assert a38["n_evals"] == want["lane_nev"].sum()
+    assert a40["status"] == 0 and a40["best_f"] == want["f"][0]
+    assert np.array_equal(a40["best_params"], want["params"][0]) and np.array_equal(a40["best_dV"], want["dV"][0])
+    assert a40["n_evals"] == want["lane_nev"].sum()

@@ -160,16 +160,16 @@ def test_plot_curve_restarts_at_each_segment():
     assert c[0] == c_obs[0] and np.array_equal(dv[0], dv_obs[0])
 
 
-SEG_BUILDS = [(n, le, tq) for n in (3, 4) for le in (True, False) for tq in (False, True)]
+SEG_BUILDS = [(n, le, tq, bc) for n in (3, 4) for le in (True, False) for tq in (False, True) for bc in (False, True)]
 
 
 def _ids(b):
-    return f"seg{b[0]}{'_onele' if b[1] else ''}{'_torque' if b[2] else ''}"
+    return f"seg{b[0]}{'_onele' if b[1] else ''}{'_torque' if b[2] else ''}{'_bc' if b[3] else ''}"
 
 
-def _build(n_seg, one_le, torque, seed=7, n=120, span=4.0):
-    v0 = variants.make_variant(torque, False, False, True, True)
-    vs = variants.ls_segments(n_seg=n_seg, one_le=one_le, torque=torque)
+def _build(n_seg, one_le, torque, bc=False, seed=7, n=120, span=4.0):
+    v0 = variants.make_variant(torque, bc, False, True, True)
+    vs = variants.ls_segments(n_seg=n_seg, one_le=one_le, torque=torque, bc=bc)
     curve, truth0 = synth.make_curve(seed, v0, n, span, n_filters=2)
     t_start = curve.mjd[0] - 0.05 + np.arange(n_seg) * span / n_seg
     pos0, pos = variants.param_positions(v0), variants.param_positions(vs)
@@ -187,10 +187,10 @@ def _build(n_seg, one_le, torque, seed=7, n=120, span=4.0):
 
 @pytest.mark.parametrize("build", SEG_BUILDS, ids=_ids)
 def test_every_segment_build_is_plain_models_on_their_observations(build):
-    n_seg, one_le, torque = build
-    v0, vs, curve, params, t_start = _build(n_seg, one_le, torque)
+    n_seg, one_le, torque, bc = build
+    v0, vs, curve, params, t_start = _build(n_seg, one_le, torque, bc)
     space = variants.default_model_space(vs)
-    assert space.n_params == 8 + 3 * torque + (6 + 3 * torque - 2 * one_le) * (n_seg - 1)  # asteroid.h:104-106
+    assert space.n_params == 8 + 3 * torque + 2 * bc + (6 + 3 * torque - 2 * one_le) * (n_seg - 1)  # asteroid.h:104-106
     pb = oracle.Problem(space, curve, t_start=t_start)
     starts = _starts(curve.mjd, t_start)
     assert len(starts) == n_seg
@@ -223,13 +223,13 @@ def test_segment_x2params_maps_each_segment_like_the_plain_model(build):
     # with every Es on one side of 1/2, segment k's parameters are the plain
     # model's map of segment k's coordinates (L from 1/P_psi with the
     # segment's own Es, cuda.c:1263)
-    n_seg, one_le, torque = build
-    v0, vs, curve, params, t_start = _build(n_seg, one_le, torque)
+    n_seg, one_le, torque, bc = build
+    v0, vs, curve, params, t_start = _build(n_seg, one_le, torque, bc)
     space, space0 = variants.default_model_space(vs), variants.default_model_space(v0)
     pb = oracle.Problem(space, curve, t_start=t_start)
     pb0 = oracle.Problem(space0, curve)
     pos0, pos = variants.param_positions(v0), variants.param_positions(vs)
-    rng = np.random.default_rng(n_seg * 10 + one_le * 2 + torque)
+    rng = np.random.default_rng(n_seg * 10 + one_le * 2 + torque + 100 * bc)
     for side in (0.25, 0.75):
         x = rng.uniform(0.05, 0.95, (50, space.n_params))
         for name, i in pos.items():
