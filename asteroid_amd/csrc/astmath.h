@@ -61,8 +61,18 @@ AST_HD double p_flipsign(double x, int flag) {
 // and holds k in two's complement in its low mantissa bits.  One add, where a
 // range-checked float-to-int conversion takes three instructions.
 AST_HD uint32_t low_word_of_int(double k) { return (uint32_t)bits_of(k + 6755399441055744.0); }
-// x with its sign bit xored by bit 31 of m (exact, not counted).
-AST_HD double p_xorsign(double x, uint32_t m) { return of_bits(bits_of(x) ^ ((uint64_t)m << 32)); }
+// x with its sign bit xored by bit 31 of m; m's other bits are ignored
+// (exact, not counted).  On the GPU the mask and the xor into the high word
+// are one v_bitop3_b32 (hi ^ (m & 0x80000000), table 0x78).
+AST_HD double p_xorsign(double x, uint32_t m) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint64_t b = bits_of(x);
+  const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(b >> 32), m, 0x80000000u, 0x78);
+  return of_bits(((uint64_t)hi << 32) | (uint32_t)b);
+#else
+  return of_bits(bits_of(x) ^ ((uint64_t)(m & 0x80000000u) << 32));
+#endif
+}
 
 // 2^k as a double, for -1022 <= k <= 1023.
 AST_HD double pow2i(int k) { return of_bits((uint64_t)(k + 1023) << 52); }
@@ -141,14 +151,13 @@ template <class R> AST_HD void sincos(R x, R* s, R* c) {
   R hz = R(0.5) * z;
   R w = R(1.0) - hz;
   R cs = w + p_fma(z * z, k_cos(z), (R(1.0) - w) - hz);
-  // quadrant q = k mod 4: sin takes the sign of bit 1 of q, cos that of bit 1
-  // of q + 1, which is bit 1 xor bit 0
+  // quadrant q = k mod 4: odd q swaps sin and cos; sin takes the sign of bit
+  // 1 of q, cos that of bit 1 of q + 1 (each shifted to bit 31)
   const uint32_t q = low_word_of_int(p_val(k));
   R s0 = (q & 1) ? cs : sn;
   R c0 = (q & 1) ? sn : cs;
-  const uint32_t ms = (q << 30) & 0x80000000u;
-  *s = p_xorsign(s0, ms);
-  *c = p_xorsign(c0, ms ^ (q << 31));
+  *s = p_xorsign(s0, q << 30);
+  *c = p_xorsign(c0, (q + 1) << 30);
 }
 
 // sin and cos for the mixed-precision form's body axes (relative error about
@@ -167,14 +176,13 @@ template <class R> AST_HD void sincos_mx(R x, R* s, R* c) {
   R hz = R(0.5) * z;
   R w = R(1.0) - hz;
   R cs = w + p_fma(z * z, pc, (R(1.0) - w) - hz);
-  // quadrant q = k mod 4: sin takes the sign of bit 1 of q, cos that of bit 1
-  // of q + 1, which is bit 1 xor bit 0
+  // quadrant q = k mod 4: odd q swaps sin and cos; sin takes the sign of bit
+  // 1 of q, cos that of bit 1 of q + 1 (each shifted to bit 31)
   const uint32_t q = low_word_of_int(p_val(k));
   R s0 = (q & 1) ? cs : sn;
   R c0 = (q & 1) ? sn : cs;
-  const uint32_t ms = (q << 30) & 0x80000000u;
-  *s = p_xorsign(s0, ms);
-  *c = p_xorsign(c0, ms ^ (q << 31));
+  *s = p_xorsign(s0, q << 30);
+  *c = p_xorsign(c0, (q + 1) << 30);
 }
 
 template <class R> AST_HD R sin(R x) { R s, c; sincos(x, &s, &c); return s; }
