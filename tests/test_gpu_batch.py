@@ -149,3 +149,29 @@ def test_batch_segment_job_matches_alone_and_oracle():
     assert r["status"] == 0 and r["best_f"] == want["f"][0]
     assert np.array_equal(r["best_params"], want["params"][0]) and np.array_equal(r["best_dV"], want["dV"][0])
     assert r["n_evals"] == want["lane_nev"].sum()
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_F32SIMPLEX])
+def test_full_size_curves_match_oracle(prec):
+    """Curves at the largest size of the batched workload (2000 points over 8
+    days, four filters; and a 1000-point three-filter BC curve over 5 days at
+    TIME_STEP 5e-3, the Stage Two shape), a few simplexes each, in one batch
+    beside a small job: every job equals the oracle's run bitwise."""
+    specs = [(variants.ls_torque(prec), 4, 2000, 8.0, 128, 1, 40, 0),
+             (variants.bc_free(prec, time_step=5e-3), 3, 1000, 5.0, 128, 2, 20, 0),
+             (variants.ls_torque(prec), 1, 60, 1.0, 256, 1, 30, 0)]
+    jobs = []
+    for k, (v, nf, n_obs, span, n_fits, nst, nsteps, reopt) in enumerate(specs):
+        curve, truth = synth.make_curve(300 + k, v, n_obs, span, n_filters=nf)
+        jobs.append(dict(space=variants.default_model_space(v), curve=curve, n_fits=n_fits, nstages=nst,
+                         n_steps=nsteps, seed=700 + k, tag=k))
+    with api.Context(variants.ls_torque(prec)) as ctx:
+        res = ctx.run_jobs(jobs, group_size=64, work_points_per_launch=4000)
+    for j, r in zip(jobs, res):
+        print("job", r["tag"], "points", j["curve"].n_data, "best", r["best_f"], flush=True)
+        want = oracle.Problem(j["space"], j["curve"]).run(j["n_fits"] // 64, 64, nstages=j["nstages"],
+                                                          n_steps=j["n_steps"], seed=j["seed"], n_threads=8)
+        g = int(np.argmin(want["f"]))
+        assert r["status"] == 0 and r["best_f"] == want["f"][g] and r["best_group"] == g
+        assert np.array_equal(r["best_params"], want["params"][g]) and np.array_equal(r["best_dV"], want["dV"][g])
+        assert r["n_evals"] == want["lane_nev"].sum()
