@@ -344,17 +344,6 @@ __device__ inline void body_axes(const SpinFrame<T>& F, T sf, T cf, T st, T ctt,
   c[2] = fmx(-a[1], b[0], a[0] * b[1]);
 }
 
-// sin and cos of an Euler angle for the body axes at an observation: the
-// canonical FP64 functions, or in the mixed form the degree-4 ones (about
-// 2e-14 relative).  The integration itself always uses the canonical ones:
-// for models whose Euler angles pass near the singular theta = 0 or pi the
-// RK4 integration is chaotic and magnifies any change in its arithmetic.
-template <bool MX>
-__device__ inline void sincos_e(double x, double* s, double* c) {
-  if (MX) astm::sincos_mx(x, s, c);
-  else astm::sincos(x, s, c);
-}
-
 // ODE right-hand side.  With FIRST the sin/cos of theta (and psi) come from
 // sc[] (those of the state, shared with the body axes), else they are computed.
 template <bool TQ, bool FIRST>
@@ -362,7 +351,7 @@ __device__ inline void ode(const double* y, double* f, const double* mu, double 
   if (TQ) {  // cuda.c:46-58
     double s5, c5, s4, c4;
     if (FIRST) { s4 = sc[0]; c4 = sc[1]; s5 = sc[2]; c5 = sc[3]; }
-    else { astm::sincos(y[5], &s5, &c5); astm::sincos(y[4], &s4, &c4); }
+    else { astm::sincos_t(y[5], &s5, &c5); astm::sincos_t(y[4], &s4, &c4); }
     f[0] = __builtin_fma(mu[0] * y[1], y[2], mu[3]);
     f[1] = __builtin_fma(mu[1] * y[2], y[0], mu[4]);
     f[2] = __builtin_fma(mu[2] * y[0], y[1], mu[5]);
@@ -371,9 +360,9 @@ __device__ inline void ode(const double* y, double* f, const double* mu, double 
     f[5] = __builtin_fma(-f[3], c4, y[2]);
   } else {   // cuda.c:69-75
     double s2, c2, s1, c1;
-    astm::sincos(2.0 * y[2], &s2, &c2);
+    astm::sincos_t(2.0 * y[2], &s2, &c2);
     if (FIRST) { s1 = sc[0]; c1 = sc[1]; }
-    else astm::sincos(y[1], &s1, &c1);
+    else astm::sincos_t(y[1], &s1, &c1);
     f[0] = mu[0] * __builtin_fma(-mu[2], c2, mu[1]);
     f[1] = (mu02 * s1) * s2;
     f[2] = c1 * (mu[0] - f[0]);
@@ -505,16 +494,17 @@ struct ModelConsts {
 
 #define AST_AI __device__ __attribute__((always_inline)) inline
 
-// Body axes at one observation (FP64) from the spin frame, phi and the sines
-// and cosines of theta (sc[0..1]) and, with torque, of psi (sc[2..3]).
-template <bool TQ, bool MX>
+// Body axes at one observation (FP64, both forms) from the spin frame, phi
+// and the sines and cosines of theta (sc[0..1]) and, with torque, of psi
+// (sc[2..3]).
+template <bool TQ>
 AST_AI void axes_at(const SpinFrame<double>& Fd, double phi, double psi, const double* sc, double* a, double* b,
                     double* c) {
   double sf, cf, sp, cp;
-  sincos_e<MX>(phi, &sf, &cf);
+  astm::sincos_t(phi, &sf, &cf);
   const double st = sc[0], ctt = sc[1];
   if (TQ) { sp = sc[2]; cp = sc[3]; }
-  else sincos_e<MX>(psi, &sp, &cp);
+  else astm::sincos_t(psi, &sp, &cp);
   body_axes<double>(Fd, sf, cf, st, ctt, sp, cp, a, b, c);
 }
 
@@ -740,8 +730,8 @@ AST_AI void begin_rotation(const double* params, int k, double Is_inv, double Ii
   using P = Pos<VC>;
   constexpr bool TQ = VarF<VC>::TQ;
   double sTM, cTM, sPM, cPM;
-  astm::sincos(segval<VC, P::theta_M>(params, k), &sTM, &cTM);
-  astm::sincos(segval<VC, P::phi_M>(params, k), &sPM, &cPM);
+  astm::sincos_t(segval<VC, P::theta_M>(params, k), &sTM, &cTM);
+  astm::sincos_t(segval<VC, P::phi_M>(params, k), &sPM, &cPM);
   Fd.M_x = sTM * cPM;
   Fd.M_y = sTM * sPM;
   Fd.M_z = cTM;
@@ -754,10 +744,10 @@ AST_AI void begin_rotation(const double* params, int k, double Is_inv, double Ii
   const double Es = segval<VC, P::Es>(params, k), psi0 = segval<VC, P::psi_0>(params, k);
   const double L = segval<VC, P::L>(params, k);
   double sP0, cP0;
-  astm::sincos(psi0, &sP0, &cP0);
+  astm::sincos_t(psi0, &sP0, &cP0);
   const double theta = astm::asin(__builtin_sqrt((Es - 1.0) / (__builtin_fma(sP0 * sP0, Ii_inv - Is_inv, Is_inv) - 1.0)));
   const double phi = segval<VC, P::phi_0>(params, k);
-  astm::sincos(theta, &sc[0], &sc[1]);
+  astm::sincos_t(theta, &sc[0], &sc[1]);
   sc[2] = sP0;
   sc[3] = cP0;
   if constexpr (TQ) {
@@ -824,13 +814,13 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
   double phi, theta, psi;
   ModelConsts K{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   if constexpr (RO) {
-    astm::sincos(params[P::phi_R], &K.sR, &K.cR);
-    astm::sincos(params[P::theta_R], &K.stR, &K.ctR);
-    astm::sincos(params[P::psi_R], &K.sPR, &K.cPR);
+    astm::sincos_t(params[P::phi_R], &K.sR, &K.cR);
+    astm::sincos_t(params[P::theta_R], &K.stR, &K.ctR);
+    astm::sincos_t(params[P::psi_R], &K.sPR, &K.cPR);
   }
   if (BW) {
-    astm::sincos(params[P::phi_R], &K.sR, &K.cR);
-    astm::sincos(params[P::theta_R], &K.stR, &K.ctR);
+    astm::sincos_t(params[P::phi_R], &K.sR, &K.cR);
+    astm::sincos_t(params[P::theta_R], &K.stR, &K.ctR);
     K.kappa = params[P::kappa];
   } else {
     const double b = BC ? params[P::b] : bt;
@@ -915,10 +905,10 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
 #pragma unroll
       for (int j = 0; j < NODE; ++j) y[j] = __builtin_fma(h6, s[j] + K4[j], y[j]);
       if (TQ) {
-        astm::sincos(y[4], &sc[0], &sc[1]);
-        astm::sincos(y[5], &sc[2], &sc[3]);
+        astm::sincos_t(y[4], &sc[0], &sc[1]);
+        astm::sincos_t(y[5], &sc[2], &sc[3]);
       } else {
-        astm::sincos(y[1], &sc[0], &sc[1]);
+        astm::sincos_t(y[1], &sc[0], &sc[1]);
       }
     }
     }  // sub-intervals
@@ -926,7 +916,7 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
     else { phi = y[0]; theta = y[1]; psi = y[2]; }
     const double E[3] = {pr.E[0], pr.E[1], pr.E[2]};
     double a[3], b[3], c[3];
-    axes_at<TQ, MX>(Fd, phi, psi, sc, a, b, c);
+    axes_at<TQ>(Fd, phi, psi, sc, a, b, c);
     if constexpr (RO) rotate_axes(a, b, c, K);
     double Vmod;
     if constexpr (RC) {

@@ -603,6 +603,7 @@ __global__ __launch_bounds__(64) void group_best_kernel(const CF* fit_f, const d
 template <class CF, int VC, int NF, bool MX, int W>
 __global__ __launch_bounds__(W == 4 ? 1024 : 256) __attribute__((amdgpu_waves_per_eu(W, W)))
 void fit_kernel(const GroupDesc<CF>* descs, SlotBufs<CF> B) {
+  astm::sincos_table_load();  // the evaluator's sin/cos table into LDS (whole workgroup)
   using P = Pos<VC>;
   AST_VFLAGS(VC);
   constexpr int N = P::N;
@@ -766,6 +767,7 @@ template <class CF, int VC, int NF, bool MX>
 __global__ __launch_bounds__(256) void chi2_batch_kernel(const PointRec* pts_g, const double* sum_w, int n_data,
                                                         int n_filters, int dof, const double* params, int n,
                                                         double* chi2, double* dV) {
+  astm::sincos_table_load();  // the evaluator's sin/cos table into LDS (whole workgroup)
   using P = Pos<VC>;
   AST_VFLAGS(VC);
   constexpr int N = P::N;
@@ -795,6 +797,7 @@ __global__ __launch_bounds__(256) void model_curve_kernel(const PointRec* pts_g,
                                                          int n_filters, int dof, const PointRec* grid_g, int n_plot,
                                                          const double* params, int n, double* chi2, double* dV,
                                                          double* vmod) {
+  astm::sincos_table_load();  // the evaluator's sin/cos table into LDS (whole workgroup)
   using P = Pos<VC>;
   AST_VFLAGS(VC);
   constexpr int N = P::N;
@@ -847,6 +850,7 @@ struct RmsdArgs {
 
 template <class CF, int VC, int NF, bool MX>
 __global__ __launch_bounds__(64) void rmsd_ref_kernel(RmsdArgs<CF> A) {
+  astm::sincos_table_load();  // the evaluator's sin/cos table into LDS (whole workgroup)
   using P = Pos<VC>;
   AST_VFLAGS(VC);
   constexpr int N = P::N;
@@ -879,6 +883,7 @@ __global__ __launch_bounds__(64) void rmsd_ref_kernel(RmsdArgs<CF> A) {
 template <class CF, int VC, int NF, bool MX, int W>
 __global__ __launch_bounds__(W == 4 ? 1024 : 256) __attribute__((amdgpu_waves_per_eu(W, W))) void rmsd_kernel(
     RmsdArgs<CF> A) {
+  astm::sincos_table_load();  // the evaluator's sin/cos table into LDS (whole workgroup)
   using P = Pos<VC>;
   AST_VFLAGS(VC);
   constexpr int N = P::N;

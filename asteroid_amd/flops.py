@@ -22,26 +22,26 @@ _W = np.array([1, 1, 2, 1, 1, 1], float)
 # TORQUE2 is the torque row: its split interval runs as two RK4 stretches,
 # counted by rk4_steps when the split fraction is given.
 TABLE = {
-    (True, "ls"): dict(per_model=(28, 41, 77, 11, 3, 4), per_point=(22, 68, 73, 5, 4, 1),
-                       per_rk4=(38, 60, 180, 4, 0, 8)),
-    (False, "ls"): dict(per_model=(27, 35, 77, 11, 3, 4), per_point=(26, 73, 88, 5, 4, 2),
-                        per_rk4=(39, 60, 142, 0, 0, 8)),
-    (True, "bw"): dict(per_model=(36, 50, 107, 9, 3, 6), per_point=(14, 44, 54, 1, 0, 1),
-                       per_rk4=(38, 60, 180, 4, 0, 8)),
-    (False, "bw"): dict(per_model=(35, 44, 107, 9, 3, 6), per_point=(18, 49, 69, 1, 0, 2),
-                        per_rk4=(39, 60, 142, 0, 0, 8)),
+    (True, "ls"): dict(per_model=(20, 45, 57, 11, 3, 4), per_point=(20, 69, 68, 5, 4, 1),
+                       per_rk4=(22, 68, 140, 4, 0, 8)),
+    (False, "ls"): dict(per_model=(19, 39, 57, 11, 3, 4), per_point=(22, 75, 78, 5, 4, 2),
+                        per_rk4=(23, 68, 102, 0, 0, 8)),
+    (True, "bw"): dict(per_model=(24, 56, 77, 9, 3, 6), per_point=(12, 45, 49, 1, 0, 1),
+                       per_rk4=(22, 68, 140, 4, 0, 8)),
+    (False, "bw"): dict(per_model=(23, 50, 77, 9, 3, 6), per_point=(14, 51, 59, 1, 0, 2),
+                        per_rk4=(23, 68, 102, 0, 0, 8)),
     # TREND: the LS torque row plus the de-trending fma (the phase angle is a
     # property of the point, computed once on the host)
-    (True, "ls+trend"): dict(per_model=(28, 41, 77, 11, 3, 4), per_point=(22, 68, 74, 5, 4, 1),
-                             per_rk4=(38, 60, 180, 4, 0, 8)),
-    (True, "ls+rot"): dict(per_model=(40, 56, 122, 11, 3, 7), per_point=(22, 86, 91, 5, 4, 1),
-                           per_rk4=(38, 60, 180, 4, 0, 8)),
-    (False, "ls+rot"): dict(per_model=(39, 50, 122, 11, 3, 7), per_point=(26, 91, 106, 5, 4, 2),
-                            per_rk4=(39, 60, 142, 0, 0, 8)),
-    (True, "rect"): dict(per_model=(28, 41, 77, 11, 3, 4), per_point=(14, 63, 72, 1, 3, 1),
-                         per_rk4=(38, 60, 180, 4, 0, 8)),
-    (False, "rect"): dict(per_model=(27, 35, 77, 11, 3, 4), per_point=(18, 68, 87, 1, 3, 2),
-                          per_rk4=(39, 60, 142, 0, 0, 8)),
+    (True, "ls+trend"): dict(per_model=(20, 45, 57, 11, 3, 4), per_point=(20, 69, 69, 5, 4, 1),
+                             per_rk4=(22, 68, 140, 4, 0, 8)),
+    (True, "ls+rot"): dict(per_model=(26, 63, 87, 11, 3, 7), per_point=(20, 87, 86, 5, 4, 1),
+                           per_rk4=(22, 68, 140, 4, 0, 8)),
+    (False, "ls+rot"): dict(per_model=(25, 57, 87, 11, 3, 7), per_point=(22, 93, 96, 5, 4, 2),
+                            per_rk4=(23, 68, 102, 0, 0, 8)),
+    (True, "rect"): dict(per_model=(20, 45, 57, 11, 3, 4), per_point=(12, 64, 67, 1, 3, 1),
+                         per_rk4=(22, 68, 140, 4, 0, 8)),
+    (False, "rect"): dict(per_model=(19, 39, 57, 11, 3, 4), per_point=(14, 70, 77, 1, 3, 2),
+                          per_rk4=(23, 68, 102, 0, 0, 8)),
 }
 
 
@@ -110,11 +110,9 @@ def eval_flops(variant, mjd, split=None, seg_t=None):
 
 # ------------------------------------------------------------ mixed form
 # The mixed-precision evaluator runs the per-model set-up, the RK4
-# integration, the scaled Sun/Earth directions and the chi^2 sums exactly as
-# the FP64 form (the rows above), takes the body axes' per-point sin/cos
-# (of phi, and of psi without torque) from the shorter sincos_mx (3 fma
-# fewer than sincos each: two-part reduction, degree-4 kernels), and
-# replaces the FP64 brightness tail by its own (RECT keeps its FP64 tail).
+# integration, the scaled Sun/Earth directions, the chi^2 sums and the body
+# axes exactly as the FP64 form (the rows above) and replaces
+# the FP64 brightness tail by its own (RECT keeps its FP64 tail).
 # Counted from ast_device.h and astmath.h:
 #   FP64 Lommel-Seeliger tail (ls_tail_f64, from |u_S|^2 to the magnitude,
 #     with the log and log10 of astmath.h): 16 add, 32 mul, 27 fma, 5 div,
@@ -130,7 +128,6 @@ MIXED = 2
 _TAIL64 = {"ls": np.array([16, 32, 27, 5, 4, 0], float), "bw": np.array([6, 8, 8, 1, 0, 0], float)}
 _TAIL32 = {"ls": np.array([3, 25, 21, 7], float), "bw": np.array([0, 0, 0, 1], float)}  # add mul fma special
 _MX_F64_EXTRA = {"ls": np.array([13, 21, 12, 0, 3, 0], float), "bw": np.array([1, 1, 0, 0, 0, 0], float)}
-_MX_SINCOS_SAVED = np.array([0, 0, 3, 0, 0, 0], float)  # per sincos_mx call
 _W4 = np.array([1, 1, 2, 1], float)
 
 
@@ -140,11 +137,11 @@ def eval_flops_mixed(variant, mjd, split=None, seg_t=None):
         parts = [eval_flops_mixed(pv, m) for pv, m in segments(variant, mjd, seg_t)]
         return sum(a for a, _ in parts), sum(b for _, b in parts)
     key = table_key(variant)
-    torque, kind = key
+    kind = key[1]
     t = TABLE[key]
     n = len(mjd)
     tail = "rect" if kind == "rect" else "bw" if kind == "bw" else "ls"
-    f64_pt = np.array(t["per_point"], float) - (1 if torque else 2) * _MX_SINCOS_SAVED
+    f64_pt = np.array(t["per_point"], float)
     f32_pt = 0.0
     if tail != "rect":
         f64_pt = f64_pt - _TAIL64[tail] + _MX_F64_EXTRA[tail]

@@ -218,7 +218,7 @@ def periodogram(t, vm, n_clusters=5):
 
 
 MATH = {"sin": 0, "cos": 1, "tan": 2, "atan2": 3, "log": 4, "log10": 5, "exp": 6, "asin": 7,
-        "atan": 8, "expf": 9, "sin_mx": 10, "cos_mx": 11, "acos": 12, "powf": 13}
+        "atan": 8, "expf": 9, "sin_t": 10, "cos_t": 11, "acos": 12, "powf": 13}
 
 
 def math(fn, x, y=None):

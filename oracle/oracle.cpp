@@ -506,7 +506,7 @@ inline void ode_canon(bool torque, const R* y, R* f, const R* mu, R mu02, const 
   if (torque) {
     R s5, c5, s4, c4;
     if (sc) { s4 = sc[0]; c4 = sc[1]; s5 = sc[2]; c5 = sc[3]; }
-    else { astm::sincos(y[5], &s5, &c5); astm::sincos(y[4], &s4, &c4); }
+    else { astm::sincos_t(y[5], &s5, &c5); astm::sincos_t(y[4], &s4, &c4); }
     f[0] = p_fma(mu[0] * y[1], y[2], mu[3]);
     f[1] = p_fma(mu[1] * y[2], y[0], mu[4]);
     f[2] = p_fma(mu[2] * y[0], y[1], mu[5]);
@@ -515,9 +515,9 @@ inline void ode_canon(bool torque, const R* y, R* f, const R* mu, R mu02, const 
     f[5] = p_fma(-f[3], c4, y[2]);
   } else {
     R s2, c2, s1, c1;
-    astm::sincos(R(2.0) * y[2], &s2, &c2);
+    astm::sincos_t(R(2.0) * y[2], &s2, &c2);
     if (sc) { s1 = sc[0]; c1 = sc[1]; }
-    else astm::sincos(y[1], &s1, &c1);
+    else astm::sincos_t(y[1], &s1, &c1);
     f[0] = mu[0] * p_fma(-mu[2], c2, mu[1]);
     f[1] = (mu02 * s1) * s2;
     f[2] = c1 * (mu[0] - f[0]);
@@ -540,8 +540,8 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
   if (!accumulate)
     for (int m = 0; m < P.n_filters; ++m) sy2[m] = sy[m] = sw[m] = R(0.0);
   R sTM, cTM, sPM, cPM;
-  astm::sincos(R(params[P.types[T.theta_M]]), &sTM, &cTM);
-  astm::sincos(R(params[P.types[T.phi_M]]), &sPM, &cPM);
+  astm::sincos_t(R(params[P.types[T.theta_M]]), &sTM, &cTM);
+  astm::sincos_t(R(params[P.types[T.phi_M]]), &sPM, &cPM);
   R M_x = sTM * cPM, M_y = sTM * sPM, M_z = cTM;
   R XM = p_sqrt(p_fma(M_x, M_x, M_z * M_z));
   R XM_x = M_z / XM;
@@ -557,7 +557,7 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
   R Ii_inv = R(1.0) / Ii;
   R Es = R(params[P.types[T.Es]]), psi0 = R(params[P.types[T.psi_0]]), L = R(params[P.types[T.L]]);
   R sP0, cP0;
-  astm::sincos(psi0, &sP0, &cP0);
+  astm::sincos_t(psi0, &sP0, &cP0);
   R theta = astm::asin(p_sqrt((Es - R(1.0)) / (p_fma(sP0 * sP0, Ii_inv - Is_inv, Is_inv) - R(1.0))));
   R phi = R(params[P.types[T.phi_0]]);
   R psi = psi0;
@@ -567,7 +567,7 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
   // sin/cos of the current theta and psi, shared by the first RK4 stage of
   // the next step and by the body axes at the observation.
   R sc[4];
-  astm::sincos(theta, &sc[0], &sc[1]);
+  astm::sincos_t(theta, &sc[0], &sc[1]);
   sc[2] = sP0;
   sc[3] = cP0;
   if (torque) {
@@ -591,13 +591,13 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
   R sR = R(0.0), cR = R(0.0), stR = R(0.0), ctR = R(0.0);
   R sPR = R(0.0), cPR = R(0.0);
   if (P.v.rotate) {
-    astm::sincos(R(params[P.types[T.phi_R]]), &sR, &cR);
-    astm::sincos(R(params[P.types[T.theta_R]]), &stR, &ctR);
-    astm::sincos(R(params[P.types[T.psi_R]]), &sPR, &cPR);
+    astm::sincos_t(R(params[P.types[T.phi_R]]), &sR, &cR);
+    astm::sincos_t(R(params[P.types[T.theta_R]]), &stR, &ctR);
+    astm::sincos_t(R(params[P.types[T.psi_R]]), &sPR, &cPR);
   }
   if (P.v.bw_ball) {
-    astm::sincos(R(params[P.types[T.phi_R]]), &sR, &cR);
-    astm::sincos(R(params[P.types[T.theta_R]]), &stR, &ctR);
+    astm::sincos_t(R(params[P.types[T.phi_R]]), &sR, &cR);
+    astm::sincos_t(R(params[P.types[T.theta_R]]), &stR, &ctR);
     kappa = R(params[P.types[T.kappa]]);
   } else {
     R b = P.bc() ? R(params[P.types[T.b]]) : bt;
@@ -646,10 +646,10 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
         ode_canon(torque, f, K, mu, mu02);
         for (int j = 0; j < n_ode; ++j) y[j] = p_fma(h6, s[j] + K[j], y[j]);
         if (torque) {
-          astm::sincos(y[4], &sc[0], &sc[1]);
-          astm::sincos(y[5], &sc[2], &sc[3]);
+          astm::sincos_t(y[4], &sc[0], &sc[1]);
+          astm::sincos_t(y[5], &sc[2], &sc[3]);
         } else {
-          astm::sincos(y[1], &sc[0], &sc[1]);
+          astm::sincos_t(y[1], &sc[0], &sc[1]);
         }
       }
       if (torque) { Om_i = y[0]; Om_s = y[1]; Om_l = y[2]; phi = y[3]; theta = y[4]; psi = y[5]; }
@@ -662,10 +662,10 @@ inline void model_canon(const Problem& P, const double* params, R* sy2, R* sy, R
       t6[3] = p_val(Om_i); t6[4] = p_val(Om_s); t6[5] = p_val(Om_l);
     }
     R sf, cf, sp, cp;
-    astm::sincos(phi, &sf, &cf);
+    astm::sincos_t(phi, &sf, &cf);
     const R st = sc[0], ctt = sc[1];
     if (torque) { sp = sc[2]; cp = sc[3]; }
-    else astm::sincos(psi, &sp, &cp);
+    else astm::sincos_t(psi, &sp, &cp);
     R N_x = p_fma(YM_x, sf, XM_x * cf);
     R N_y = YM_y * sf;
     R N_z = p_fma(YM_z, sf, XM_z * cf);
@@ -1764,7 +1764,7 @@ int orc_run(const orc_problem* pb, int form, int n_groups, int group_size, int n
 }
 
 // Elementary functions over arrays (fn: 0 sin, 1 cos, 2 tan, 3 atan2(x,y as y,x), 4 log,
-// 5 log10, 6 exp, 7 asin, 8 atan, 9 expf).  For atan2, x holds y and y holds x.
+// 5 log10, 6 exp, 7 asin, 8 atan, 9 expf, 10/11 sin/cos of sincos_t, 12 acos, 13 powf).  For atan2, x holds y and y holds x.
 int orc_math(int fn, const double* x, const double* y, double* out, int n) {
   for (int k = 0; k < n; ++k) {
     double a = x[k];
@@ -1779,8 +1779,8 @@ int orc_math(int fn, const double* x, const double* y, double* out, int n) {
       case 7: out[k] = astm::asin(a); break;
       case 8: out[k] = astm::atan(a); break;
       case 9: out[k] = astm::expf((float)a); break;
-      case 10: { double sn, cs; astm::sincos_mx(a, &sn, &cs); out[k] = sn; break; }
-      case 11: { double sn, cs; astm::sincos_mx(a, &sn, &cs); out[k] = cs; break; }
+      case 10: { double sn, cs; astm::sincos_t(a, &sn, &cs); out[k] = sn; break; }
+      case 11: { double sn, cs; astm::sincos_t(a, &sn, &cs); out[k] = cs; break; }
       case 12: out[k] = astm::acos(a); break;
       case 13: out[k] = astm::powf((float)a, (float)y[k]); break;
       default: return AST_E_ARG;

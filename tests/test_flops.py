@@ -51,29 +51,11 @@ def test_mixed_table_consistent():
     f64, f32 = flops.eval_flops_mixed(vmx, curve.mjd)
     w = np.array([1, 1, 2, 1, 1, 1], float)
     expect64 = flops.eval_flops(v64, curve.mjd) - 50 * (flops._TAIL64["ls"] - flops._MX_F64_EXTRA["ls"]) @ w
-    expect64 -= 50 * 6  # sincos_mx of phi: 3 fma fewer
     assert f64 == expect64 and f32 == 50 * 77 and flops.eval_flops(vmx, curve.mjd) == f64 + f32
-    # RECT keeps its FP64 tail: only the body axes' sin/cos change
+    # RECT keeps its FP64 tail, and the body axes are the FP64 form's: the same count
     r64, rmx = variants.rect(abi.PREC_FP64), variants.rect(abi.PREC_MIXED)
     f64, f32 = flops.eval_flops_mixed(rmx, curve.mjd)
-    assert f32 == 0 and f64 == flops.eval_flops(r64, curve.mjd) - 50 * 12  # psi and phi without torque
-
-
-def test_sincos_mx_saves_three_fma():
-    """The mixed body axes' sin/cos: the FP64 sincos has a three-part pi/2
-    reduction and degree-5 kernels, sincos_mx a two-part one and degree-4
-    kernels; their fma counts differ by 3, read off astmath.h's source."""
-    import os
-    import re
-    src = open(os.path.join(os.path.dirname(flops.__file__), "csrc", "astmath.h")).read()
-
-    def body(name):
-        m = re.search(r"template <class R> AST_HD (?:R|void) %s\(R [a-z]\w*(?:, R\* s, R\* c)?\) \{(.*?)\n\}" % name,
-                      src, re.S)
-        return m.group(1)
-    n_full = body("sincos").count("p_fma") + body("k_sin").count("p_fma") + body("k_cos").count("p_fma")
-    n_mx = body("sincos_mx").count("p_fma")
-    assert n_full - n_mx == flops._MX_SINCOS_SAVED[2] == 3
+    assert f32 == 0 and f64 == flops.eval_flops(r64, curve.mjd)
 
 
 def test_segment_counts_are_the_sum_over_segments():
