@@ -344,6 +344,27 @@ __device__ inline void body_axes(const SpinFrame<T>& F, T sf, T cf, T st, T ctt,
   c[2] = fmx(-a[1], b[0], a[0] * b[1]);
 }
 
+// n / d rounded as IEEE division rounds it, in 8 instructions where the
+// compiler's division takes 11: the hardware reciprocal estimate refined by
+// two Newton steps, the quotient and one residual correction, the same
+// operations in the same order as the compiler's sequence without its range
+// scaling (v_div_scale, v_div_fmas) and special-case fix-up (v_div_fixup).
+// Those act only when d or the quotient is subnormal, |n| < 2^-969, the
+// quotient overflows, or an operand is 0, infinite or NaN; the one caller,
+// the torque ODE's division by sin(theta), never meets them with a finite
+// state (theta would have to lie within 1e-300 of a multiple of pi), and
+// where sin(theta) is exactly 0 this gives NaN where the division gives an
+// infinity that turns into NaN at the next sin/cos.
+__device__ inline double div_rn(double n, double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = __builtin_fma(-d, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-d, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  const double q = n * r;
+  return __builtin_fma(__builtin_fma(-d, q, n), r, q);
+}
+
 // ODE right-hand side.  With FIRST the sin/cos of theta (and psi) come from
 // sc[] (those of the state, shared with the body axes), else they are computed.
 template <bool TQ, bool FIRST>
@@ -355,7 +376,7 @@ __device__ inline void ode(const double* y, double* f, const double* mu, double 
     f[0] = __builtin_fma(mu[0] * y[1], y[2], mu[3]);
     f[1] = __builtin_fma(mu[1] * y[2], y[0], mu[4]);
     f[2] = __builtin_fma(mu[2] * y[0], y[1], mu[5]);
-    f[3] = __builtin_fma(y[1], c5, y[0] * s5) / s4;
+    f[3] = div_rn(__builtin_fma(y[1], c5, y[0] * s5), s4);
     f[4] = __builtin_fma(-y[1], s5, y[0] * c5);
     f[5] = __builtin_fma(-f[3], c4, y[2]);
   } else {   // cuda.c:69-75

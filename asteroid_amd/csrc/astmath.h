@@ -60,10 +60,13 @@ AST_HD double p_flipsign(double x, int flag) {
   return of_bits(bits_of(x) ^ ((uint64_t)(uint32_t)(flag & 2) << 62));
 }
 
-// Low word of an integer-valued double k, |k| < 2^51: k + 1.5*2^52 is exact
-// and holds k in two's complement in its low mantissa bits.  One add, where a
-// range-checked float-to-int conversion takes three instructions.
-AST_HD uint32_t low_word_of_int(double k) { return (uint32_t)bits_of(k + 6755399441055744.0); }
+// 1.5*2^52: for |v| < 2^51, v + RINT_SHIFT rounds v to the nearest integer
+// k (ties to even) and holds k in two's complement in its low mantissa bits.
+constexpr double RINT_SHIFT = 6755399441055744.0;
+// Low word of an integer-valued double k, |k| < 2^51: k + RINT_SHIFT is exact.
+// One add, where a range-checked float-to-int conversion takes three
+// instructions.
+AST_HD uint32_t low_word_of_int(double k) { return (uint32_t)bits_of(k + RINT_SHIFT); }
 // x with its sign bit xored by bit 31 of m; m's other bits are ignored
 // (exact, not counted).  On the GPU the mask and the xor into the high word
 // are one v_bitop3_b32 (hi ^ (m & 0x80000000), table 0x78).
@@ -196,15 +199,19 @@ inline void sctab_get(uint32_t i, double* s, double* c) {
 // C = cos of (k mod 128) pi/64 from the table, sin r = r + r^3 ST(r^2) and
 // cos r - 1 = r^2 (-1/2 + r^2 CT(r^2)), then
 //   sin x = S + (S (cos r - 1) + C sin r),  cos x = C + (C (cos r - 1) - S sin r).
-// About 1 ulp, with no quadrant logic: 23 instructions and one table read
-// on the GPU against 39 for sincos.  Non-finite x gives NaN for both.
+// k is x 64/pi rounded once, by an fma onto RINT_SHIFT (which also leaves k
+// in the low word for the table index), and RINT_SHIFT taken off again.
+// About 1 ulp for |x| < 2^51 pi/64, with no quadrant logic: 22 instructions
+// and one table read on the GPU against 39 for sincos.  Non-finite x gives
+// NaN for both.
 template <class R> AST_HD void sincos_t(R x, R* s, R* c) {
-  R k = p_rint(x * R(AST_64_OVER_PI));
+  const R kb = p_fma(x, R(AST_64_OVER_PI), R(RINT_SHIFT));
+  const R k = kb - R(RINT_SHIFT);
   R r = p_fma(-k, R(AST_PIO64_1), x);
   r = p_fma(-k, R(AST_PIO64_2), r);
   r = p_fma(-k, R(AST_PIO64_3), r);
   double tsv, tcv;
-  sctab_get(low_word_of_int(p_val(k)) & (AST_SCTAB_N - 1), &tsv, &tcv);
+  sctab_get((uint32_t)bits_of(p_val(kb)) & (AST_SCTAB_N - 1), &tsv, &tcv);
   const R S(tsv), C(tcv);
   R z = r * r;
   R sr = p_fma(r * z, p_fma(z, p_fma(z, R(AST_SINT_2), R(AST_SINT_1)), R(AST_SINT_0)), r);
