@@ -11,8 +11,8 @@
 // Methods (coefficients from gen_math_coeffs.py, Chebyshev fits at 60 digits):
 //   sincos: Cody-Waite reduction by pi/2 (3-part constant, fma), odd sine and
 //           even cosine kernels on |r| <= pi/4, quadrant swap/sign.
-//   sincos_t: table-driven (Tang's method): x = k pi/64 + r (3-part pi/64,
-//           |r| <= pi/128), sin and cos of k pi/64 from a 128-entry table,
+//   sincos_t: table-driven (Tang's method): x = k pi/512 + r (3-part pi/512,
+//           |r| <= pi/1024), sin and cos of k pi/512 from a 1024-entry table,
 //           short kernels for sin r and cos r - 1, angle addition.
 //   atan2 : ratio t = min/max in [0,1]; for t > tan(pi/8) the identity
 //           atan t = pi/4 + atan((t-1)/(t+1)); odd kernel on |u| <= tan(pi/8);
@@ -166,8 +166,8 @@ template <class R> AST_HD void sincos(R x, R* s, R* c) {
   *c = p_xorsign(c0, (q + 1) << 30);
 }
 
-// Table of sin and cos of k pi/64, k = 0..127, for sincos_t.  On the host a
-// constant array; on the GPU a copy in LDS, so that 64 lanes with 64
+// Table of sin and cos of k pi/512, k = 0..1023, for sincos_t.  On the host a
+// constant array; on the GPU a 16 KB copy in LDS, so that 64 lanes with 64
 // different k read it in one ds_read_b128 each: a kernel that calls sincos_t
 // must call sincos_table_load() first, with every thread of the workgroup
 // (it ends with a barrier); until then the copy holds garbage.
@@ -194,28 +194,28 @@ inline void sctab_get(uint32_t i, double* s, double* c) {
 }
 #endif
 
-// sin and cos of x, table-driven (Tang's method): x = k pi/64 + r by a
-// Cody-Waite reduction with a three-part pi/64 (|r| <= pi/128), S = sin and
-// C = cos of (k mod 128) pi/64 from the table, sin r = r + r^3 ST(r^2) and
-// cos r - 1 = r^2 (-1/2 + r^2 CT(r^2)), then
+// sin and cos of x, table-driven (Tang's method): x = k pi/512 + r by a
+// Cody-Waite reduction with a three-part pi/512 (|r| <= pi/1024), S = sin and
+// C = cos of (k mod 1024) pi/512 from the table, sin r = r + r^3 ST(r^2) and
+// cos r - 1 = r^2 (-1/2 + r^2 CT), then
 //   sin x = S + (S (cos r - 1) + C sin r),  cos x = C + (C (cos r - 1) - S sin r).
-// k is x 64/pi rounded once, by an fma onto RINT_SHIFT (which also leaves k
+// k is x 512/pi rounded once, by an fma onto RINT_SHIFT (which also leaves k
 // in the low word for the table index), and RINT_SHIFT taken off again.
-// About 1 ulp for |x| < 2^51 pi/64, with no quadrant logic: 22 instructions
+// About 1 ulp for |x| < 2^51 pi/512, with no quadrant logic: 20 instructions
 // and one table read on the GPU against 39 for sincos.  Non-finite x gives
 // NaN for both.
 template <class R> AST_HD void sincos_t(R x, R* s, R* c) {
-  const R kb = p_fma(x, R(AST_64_OVER_PI), R(RINT_SHIFT));
+  const R kb = p_fma(x, R(AST_SCTAB_INV_STEP), R(RINT_SHIFT));
   const R k = kb - R(RINT_SHIFT);
-  R r = p_fma(-k, R(AST_PIO64_1), x);
-  r = p_fma(-k, R(AST_PIO64_2), r);
-  r = p_fma(-k, R(AST_PIO64_3), r);
+  R r = p_fma(-k, R(AST_SCTAB_STEP_1), x);
+  r = p_fma(-k, R(AST_SCTAB_STEP_2), r);
+  r = p_fma(-k, R(AST_SCTAB_STEP_3), r);
   double tsv, tcv;
   sctab_get((uint32_t)bits_of(p_val(kb)) & (AST_SCTAB_N - 1), &tsv, &tcv);
   const R S(tsv), C(tcv);
   R z = r * r;
-  R sr = p_fma(r * z, p_fma(z, p_fma(z, R(AST_SINT_2), R(AST_SINT_1)), R(AST_SINT_0)), r);
-  R cm = z * p_fma(z, p_fma(z, R(AST_COST_1), R(AST_COST_0)), R(-0.5));
+  R sr = p_fma(r * z, p_fma(z, R(AST_SINT_1), R(AST_SINT_0)), r);
+  R cm = z * p_fma(z, R(AST_COST_0), R(-0.5));
   *s = S + p_fma(S, cm, C * sr);
   *c = C + p_fma(C, cm, -(S * sr));
 }

@@ -112,17 +112,22 @@ def test_coefficients_regenerate():
 
 @pytest.mark.parametrize("name,ref", [("sin_t", mp.sin), ("cos_t", mp.cos)])
 def test_table_driven_sincos(name, ref):
-    """The evaluator's table-driven sin/cos (Tang's method, 128-entry table of
-    k pi/64) within 1 ulp of mpmath on small and large arguments (the Euler
-    angles reach ~1e4 rad after days of fast rotation), exact at the table
-    points, NaN for non-finite arguments."""
+    """The evaluator's table-driven sin/cos (Tang's method, 1024-entry table of
+    k pi/512) within 1 ulp of mpmath on small and large arguments (the Euler
+    angles reach ~1e4 rad after days of fast rotation), at the table points
+    and half-way between them (where the reduction's |r| is largest and k's
+    rounding decides), NaN for non-finite arguments."""
     rng = np.random.default_rng(21)
     x = np.concatenate([rng.uniform(-4, 4, 3000), rng.uniform(-2e4, 2e4, 3000), rng.uniform(-1e-3, 1e-3, 500)])
     got = oracle.math(name, x)
     err = _ulps(got, [ref(mp.mpf(float(v))) for v in x])
     assert err.max() <= 1.0, err.max()
-    k = np.arange(-300, 300)
-    pts = k * (np.pi / 64)  # the rounded k pi/64: within 1 ulp of the true values there too
+    k = np.arange(-3000, 3000)
+    pts = k * (np.pi / 512)  # the rounded k pi/512: within 1 ulp of the true values there too
+    err = _ulps(oracle.math(name, pts), [ref(mp.mpf(float(v))) for v in pts])
+    assert err.max() <= 1.0
+    half = (k + 0.5) * (np.pi / 512)
+    pts = np.concatenate([half, np.nextafter(half, np.inf), np.nextafter(half, -np.inf)])
     err = _ulps(oracle.math(name, pts), [ref(mp.mpf(float(v))) for v in pts])
     assert err.max() <= 1.0
     assert np.isnan(oracle.math(name, np.array([np.inf, -np.inf, np.nan]))).all()
