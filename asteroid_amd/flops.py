@@ -22,26 +22,26 @@ _W = np.array([1, 1, 2, 1, 1, 1], float)
 # TORQUE2 is the torque row: its split interval runs as two RK4 stretches,
 # counted by rk4_steps when the split fraction is given.
 TABLE = {
-    (True, "ls"): dict(per_model=(24, 41, 53, 11, 3, 0), per_point=(21, 68, 67, 5, 4, 0),
-                       per_rk4=(30, 60, 132, 4, 0, 0)),
-    (False, "ls"): dict(per_model=(23, 35, 53, 11, 3, 0), per_point=(24, 73, 76, 5, 4, 0),
-                        per_rk4=(31, 60, 94, 0, 0, 0)),
-    (True, "bw"): dict(per_model=(30, 50, 71, 9, 3, 0), per_point=(13, 44, 48, 1, 0, 0),
-                       per_rk4=(30, 60, 132, 4, 0, 0)),
-    (False, "bw"): dict(per_model=(29, 44, 71, 9, 3, 0), per_point=(16, 49, 57, 1, 0, 0),
-                        per_rk4=(31, 60, 94, 0, 0, 0)),
+    (True, "ls"): dict(per_model=(16, 33, 61, 11, 3, 0), per_point=(19, 66, 69, 5, 4, 0),
+                       per_rk4=(14, 44, 148, 4, 0, 0)),
+    (False, "ls"): dict(per_model=(15, 27, 61, 11, 3, 0), per_point=(20, 69, 80, 5, 4, 0),
+                        per_rk4=(15, 44, 110, 0, 0, 0)),
+    (True, "bw"): dict(per_model=(18, 38, 83, 9, 3, 0), per_point=(11, 42, 50, 1, 0, 0),
+                       per_rk4=(14, 44, 148, 4, 0, 0)),
+    (False, "bw"): dict(per_model=(17, 32, 83, 9, 3, 0), per_point=(12, 45, 61, 1, 0, 0),
+                        per_rk4=(15, 44, 110, 0, 0, 0)),
     # TREND: the LS torque row plus the de-trending fma (the phase angle is a
     # property of the point, computed once on the host)
-    (True, "ls+trend"): dict(per_model=(24, 41, 53, 11, 3, 0), per_point=(21, 68, 68, 5, 4, 0),
-                             per_rk4=(30, 60, 132, 4, 0, 0)),
-    (True, "ls+rot"): dict(per_model=(33, 56, 80, 11, 3, 0), per_point=(21, 86, 85, 5, 4, 0),
-                           per_rk4=(30, 60, 132, 4, 0, 0)),
-    (False, "ls+rot"): dict(per_model=(32, 50, 80, 11, 3, 0), per_point=(24, 91, 94, 5, 4, 0),
-                            per_rk4=(31, 60, 94, 0, 0, 0)),
-    (True, "rect"): dict(per_model=(24, 41, 53, 11, 3, 0), per_point=(13, 63, 66, 1, 3, 0),
-                         per_rk4=(30, 60, 132, 4, 0, 0)),
-    (False, "rect"): dict(per_model=(23, 35, 53, 11, 3, 0), per_point=(16, 68, 75, 1, 3, 0),
-                          per_rk4=(31, 60, 94, 0, 0, 0)),
+    (True, "ls+trend"): dict(per_model=(16, 33, 61, 11, 3, 0), per_point=(19, 66, 70, 5, 4, 0),
+                             per_rk4=(14, 44, 148, 4, 0, 0)),
+    (True, "ls+rot"): dict(per_model=(19, 42, 94, 11, 3, 0), per_point=(19, 84, 87, 5, 4, 0),
+                           per_rk4=(14, 44, 148, 4, 0, 0)),
+    (False, "ls+rot"): dict(per_model=(18, 36, 94, 11, 3, 0), per_point=(20, 87, 98, 5, 4, 0),
+                            per_rk4=(15, 44, 110, 0, 0, 0)),
+    (True, "rect"): dict(per_model=(16, 33, 61, 11, 3, 0), per_point=(11, 61, 68, 1, 3, 0),
+                         per_rk4=(14, 44, 148, 4, 0, 0)),
+    (False, "rect"): dict(per_model=(15, 27, 61, 11, 3, 0), per_point=(12, 64, 79, 1, 3, 0),
+                          per_rk4=(15, 44, 110, 0, 0, 0)),
 }
 
 
