@@ -2551,15 +2551,23 @@ int ast_autotune(ast_ctx* c, const ast_job* jobs, int n_jobs, const ast_batch_cf
     double ms[3] = {0, 0, 0};
     int best = 4;
     double best_ms = 0;
-    for (int w = 0; w < 3; ++w) {
-      if (cand[w] < 4 && G > 256) continue;
-      c->waves[k] = cand[w];
-      int launches = 0;
-      // the same jobs from the same start each time: identical work, so the
-      // kernel times compare the register budgets alone
-      RETIF(batch_run(c, kv.second, G, work, cycles, nullptr, &ms[w], &launches, false));
-      if (best_ms == 0 || ms[w] < best_ms) { best_ms = ms[w]; best = cand[w]; }
+    // Two interleaved passes over the candidates, each kept at its faster
+    // pass: the builds often run within a few percent of each other, so one
+    // timing each would let clock drift between them pick the build.
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int w = 0; w < 3; ++w) {
+        if (cand[w] < 4 && G > 256) continue;
+        c->waves[k] = cand[w];
+        int launches = 0;
+        double t = 0;
+        // the same jobs from the same start each time: identical work, so the
+        // kernel times compare the register budgets alone
+        RETIF(batch_run(c, kv.second, G, work, cycles, nullptr, &t, &launches, false));
+        if (pass == 0 || t < ms[w]) ms[w] = t;
+      }
     }
+    for (int w = 0; w < 3; ++w)
+      if (ms[w] > 0 && (best_ms == 0 || ms[w] < best_ms)) { best_ms = ms[w]; best = cand[w]; }
     c->waves[k] = best;
     if (*n_out < max_out) {
       ast_tune_entry& e = out[(*n_out)++];

@@ -496,7 +496,8 @@ int ast_run_jobs(ast_ctx* ctx, const ast_job* jobs, int n_jobs, const ast_batch_
  * of up to 256.  They compute exactly the same results.  ast_autotune runs
  * `cycles` launch cycles of the given jobs (one class at a time, from the
  * same start for every budget, so the work is identical) at each budget,
- * keeps the fastest for each class in the context, and reports the kernel
+ * twice, interleaved, keeping each budget's faster time; it keeps the
+ * fastest budget for each class in the context and reports those kernel
  * times; ast_set_waves sets one class by hand.  Untuned classes use 4. */
 typedef struct {
   int torque, bc, bw_ball, precision, filter_capacity;
