@@ -391,6 +391,11 @@ __device__ inline void ode(const double* y, double* f, const double* mu, double 
 }
 
 typedef __attribute__((address_space(4))) const PointRec* PointPtr;
+
+// An empty asm that takes a wave-uniform value in scalar registers and
+// "returns" it: the value must be computed (loaded) before this point and
+// stays in registers after it.  No instruction is emitted.
+template <class T> __device__ inline void pin_sgpr(T& v) { asm volatile("" : "+s"(v)); }
 typedef __attribute__((address_space(4))) const TailRec* TailPtr;
 
 // ------------------------------------------------------------------ NUDGE
@@ -867,14 +872,24 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
   // inside the table; its values are never used).
   int nx_steps = pts[0].nsteps;  // 0 for the first observation
   double nx_h = pts[0].h, nx_hh = pts[0].hh, nx_h6 = pts[0].h6;
+  // The observation's own fields are read one point ahead too, and all of
+  // the next record is read right after the body axes: the scalar loads and
+  // the LDS table reads share one wait counter, so loads issued anywhere
+  // else would be waited for at the next sin/cos, a few instructions on,
+  // while after the axes the point's brightness and chi^2 (no table read)
+  // cover their latency.
+  double nx_E[3] = {pts[0].E[0], pts[0].E[1], pts[0].E[2]};
+  double nx_S[3] = {pts[0].S[0], pts[0].S[1], pts[0].S[2]};
+  float nx_V = pts[0].V, nx_w = pts[0].w;
+  int nx_filter = pts[0].filter;
   for (int i = 0; i < n_data; ++i) {
     const auto& pr = pts[i];
     int nsteps = nx_steps;
     double h = nx_h, hh = nx_hh, h6 = nx_h6;
-    nx_steps = pts[i + 1].nsteps;
-    nx_h = pts[i + 1].h;
-    nx_hh = pts[i + 1].hh;
-    nx_h6 = pts[i + 1].h6;
+    const double E[3] = {nx_E[0], nx_E[1], nx_E[2]};
+    const double S[3] = {nx_S[0], nx_S[1], nx_S[2]};
+    const float pV = nx_V, pw = nx_w;
+    const int pfilter = nx_filter;
     if constexpr (SG) {
       // The first point of segment k (its record's step count is -1 - k;
       // a table that opens with segment 0 has no marker there): the
@@ -935,9 +950,19 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
     }  // sub-intervals
     if (TQ) { phi = y[3]; theta = y[4]; psi = y[5]; }
     else { phi = y[0]; theta = y[1]; psi = y[2]; }
-    const double E[3] = {pr.E[0], pr.E[1], pr.E[2]};
     double a[3], b[3], c[3];
     axes_at<TQ>(Fd, phi, psi, sc, a, b, c);
+    __builtin_amdgcn_sched_barrier(0);  // keep the next record's loads below the axes' table read
+    nx_steps = pts[i + 1].nsteps;
+    nx_h = pts[i + 1].h;
+    nx_hh = pts[i + 1].hh;
+    nx_h6 = pts[i + 1].h6;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { nx_E[k] = pts[i + 1].E[k]; nx_S[k] = pts[i + 1].S[k]; }
+    nx_V = pts[i + 1].V;
+    nx_w = pts[i + 1].w;
+    nx_filter = pts[i + 1].filter;
+    __builtin_amdgcn_sched_barrier(0);
     if constexpr (RO) rotate_axes(a, b, c, K);
     double Vmod;
     if constexpr (RC) {
@@ -946,7 +971,6 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
       const double arg = bw_arg_f64(a, b, c, E, K);
       Vmod = MX ? M2P5_LOG10_2 * log2_f32x((float)arg) : -2.5 * astm::log10(arg);
     } else {
-      const double S[3] = {pr.S[0], pr.S[1], pr.S[2]};
       const ScaledDirs u = scaled_dirs(a, b, c, E, S, K);
       if (MX) {
         Vmod = ls_tail_mx(u, bcpf);
@@ -955,6 +979,19 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
       }
     }
     if constexpr (TR) Vmod = __builtin_fma(-trendA, pr.aux, Vmod);  // cuda.c:692-697
+    // The next record's values are pinned in SGPRs here, after the
+    // brightness (an empty asm that "changes" them): the compiler can then
+    // neither sink their loads to this point nor drop them and read the
+    // record again next to a table read.
+    pin_sgpr(nx_steps);
+    pin_sgpr(nx_h);
+    pin_sgpr(nx_hh);
+    pin_sgpr(nx_h6);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { pin_sgpr(nx_E[k]); pin_sgpr(nx_S[k]); }
+    pin_sgpr(nx_V);
+    pin_sgpr(nx_w);
+    pin_sgpr(nx_filter);
     if constexpr (FORM == FORM_PLOT) {
       // Under NUDGE the reference stops the curve at a failed model
       // (cuda.c:735-738); the rest of the curve is NaN here.
@@ -962,9 +999,9 @@ __device__ CF chi2_model(const double* params, PointPtr pts, int n_data, int n_f
       if constexpr (NU) nu.point(i, pr.aux, Vmod, mr);
       continue;
     }
-    const double w = (double)pr.w;
-    const double yv = (double)pr.V - Vmod;
-    const int m = pr.filter;
+    const double w = (double)pw;
+    const double yv = (double)pV - Vmod;
+    const int m = pfilter;
     if (NF == 1) {
       sy2[0] = __builtin_fma(yv * yv, w, sy2[0]);
       sy[0] = __builtin_fma(yv, w, sy[0]);

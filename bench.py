@@ -138,8 +138,18 @@ def run_stage_one(prec, args):
     ctx = api.Context(v)
     ctx.set_model_space(space)
     ctx.upload_curve(curve)
-    tune = ctx.autotune([dict(space=space, curve=curve, n_fits=args.fits, n_steps=5000, eval_budget=args.budget,
-                              size_min=SIZE_MIN, seed=1)], cycles=4) if args.autotune else []
+    if args.waves:
+        ctx.set_waves(v, 1, args.waves)
+        tune = [{"waves": args.waves, "set_by_hand": True}]
+    elif args.autotune:
+        # one launch per register budget and pass, of args.ticks x n_obs
+        # model-points of work (about 50 evaluation rounds here: the launch size
+        # also counts each RK4 step as ~1.9 observations)
+        tune = ctx.autotune([dict(space=space, curve=curve, n_fits=args.fits, n_steps=5000, eval_budget=args.budget,
+                                  size_min=SIZE_MIN, seed=1)],
+                            work_points_per_launch=float(args.ticks * len(curve.mjd)), cycles=1)
+    else:
+        tune = []
     ctx.rng_seed(1)
     name = PREC_NAMES[prec]
     # enough fits in the queue that it cannot run dry during the steps
@@ -881,6 +891,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
+    ap.add_argument("--waves", type=int, choices=[0, 2, 3, 4], default=0,
+                    help="Stage One: run the fitting kernel's build for this many waves per SIMD (0: autotune)")
     ap.add_argument("--no-graphs", dest="graphs", action="store_false")
     args = ap.parse_args()
 
