@@ -344,27 +344,6 @@ __device__ inline void body_axes(const SpinFrame<T>& F, T sf, T cf, T st, T ctt,
   c[2] = fmx(-a[1], b[0], a[0] * b[1]);
 }
 
-// n / d rounded as IEEE division rounds it, in 8 instructions where the
-// compiler's division takes 11: the hardware reciprocal estimate refined by
-// two Newton steps, the quotient and one residual correction, the same
-// operations in the same order as the compiler's sequence without its range
-// scaling (v_div_scale, v_div_fmas) and special-case fix-up (v_div_fixup).
-// Those act only when d or the quotient is subnormal, |n| < 2^-969, the
-// quotient overflows, or an operand is 0, infinite or NaN; the one caller,
-// the torque ODE's division by sin(theta), never meets them with a finite
-// state (theta would have to lie within 1e-300 of a multiple of pi), and
-// where sin(theta) is exactly 0 this gives NaN where the division gives an
-// infinity that turns into NaN at the next sin/cos.
-__device__ inline double div_rn(double n, double d) {
-  double r = __builtin_amdgcn_rcp(d);
-  double e = __builtin_fma(-d, r, 1.0);
-  r = __builtin_fma(r, e, r);
-  e = __builtin_fma(-d, r, 1.0);
-  r = __builtin_fma(r, e, r);
-  const double q = n * r;
-  return __builtin_fma(__builtin_fma(-d, q, n), r, q);
-}
-
 // ODE right-hand side.  With FIRST the sin/cos of theta (and psi) come from
 // sc[] (those of the state, shared with the body axes), else they are computed.
 template <bool TQ, bool FIRST>
@@ -376,7 +355,7 @@ __device__ inline void ode(const double* y, double* f, const double* mu, double 
     f[0] = __builtin_fma(mu[0] * y[1], y[2], mu[3]);
     f[1] = __builtin_fma(mu[1] * y[2], y[0], mu[4]);
     f[2] = __builtin_fma(mu[2] * y[0], y[1], mu[5]);
-    f[3] = div_rn(__builtin_fma(y[1], c5, y[0] * s5), s4);
+    f[3] = astm::div_rn(__builtin_fma(y[1], c5, y[0] * s5), s4);
     f[4] = __builtin_fma(-y[1], s5, y[0] * c5);
     f[5] = __builtin_fma(-f[3], c4, y[2]);
   } else {   // cuda.c:69-75
@@ -588,7 +567,7 @@ AST_AI double rect_mag(const double* a, const double* b, const double* c, const 
   if (sab < 0.0) sab = 0.0;
   if (sbc < 0.0) sbc = 0.0;
   if (sac < 0.0) sac = 0.0;
-  return -2.5 * astm::log10(4.0 * ((__builtin_sqrt(sab) + __builtin_sqrt(sbc)) + __builtin_sqrt(sac)));
+  return -2.5 * astm::log10(4.0 * ((astm::sqrt_rn(sab) + astm::sqrt_rn(sbc)) + astm::sqrt_rn(sac)));
 }
 
 // Sun and Earth directions in the body frame scaled by the ellipsoid's axes
@@ -615,17 +594,17 @@ AST_AI ScaledDirs scaled_dirs(const double* a, const double* b, const double* c,
 AST_AI double ls_tail_f64(const ScaledDirs& u, double bcp) {
   const double sS2 = dot3(u.uSb, u.uSc, u.uSa, u.uSb, u.uSc, u.uSa);
   const double sE2 = dot3(u.uEb, u.uEc, u.uEa, u.uEb, u.uEc, u.uEa);
-  const double sS = __builtin_sqrt(sS2);
-  const double sE = __builtin_sqrt(sE2);
-  const double ca = dot3(u.uSb, u.uSc, u.uSa, u.uEb, u.uEc, u.uEa) / (sS * sE);
-  const double sa = __builtin_sqrt(__builtin_fma(-ca, ca, 1.0));
-  const double scn = __builtin_sqrt(__builtin_fma((2.0 * sS) * sE, ca, sS2 + sE2));
-  const double isc = 1.0 / scn;
+  const double sS = astm::sqrt_rn(sS2);
+  const double sE = astm::sqrt_rn(sE2);
+  const double ca = astm::div_rn(dot3(u.uSb, u.uSc, u.uSa, u.uEb, u.uEc, u.uEa), sS * sE);
+  const double sa = astm::sqrt_rn(__builtin_fma(-ca, ca, 1.0));
+  const double scn = astm::sqrt_rn(__builtin_fma((2.0 * sS) * sE, ca, sS2 + sE2));
+  const double isc = astm::div_rn(1.0, scn);
   const double cl = __builtin_fma(sE, ca, sS) * isc;
   const double sl = (sE * sa) * isc;
   const double cd = __builtin_fma(cl, ca, sl * sa);
   const double sd = __builtin_fma(-cl, sa, sl * ca);
-  const double lg = astm::log(((1.0 + cl) * (1.0 + cd)) / (sl * (-sd)));
+  const double lg = astm::log(astm::div_rn((1.0 + cl) * (1.0 + cd), sl * (-sd)));
   const double inner = __builtin_fma(sl * sd, lg, cd + cl);
   return -2.5 * astm::log10((((bcp * sS) * sE) * isc) * inner);
 }

@@ -1494,6 +1494,7 @@ inline R operator/(R a, R b) { K.div++; return R(a.v / b.v); }
 inline R operator-(R a) { return R(-a.v); }  // sign flip: an operand modifier, not an operation
 inline R p_fma(R a, R b, R c) { K.fma++; return R(__builtin_fma(a.v, b.v, c.v)); }
 inline R p_sqrt(R a) { K.sqrt++; return R(__builtin_sqrt(a.v)); }
+inline R p_div(R a, R b) { K.div++; return R(a.v / b.v); }
 inline R p_rint(R a) { K.rnd++; return R(__builtin_rint(a.v)); }
 inline R p_floor(R a) { K.rnd++; return R(__builtin_floor(a.v)); }
 inline R p_fabs(R a) { return R(__builtin_fabs(a.v)); }
