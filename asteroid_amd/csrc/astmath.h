@@ -309,8 +309,33 @@ template <class R> AST_HD R acos(R x) {
 }
 
 // Natural logarithm: -inf at 0, NaN below 0 or for NaN, +inf at +inf.
+// log(m 2^e) for m in (sqrt(1/2), sqrt(2)]: the kernel and the e ln2 terms.
+template <class R> AST_HD R log_split(double mv, int e) {
+  R m = R(mv);
+  R f = m - R(1.0);
+  R s = p_div(f, R(2.0) + f);
+  R z = s * s;
+  R rz = z * k_log(z);
+  R hfsq = R(0.5) * f * f;
+  R dk = R((double)e);
+  return dk * R(AST_LN2_HI) - ((hfsq - p_fma(s, hfsq + rz, dk * R(AST_LN2_LO))) - f);
+}
+
 template <class R> AST_HD R log(R x) {
   double xv = p_val(x);
+#if defined(__HIP_DEVICE_COMPILE__)
+  // Without branches on the GPU: the hardware splits x = m 2^E with m in
+  // [0.5, 1) (exactly, subnormals included), which gives the same m, e as
+  // the exponent-field split below, and the special cases are selected at
+  // the end.
+  const double m = __builtin_amdgcn_frexp_mant(xv);
+  const int E = __builtin_amdgcn_frexp_exp(xv);
+  const double m2 = 2.0 * m;
+  const bool big = m2 > 1.4142135623730951;
+  const R r = log_split<R>(big ? m : m2, big ? E : E - 1);
+  const double sp = xv == 0.0 ? -__builtin_inf() : (xv == __builtin_inf() ? xv : __builtin_nan(""));
+  return __builtin_amdgcn_class(xv, 0x180) ? r : R(sp);  // class bits of +subnormal, +normal
+#else
   if (!(xv > 0.0)) return R(xv == 0.0 ? -__builtin_inf() : __builtin_nan(""));
   if (xv == __builtin_inf()) return x;
   uint64_t b = bits_of(xv);
@@ -324,14 +349,8 @@ template <class R> AST_HD R log(R x) {
   int e = ex - 1023 - sub;
   double mv = of_bits((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);  // [1,2)
   if (mv > 1.4142135623730951) { mv *= 0.5; e += 1; }
-  R m = R(mv);
-  R f = m - R(1.0);
-  R s = p_div(f, R(2.0) + f);
-  R z = s * s;
-  R rz = z * k_log(z);
-  R hfsq = R(0.5) * f * f;
-  R dk = R((double)e);
-  return dk * R(AST_LN2_HI) - ((hfsq - p_fma(s, hfsq + rz, dk * R(AST_LN2_LO))) - f);
+  return log_split<R>(mv, e);
+#endif
 }
 
 template <class R> AST_HD R log10(R x) {
