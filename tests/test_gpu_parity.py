@@ -56,6 +56,33 @@ def test_chi2_batch_bitwise(name, prec):
     assert np.array_equal(got_dv[ok], want_dv[ok])
 
 
+@pytest.mark.parametrize("name", ["ls_torque", "bc", "bw_torque", "rect_torque", "torque2"])
+def test_chi2_batch_bitwise_wide(name):
+    """A wider sweep for the evaluator's hardware-refined division, square root
+    and log: 4096 models over the whole scale-free cube (its faces x = 0 and
+    x = 1 included), a 400-point curve over 3 days, FP64 form, bitwise."""
+    v, curve, truth, space = _setup(name, "fp64", n_obs=400, span=3.0, seed=29)
+    pb = oracle.Problem(space, curve)
+    rng = np.random.default_rng(8)
+    x = rng.uniform(0.0, 1.0, size=(4096, space.n_params))
+    faces = rng.random(size=x.shape)
+    x[faces < 0.05] = 0.0
+    x[faces > 0.95] = 1.0
+    params, rc = pb.x2params(x, reopt=0)
+    params = np.vstack([truth, params[rc == 0]])
+    want_c, want_dv = pb.chi2(params, form=oracle.CANON)
+    with api.Context(v) as ctx:
+        ctx.set_model_space(space)
+        ctx.upload_curve(curve)
+        got_c, got_dv = ctx.chi2_batch(params)
+    print(name, "n", len(params), "nan", int(np.isnan(want_c).sum()))
+    assert len(params) > 1000
+    np.testing.assert_array_equal(np.isnan(got_c), np.isnan(want_c))
+    ok = ~np.isnan(want_c)
+    assert np.array_equal(got_c[ok], want_c[ok])
+    assert np.array_equal(got_dv[ok], want_dv[ok])
+
+
 @pytest.mark.parametrize("prec", list(PRECS))
 @pytest.mark.parametrize("name,nstages,reopt", [("ls_torque", 2, 0), ("bc", 1, 1), ("bw_torque", 1, 0),
                                                 ("ls_plain", 2, 0), ("trend", 1, 0), ("torque2", 1, 0),
