@@ -64,6 +64,36 @@ def test_mixed_chi2_within_tolerance(name):
     assert rel.max() <= REL_TOL
 
 
+@pytest.mark.parametrize("name", list(CASES))
+def test_mixed_chi2_within_tolerance_whole_cube(name):
+    """The gate over the whole scale-free cube, its faces (x = 0, x = 1)
+    included: the extreme shapes and spins there still stay within 1e-6, and
+    the models whose chi^2 is not finite are the same in both forms."""
+    mk, nf, n_obs, span = CASES[name]
+    v = mk(abi.PREC_MIXED)
+    curve, truth = synth.make_curve(7, v, n_obs, span, n_filters=nf)
+    space = variants.default_model_space(v)
+    pb = oracle.Problem(space, curve)
+    rng = np.random.default_rng(12)
+    x = rng.uniform(0.0, 1.0, size=(2048, space.n_params))
+    faces = rng.random(size=x.shape)
+    x[faces < 0.05] = 0.0
+    x[faces > 0.95] = 1.0
+    params, rc = pb.x2params(x, reopt=0)
+    params = params[rc == 0]
+    want_c, _ = pb.chi2(params, form=oracle.CANON)
+    with api.Context(v) as ctx:
+        ctx.set_model_space(space)
+        ctx.upload_curve(curve)
+        got_c, _ = ctx.chi2_batch(params)
+    ok = np.isfinite(want_c)
+    assert ok.sum() > 1000
+    assert np.array_equal(np.isfinite(got_c), ok)
+    rel = np.abs(got_c[ok] - want_c[ok]) / want_c[ok]
+    print(f"{name}: n {ok.sum()} of {len(params)} finite, max rel chi2 err {rel.max():.2e}")
+    assert rel.max() <= REL_TOL
+
+
 @pytest.mark.parametrize("name", ["ls_torque_400x3d", "bc_818x5d_3f"])
 def test_mixed_fit_best_matches_fp64_chi2(name):
     """A mixed-precision fit's reported best chi^2 is the FP64 chi^2 of its
